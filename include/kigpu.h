@@ -1,0 +1,169 @@
+/*
+ * kigpu.h -- C ABI of the MI355X recursive-skeletonization factor/solve.
+ *
+ * Plain C types only: no exceptions, no std:: types and no torch types cross
+ * this boundary.  Every entry point returns a kigpu_status (0 = success);
+ * kigpu_status_string() names the code and kigpu_last_error() gives the
+ * detail of the last failure on the calling thread.
+ *
+ * Host buffers are caller-owned, FP64, column-major; DOF indices are 32-bit.
+ * Device memory is owned by the handles.  Entry points ending in _device
+ * take device pointers (memory already resident in HBM) and run on the
+ * context's stream.
+ *
+ * The reference interface each entry point replaces is cited beside it
+ * (paths relative to the reference repository root).
+ */
+#ifndef KIGPU_H_
+#define KIGPU_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int kigpu_status;
+enum {
+  KIGPU_OK = 0,
+  KIGPU_ERR_ARG = 1,          /* invalid argument (null pointer, bad size) */
+  KIGPU_ERR_HIP = 2,          /* a HIP runtime call failed */
+  KIGPU_ERR_NOMEM = 3,        /* device allocation failed */
+  KIGPU_ERR_STATE = 4,        /* wrong call order (e.g. solve before factor) */
+  KIGPU_ERR_RANK = 5,         /* ID matrix with fewer rows than columns */
+  KIGPU_ERR_NOCOMPRESS = 6,   /* bordered system with nothing compressed */
+  KIGPU_ERR_NOTSUPP = 7,      /* combination not supported */
+  KIGPU_ERR_NOCONV = 8        /* iterative refinement did not converge */
+};
+
+/* Kernel::Pde (kern_interp/kernel/kernel.h:16-20) */
+enum { KIGPU_LAPLACE = 0, KIGPU_LAPLACE_NEUMANN = 1, KIGPU_STOKES = 2 };
+
+/* arithmetic of the factorization */
+enum {
+  KIGPU_FP64 = 0,         /* everything in FP64 (the reference's arithmetic) */
+  KIGPU_FP32_REFINE = 1   /* FP32 factor/solve; FP64 norms, root LU, residuals */
+};
+
+typedef struct kigpu_ctx kigpu_ctx;
+typedef struct kigpu_tree kigpu_tree;
+
+/* circular hole: kern_interp/boundaries/boundary.h:12-16 */
+typedef struct {
+  double cx, cy, radius;
+  int num_nodes;
+} kigpu_hole;
+
+typedef struct {
+  int pde;               /* KIGPU_LAPLACE / KIGPU_LAPLACE_NEUMANN / KIGPU_STOKES */
+  double id_tol;         /* SkelFactorization::id_tol (skel_factorization.h:21) */
+  int precision;         /* KIGPU_FP64 or KIGPU_FP32_REFINE */
+  int min_dofs;          /* MIN_DOFS_TO_COMPRESS, 16 (skel_factorization.h:11) */
+} kigpu_factor_opts;
+
+typedef struct {
+  int max_iters;         /* refinement steps (FP32_REFINE only); 0 = plain solve */
+  double rel_residual;   /* stop when ||f - K mu|| / ||f|| <= this */
+} kigpu_refine_opts;
+
+typedef struct {
+  int iterations;        /* refinement steps taken */
+  double rel_residual;   /* last FP64 residual measured (-1 if none) */
+} kigpu_solve_info;
+
+typedef struct {
+  int num_levels, num_nodes, num_points, solution_dim, leaf_dofs;
+  double min, max;       /* QuadTree::min/max (quadtree.h:78) */
+} kigpu_tree_info;
+
+typedef struct {
+  double t_tree, t_lists, t_id, t_decouple, t_root, t_solve, t_refine_apply;
+  int syncs;
+} kigpu_timings;
+
+/* ---- context ---------------------------------------------------------- */
+kigpu_status kigpu_context_create(int device, kigpu_ctx** ctx);
+kigpu_status kigpu_context_destroy(kigpu_ctx* ctx);
+const char* kigpu_status_string(kigpu_status s);
+const char* kigpu_last_error(void);
+/* device-side phase timings of the last factor/solve on this context */
+kigpu_status kigpu_last_timings(kigpu_ctx* ctx, kigpu_timings* out);
+
+/* ---- geometry + tree -------------------------------------------------- */
+/* QuadTree::initialize_tree (kern_interp/quadtree/quadtree.cpp:72-114) plus
+ * the Kernel's copies of the boundary arrays (kern_interp/kernel/kernel.h:22-32).
+ * points/normals: 2*npts interleaved; weights/curvatures: npts.
+ * leaf_dofs replaces MAX_LEAF_DOFS (quadtree.h:10).  Built on the GPU. */
+kigpu_status kigpu_tree_build(kigpu_ctx* ctx, const double* points,
+                              const double* normals, const double* weights,
+                              const double* curvatures, int npts, int sd,
+                              int leaf_dofs, kigpu_tree** tree);
+kigpu_status kigpu_tree_destroy(kigpu_tree* tree);
+kigpu_status kigpu_tree_get_info(const kigpu_tree* tree, kigpu_tree_info* info);
+
+/* Node arrays in level order (node id = position in levels[0], levels[1], ...):
+ * level_offsets[num_levels+1]; parent[num_nodes]; children[4*num_nodes] (-1 =
+ * none); is_leaf[num_nodes]; corners[8*num_nodes] ({BL,TL,TR,BR}, quadtree.h:55);
+ * side[num_nodes].  Any pointer may be NULL to skip that array. */
+kigpu_status kigpu_tree_export_nodes(const kigpu_tree* tree, int64_t* level_offsets,
+                                     int* parent, int* children, int* is_leaf,
+                                     double* corners, double* side);
+/* InteractionLists::original_box of every node (quadtree.h:15) as CSR:
+ * offsets[num_nodes+1], then dofs[offsets[num_nodes]].  Call with dofs=NULL
+ * to get the offsets (and so the size) first. */
+kigpu_status kigpu_tree_export_boxes(const kigpu_tree* tree, int64_t* offsets, int* dofs);
+/* QuadTreeNode::neighbors (quadtree.h:50) as CSR of node ids, in the order
+ * compute_neighbor_lists builds them (quadtree.cpp:27-69, 119-161). */
+kigpu_status kigpu_tree_export_neighbors(const kigpu_tree* tree, int64_t* offsets, int* nbrs);
+
+/* ---- factorization ---------------------------------------------------- */
+/* Border matrices for holes (QuadTree::U / QuadTree::Psi, quadtree.h:81),
+ * column-major: U is ndof x nborder, Psi is nborder x ndof.  Set before
+ * kigpu_factor; nborder = 0 clears them.  (linear_solve.cpp:154-163) */
+kigpu_status kigpu_set_border(kigpu_tree* tree, const double* U, const double* Psi,
+                              int nborder);
+/* SkelFactorization::skeletonize (kern_interp/skel_factorization/
+ * skel_factorization.cpp:94-243): one batched launch per tree level. */
+kigpu_status kigpu_factor(kigpu_ctx* ctx, kigpu_tree* tree, const kigpu_factor_opts* opts);
+/* Per-node results: rank[num_nodes] = #skeleton DOFs (-1: not compressed),
+ * active[num_nodes] = |active_box| when the node was processed. */
+kigpu_status kigpu_factor_ranks(const kigpu_tree* tree, int* rank, int* active);
+/* skel / redundant DOF lists of every node as CSR (InteractionLists::skel and
+ * ::redundant, quadtree.h:17-18; filled by set_rs_ranges, quadtree.h:23-35). */
+kigpu_status kigpu_factor_export_lists(const kigpu_tree* tree, int64_t* skel_off, int* skel,
+                                       int64_t* red_off, int* red);
+/* the root skeleton QuadTree::root->dof_lists.active_box after factoring */
+kigpu_status kigpu_factor_allskel(const kigpu_tree* tree, int* n, int* allskel);
+
+/* ---- solve ------------------------------------------------------------ */
+/* SkelFactorization::solve (skel_factorization.cpp:341-403) for nrhs columns,
+ * or multiply_connected_solve (:406-551) when a border is set (then alpha,
+ * nborder x nrhs, receives the hole unknowns).  With precision FP32_REFINE and
+ * refine->max_iters > 0, FP64 iterative refinement against the matrix-free
+ * FP64 kernel apply follows.  Host buffers. */
+kigpu_status kigpu_solve(kigpu_ctx* ctx, kigpu_tree* tree, const double* b, int ldb,
+                         int nrhs, double* x, int ldx, double* alpha,
+                         const kigpu_refine_opts* refine, kigpu_solve_info* info);
+/* the same with device pointers (b, x: ndof x nrhs, leading dimension ndof) */
+kigpu_status kigpu_solve_device(kigpu_ctx* ctx, kigpu_tree* tree, const double* d_b,
+                                int nrhs, double* d_x, double* d_alpha,
+                                const kigpu_refine_opts* refine, kigpu_solve_info* info);
+
+/* ---- matrix-free kernel apply ----------------------------------------- */
+/* y = K(all, all) mu in FP64 on the fly, Kernel::operator() over all DOFs
+ * (kernel.cpp:99-150, 47-96).  mu, y: ndof x nrhs. */
+kigpu_status kigpu_kernel_apply(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
+                                const double* mu, int nrhs, double* y);
+kigpu_status kigpu_kernel_apply_device(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
+                                       const double* d_mu, int nrhs, double* d_y);
+/* u = Kernel::forward() mu (kernel.cpp:337-357): boundary -> domain targets,
+ * targets: 2*ntgt interleaved.  u: (sd*ntgt) x nrhs. */
+kigpu_status kigpu_forward_apply(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
+                                 const double* targets, int ntgt, const double* mu,
+                                 int nrhs, double* u);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KIGPU_H_ */

@@ -1,0 +1,429 @@
+// extern "C" entry points of libkigpu (declared in include/kigpu.h).
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "factor.h"
+
+using namespace kigpu;
+
+struct kigpu_ctx {
+  Context c;
+};
+
+struct kigpu_tree {
+  TreeData t;
+};
+
+namespace kigpu {
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+}  // namespace kigpu
+
+namespace {
+
+template <typename F>
+kigpu_status guard(F&& f) {
+  try {
+    f();
+    return KIGPU_OK;
+  } catch (const HipError& e) {
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error("host allocation failed");
+    return KIGPU_ERR_NOMEM;
+  } catch (...) {
+    set_error("unexpected exception");
+    return KIGPU_ERR_STATE;
+  }
+}
+
+#define KI_REQUIRE(cond, msg)              \
+  do {                                     \
+    if (!(cond)) {                         \
+      set_error(msg);                      \
+      return KIGPU_ERR_ARG;                \
+    }                                      \
+  } while (0)
+
+// get_descendent_neighbors wall test (quadtree.cpp:121-155), host side
+bool wall_touch_host(const double* bc, const double* dc) {
+  double top = bc[3], bottom = bc[1], left = bc[0], right = bc[4];
+  for (int i = 0; i < 8; i += 2) {
+    double x = dc[i], y = dc[i + 1];
+    if (x > left && x < right) {
+      if (std::fabs(y - top) < 1e-14) return true;
+      if (std::fabs(y - bottom) < 1e-14) return true;
+    }
+    if (y < top && y < bottom) {
+      if (std::fabs(x - left) < 1e-14) return true;
+      if (std::fabs(x - right) < 1e-14) return true;
+    }
+  }
+  return false;
+}
+
+void finer_pushes(const TreeData& t, int big, int small, std::vector<int>* out) {
+  if (wall_touch_host(&t.h_corners[8 * (size_t)big], &t.h_corners[8 * (size_t)small]))
+    out->push_back(small);
+  for (int s = 0; s < 4; ++s) {
+    int c = t.h_child[4 * (size_t)small + s];
+    if (c >= 0) finer_pushes(t, big, c, out);
+  }
+}
+
+std::vector<int> full_neighbors(const TreeData& t, int x) {
+  std::vector<int> v;
+  for (int64_t e = t.h_cp_off[x]; e < t.h_cp_off[x + 1]; ++e) v.push_back(t.h_cp_nbr[e]);
+  for (int j = 0; j < t.h_sl_cnt[x]; ++j) v.push_back(t.h_sl_nbr[kMaxNbrs * (size_t)x + j]);
+  if (t.h_is_leaf[x]) {
+    for (int j = 0; j < t.h_sl_cnt[x]; ++j) {
+      int nb = t.h_sl_nbr[kMaxNbrs * (size_t)x + j];
+      for (int s = 0; s < 4; ++s) {
+        int c = t.h_child[4 * (size_t)nb + s];
+        if (c >= 0) finer_pushes(t, x, c, &v);
+      }
+    }
+  }
+  return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* kigpu_status_string(kigpu_status s) {
+  switch (s) {
+    case KIGPU_OK: return "ok";
+    case KIGPU_ERR_ARG: return "invalid argument";
+    case KIGPU_ERR_HIP: return "HIP runtime error";
+    case KIGPU_ERR_NOMEM: return "out of device memory";
+    case KIGPU_ERR_STATE: return "invalid state";
+    case KIGPU_ERR_RANK: return "ID matrix has fewer rows than columns";
+    case KIGPU_ERR_NOCOMPRESS: return "no compression possible";
+    case KIGPU_ERR_NOTSUPP: return "not supported";
+    case KIGPU_ERR_NOCONV: return "refinement did not converge";
+    default: return "unknown status";
+  }
+}
+
+const char* kigpu_last_error(void) { return g_err.c_str(); }
+
+kigpu_status kigpu_context_create(int device, kigpu_ctx** out) {
+  KI_REQUIRE(out, "null output handle");
+  *out = nullptr;
+  auto ctx = std::make_unique<kigpu_ctx>();
+  kigpu_status st = guard([&] {
+    Context& c = ctx->c;
+    c.device = device;
+    KI_HIP(hipSetDevice(device));
+    KI_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    for (auto& e : c.ev) KI_HIP(hipEventCreate(&e));
+    hipDeviceProp_t prop;
+    KI_HIP(hipGetDeviceProperties(&prop, device));
+    c.lds_max = prop.sharedMemPerBlock;
+    // proxy ring unit vectors with the host libm (kernel.cpp:242-252)
+    std::vector<double> ring(2 * kProxyPoints);
+    for (int i = 0; i < kProxyPoints; ++i) {
+      double ang = 2 * M_PI * i * (1.0 / kProxyPoints);
+      ring[i] = std::cos(ang);
+      ring[kProxyPoints + i] = std::sin(ang);
+    }
+    c.ring.upload(ring, c.stream);
+    KI_HIP(hipStreamSynchronize(c.stream));
+  });
+  if (st == KIGPU_OK) *out = ctx.release();
+  return st;
+}
+
+kigpu_status kigpu_context_destroy(kigpu_ctx* ctx) {
+  if (!ctx) return KIGPU_OK;
+  (void)hipStreamSynchronize(ctx->c.stream);
+  ctx->c.ring.release();
+  for (auto& e : ctx->c.ev) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(ctx->c.stream);
+  delete ctx;
+  return KIGPU_OK;
+}
+
+kigpu_status kigpu_last_timings(kigpu_ctx* ctx, kigpu_timings* out) {
+  KI_REQUIRE(ctx && out, "null argument");
+  *out = ctx->c.tim;
+  return KIGPU_OK;
+}
+
+kigpu_status kigpu_tree_build(kigpu_ctx* ctx, const double* points, const double* normals,
+                              const double* weights, const double* curvatures, int npts, int sd,
+                              int leaf_dofs, kigpu_tree** out) {
+  KI_REQUIRE(ctx && out, "null argument");
+  KI_REQUIRE(points && normals && weights && curvatures, "null geometry array");
+  KI_REQUIRE(npts > 0, "number of boundary points to init tree cannot be 0");
+  KI_REQUIRE(sd == 1 || sd == 2, "solution dimension must be 1 or 2");
+  KI_REQUIRE(leaf_dofs >= sd, "leaf_dofs must hold at least one point");
+  *out = nullptr;
+  auto tr = std::make_unique<kigpu_tree>();
+  Context& c = ctx->c;
+  kigpu_status st = guard([&] {
+    TreeData& t = tr->t;
+    t.npts = npts;
+    t.sd = sd;
+    t.leaf_dofs = leaf_dofs;
+    t.ndof = npts * sd;
+    t.h_points.assign(points, points + 2 * (size_t)npts);
+    t.h_normals.assign(normals, normals + 2 * (size_t)npts);
+    t.h_weights.assign(weights, weights + npts);
+    t.h_curv.assign(curvatures, curvatures + npts);
+    std::vector<double> a(npts), b(npts), na(npts), nb(npts);
+    for (int i = 0; i < npts; ++i) {
+      a[i] = points[2 * i];
+      b[i] = points[2 * i + 1];
+      na[i] = normals[2 * i];
+      nb[i] = normals[2 * i + 1];
+    }
+    t.px.upload(a, c.stream);
+    t.py.upload(b, c.stream);
+    t.nx.upload(na, c.stream);
+    t.ny.upload(nb, c.stream);
+    t.w.upload(weights, npts, c.stream);
+    t.kappa.upload(curvatures, npts, c.stream);
+    hipEventRecord(c.ev[10], c.stream);
+    c.tim.syncs = 0;
+    build_tree(t, c.stream, &c.tim.syncs);
+    hipEventRecord(c.ev[11], c.stream);
+    KI_HIP(hipStreamSynchronize(c.stream));
+    float ms = 0;
+    hipEventElapsedTime(&ms, c.ev[10], c.ev[11]);
+    c.tim.t_tree = ms * 1e-3;
+  });
+  if (st == KIGPU_OK) *out = tr.release();
+  return st;
+}
+
+kigpu_status kigpu_tree_destroy(kigpu_tree* tree) {
+  delete tree;
+  return KIGPU_OK;
+}
+
+kigpu_status kigpu_tree_get_info(const kigpu_tree* tree, kigpu_tree_info* info) {
+  KI_REQUIRE(tree && info, "null argument");
+  const TreeData& t = tree->t;
+  info->num_levels = t.nlevels;
+  info->num_nodes = t.nnodes;
+  info->num_points = t.npts;
+  info->solution_dim = t.sd;
+  info->leaf_dofs = t.leaf_dofs;
+  info->min = t.min;
+  info->max = t.max;
+  return KIGPU_OK;
+}
+
+kigpu_status kigpu_tree_export_nodes(const kigpu_tree* tree, int64_t* level_offsets, int* parent,
+                                     int* children, int* is_leaf, double* corners, double* side) {
+  KI_REQUIRE(tree, "null tree");
+  const TreeData& t = tree->t;
+  const size_t nn = t.nnodes;
+  if (level_offsets) std::memcpy(level_offsets, t.level_off.data(), sizeof(int64_t) * (t.nlevels + 1));
+  if (parent) std::memcpy(parent, t.h_parent.data(), sizeof(int) * nn);
+  if (children) std::memcpy(children, t.h_child.data(), sizeof(int) * 4 * nn);
+  if (is_leaf) std::memcpy(is_leaf, t.h_is_leaf.data(), sizeof(int) * nn);
+  if (corners) std::memcpy(corners, t.h_corners.data(), sizeof(double) * 8 * nn);
+  if (side) std::memcpy(side, t.h_side.data(), sizeof(double) * nn);
+  return KIGPU_OK;
+}
+
+kigpu_status kigpu_tree_export_boxes(const kigpu_tree* tree, int64_t* offsets, int* dofs) {
+  KI_REQUIRE(tree && offsets, "null argument");
+  const TreeData& t = tree->t;
+  int64_t o = 0;
+  for (int x = 0; x < t.nnodes; ++x) {
+    offsets[x] = o;
+    const int cnt = t.h_pcount[x];
+    if (dofs) {
+      const int* pp = &t.h_perm[t.h_pstart[x]];
+      for (int j = 0; j < cnt; ++j)
+        for (int c = 0; c < t.sd; ++c) dofs[o + (int64_t)t.sd * j + c] = t.sd * pp[j] + c;
+    }
+    o += (int64_t)cnt * t.sd;
+  }
+  offsets[t.nnodes] = o;
+  return KIGPU_OK;
+}
+
+kigpu_status kigpu_tree_export_neighbors(const kigpu_tree* tree, int64_t* offsets, int* nbrs) {
+  KI_REQUIRE(tree && offsets, "null argument");
+  const TreeData& t = tree->t;
+  int64_t o = 0;
+  for (int x = 0; x < t.nnodes; ++x) {
+    offsets[x] = o;
+    std::vector<int> v = full_neighbors(t, x);
+    if (nbrs) std::memcpy(nbrs + o, v.data(), sizeof(int) * v.size());
+    o += (int64_t)v.size();
+  }
+  offsets[t.nnodes] = o;
+  return KIGPU_OK;
+}
+
+kigpu_status kigpu_set_border(kigpu_tree* tree, const double* U, const double* Psi, int nborder) {
+  KI_REQUIRE(tree, "null tree");
+  KI_REQUIRE(nborder >= 0, "negative border size");
+  TreeData& t = tree->t;
+  if (nborder == 0) {
+    t.nborder = 0;
+    t.h_U.clear();
+    t.h_Psi.clear();
+    return KIGPU_OK;
+  }
+  KI_REQUIRE(U && Psi, "null border matrix");
+  set_error("bordered (multiply connected) systems are not supported yet");
+  return KIGPU_ERR_NOTSUPP;
+}
+
+kigpu_status kigpu_factor(kigpu_ctx* ctx, kigpu_tree* tree, const kigpu_factor_opts* opts) {
+  KI_REQUIRE(ctx && tree && opts, "null argument");
+  KI_REQUIRE(opts->id_tol > 0, "id_tol must be greater than zero");
+  KI_REQUIRE(opts->pde >= KIGPU_LAPLACE && opts->pde <= KIGPU_STOKES, "unknown pde");
+  KI_REQUIRE((opts->pde == KIGPU_STOKES) == (tree->t.sd == 2),
+             "solution dimension does not match the pde");
+  KI_REQUIRE(opts->precision == KIGPU_FP64 || opts->precision == KIGPU_FP32_REFINE,
+             "unknown precision");
+  if (opts->pde == KIGPU_LAPLACE_NEUMANN) {
+    set_error("the Neumann kernel is not supported yet");
+    return KIGPU_ERR_NOTSUPP;
+  }
+  return guard([&] {
+    kigpu_factor_opts o = *opts;
+    if (o.min_dofs <= 0) o.min_dofs = 16;
+    tree->t.factored = false;
+    factor_tree(ctx->c, tree->t, o);
+  });
+}
+
+kigpu_status kigpu_factor_ranks(const kigpu_tree* tree, int* rank, int* active) {
+  KI_REQUIRE(tree, "null tree");
+  const TreeData& t = tree->t;
+  if (!t.factored) {
+    set_error("tree is not factored");
+    return KIGPU_ERR_STATE;
+  }
+  if (rank) std::memcpy(rank, t.h_rank.data(), sizeof(int) * t.nnodes);
+  if (active) std::memcpy(active, t.h_nact.data(), sizeof(int) * t.nnodes);
+  return KIGPU_OK;
+}
+
+kigpu_status kigpu_factor_export_lists(const kigpu_tree* tree, int64_t* skel_off, int* skel,
+                                       int64_t* red_off, int* red) {
+  KI_REQUIRE(tree && skel_off && red_off, "null argument");
+  const TreeData& t = tree->t;
+  if (!t.factored) {
+    set_error("tree is not factored");
+    return KIGPU_ERR_STATE;
+  }
+  int64_t os = 0, orr = 0;
+  for (int x = 0; x < t.nnodes; ++x) {
+    skel_off[x] = os;
+    red_off[x] = orr;
+    const int k = t.h_rank[x];
+    if (k < 0) continue;
+    const int n = t.h_nact[x];
+    const int* sr = &t.h_sr[t.h_act_off[x]];
+    if (skel) std::memcpy(skel + os, sr, sizeof(int) * k);
+    if (red) std::memcpy(red + orr, sr + k, sizeof(int) * (n - k));
+    os += k;
+    orr += n - k;
+  }
+  skel_off[t.nnodes] = os;
+  red_off[t.nnodes] = orr;
+  return KIGPU_OK;
+}
+
+kigpu_status kigpu_factor_allskel(const kigpu_tree* tree, int* n, int* allskel) {
+  KI_REQUIRE(tree && n, "null argument");
+  const TreeData& t = tree->t;
+  if (!t.factored) {
+    set_error("tree is not factored");
+    return KIGPU_ERR_STATE;
+  }
+  *n = t.nroot;
+  if (allskel && t.nroot) std::memcpy(allskel, &t.h_act[t.h_act_off[0]], sizeof(int) * t.nroot);
+  return KIGPU_OK;
+}
+
+kigpu_status kigpu_solve_device(kigpu_ctx* ctx, kigpu_tree* tree, const double* d_b, int nrhs,
+                                double* d_x, double* d_alpha, const kigpu_refine_opts* refine,
+                                kigpu_solve_info* info) {
+  KI_REQUIRE(ctx && tree && d_b && d_x, "null argument");
+  KI_REQUIRE(nrhs > 0, "nrhs must be positive");
+  if (!tree->t.factored) {
+    set_error("solve before factor");
+    return KIGPU_ERR_STATE;
+  }
+  return guard([&] { solve_tree(ctx->c, tree->t, d_b, nrhs, d_x, d_alpha, refine, info); });
+}
+
+kigpu_status kigpu_solve(kigpu_ctx* ctx, kigpu_tree* tree, const double* b, int ldb, int nrhs,
+                         double* x, int ldx, double* alpha, const kigpu_refine_opts* refine,
+                         kigpu_solve_info* info) {
+  KI_REQUIRE(ctx && tree && b && x, "null argument");
+  KI_REQUIRE(nrhs > 0, "nrhs must be positive");
+  const int nd = tree->t.ndof;
+  KI_REQUIRE(ldb >= nd && ldx >= nd, "leading dimension smaller than the number of DOFs");
+  if (!tree->t.factored) {
+    set_error("solve before factor");
+    return KIGPU_ERR_STATE;
+  }
+  (void)alpha;
+  return guard([&] {
+    Context& c = ctx->c;
+    DBuf<double> db, dx;
+    db.alloc((size_t)nd * nrhs);
+    dx.alloc((size_t)nd * nrhs);
+    KI_HIP(hipMemcpy2DAsync(db.p, sizeof(double) * nd, b, sizeof(double) * ldb,
+                            sizeof(double) * nd, nrhs, hipMemcpyHostToDevice, c.stream));
+    solve_tree(c, tree->t, db.p, nrhs, dx.p, nullptr, refine, info);
+    KI_HIP(hipMemcpy2DAsync(x, sizeof(double) * ldx, dx.p, sizeof(double) * nd,
+                            sizeof(double) * nd, nrhs, hipMemcpyDeviceToHost, c.stream));
+    KI_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+kigpu_status kigpu_kernel_apply_device(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
+                                       const double* d_mu, int nrhs, double* d_y) {
+  KI_REQUIRE(ctx && tree && d_mu && d_y && nrhs > 0, "invalid argument");
+  return guard([&] { kernel_apply(ctx->c, tree->t, pde, d_mu, nrhs, d_y); });
+}
+
+kigpu_status kigpu_kernel_apply(kigpu_ctx* ctx, const kigpu_tree* tree, int pde, const double* mu,
+                                int nrhs, double* y) {
+  KI_REQUIRE(ctx && tree && mu && y && nrhs > 0, "invalid argument");
+  return guard([&] {
+    Context& c = ctx->c;
+    const size_t len = (size_t)tree->t.ndof * nrhs;
+    DBuf<double> dm, dy;
+    dm.upload(mu, len, c.stream);
+    dy.alloc(len);
+    kernel_apply(c, tree->t, pde, dm.p, nrhs, dy.p);
+    dy.download(y, len, c.stream);
+    KI_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+kigpu_status kigpu_forward_apply(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
+                                 const double* targets, int ntgt, const double* mu, int nrhs,
+                                 double* u) {
+  KI_REQUIRE(ctx && tree && targets && mu && u && ntgt > 0 && nrhs > 0, "invalid argument");
+  return guard([&] {
+    Context& c = ctx->c;
+    const TreeData& t = tree->t;
+    const int sd = pde == KIGPU_STOKES ? 2 : 1;
+    DBuf<double> dt, dm, du;
+    dt.upload(targets, 2 * (size_t)ntgt, c.stream);
+    dm.upload(mu, (size_t)t.ndof * nrhs, c.stream);
+    du.alloc((size_t)sd * ntgt * nrhs);
+    forward_apply(c, t, pde, dt.p, ntgt, dm.p, nrhs, du.p);
+    du.download(u, (size_t)sd * ntgt * nrhs, c.stream);
+    KI_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,102 @@
+// Shared definitions for the HIP sources of libkigpu (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "kigpu.h"
+
+namespace kigpu {
+
+constexpr int kProxyPoints = 128;     // NUM_PROXY_POINTS, kernel.h:10
+constexpr double kRadiusRatio = 1.5;  // RADIUS_RATIO, kernel.h:11
+constexpr int kMaxNbrs = 8;           // same-level neighbors of a box (8-connectivity)
+constexpr int kWave = 64;
+
+// thread-local error detail returned by kigpu_last_error()
+void set_error(const std::string& msg);
+
+struct HipError {
+  kigpu_status code;
+};
+
+#define KI_HIP(call)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      ::kigpu::set_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+      throw ::kigpu::HipError{e_ == hipErrorOutOfMemory ? KIGPU_ERR_NOMEM   \
+                                                        : KIGPU_ERR_HIP};   \
+    }                                                                       \
+  } while (0)
+
+#define KI_FAIL(code, msg)          \
+  do {                              \
+    ::kigpu::set_error(msg);        \
+    throw ::kigpu::HipError{code};  \
+  } while (0)
+
+// owning device buffer
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count <= n && p) return;
+    release();
+    size_t bytes = count ? count * sizeof(T) : sizeof(T);
+    KI_HIP(hipMalloc(&p, bytes));
+    n = count;
+  }
+  void upload(const T* h, size_t count, hipStream_t s) {
+    alloc(count);
+    if (count) KI_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void upload(const std::vector<T>& h, hipStream_t s) { upload(h.data(), h.size(), s); }
+  void download(T* h, size_t count, hipStream_t s) const {
+    if (count) KI_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+  }
+  void zero(hipStream_t s) {
+    if (n) KI_HIP(hipMemsetAsync(p, 0, n * sizeof(T), s));
+  }
+};
+
+// Boundary arrays on the device, SoA for coalesced gathers.
+struct GeomView {
+  const double* px;
+  const double* py;
+  const double* nx;
+  const double* ny;
+  const double* w;
+  const double* kappa;
+  int npts;
+  int sd;
+};
+
+// device-side scan helpers (scan.hip)
+void exclusive_scan_i64(const int64_t* in, int64_t* out, int n, hipStream_t s);
+void exclusive_scan_i32_to_i64(const int* in, int64_t* out, int n, hipStream_t s);
+
+inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+// allow a kernel to use `bytes` of dynamic LDS (above the default limit)
+template <typename K>
+inline void allow_lds(K kern, size_t bytes) {
+  if (bytes > 0)
+    KI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+}  // namespace kigpu

@@ -1,0 +1,46 @@
+// Factorization / solve internals shared by factor.hip and capi.cpp.
+#pragma once
+
+#include "tree.h"
+
+namespace kigpu {
+
+// one box to compress at a level
+struct IdWork {
+  int node;
+  int n;          // |active_box|
+  int q;          // #near (inner-circle) or outside rows
+  int m;          // ID matrix rows: 2q (+ proxy rows)
+  int proxy;      // proxy block appended
+  int in_lds;     // ID matrix kept in LDS
+  int64_t rows_off;
+  int64_t a_off;  // global workspace offset when !in_lds
+  double cx, cy, rad, pw;
+};
+
+struct LevelWork {
+  int level = 0;
+  int a = 0, b = 0;         // node id range
+  int nmax = 0;             // largest |active_box| on the level
+  DBuf<double> slab64;      // factor slabs of this level's nodes
+  DBuf<float> slab32;
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[16];
+  kigpu_timings tim{};
+  DBuf<double> ring;        // proxy unit vectors cos | sin (host libm)
+  size_t lds_max = 65536;
+};
+
+void factor_tree(Context& c, TreeData& t, const kigpu_factor_opts& o);
+void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_x,
+                double* d_alpha, const kigpu_refine_opts* ro, kigpu_solve_info* info);
+void kernel_apply(Context& c, const TreeData& t, int pde, const double* d_mu, int nrhs,
+                  double* d_y);
+void forward_apply(Context& c, const TreeData& t, int pde, const double* d_tgt, int ntgt,
+                   const double* d_mu, int nrhs, double* d_u);
+
+}  // namespace kigpu

@@ -1,0 +1,505 @@
+// Solve sweeps of the skeletonized factorization (skel_factorization.cpp:
+// 341-403), the FP64 matrix-free kernel apply used for iterative refinement
+// and the boundary -> domain evaluation (kernel.cpp:337-357).
+//
+// The upward pass applies, per compressed box and level (finest first),
+//   x_r -= T^T x_s ;  x_s -= L x_r ;  x_r <- X_rr^{-1} x_r
+// where the third step is the reference's separate D^{-1} pass fused in: a
+// box's redundant DOFs are touched by no later (coarser) box, so the order of
+// the operations on each DOF is unchanged.  Then the root solve and the
+// downward pass, levels coarse to fine:  x_r -= U x_s ;  x_s -= T x_r.
+#include <algorithm>
+#include <cmath>
+
+#include "device_util.h"
+#include "entries.h"
+#include "factor.h"
+
+namespace kigpu {
+
+namespace {
+
+constexpr int kTB = 256;
+constexpr int kNW = kTB / kWave;
+constexpr int kRC = 16;  // right-hand sides per block
+
+template <typename T>
+__global__ __launch_bounds__(kTB) void k_up(int a, const int* rank, const int* nact,
+                                            const int64_t* act_off, const int* sr_pool,
+                                            const int* piv_pool, const uint64_t* fac_ptr, T* x,
+                                            int ldx, int nrhs) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int node = a + blockIdx.x;
+  const int k = rank[node];
+  if (k < 0) return;
+  const int n = nact[node], r = n - k;
+  const int c0 = blockIdx.y * kRC, nc = min(kRC, nrhs - c0);
+  const int* dofs = sr_pool + act_off[node];
+  const int* piv = piv_pool + act_off[node];
+  const T* Tm = reinterpret_cast<const T*>(fac_ptr[node]);
+  const T* Lm = Tm + (size_t)k * r;
+  const T* Xlu = Tm + 3 * (size_t)k * r;
+  T* v = reinterpret_cast<T*>(smem);
+  const int tid = threadIdx.x;
+  for (int e = tid; e < n * nc; e += kTB) {
+    int i = e % n, c = e / n;
+    v[e] = x[dofs[i] + (size_t)(c0 + c) * ldx];
+  }
+  __syncthreads();
+  for (int e = tid; e < r * nc; e += kTB) {  // x_r -= T^T x_s
+    int i = e % r, c = e / r;
+    const T* vs = v + (size_t)c * n;
+    T s = vs[k + i];
+    for (int t = 0; t < k; ++t) s -= Tm[t + (size_t)i * k] * vs[t];
+    v[k + i + (size_t)c * n] = s;
+  }
+  __syncthreads();
+  for (int e = tid; e < k * nc; e += kTB) {  // x_s -= L x_r
+    int t = e % k, c = e / k;
+    const T* vs = v + (size_t)c * n;
+    T s = vs[t];
+    for (int i = 0; i < r; ++i) s -= Lm[t + (size_t)i * k] * vs[k + i];
+    v[t + (size_t)c * n] = s;
+  }
+  __syncthreads();
+  block_lu_solve<T, kNW>(Xlu, r, piv, r, v + k, n, 1, nc, false);  // x_r <- X_rr^{-1} x_r
+  __syncthreads();
+  for (int e = tid; e < n * nc; e += kTB) {
+    int i = e % n, c = e / n;
+    x[dofs[i] + (size_t)(c0 + c) * ldx] = v[e];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTB) void k_down(int a, const int* rank, const int* nact,
+                                              const int64_t* act_off, const int* sr_pool,
+                                              const uint64_t* fac_ptr, T* x, int ldx, int nrhs) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int node = a + blockIdx.x;
+  const int k = rank[node];
+  if (k < 0) return;
+  const int n = nact[node], r = n - k;
+  const int c0 = blockIdx.y * kRC, nc = min(kRC, nrhs - c0);
+  const int* dofs = sr_pool + act_off[node];
+  const T* Tm = reinterpret_cast<const T*>(fac_ptr[node]);
+  const T* Um = Tm + 2 * (size_t)k * r;
+  T* v = reinterpret_cast<T*>(smem);
+  const int tid = threadIdx.x;
+  for (int e = tid; e < n * nc; e += kTB) {
+    int i = e % n, c = e / n;
+    v[e] = x[dofs[i] + (size_t)(c0 + c) * ldx];
+  }
+  __syncthreads();
+  for (int e = tid; e < r * nc; e += kTB) {  // x_r -= U x_s
+    int i = e % r, c = e / r;
+    const T* vs = v + (size_t)c * n;
+    T s = vs[k + i];
+    for (int t = 0; t < k; ++t) s -= Um[i + (size_t)t * r] * vs[t];
+    v[k + i + (size_t)c * n] = s;
+  }
+  __syncthreads();
+  for (int e = tid; e < k * nc; e += kTB) {  // x_s -= T x_r
+    int t = e % k, c = e / k;
+    const T* vs = v + (size_t)c * n;
+    T s = vs[t];
+    for (int i = 0; i < r; ++i) s -= Tm[t + (size_t)i * k] * vs[k + i];
+    v[t + (size_t)c * n] = s;
+  }
+  __syncthreads();
+  for (int e = tid; e < r * nc; e += kTB) {  // only the redundant part changed last
+    int i = e % r, c = e / r;
+    x[dofs[k + i] + (size_t)(c0 + c) * ldx] = v[k + i + (size_t)c * n];
+  }
+  for (int e = tid; e < k * nc; e += kTB) {
+    int t = e % k, c = e / k;
+    x[dofs[t] + (size_t)(c0 + c) * ldx] = v[t + (size_t)c * n];
+  }
+}
+
+// x[allskel] <- allskel_mat^{-1} x[allskel] in FP64, one wave per column.
+// Lane l holds elements l, l + 64, ... (E per lane, n <= 64 E).
+template <typename T, int E>
+__global__ __launch_bounds__(64) void k_root_solve(const int* dofs, int n, const double* lu,
+                                                   const int* piv, T* x, int ldx) {
+  __shared__ double buf[64 * E];
+  const int lane = threadIdx.x;
+  T* xc = x + (size_t)blockIdx.x * ldx;
+  for (int i = lane; i < n; i += 64) buf[i] = (double)xc[dofs[i]];
+  __syncthreads();
+  if (lane == 0)
+    for (int c = 0; c < n; ++c) {
+      int p = piv[c];
+      if (p != c) {
+        double t0 = buf[c];
+        buf[c] = buf[p];
+        buf[p] = t0;
+      }
+    }
+  __syncthreads();
+  double u[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) u[e] = e * 64 + lane < n ? buf[e * 64 + lane] : 0.0;
+#pragma unroll
+  for (int tb = 0; tb < E; ++tb) {
+    for (int tl = 0; tl < 64; ++tl) {
+      const int t = tb * 64 + tl;
+      if (t >= n - 1) break;
+      double ut = __shfl(u[tb], tl, 64);
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        int i = e * 64 + lane;
+        if (i > t && i < n) u[e] -= lu[i + (size_t)t * n] * ut;
+      }
+    }
+  }
+#pragma unroll
+  for (int tb = E - 1; tb >= 0; --tb) {
+    for (int tl = 63; tl >= 0; --tl) {
+      const int t = tb * 64 + tl;
+      if (t >= n) continue;
+      if (lane == tl) u[tb] /= lu[t + (size_t)t * n];
+      double ut = __shfl(u[tb], tl, 64);
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        int i = e * 64 + lane;
+        if (i < t) u[e] -= lu[i + (size_t)t * n] * ut;
+      }
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    int i = e * 64 + lane;
+    if (i < n) xc[dofs[i]] = (T)u[e];
+  }
+}
+
+// ---- FP64 matrix-free apply -----------------------------------------------
+constexpr int kAT = 256;  // targets per block = sources per tile
+constexpr int kAR = 4;    // right-hand sides per pass
+
+struct Targets {
+  const double* x;
+  const double* y;
+  const double* nx;  // null for domain targets (forward)
+  const double* ny;
+  int n;             // target points
+};
+
+// y_part[split] = sum over this split's sources; one thread per target point
+template <int PDE, bool FWD>
+__global__ __launch_bounds__(kAT) void k_apply(Targets tg, GeomView g, const double* mu,
+                                               int ldm, int nrhs, double* part, int ldp,
+                                               int src_per_split) {
+#pragma clang fp contract(off)
+  __shared__ double sx[kAT], sy[kAT], snx[kAT], sny[kAT], sw[kAT], sk[kAT];
+  __shared__ double smu[kAT * 2 * kAR];
+  constexpr int SD = PDE == KIGPU_STOKES ? 2 : 1;
+  const int ti = blockIdx.x * kAT + threadIdx.x;
+  const bool valid = ti < tg.n;
+  const double tx = valid ? tg.x[ti] : 0, ty = valid ? tg.y[ti] : 0;
+  const double tnx = (valid && !FWD) ? tg.nx[ti] : 0, tny = (valid && !FWD) ? tg.ny[ti] : 0;
+  const int s0 = blockIdx.y * src_per_split;
+  const int s1 = min(g.npts, s0 + src_per_split);
+  for (int c0 = 0; c0 < nrhs; c0 += kAR) {
+    const int nc = min(kAR, nrhs - c0);
+    double acc[kAR][SD];
+#pragma unroll
+    for (int c = 0; c < kAR; ++c)
+#pragma unroll
+      for (int a = 0; a < SD; ++a) acc[c][a] = 0;
+    for (int b0 = s0; b0 < s1; b0 += kAT) {
+      const int nb = min(kAT, s1 - b0);
+      __syncthreads();
+      if (threadIdx.x < nb) {
+        int sp = b0 + threadIdx.x;
+        sx[threadIdx.x] = g.px[sp];
+        sy[threadIdx.x] = g.py[sp];
+        snx[threadIdx.x] = g.nx[sp];
+        sny[threadIdx.x] = g.ny[sp];
+        sw[threadIdx.x] = g.w[sp];
+        sk[threadIdx.x] = g.kappa[sp];
+        for (int c = 0; c < nc; ++c)
+          for (int b = 0; b < SD; ++b)
+            smu[(threadIdx.x * SD + b) * kAR + c] = mu[SD * sp + b + (size_t)(c0 + c) * ldm];
+      }
+      __syncthreads();
+      if (!valid) continue;
+      for (int j = 0; j < nb; ++j) {
+        const double r1 = tx - sx[j], r2 = ty - sy[j];
+        const double w = sw[j];
+        if (PDE != KIGPU_STOKES) {
+          double kv;
+          if (FWD && PDE == KIGPU_LAPLACE_NEUMANN) {
+            kv = w * kInv2Pi * log(sqrt(r1 * r1 + r2 * r2));
+          } else {
+            kv = laplace_entry(PDE, r1, r2, w, sk[j], snx[j], sny[j], tnx, tny, FWD);
+          }
+#pragma unroll
+          for (int c = 0; c < kAR; ++c)
+            if (c < nc) acc[c][0] += kv * smu[j * kAR + c];
+        } else {
+#pragma unroll
+          for (int ta = 0; ta < SD; ++ta)
+#pragma unroll
+            for (int sb = 0; sb < SD; ++sb) {
+              double kv = stokes_entry(ta, sb, tx, ty, tnx, tny, sx[j], sy[j], snx[j], sny[j], w,
+                                       sk[j]);
+#pragma unroll
+              for (int c = 0; c < kAR; ++c)
+                if (c < nc) acc[c][ta] += kv * smu[(j * SD + sb) * kAR + c];
+            }
+        }
+      }
+    }
+    if (valid) {
+      for (int c = 0; c < nc; ++c)
+        for (int a = 0; a < SD; ++a)
+          part[(size_t)blockIdx.y * ldp * nrhs + SD * ti + a + (size_t)(c0 + c) * ldp] =
+              acc[c][a];
+    }
+  }
+}
+
+__global__ void k_sum_parts(const double* part, int nsplit, size_t len, double* y) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  double s = 0;
+  for (int p = 0; p < nsplit; ++p) s += part[(size_t)p * len + i];
+  y[i] = s;
+}
+
+// r = f - y (FP64); per-column partial sums of squares of r and f
+__global__ void k_residual(const double* f, const double* y, double* r, int n, int nrhs,
+                           double* sums) {
+  __shared__ double red[2 * kNW];
+  const int c = blockIdx.y;
+  double sr = 0, sf = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double fv = f[i + (size_t)c * n];
+    double rv = fv - y[i + (size_t)c * n];
+    r[i + (size_t)c * n] = rv;
+    sr += rv * rv;
+    sf += fv * fv;
+  }
+  sr = wave_sum(sr);
+  sf = wave_sum(sf);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[wv] = sr;
+    red[kNW + wv] = sf;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0;
+    for (int i = 0; i < kNW; ++i) {
+      a += red[i];
+      b += red[kNW + i];
+    }
+    sums[2 * ((size_t)c * gridDim.x + blockIdx.x)] = a;
+    sums[2 * ((size_t)c * gridDim.x + blockIdx.x) + 1] = b;
+  }
+}
+
+template <typename A, typename B>
+__global__ void k_convert(const A* in, B* out, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (B)in[i];
+}
+
+__global__ void k_axpy_f32(double* x, const float* d, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += (double)d[i];
+}
+
+template <typename T>
+void sweeps(Context& c, TreeData& t, T* x, int ldx, int nrhs) {
+  hipStream_t s = c.stream;
+  const int ncb = ceil_div(nrhs, kRC);
+  const int rcol = std::min(nrhs, kRC);
+  for (int l = t.nlevels - 1; l >= 1; --l) {
+    LevelWork& w = *t.work[l];
+    if (w.nmax == 0) continue;
+    size_t lds = sizeof(T) * (size_t)w.nmax * rcol;
+    allow_lds(k_up<T>, lds);
+    k_up<T><<<dim3(w.b - w.a, ncb), kTB, lds, s>>>(w.a, t.rank.p, t.nact.p, t.act_off.p,
+                                                   t.sr_pool.p, t.piv_pool.p, t.fac_ptr.p, x,
+                                                   ldx, nrhs);
+  }
+  if (t.nroot > 0) {
+    const int n = t.nroot;
+    if (n <= 64)
+      k_root_solve<T, 1><<<nrhs, 64, 0, s>>>(t.root_dofs.p, n, t.root_lu.p, t.root_piv.p, x, ldx);
+    else if (n <= 128)
+      k_root_solve<T, 2><<<nrhs, 64, 0, s>>>(t.root_dofs.p, n, t.root_lu.p, t.root_piv.p, x, ldx);
+    else if (n <= 256)
+      k_root_solve<T, 4><<<nrhs, 64, 0, s>>>(t.root_dofs.p, n, t.root_lu.p, t.root_piv.p, x, ldx);
+    else if (n <= 512)
+      k_root_solve<T, 8><<<nrhs, 64, 0, s>>>(t.root_dofs.p, n, t.root_lu.p, t.root_piv.p, x, ldx);
+    else
+      KI_FAIL(KIGPU_ERR_NOTSUPP, "root skeleton larger than 512 DOFs");
+  }
+  for (int l = 1; l < t.nlevels; ++l) {
+    LevelWork& w = *t.work[l];
+    if (w.nmax == 0) continue;
+    size_t lds = sizeof(T) * (size_t)w.nmax * rcol;
+    allow_lds(k_down<T>, lds);
+    k_down<T><<<dim3(w.b - w.a, ncb), kTB, lds, s>>>(w.a, t.rank.p, t.nact.p, t.act_off.p,
+                                                     t.sr_pool.p, t.fac_ptr.p, x, ldx, nrhs);
+  }
+  KI_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void kernel_apply(Context& c, const TreeData& t, int pde, const double* d_mu, int nrhs,
+                  double* d_y) {
+  hipStream_t s = c.stream;
+  const GeomView g{t.px.p, t.py.p, t.nx.p, t.ny.p, t.w.p, t.kappa.p, t.npts, t.sd};
+  Targets tg{t.px.p, t.py.p, t.nx.p, t.ny.p, t.npts};
+  const int tb = ceil_div(t.npts, kAT);
+  int split = std::max(1, std::min(64, ceil_div(2048, tb)));
+  int per = ceil_div(t.npts, split);
+  per = ceil_div(per, kAT) * kAT;
+  split = ceil_div(t.npts, per);
+  const size_t len = (size_t)t.ndof * nrhs;
+  DBuf<double> part;
+  part.alloc(len * split);
+  dim3 grid(tb, split);
+  if (pde == KIGPU_LAPLACE)
+    k_apply<KIGPU_LAPLACE, false><<<grid, kAT, 0, s>>>(tg, g, d_mu, t.ndof, nrhs, part.p, t.ndof, per);
+  else if (pde == KIGPU_LAPLACE_NEUMANN)
+    k_apply<KIGPU_LAPLACE_NEUMANN, false><<<grid, kAT, 0, s>>>(tg, g, d_mu, t.ndof, nrhs, part.p,
+                                                               t.ndof, per);
+  else
+    k_apply<KIGPU_STOKES, false><<<grid, kAT, 0, s>>>(tg, g, d_mu, t.ndof, nrhs, part.p, t.ndof, per);
+  k_sum_parts<<<ceil_div((long)len, 256), 256, 0, s>>>(part.p, split, len, d_y);
+  KI_HIP(hipGetLastError());
+  KI_HIP(hipStreamSynchronize(s));
+}
+
+void forward_apply(Context& c, const TreeData& t, int pde, const double* d_tgt, int ntgt,
+                   const double* d_mu, int nrhs, double* d_u) {
+  hipStream_t s = c.stream;
+  const GeomView g{t.px.p, t.py.p, t.nx.p, t.ny.p, t.w.p, t.kappa.p, t.npts, t.sd};
+  DBuf<double> tx, ty;
+  tx.alloc(ntgt);
+  ty.alloc(ntgt);
+  std::vector<double> h(2 * (size_t)ntgt), hx(ntgt), hy(ntgt);
+  KI_HIP(hipMemcpyAsync(h.data(), d_tgt, sizeof(double) * 2 * ntgt, hipMemcpyDeviceToHost, s));
+  KI_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < ntgt; ++i) {
+    hx[i] = h[2 * i];
+    hy[i] = h[2 * i + 1];
+  }
+  tx.upload(hx, s);
+  ty.upload(hy, s);
+  Targets tg{tx.p, ty.p, nullptr, nullptr, ntgt};
+  const int sd = pde == KIGPU_STOKES ? 2 : 1;
+  const int tb = ceil_div(ntgt, kAT);
+  int split = std::max(1, std::min(256, ceil_div(2048, tb)));
+  int per = ceil_div(t.npts, split);
+  per = ceil_div(per, kAT) * kAT;
+  split = ceil_div(t.npts, per);
+  const int ldp = sd * ntgt;
+  const size_t len = (size_t)ldp * nrhs;
+  DBuf<double> part;
+  part.alloc(len * split);
+  dim3 grid(tb, split);
+  if (pde == KIGPU_LAPLACE)
+    k_apply<KIGPU_LAPLACE, true><<<grid, kAT, 0, s>>>(tg, g, d_mu, t.ndof, nrhs, part.p, ldp, per);
+  else if (pde == KIGPU_LAPLACE_NEUMANN)
+    k_apply<KIGPU_LAPLACE_NEUMANN, true><<<grid, kAT, 0, s>>>(tg, g, d_mu, t.ndof, nrhs, part.p,
+                                                              ldp, per);
+  else
+    k_apply<KIGPU_STOKES, true><<<grid, kAT, 0, s>>>(tg, g, d_mu, t.ndof, nrhs, part.p, ldp, per);
+  k_sum_parts<<<ceil_div((long)len, 256), 256, 0, s>>>(part.p, split, len, d_u);
+  KI_HIP(hipGetLastError());
+  KI_HIP(hipStreamSynchronize(s));
+}
+
+static void residual_norms(Context& c, const double* f, const double* y, double* r, int n,
+                           int nrhs, double* rel) {
+  hipStream_t s = c.stream;
+  const int nb = 64;
+  DBuf<double> sums;
+  sums.alloc(2 * (size_t)nb * nrhs);
+  k_residual<<<dim3(nb, nrhs), kTB, 0, s>>>(f, y, r, n, nrhs, sums.p);
+  std::vector<double> h(2 * (size_t)nb * nrhs);
+  sums.download(h.data(), h.size(), s);
+  KI_HIP(hipStreamSynchronize(s));
+  double worst = 0;
+  for (int col = 0; col < nrhs; ++col) {
+    double a = 0, b = 0;
+    for (int i = 0; i < nb; ++i) {
+      a += h[2 * ((size_t)col * nb + i)];
+      b += h[2 * ((size_t)col * nb + i) + 1];
+    }
+    worst = std::max(worst, b > 0 ? std::sqrt(a / b) : std::sqrt(a));
+  }
+  *rel = worst;
+}
+
+void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_x, double* d_alpha,
+                const kigpu_refine_opts* ro, kigpu_solve_info* info) {
+  (void)d_alpha;
+  hipStream_t s = c.stream;
+  const size_t len = (size_t)t.ndof * nrhs;
+  if (info) {
+    info->iterations = 0;
+    info->rel_residual = -1;
+  }
+  hipEventRecord(c.ev[6], s);
+  if (t.opts.precision == KIGPU_FP64) {
+    if (d_x != d_b)
+      KI_HIP(hipMemcpyAsync(d_x, d_b, sizeof(double) * len, hipMemcpyDeviceToDevice, s));
+    sweeps<double>(c, t, d_x, t.ndof, nrhs);
+    hipEventRecord(c.ev[7], s);
+    KI_HIP(hipStreamSynchronize(s));
+    float ms = 0;
+    hipEventElapsedTime(&ms, c.ev[6], c.ev[7]);
+    c.tim.t_solve = ms * 1e-3;
+    c.tim.t_refine_apply = 0;
+    return;
+  }
+  // FP32 factor: solve in FP32, then FP64 refinement mu += F^{-1}(f - K mu)
+  t.x32.alloc(len);
+  const int g = ceil_div((long)len, 256);
+  k_convert<double, float><<<g, 256, 0, s>>>(d_b, t.x32.p, len);
+  sweeps<float>(c, t, t.x32.p, t.ndof, nrhs);
+  k_convert<float, double><<<g, 256, 0, s>>>(t.x32.p, d_x, len);
+  hipEventRecord(c.ev[7], s);
+  KI_HIP(hipStreamSynchronize(s));
+  float ms = 0;
+  hipEventElapsedTime(&ms, c.ev[6], c.ev[7]);
+  c.tim.t_solve = ms * 1e-3;
+  c.tim.t_refine_apply = 0;
+  const int iters = ro ? ro->max_iters : 0;
+  if (iters <= 0) return;
+  t.r64.alloc(len);
+  t.f64.alloc(len);
+  double rel = 0;
+  for (int it = 0; it <= iters; ++it) {
+    hipEventRecord(c.ev[8], s);
+    kernel_apply(c, t, t.opts.pde, d_x, nrhs, t.f64.p);
+    residual_norms(c, d_b, t.f64.p, t.r64.p, t.ndof, nrhs, &rel);
+    hipEventRecord(c.ev[9], s);
+    KI_HIP(hipStreamSynchronize(s));
+    hipEventElapsedTime(&ms, c.ev[8], c.ev[9]);
+    c.tim.t_refine_apply += ms * 1e-3;
+    if (info) {
+      info->rel_residual = rel;
+      info->iterations = it;
+    }
+    if (rel <= ro->rel_residual || it == iters) break;
+    hipEventRecord(c.ev[8], s);
+    k_convert<double, float><<<g, 256, 0, s>>>(t.r64.p, t.x32.p, len);
+    sweeps<float>(c, t, t.x32.p, t.ndof, nrhs);
+    k_axpy_f32<<<g, 256, 0, s>>>(d_x, t.x32.p, len);
+    hipEventRecord(c.ev[9], s);
+    KI_HIP(hipStreamSynchronize(s));
+    hipEventElapsedTime(&ms, c.ev[8], c.ev[9]);
+    c.tim.t_solve += ms * 1e-3;
+  }
+}
+
+}  // namespace kigpu

@@ -1,0 +1,86 @@
+// Device-resident quadtree + factorization state owned by a kigpu_tree handle.
+#pragma once
+
+#include <memory>
+
+#include "common.h"
+
+namespace kigpu {
+
+struct LevelWork;  // per-level factor work lists (factor.cpp)
+
+struct TreeData {
+  // ---- geometry (Kernel's copies of the boundary arrays) ----
+  int npts = 0, sd = 1, leaf_dofs = 32, ndof = 0;
+  double min = 0, max = 0;
+  DBuf<double> px, py, nx, ny, w, kappa;
+  std::vector<double> h_points, h_normals, h_weights, h_curv;  // interleaved copies
+
+  // ---- tree, node ids in level order ----
+  int nlevels = 0, nnodes = 0;
+  std::vector<int64_t> level_off;  // nlevels + 1
+  DBuf<int> parent, child, is_leaf, level, slot;
+  DBuf<double> corners, side;
+  DBuf<int64_t> pstart;            // node's points: perm_all[pstart .. pstart + pcount)
+  DBuf<int> pcount, perm_all;
+  DBuf<int> sl_nbr, sl_cnt;        // same-level neighbors, kMaxNbrs per node, in list order
+  DBuf<int64_t> cp_off;            // coarse leaves pushed into a node (CSR)
+  DBuf<int> cp_nbr;
+
+  // host mirrors
+  std::vector<int> h_parent, h_child, h_is_leaf, h_level, h_pcount, h_perm, h_sl_nbr, h_sl_cnt,
+      h_cp_nbr;
+  std::vector<int64_t> h_pstart, h_cp_off;
+  std::vector<double> h_corners, h_side;
+
+  // ---- factorization ----
+  bool factored = false;
+  kigpu_factor_opts opts{};
+  DBuf<int64_t> obox_off;  // original_box DOF list of each leaf in obox_pool
+  DBuf<int> obox_pool;
+  // per-node factor bookkeeping (device), node ids as above
+  DBuf<int64_t> act_off;   // offset of the node's active list in act_pool / sr_pool
+  DBuf<int> nact;          // |active_box|
+  DBuf<int> nrows;         // ID rows (inner-circle or outside DOFs) of the node
+  DBuf<int> rank;          // #skeleton DOFs, -1 = not compressed
+  DBuf<int> osize;         // size of the list handed to the parent
+  DBuf<uint64_t> fac_ptr;  // device address of the node's factor slab
+  DBuf<uint64_t> acc_ptr;  // accumulated Schur update on the parent-facing list (0 = none)
+  DBuf<int> acc_ld;
+  DBuf<int> act_pool;      // active DOF lists
+  DBuf<int> sr_pool;       // active lists permuted to [skel | redundant]
+  DBuf<int> pos_pool;      // the ID pivots (positions in the active list)
+  DBuf<int> piv_pool;      // LU pivots of X_rr
+  DBuf<int> rows_pool;     // ID row lists of the current level
+  DBuf<double> ws64;       // ID / elimination workspace when a box exceeds LDS
+  DBuf<float> ws32;
+  DBuf<int> status;        // device status word
+  int64_t act_pool_size = 0;
+  std::vector<int> h_rank, h_nact;
+  std::vector<int64_t> h_act_off;
+  std::vector<int> h_sr;   // host copy of sr_pool after factoring
+  std::vector<int> h_act;  // host copy of act_pool after factoring
+
+  // root
+  int nroot = 0;
+  DBuf<int> root_dofs;
+  DBuf<double> root_lu;
+  DBuf<int> root_piv;
+
+  // per-level compressed-node lists for the solve sweeps (device), by level
+  std::vector<std::unique_ptr<LevelWork>> work;
+
+  // border (holes)
+  int nborder = 0;
+  std::vector<double> h_U, h_Psi;
+
+  // solve scratch
+  DBuf<double> x64, r64, f64;
+  DBuf<float> x32;
+  DBuf<double> alpha_dev;
+};
+
+// tree.hip
+void build_tree(TreeData& t, hipStream_t s, int* syncs);
+
+}  // namespace kigpu

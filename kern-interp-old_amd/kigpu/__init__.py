@@ -1,0 +1,286 @@
+"""ctypes binding of libkigpu.so (the C ABI declared in include/kigpu.h).
+
+The Python side is plumbing: it moves numpy arrays across the C ABI.  All
+computation happens in the HIP library; there is no CPU fallback, and
+loading fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libkigpu.so")
+
+LAPLACE, LAPLACE_NEUMANN, STOKES = 0, 1, 2
+FP64, FP32_REFINE = 0, 1
+
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+class FactorOpts(C.Structure):
+    _fields_ = [("pde", C.c_int), ("id_tol", C.c_double), ("precision", C.c_int),
+                ("min_dofs", C.c_int)]
+
+
+class RefineOpts(C.Structure):
+    _fields_ = [("max_iters", C.c_int), ("rel_residual", C.c_double)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("rel_residual", C.c_double)]
+
+
+class TreeInfo(C.Structure):
+    _fields_ = [("num_levels", C.c_int), ("num_nodes", C.c_int), ("num_points", C.c_int),
+                ("solution_dim", C.c_int), ("leaf_dofs", C.c_int), ("min", C.c_double),
+                ("max", C.c_double)]
+
+
+class Timings(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("t_tree", "t_lists", "t_id", "t_decouple", "t_root",
+                                          "t_solve", "t_refine_apply")] + [("syncs", C.c_int)]
+
+
+# every symbol include/kigpu.h declares
+EXPORTS = [
+    "kigpu_context_create", "kigpu_context_destroy", "kigpu_status_string", "kigpu_last_error",
+    "kigpu_last_timings", "kigpu_tree_build", "kigpu_tree_destroy", "kigpu_tree_get_info",
+    "kigpu_tree_export_nodes", "kigpu_tree_export_boxes", "kigpu_tree_export_neighbors",
+    "kigpu_set_border", "kigpu_factor", "kigpu_factor_ranks", "kigpu_factor_export_lists",
+    "kigpu_factor_allskel", "kigpu_solve", "kigpu_solve_device", "kigpu_kernel_apply",
+    "kigpu_kernel_apply_device", "kigpu_forward_apply",
+]
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libkigpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with "
+                           "`python kern-interp-old_amd/build.py` (or __graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    sig = {
+        "kigpu_context_create": (C.c_int, [C.c_int, C.POINTER(P)]),
+        "kigpu_context_destroy": (C.c_int, [P]),
+        "kigpu_status_string": (C.c_char_p, [C.c_int]),
+        "kigpu_last_error": (C.c_char_p, []),
+        "kigpu_last_timings": (C.c_int, [P, C.POINTER(Timings)]),
+        "kigpu_tree_build": (C.c_int, [P, _f64p, _f64p, _f64p, _f64p, C.c_int, C.c_int, C.c_int,
+                                       C.POINTER(P)]),
+        "kigpu_tree_destroy": (C.c_int, [P]),
+        "kigpu_tree_get_info": (C.c_int, [P, C.POINTER(TreeInfo)]),
+        "kigpu_tree_export_nodes": (C.c_int, [P, _i64p, _i32p, _i32p, _i32p, _f64p, _f64p]),
+        "kigpu_tree_export_boxes": (C.c_int, [P, _i64p, P]),
+        "kigpu_tree_export_neighbors": (C.c_int, [P, _i64p, P]),
+        "kigpu_set_border": (C.c_int, [P, P, P, C.c_int]),
+        "kigpu_factor": (C.c_int, [P, P, C.POINTER(FactorOpts)]),
+        "kigpu_factor_ranks": (C.c_int, [P, _i32p, _i32p]),
+        "kigpu_factor_export_lists": (C.c_int, [P, _i64p, P, _i64p, P]),
+        "kigpu_factor_allskel": (C.c_int, [P, C.POINTER(C.c_int), P]),
+        "kigpu_solve": (C.c_int, [P, P, _f64p, C.c_int, C.c_int, _f64p, C.c_int, P,
+                                  C.POINTER(RefineOpts), C.POINTER(SolveInfo)]),
+        "kigpu_solve_device": (C.c_int, [P, P, P, C.c_int, P, P, C.POINTER(RefineOpts),
+                                         C.POINTER(SolveInfo)]),
+        "kigpu_kernel_apply": (C.c_int, [P, P, C.c_int, _f64p, C.c_int, _f64p]),
+        "kigpu_kernel_apply_device": (C.c_int, [P, P, C.c_int, P, C.c_int, P]),
+        "kigpu_forward_apply": (C.c_int, [P, P, C.c_int, _f64p, C.c_int, _f64p, C.c_int, _f64p]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+class KigpuError(RuntimeError):
+    def __init__(self, code, what):
+        lib = load()
+        msg = lib.kigpu_status_string(code).decode()
+        detail = lib.kigpu_last_error().decode()
+        super().__init__(f"{what}: {msg} ({detail})")
+        self.code = code
+
+
+def _check(code, what):
+    if code != 0:
+        raise KigpuError(code, what)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    """A device + HIP stream (kigpu_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        _check(self.lib.kigpu_context_create(device, C.byref(h)), "kigpu_context_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.kigpu_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def timings(self) -> dict:
+        t = Timings()
+        _check(self.lib.kigpu_last_timings(self.h, C.byref(t)), "kigpu_last_timings")
+        return {n: getattr(t, n) for n, _ in Timings._fields_}
+
+    def build_tree(self, points, normals, weights, curvatures, sd=1, leaf_dofs=32):
+        return Tree(self, points, normals, weights, curvatures, sd, leaf_dofs)
+
+
+class Tree:
+    """Geometry + quadtree + factorization handle (kigpu_tree)."""
+
+    def __init__(self, ctx, points, normals, weights, curvatures, sd, leaf_dofs):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        pts, nrm = _f64(points).reshape(-1), _f64(normals).reshape(-1)
+        w, k = _f64(weights), _f64(curvatures)
+        self.npts = w.size
+        self.sd = sd
+        self.ndof = sd * self.npts
+        h = C.c_void_p()
+        _check(self.lib.kigpu_tree_build(ctx.h, pts, nrm, w, k, self.npts, sd, leaf_dofs,
+                                         C.byref(h)), "kigpu_tree_build")
+        self.h = h
+        self.pde = None
+
+    def close(self):
+        if self.h:
+            self.lib.kigpu_tree_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        i = TreeInfo()
+        _check(self.lib.kigpu_tree_get_info(self.h, C.byref(i)), "kigpu_tree_get_info")
+        return {n: getattr(i, n) for n, _ in TreeInfo._fields_}
+
+    def export_nodes(self) -> dict:
+        inf = self.info()
+        L, N = inf["num_levels"], inf["num_nodes"]
+        out = {"level_offsets": np.zeros(L + 1, np.int64), "parent": np.zeros(N, np.int32),
+               "children": np.zeros(4 * N, np.int32), "is_leaf": np.zeros(N, np.int32),
+               "corners": np.zeros(8 * N), "side": np.zeros(N)}
+        _check(self.lib.kigpu_tree_export_nodes(self.h, out["level_offsets"], out["parent"],
+                                                out["children"], out["is_leaf"], out["corners"],
+                                                out["side"]), "kigpu_tree_export_nodes")
+        out["children"] = out["children"].reshape(N, 4)
+        out["corners"] = out["corners"].reshape(N, 8)
+        return out
+
+    def _csr(self, fn, what):
+        N = self.info()["num_nodes"]
+        off = np.zeros(N + 1, np.int64)
+        _check(fn(self.h, off, None), what)
+        data = np.zeros(max(int(off[-1]), 1), np.int32)
+        _check(fn(self.h, off, data.ctypes.data_as(C.c_void_p)), what)
+        return off, data[: off[-1]]
+
+    def export_boxes(self):
+        return self._csr(self.lib.kigpu_tree_export_boxes, "kigpu_tree_export_boxes")
+
+    def export_neighbors(self):
+        return self._csr(self.lib.kigpu_tree_export_neighbors, "kigpu_tree_export_neighbors")
+
+    def factor(self, pde=LAPLACE, id_tol=1e-6, precision=FP64, min_dofs=16):
+        o = FactorOpts(pde, id_tol, precision, min_dofs)
+        _check(self.lib.kigpu_factor(self.ctx.h, self.h, C.byref(o)), "kigpu_factor")
+        self.pde = pde
+
+    def ranks(self):
+        N = self.info()["num_nodes"]
+        r = np.zeros(N, np.int32)
+        a = np.zeros(N, np.int32)
+        _check(self.lib.kigpu_factor_ranks(self.h, r, a), "kigpu_factor_ranks")
+        return r, a
+
+    def lists(self):
+        N = self.info()["num_nodes"]
+        so = np.zeros(N + 1, np.int64)
+        ro = np.zeros(N + 1, np.int64)
+        _check(self.lib.kigpu_factor_export_lists(self.h, so, None, ro, None), "export_lists")
+        s = np.zeros(max(int(so[-1]), 1), np.int32)
+        r = np.zeros(max(int(ro[-1]), 1), np.int32)
+        _check(self.lib.kigpu_factor_export_lists(self.h, so, s.ctypes.data_as(C.c_void_p), ro,
+                                                  r.ctypes.data_as(C.c_void_p)), "export_lists")
+        return so, s[: so[-1]], ro, r[: ro[-1]]
+
+    def allskel(self):
+        n = C.c_int()
+        _check(self.lib.kigpu_factor_allskel(self.h, C.byref(n), None), "allskel")
+        a = np.zeros(max(n.value, 1), np.int32)
+        _check(self.lib.kigpu_factor_allskel(self.h, C.byref(n), a.ctypes.data_as(C.c_void_p)),
+               "allskel")
+        return a[: n.value]
+
+    def solve(self, b, refine_iters=0, rel_residual=1e-10):
+        b = np.asarray(b, np.float64)
+        one = b.ndim == 1
+        B = np.asfortranarray(b.reshape(self.ndof, -1))
+        nrhs = B.shape[1]
+        bflat = np.ascontiguousarray(B.T).reshape(-1)   # column-major
+        x = np.zeros_like(bflat)
+        ro = RefineOpts(refine_iters, rel_residual)
+        info = SolveInfo()
+        _check(self.lib.kigpu_solve(self.ctx.h, self.h, bflat, self.ndof, nrhs, x, self.ndof,
+                                    None, C.byref(ro), C.byref(info)), "kigpu_solve")
+        X = x.reshape(nrhs, self.ndof).T
+        self.last_info = {"iterations": info.iterations, "rel_residual": info.rel_residual}
+        return X[:, 0].copy() if one else np.array(X)
+
+    def apply(self, mu, pde=None):
+        pde = self.pde if pde is None else pde
+        mu = np.asarray(mu, np.float64)
+        one = mu.ndim == 1
+        M = mu.reshape(self.ndof, -1)
+        nrhs = M.shape[1]
+        flat = np.ascontiguousarray(M.T).reshape(-1)
+        y = np.zeros_like(flat)
+        _check(self.lib.kigpu_kernel_apply(self.ctx.h, self.h, pde, flat, nrhs, y),
+               "kigpu_kernel_apply")
+        Y = y.reshape(nrhs, self.ndof).T
+        return Y[:, 0].copy() if one else np.array(Y)
+
+    def forward(self, targets, mu, pde=None):
+        pde = self.pde if pde is None else pde
+        t = _f64(targets).reshape(-1)
+        ntgt = t.size // 2
+        mu = np.asarray(mu, np.float64)
+        one = mu.ndim == 1
+        M = mu.reshape(self.ndof, -1)
+        nrhs = M.shape[1]
+        flat = np.ascontiguousarray(M.T).reshape(-1)
+        sd = 2 if pde == STOKES else 1
+        u = np.zeros(sd * ntgt * nrhs)
+        _check(self.lib.kigpu_forward_apply(self.ctx.h, self.h, pde, t, ntgt, flat, nrhs, u),
+               "kigpu_forward_apply")
+        U = u.reshape(nrhs, sd * ntgt).T
+        return U[:, 0].copy() if one else np.array(U)

@@ -1,0 +1,120 @@
+"""GPU factor/solve vs the oracle on the same inputs.
+
+Bars: per-box ranks, skeleton lists (ordered) and the root skeleton identical
+to the oracle on the FP64 path; redundant lists equal as sets (the GPU QR
+stops at the rank-revealing step, so their order is not the dgeqp3 order);
+mu within 1e-9 relative of the oracle's (both factor the same compressed
+operator; differences are rounding); backward error <= 10 id_tol
+(kern_interp_test.cpp:90-93)."""
+import numpy as np
+import pytest
+
+import kigpu
+from oracle import geometry as G
+from oracle import kernel as K
+from oracle import linear_solve as LS
+
+from helpers import exact_point_source, gpu_tree, star_problem
+
+
+def oracle_arrays(tree):
+    nodes = list(tree.all_nodes())
+    rank = np.array([len(n.skel) if n.compressed else -1 for n in nodes], np.int32)
+    return nodes, rank
+
+
+def check_factor_parity(ctx, b, pde, sd, leaf, tol):
+    otree, okern, ofac = LS.factor(b, pde, tol, leaf)
+    t = gpu_tree(ctx, b, sd, leaf)
+    t.factor(pde, tol, kigpu.FP64)
+    rank, active = t.ranks()
+    nodes, orank = oracle_arrays(otree)
+    assert np.array_equal(rank, orank), "per-box ranks differ"
+    so, s, ro, r = t.lists()
+    for i, nd in enumerate(nodes):
+        if not nd.compressed:
+            continue
+        assert active[i] == len(nd.active_box)
+        assert list(s[so[i]:so[i + 1]]) == nd.skel
+        assert sorted(r[ro[i]:ro[i + 1]]) == sorted(nd.redundant)
+    assert list(t.allskel()) == otree.allskel
+    return otree, ofac, t
+
+
+@pytest.mark.gpu
+def test_star4096_fp64_parity(ctx):
+    print("[factor] star N=4096 leaf 64 tol 1e-6 FP64", flush=True)
+    b = star_problem(4096)
+    otree, ofac, t = check_factor_parity(ctx, b, K.LAPLACE, 1, 64, 1e-6)
+    f = b.boundary_values
+    mu = t.solve(f)
+    mu_ref = ofac.solve(otree, f)
+    assert np.linalg.norm(mu - mu_ref) / np.linalg.norm(mu_ref) < 1e-9
+    assert LS.backward_error(b, K.LAPLACE, mu) <= 10 * 1e-6
+    tgt = G.interior_disc_targets(1000, 1234)
+    u = t.forward(tgt, mu)
+    ex = exact_point_source(tgt)
+    assert np.linalg.norm(u - ex) / np.linalg.norm(ex) <= 1e-6
+
+
+@pytest.mark.gpu
+def test_circle_leaf32_parity(ctx):
+    print("[factor] circle N=1024 leaf 32 FP64", flush=True)
+    b = G.circle(1024)
+    b.boundary_values = G.bc_single_electron(b)
+    otree, ofac, t = check_factor_parity(ctx, b, K.LAPLACE, 1, 32, 1e-6)
+    mu = t.solve(b.boundary_values)
+    assert LS.backward_error(b, K.LAPLACE, mu) <= 10 * 1e-6
+
+
+@pytest.mark.gpu
+def test_multi_rhs(ctx):
+    print("[factor] star N=4096 16 RHS", flush=True)
+    b = star_problem(4096)
+    t = gpu_tree(ctx, b)
+    t.factor(K.LAPLACE, 1e-6, kigpu.FP64)
+    src = G.point_sources(16, 99)
+    F = np.stack([G.bc_point_source(b, tuple(s)) for s in src], 1)
+    X = t.solve(F)
+    for j in range(F.shape[1]):
+        xj = t.solve(F[:, j])
+        assert np.allclose(X[:, j], xj, rtol=0, atol=1e-13 * np.abs(xj).max())
+
+
+@pytest.mark.gpu
+def test_fp32_refine(ctx):
+    print("[factor] star N=4096 FP32 + refinement", flush=True)
+    b = star_problem(4096)
+    t = gpu_tree(ctx, b)
+    t.factor(K.LAPLACE, 1e-6, kigpu.FP32_REFINE)
+    f = b.boundary_values
+    mu = t.solve(f, refine_iters=10, rel_residual=1e-10)
+    assert t.last_info["rel_residual"] <= 1e-10
+    kern = K.Kernel(1, K.LAPLACE, b)
+    Kd = kern(np.arange(4096), np.arange(4096))
+    assert np.linalg.norm(Kd @ mu - f) / np.linalg.norm(f) <= 2e-10
+    tgt = G.interior_disc_targets(1000, 1234)
+    ex = exact_point_source(tgt)
+    u = t.forward(tgt, mu)
+    assert np.linalg.norm(u - ex) / np.linalg.norm(ex) <= 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pde", [K.LAPLACE, K.STOKES])
+def test_apply_matches_dense(ctx, pde):
+    print(f"[apply] pde {pde}", flush=True)
+    b = G.circle(512) if pde == K.STOKES else G.star(1024)
+    sd = 2 if pde == K.STOKES else 1
+    t = gpu_tree(ctx, b, sd, 32)
+    nd = sd * b.num_points
+    rng = np.random.default_rng(5)
+    mu = rng.standard_normal((nd, 3))
+    kern = K.Kernel(sd, pde, b)
+    Kd = kern(np.arange(nd), np.arange(nd))
+    y = t.apply(mu, pde)
+    assert np.allclose(y, Kd @ mu, rtol=0, atol=1e-12 * np.abs(Kd @ mu).max())
+    tgt = G.interior_disc_targets(300, 3) * 0.2 + (0.5 if pde == K.STOKES else 0.0)
+    kd = K.Kernel(sd, pde, b, tgt)
+    u = t.forward(tgt, mu, pde)
+    ref = kd.forward() @ mu
+    assert np.allclose(u, ref, rtol=0, atol=1e-12 * np.abs(ref).max())

@@ -1,0 +1,45 @@
+"""GPU quadtree vs the sequential-insertion oracle: bit-exact node order,
+corners, DOF lists and neighbor lists (quadtree.cpp:27-285)."""
+import numpy as np
+import pytest
+
+from oracle import geometry as G
+from oracle.tree import QuadTree
+
+from helpers import gpu_tree
+
+CASES = [
+    ("star4096_leaf64", lambda: G.star(4096), 1, 64),
+    ("circle1024_leaf32", lambda: G.circle(1024), 1, 32),
+    ("annulus1024_leaf32", lambda: G.annulus(1024), 1, 32),
+    ("circle1024_stokes_leaf32", lambda: G.circle(1024), 2, 32),
+    ("holes_stokes", lambda: G.multi_hole_disc(2048, 256, 8, 7), 2, 64),
+    ("rstar_leaf64", lambda: G.random_star(3008, 11), 1, 64),
+    ("tiny3", lambda: G.circle(3), 1, 32),
+    ("star65536_leaf64", lambda: G.star(65536), 1, 64),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,make,sd,leaf", CASES, ids=[c[0] for c in CASES])
+def test_tree_bit_exact(ctx, name, make, sd, leaf):
+    print(f"[tree] {name}", flush=True)
+    b = make()
+    ref = QuadTree(b.points, sd, leaf).export()
+    t = gpu_tree(ctx, b, sd, leaf)
+    got = t.export_nodes()
+    assert np.array_equal(got["level_offsets"], ref["level_offsets"])
+    assert np.array_equal(got["parent"], ref["parent"])
+    assert np.array_equal(got["children"], ref["children"])
+    assert np.array_equal(got["is_leaf"], ref["is_leaf"])
+    # corners and sides are compared bit for bit
+    assert np.array_equal(got["corners"].view(np.int64), ref["corners"].view(np.int64))
+    assert np.array_equal(got["side"].view(np.int64), ref["side"].view(np.int64))
+    off, dofs = t.export_boxes()
+    assert np.array_equal(off, ref["box_offsets"])
+    assert np.array_equal(dofs, ref["box_dofs"])
+    noff, nbrs = t.export_neighbors()
+    assert np.array_equal(noff, ref["nbr_offsets"])
+    assert np.array_equal(nbrs, ref["nbrs"])
+    info = t.info()
+    assert info["min"] == float(np.min(b.points)) and info["max"] == float(np.max(b.points))
