@@ -1,0 +1,199 @@
+"""Factor + solve benchmark on one MI355X.
+
+Workload: 2D Laplace interior Dirichlet problem, double-layer kernel on the star
+r(theta) = 1 + 0.3 cos(5 theta), N = 65536 trapezoidal nodes, leaf 64, ID tol
+1e-6, boundary data from a point source at (3, 3).  One step = quadtree
+build + skeletonization + solve (+ FP64 iterative refinement for the FP32
+path), from geometry and right-hand side resident in HBM to the density in
+HBM.  Both arithmetic paths are timed: FP64, and FP32 elimination/solve with
+FP64 ID and FP64 iterative refinement to relative residual 1e-10.
+
+Prints one JSON line.  `value` is DOF/s = N / (seconds per step) of the
+FP32+refinement path; the FP64 path's numbers sit beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+# the CPU baseline runs the oracle with single-threaded BLAS
+os.environ["OPENBLAS_NUM_THREADS"] = "1"
+os.environ.setdefault("MKL_NUM_THREADS", "1")
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "kern-interp-old_amd"))
+
+import numpy as np  # noqa: E402
+
+PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=65536)
+    ap.add_argument("--nrhs", type=int, default=1)
+    ap.add_argument("--leaf", type=int, default=64)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-n", type=int, default=65536)
+    return ap.parse_args()
+
+
+class DeviceProblem:
+    """Geometry and right-hand sides resident in device memory (torch is only
+    the allocator here; the library runs on its own HIP stream)."""
+
+    def __init__(self, torch, b, F):
+        dev = torch.device("cuda:0")
+        xy = b.points.reshape(-1, 2)
+        nm = b.normals.reshape(-1, 2)
+        self.arrays = [torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
+                       for a in (xy[:, 0], xy[:, 1], nm[:, 0], nm[:, 1], b.weights, b.curvatures)]
+        self.F = torch.tensor(np.ascontiguousarray(F.T), dtype=torch.float64, device=dev)
+        self.X = torch.zeros_like(self.F)
+        self.n = b.num_points
+        self.nrhs = F.shape[1]
+        torch.cuda.synchronize()
+
+
+def run_step(kigpu, ctx, prob, precision, leaf, tol):
+    import ctypes as C
+    lib = ctx.lib
+    h = C.c_void_p()
+    ptrs = [C.c_void_p(a.data_ptr()) for a in prob.arrays]
+    kigpu._check(lib.kigpu_tree_build_device(ctx.h, *ptrs, prob.n, 1, leaf, C.byref(h)),
+                 "kigpu_tree_build_device")
+    tim_tree = ctx.timings()["t_tree"]
+    opts = kigpu.FactorOpts(kigpu.LAPLACE, tol, precision, 16)
+    kigpu._check(lib.kigpu_factor(ctx.h, h, C.byref(opts)), "kigpu_factor")
+    ro = kigpu.RefineOpts(20 if precision == kigpu.FP32_REFINE else 0, 1e-10)
+    info = kigpu.SolveInfo()
+    kigpu._check(lib.kigpu_solve_device(ctx.h, h, C.c_void_p(prob.F.data_ptr()), prob.nrhs,
+                                        C.c_void_p(prob.X.data_ptr()), None, C.byref(ro),
+                                        C.byref(info)), "kigpu_solve_device")
+    tim = ctx.timings()
+    tim["t_tree"] = tim_tree
+    return h, tim, {"iterations": info.iterations, "rel_residual": info.rel_residual}
+
+
+def time_path(torch, kigpu, ctx, prob, precision, args):
+    lib = ctx.lib
+    for _ in range(args.warmup):
+        h, _, _ = run_step(kigpu, ctx, prob, precision, args.leaf, args.tol)
+        lib.kigpu_tree_destroy(h)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tims = []
+    info = None
+    h = None
+    for i in range(args.steps):
+        if h is not None:
+            lib.kigpu_tree_destroy(h)
+        h, tim, info = run_step(kigpu, ctx, prob, precision, args.leaf, args.tol)
+        tims.append(tim)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    mu = prob.X.cpu().numpy().T.copy()
+    return dt, tims, info, h, mu
+
+
+def interior_error(kigpu, ctx, h, mu, b):
+    from oracle import geometry as G
+    import ctypes as C
+    tgt = G.interior_disc_targets(1000, 1234)
+    xy = tgt.reshape(-1, 2)
+    ex = np.log(np.sqrt((xy[:, 0] - 3) ** 2 + (xy[:, 1] - 3) ** 2)) / (2 * np.pi)
+    u = np.zeros(1000)
+    m0 = np.ascontiguousarray(mu[:, 0])
+    kigpu._check(ctx.lib.kigpu_forward_apply(ctx.h, h, kigpu.LAPLACE, tgt, 1000, m0, 1, u),
+                 "kigpu_forward_apply")
+    return float(np.linalg.norm(u - ex) / np.linalg.norm(ex))
+
+
+def cpu_baseline(n, leaf, tol):
+    """The oracle (numpy/scipy restatement, single-threaded BLAS) on the same workload."""
+    from oracle import geometry as G
+    from oracle import kernel as K
+    from oracle import linear_solve as LS
+    b = G.star(n)
+    b.boundary_values = G.bc_point_source(b, (3.0, 3.0))
+    t0 = time.perf_counter()
+    tree, kern, fac = LS.factor(b, K.LAPLACE, tol, leaf)
+    mu, _ = LS.linear_solve(fac, tree, b.boundary_values)
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "DOF/s", "cores": 1, "kind": "port",
+            "sample": f"one FP64 tree+factor+solve of the same star workload at N={n} "
+                      f"(numpy/scipy oracle, OpenBLAS 1 thread): {dt:.2f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path "
+              f"(--gpus {args.gpus} requested)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import kigpu
+    from oracle import geometry as G
+
+    b = G.star(args.n)
+    srcs = [(3.0, 3.0)] + [tuple(s) for s in G.point_sources(max(args.nrhs - 1, 0), 7)]
+    F = np.stack([G.bc_point_source(b, s) for s in srcs[: args.nrhs]], 1)
+    ctx = kigpu.Context(0)
+    prob = DeviceProblem(torch, b, F)
+
+    res = {}
+    for name, prec in (("fp64", kigpu.FP64), ("fp32_refine", kigpu.FP32_REFINE)):
+        dt, tims, info, h, mu = time_path(torch, kigpu, ctx, prob, prec, args)
+        err = interior_error(kigpu, ctx, h, mu, b)
+        ctx.lib.kigpu_tree_destroy(h)
+        tm = {k: float(np.median([t[k] for t in tims])) for k in tims[0]}
+        res[name] = {"s_per_step": dt, "dof_per_s": args.n / dt, "interior_rel_err": err,
+                     "refine": info, "phases_s": {k: tm[k] for k in (
+                         "t_tree", "t_lists", "t_id", "t_decouple", "t_root", "t_solve",
+                         "t_refine_apply")}, "timings": tm}
+
+    # roofline of the dominant kernel (the LDS-resident ID kernel, FP64)
+    tm = res["fp64"]["timings"]
+    per_launch_flops = tm["id_flops"] / max(tm["id_launches"], 1)
+    per_launch_s = tm["id_kernel_s"] / max(tm["id_launches"], 1)
+    achieved = per_launch_flops / per_launch_s / 1e12 if per_launch_s > 0 else 0.0
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                "kernel": "k_id<double,double,true> (ID matrix + pivoted QR + T, FP64)",
+                "avg_launch_us": per_launch_s * 1e6, "launches_per_factor": tm["id_launches"]}
+
+    fast = res["fp32_refine"]
+    out = {
+        "metric": "factor+solve s & DOF/s at N=65536 FP32+refine vs FP64 GPU, rel err<=1e-6",
+        "value": fast["dof_per_s"], "unit": "DOF/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": fast["s_per_step"] * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 elimination/solve + f64 ID, root LU and refinement",
+        "data": "synthetic",
+        "config": {"workload": f"laplace_star_N{args.n}_leaf{args.leaf}_tol{args.tol:g}",
+                   "N": args.n, "nrhs": args.nrhs, "leaf": args.leaf, "id_tol": args.tol,
+                   "refine_to": 1e-10},
+        "fp64": {k: res["fp64"][k] for k in ("s_per_step", "dof_per_s", "interior_rel_err",
+                                               "phases_s")},
+        "fp32_refine": {k: fast[k] for k in ("s_per_step", "dof_per_s", "interior_rel_err",
+                                             "refine", "phases_s")},
+        "speedup_fp32_refine_vs_fp64": res["fp64"]["s_per_step"] / fast["s_per_step"],
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.leaf, args.tol)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

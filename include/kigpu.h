@@ -77,8 +77,16 @@ typedef struct {
 } kigpu_tree_info;
 
 typedef struct {
+  /* device time per phase of the last tree build / factor / solve (seconds) */
   double t_tree, t_lists, t_id, t_decouple, t_root, t_solve, t_refine_apply;
-  int syncs;
+  int syncs;             /* host synchronizations in the last tree build + factor */
+  /* the ID kernel (ID matrix generation + pivoted QR + T), LDS-resident
+   * variant: launches, summed launch durations (s) and algorithmic flops */
+  int id_launches;
+  double id_kernel_s, id_flops;
+  /* the same for the elimination kernel */
+  int dec_launches;
+  double dec_kernel_s, dec_flops;
 } kigpu_timings;
 
 /* ---- context ---------------------------------------------------------- */
@@ -98,6 +106,12 @@ kigpu_status kigpu_tree_build(kigpu_ctx* ctx, const double* points,
                               const double* normals, const double* weights,
                               const double* curvatures, int npts, int sd,
                               int leaf_dofs, kigpu_tree** tree);
+/* the same from device arrays (SoA, already resident in HBM): x, y, normal
+ * x, normal y, weights, curvatures, each npts doubles */
+kigpu_status kigpu_tree_build_device(kigpu_ctx* ctx, const double* d_px, const double* d_py,
+                                     const double* d_nx, const double* d_ny, const double* d_w,
+                                     const double* d_kappa, int npts, int sd, int leaf_dofs,
+                                     kigpu_tree** tree);
 kigpu_status kigpu_tree_destroy(kigpu_tree* tree);
 kigpu_status kigpu_tree_get_info(const kigpu_tree* tree, kigpu_tree_info* info);
 

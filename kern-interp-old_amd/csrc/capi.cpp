@@ -1,6 +1,7 @@
 // extern "C" entry points of libkigpu (declared in include/kigpu.h).
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
 
@@ -153,11 +154,10 @@ kigpu_status kigpu_last_timings(kigpu_ctx* ctx, kigpu_timings* out) {
   return KIGPU_OK;
 }
 
-kigpu_status kigpu_tree_build(kigpu_ctx* ctx, const double* points, const double* normals,
-                              const double* weights, const double* curvatures, int npts, int sd,
-                              int leaf_dofs, kigpu_tree** out) {
+static kigpu_status tree_build_common(kigpu_ctx* ctx, int npts, int sd, int leaf_dofs,
+                                      kigpu_tree** out,
+                                      const std::function<void(TreeData&, hipStream_t)>& load) {
   KI_REQUIRE(ctx && out, "null argument");
-  KI_REQUIRE(points && normals && weights && curvatures, "null geometry array");
   KI_REQUIRE(npts > 0, "number of boundary points to init tree cannot be 0");
   KI_REQUIRE(sd == 1 || sd == 2, "solution dimension must be 1 or 2");
   KI_REQUIRE(leaf_dofs >= sd, "leaf_dofs must hold at least one point");
@@ -170,10 +170,25 @@ kigpu_status kigpu_tree_build(kigpu_ctx* ctx, const double* points, const double
     t.sd = sd;
     t.leaf_dofs = leaf_dofs;
     t.ndof = npts * sd;
-    t.h_points.assign(points, points + 2 * (size_t)npts);
-    t.h_normals.assign(normals, normals + 2 * (size_t)npts);
-    t.h_weights.assign(weights, weights + npts);
-    t.h_curv.assign(curvatures, curvatures + npts);
+    KI_HIP(hipEventRecord(c.ev[10], c.stream));
+    load(t, c.stream);
+    c.tim.syncs = 0;
+    build_tree(t, c.stream, &c.tim.syncs);
+    KI_HIP(hipEventRecord(c.ev[11], c.stream));
+    KI_HIP(hipStreamSynchronize(c.stream));
+    float ms = 0;
+    KI_HIP(hipEventElapsedTime(&ms, c.ev[10], c.ev[11]));
+    c.tim.t_tree = ms * 1e-3;
+  });
+  if (st == KIGPU_OK) *out = tr.release();
+  return st;
+}
+
+kigpu_status kigpu_tree_build(kigpu_ctx* ctx, const double* points, const double* normals,
+                              const double* weights, const double* curvatures, int npts, int sd,
+                              int leaf_dofs, kigpu_tree** out) {
+  KI_REQUIRE(points && normals && weights && curvatures, "null geometry array");
+  return tree_build_common(ctx, npts, sd, leaf_dofs, out, [&](TreeData& t, hipStream_t s) {
     std::vector<double> a(npts), b(npts), na(npts), nb(npts);
     for (int i = 0; i < npts; ++i) {
       a[i] = points[2 * i];
@@ -181,23 +196,30 @@ kigpu_status kigpu_tree_build(kigpu_ctx* ctx, const double* points, const double
       na[i] = normals[2 * i];
       nb[i] = normals[2 * i + 1];
     }
-    t.px.upload(a, c.stream);
-    t.py.upload(b, c.stream);
-    t.nx.upload(na, c.stream);
-    t.ny.upload(nb, c.stream);
-    t.w.upload(weights, npts, c.stream);
-    t.kappa.upload(curvatures, npts, c.stream);
-    hipEventRecord(c.ev[10], c.stream);
-    c.tim.syncs = 0;
-    build_tree(t, c.stream, &c.tim.syncs);
-    hipEventRecord(c.ev[11], c.stream);
-    KI_HIP(hipStreamSynchronize(c.stream));
-    float ms = 0;
-    hipEventElapsedTime(&ms, c.ev[10], c.ev[11]);
-    c.tim.t_tree = ms * 1e-3;
+    t.px.upload(a, s);
+    t.py.upload(b, s);
+    t.nx.upload(na, s);
+    t.ny.upload(nb, s);
+    t.w.upload(weights, npts, s);
+    t.kappa.upload(curvatures, npts, s);
+    KI_HIP(hipStreamSynchronize(s));
   });
-  if (st == KIGPU_OK) *out = tr.release();
-  return st;
+}
+
+kigpu_status kigpu_tree_build_device(kigpu_ctx* ctx, const double* d_px, const double* d_py,
+                                     const double* d_nx, const double* d_ny, const double* d_w,
+                                     const double* d_kappa, int npts, int sd, int leaf_dofs,
+                                     kigpu_tree** out) {
+  KI_REQUIRE(d_px && d_py && d_nx && d_ny && d_w && d_kappa, "null geometry array");
+  return tree_build_common(ctx, npts, sd, leaf_dofs, out, [&](TreeData& t, hipStream_t s) {
+    const size_t b = sizeof(double) * npts;
+    DBuf<double>* dst[6] = {&t.px, &t.py, &t.nx, &t.ny, &t.w, &t.kappa};
+    const double* src[6] = {d_px, d_py, d_nx, d_ny, d_w, d_kappa};
+    for (int i = 0; i < 6; ++i) {
+      dst[i]->alloc(npts);
+      KI_HIP(hipMemcpyAsync(dst[i]->p, src[i], b, hipMemcpyDeviceToDevice, s));
+    }
+  });
 }
 
 kigpu_status kigpu_tree_destroy(kigpu_tree* tree) {

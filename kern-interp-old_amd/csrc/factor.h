@@ -22,6 +22,7 @@ struct LevelWork {
   int level = 0;
   int a = 0, b = 0;         // node id range
   int nmax = 0;             // largest |active_box| on the level
+  std::vector<IdWork> id_lds;  // boxes whose ID ran in the LDS-resident kernel
   DBuf<double> slab64;      // factor slabs of this level's nodes
   DBuf<float> slab32;
 };

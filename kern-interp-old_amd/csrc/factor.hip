@@ -212,7 +212,10 @@ __host__ __device__ inline IdLds id_lds_layout(int m, int n, int q, bool a_in_ld
   return L;
 }
 
-template <typename T, bool LDS>
+// T: arithmetic of the QR (FP64: an ID at tol ~1e-6 needs it, since
+// T = R11^{-1} R12 inherits cond(R11) ~ 1/tol); TS: storage type of the
+// interpolation matrix in the factor slab.
+template <typename T, typename TS, bool LDS>
 __global__ __launch_bounds__(kTB) void k_id(const IdWork* work, GeomView g, int pde, double tol,
                                             const int* rows_pool, const int* act_pool,
                                             const int64_t* act_off, const double* ring, T* ws,
@@ -395,15 +398,16 @@ __global__ __launch_bounds__(kTB) void k_id(const IdWork* work, GeomView g, int 
     sr_pool[aoff + j] = s_act[perm[j]];
   }
   const int r = n - k;
-  T* Tm = reinterpret_cast<T*>(fac_ptr[w.node]);
-  for (int c = tid; c < r; c += kTB) {
-    const T* rc = A + (size_t)(k + c) * lda;
-    T* tc = Tm + (size_t)c * k;
+  TS* Tm = reinterpret_cast<TS*>(fac_ptr[w.node]);
+  for (int c = tid; c < r; c += kTB) {  // back substitution in place of R12's column
+    T* rc = A + (size_t)(k + c) * lda;
     for (int row = k - 1; row >= 0; --row) {
       T s = rc[row];
-      for (int j = row + 1; j < k; ++j) s -= A[row + (size_t)j * lda] * tc[j];
-      tc[row] = s / A[row + (size_t)row * lda];
+      for (int j = row + 1; j < k; ++j) s -= A[row + (size_t)j * lda] * rc[j];
+      rc[row] = s / A[row + (size_t)row * lda];
     }
+    TS* tc = Tm + (size_t)c * k;
+    for (int row = 0; row < k; ++row) tc[row] = (TS)rc[row];
   }
   if (tid == 0) rank[w.node] = k;
 }
@@ -825,7 +829,7 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
       w.cy = cc[1] + t.h_side[x] / 2.0;
       w.rad = t.h_side[x] * kRadiusRatio;
       w.pw = 2.0 * M_PI * w.rad / kProxyPoints;
-      IdLds L = id_lds_layout(w.m, w.n, w.q, true, sizeof(T));
+      IdLds L = id_lds_layout(w.m, w.n, w.q, true, sizeof(double));
       w.in_lds = L.total <= lds_cap;
       w.a_off = atot;
       if (!w.in_lds) atot += (int64_t)w.m * w.n;
@@ -881,35 +885,38 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
       for (auto& w : wl) {
         if (w.in_lds) {
           wl_lds.push_back(w);
-          lds_bytes = std::max(lds_bytes, id_lds_layout(w.m, w.n, w.q, true, sizeof(T)).total);
+          lds_bytes = std::max(lds_bytes, id_lds_layout(w.m, w.n, w.q, true, sizeof(double)).total);
         } else {
           wl_glb.push_back(w);
-          glds = std::max(glds, id_lds_layout(w.m, w.n, w.q, false, sizeof(T)).total);
+          glds = std::max(glds, id_lds_layout(w.m, w.n, w.q, false, sizeof(double)).total);
         }
       }
-      T* ws = nullptr;
+      double* ws = nullptr;
       if (atot) {
-        if (sizeof(T) == 8) {
-          t.ws64.alloc(atot);
-          ws = reinterpret_cast<T*>(t.ws64.p);
-        } else {
-          t.ws32.alloc(atot);
-          ws = reinterpret_cast<T*>(t.ws32.p);
-        }
+        t.ws64.alloc(atot);
+        ws = t.ws64.p;
       }
       DBuf<IdWork> d1, d2;
       if (!wl_lds.empty()) {
         d1.upload(wl_lds, s);
-        allow_lds(k_id<T, true>, lds_bytes);
-        k_id<T, true><<<(int)wl_lds.size(), kTB, lds_bytes, s>>>(
+        allow_lds(k_id<double, T, true>, lds_bytes);
+        KI_HIP(hipEventRecord(c.ev[12], s));
+        k_id<double, T, true><<<(int)wl_lds.size(), kTB, lds_bytes, s>>>(
             d1.p, g, pde, o.id_tol, t.rows_pool.p, t.act_pool.p, t.act_off.p, c.ring.p, ws,
             t.rank.p, t.sr_pool.p, t.pos_pool.p, t.fac_ptr.p, t.status.p);
         KI_HIP(hipGetLastError());
+        KI_HIP(hipEventRecord(c.ev[13], s));
+        KI_HIP(hipEventSynchronize(c.ev[13]));
+        float kms = 0;
+        KI_HIP(hipEventElapsedTime(&kms, c.ev[12], c.ev[13]));
+        c.tim.id_kernel_s += kms * 1e-3;
+        c.tim.id_launches += 1;
+        lw->id_lds = wl_lds;
       }
       if (!wl_glb.empty()) {
         d2.upload(wl_glb, s);
-        allow_lds(k_id<T, false>, glds);
-        k_id<T, false><<<(int)wl_glb.size(), kTB, glds, s>>>(
+        allow_lds(k_id<double, T, false>, glds);
+        k_id<double, T, false><<<(int)wl_glb.size(), kTB, glds, s>>>(
             d2.p, g, pde, o.id_tol, t.rows_pool.p, t.act_pool.p, t.act_off.p, c.ring.p, ws,
             t.rank.p, t.sr_pool.p, t.pos_pool.p, t.fac_ptr.p, t.status.p);
         KI_HIP(hipGetLastError());
@@ -920,6 +927,7 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     // ---- elimination for every node of the level
     {
       DecLds L = dec_layout(std::max(nmax, 1), sizeof(T));
+      KI_HIP(hipEventRecord(c.ev[14], s));
       if (L.total <= lds_cap) {
         allow_lds(k_decouple<T, true>, L.total);
         k_decouple<T, true><<<cn, kTB, L.total, s>>>(
@@ -942,6 +950,12 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
             t.fac_ptr.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, ws, stride, t.status.p);
       }
       KI_HIP(hipGetLastError());
+      KI_HIP(hipEventRecord(c.ev[15], s));
+      KI_HIP(hipEventSynchronize(c.ev[15]));
+      float kms = 0;
+      KI_HIP(hipEventElapsedTime(&kms, c.ev[14], c.ev[15]));
+      c.tim.dec_kernel_s += kms * 1e-3;
+      c.tim.dec_launches += 1;
     }
     hipEventRecord(c.ev[3], s);
     KI_HIP(hipStreamSynchronize(s));
@@ -990,8 +1004,32 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
   c.tim.t_root += ms * 1e-3;
 }
 
+// Algorithmic flops of the ID kernel for one box, as executed: entry
+// generation (c_gen flops per entry), initial column norms, the QR steps up to
+// the rank-revealing one (all n steps when the box is not compressed) and the
+// T = R11^{-1} R12 solve.
+static double id_flops(int m, int n, int k, double c_gen) {
+  double f = c_gen * m * n + 2.0 * m * n;
+  const int steps = k > 0 ? k + 1 : n;
+  for (int i = 0; i < steps; ++i) {
+    f += 2.0 * (m - i);
+    if (i + 1 < steps) f += 4.0 * (m - i) * (n - i - 1);
+  }
+  if (k > 0) f += (double)k * k * (n - k);
+  return f;
+}
+
+// elimination: K_BN generation plus 6k^2r + 8kr^2 + 2/3 r^3
+static double dec_flops(int n, int k, double c_gen) {
+  if (k <= 0) return 0;
+  const double r = n - k;
+  return c_gen * n * n + 6.0 * k * k * r + 8.0 * k * r * r + 2.0 / 3.0 * r * r * r;
+}
+
 void factor_tree(Context& c, TreeData& t, const kigpu_factor_opts& o) {
   c.tim.t_lists = c.tim.t_id = c.tim.t_decouple = c.tim.t_root = 0;
+  c.tim.id_launches = c.tim.dec_launches = 0;
+  c.tim.id_kernel_s = c.tim.id_flops = c.tim.dec_kernel_s = c.tim.dec_flops = 0;
   if (o.precision == KIGPU_FP32_REFINE) factor_levels<float>(c, t, o);
   else factor_levels<double>(c, t, o);
   int st = 0;
@@ -1011,6 +1049,12 @@ void factor_tree(Context& c, TreeData& t, const kigpu_factor_opts& o) {
   t.sr_pool.download(t.h_sr.data(), t.act_pool_size, c.stream);
   t.act_pool.download(t.h_act.data(), t.act_pool_size, c.stream);
   KI_HIP(hipStreamSynchronize(c.stream));
+  const double c_gen = o.pde == KIGPU_STOKES ? 20.0 : 11.0;
+  for (auto& lw : t.work) {
+    if (!lw) continue;
+    for (const IdWork& w : lw->id_lds) c.tim.id_flops += id_flops(w.m, w.n, t.h_rank[w.node], c_gen);
+    for (int x = lw->a; x < lw->b; ++x) c.tim.dec_flops += dec_flops(t.h_nact[x], t.h_rank[x], c_gen);
+  }
   t.opts = o;
   t.factored = true;
 }

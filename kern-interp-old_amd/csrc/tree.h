@@ -14,7 +14,6 @@ struct TreeData {
   int npts = 0, sd = 1, leaf_dofs = 32, ndof = 0;
   double min = 0, max = 0;
   DBuf<double> px, py, nx, ny, w, kappa;
-  std::vector<double> h_points, h_normals, h_weights, h_curv;  // interleaved copies
 
   // ---- tree, node ids in level order ----
   int nlevels = 0, nnodes = 0;

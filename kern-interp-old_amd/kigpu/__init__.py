@@ -43,13 +43,16 @@ class TreeInfo(C.Structure):
 
 class Timings(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("t_tree", "t_lists", "t_id", "t_decouple", "t_root",
-                                          "t_solve", "t_refine_apply")] + [("syncs", C.c_int)]
+                                          "t_solve", "t_refine_apply")] + [
+        ("syncs", C.c_int), ("id_launches", C.c_int), ("id_kernel_s", C.c_double),
+        ("id_flops", C.c_double), ("dec_launches", C.c_int), ("dec_kernel_s", C.c_double),
+        ("dec_flops", C.c_double)]
 
 
 # every symbol include/kigpu.h declares
 EXPORTS = [
     "kigpu_context_create", "kigpu_context_destroy", "kigpu_status_string", "kigpu_last_error",
-    "kigpu_last_timings", "kigpu_tree_build", "kigpu_tree_destroy", "kigpu_tree_get_info",
+    "kigpu_last_timings", "kigpu_tree_build", "kigpu_tree_build_device", "kigpu_tree_destroy", "kigpu_tree_get_info",
     "kigpu_tree_export_nodes", "kigpu_tree_export_boxes", "kigpu_tree_export_neighbors",
     "kigpu_set_border", "kigpu_factor", "kigpu_factor_ranks", "kigpu_factor_export_lists",
     "kigpu_factor_allskel", "kigpu_solve", "kigpu_solve_device", "kigpu_kernel_apply",
@@ -77,6 +80,8 @@ def load() -> C.CDLL:
         "kigpu_last_timings": (C.c_int, [P, C.POINTER(Timings)]),
         "kigpu_tree_build": (C.c_int, [P, _f64p, _f64p, _f64p, _f64p, C.c_int, C.c_int, C.c_int,
                                        C.POINTER(P)]),
+        "kigpu_tree_build_device": (C.c_int, [P, P, P, P, P, P, P, C.c_int, C.c_int, C.c_int,
+                                              C.POINTER(P)]),
         "kigpu_tree_destroy": (C.c_int, [P]),
         "kigpu_tree_get_info": (C.c_int, [P, C.POINTER(TreeInfo)]),
         "kigpu_tree_export_nodes": (C.c_int, [P, _i64p, _i32p, _i32p, _i32p, _f64p, _f64p]),
