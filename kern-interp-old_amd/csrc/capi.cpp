@@ -1,4 +1,5 @@
 // extern "C" entry points of libkigpu (declared in include/kigpu.h).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -123,7 +124,11 @@ kigpu_status kigpu_context_create(int device, kigpu_ctx** out) {
     for (auto& e : c.ev) KI_HIP(hipEventCreate(&e));
     hipDeviceProp_t prop;
     KI_HIP(hipGetDeviceProperties(&prop, device));
-    c.lds_max = prop.sharedMemPerBlock;
+    int lds_cu = 0;
+    KI_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                 device));
+    // a workgroup may use the whole LDS of its CU (160 KiB on gfx950)
+    c.lds_max = std::max<size_t>(prop.sharedMemPerBlock, (size_t)lds_cu);
     // proxy ring unit vectors with the host libm (kernel.cpp:242-252)
     std::vector<double> ring(2 * kProxyPoints);
     for (int i = 0; i < kProxyPoints; ++i) {
@@ -223,6 +228,8 @@ kigpu_status kigpu_tree_build_device(kigpu_ctx* ctx, const double* d_px, const d
 }
 
 kigpu_status kigpu_tree_destroy(kigpu_tree* tree) {
+  // device blocks go back to the allocator cache: no work may still use them
+  (void)hipDeviceSynchronize();
   delete tree;
   return KIGPU_OK;
 }

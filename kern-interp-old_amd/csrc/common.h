@@ -39,25 +39,38 @@ struct HipError {
     throw ::kigpu::HipError{code};  \
   } while (0)
 
+// Caching device allocator: freed blocks go to a free list keyed by their
+// rounded size and are reused, so the per-level buffers of tree builds,
+// factorizations and solves never call hipMalloc/hipFree in steady state.
+void* dev_alloc(size_t bytes);
+void dev_free(void* p, size_t bytes);
+size_t dev_round(size_t bytes);
+
 // owning device buffer
 template <typename T>
 struct DBuf {
   T* p = nullptr;
   size_t n = 0;
+  size_t cap = 0;  // bytes held
   DBuf() = default;
   DBuf(const DBuf&) = delete;
   DBuf& operator=(const DBuf&) = delete;
   ~DBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) dev_free(p, cap);
     p = nullptr;
     n = 0;
+    cap = 0;
   }
   void alloc(size_t count) {
-    if (count <= n && p) return;
-    release();
     size_t bytes = count ? count * sizeof(T) : sizeof(T);
-    KI_HIP(hipMalloc(&p, bytes));
+    if (p && bytes <= cap) {
+      n = count;
+      return;
+    }
+    release();
+    cap = dev_round(bytes);
+    p = static_cast<T*>(dev_alloc(cap));
     n = count;
   }
   void upload(const T* h, size_t count, hipStream_t s) {

@@ -25,15 +25,33 @@ struct LevelWork {
   std::vector<IdWork> id_lds;  // boxes whose ID ran in the LDS-resident kernel
   DBuf<double> slab64;      // factor slabs of this level's nodes
   DBuf<float> slab32;
+  // per-level device temporaries, kept until the tree is freed so that no
+  // hipFree stalls the level loop
+  DBuf<int64_t> soff, rows_off;
+  DBuf<int> rows;
+  DBuf<IdWork> wl_lds, wl_glb;
+  DBuf<double> ws, dws64;
+  DBuf<float> dws32;
 };
 
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[16];
+  std::vector<hipEvent_t> evpool;  // per-level phase markers of a factorization
   kigpu_timings tim{};
   DBuf<double> ring;        // proxy unit vectors cos | sin (host libm)
   size_t lds_max = 65536;
+  void ensure_events(size_t n) {
+    while (evpool.size() < n) {
+      hipEvent_t e;
+      KI_HIP(hipEventCreate(&e));
+      evpool.push_back(e);
+    }
+  }
+  ~Context() {
+    for (auto e : evpool) (void)hipEventDestroy(e);
+  }
 };
 
 void factor_tree(Context& c, TreeData& t, const kigpu_factor_opts& o);

@@ -186,16 +186,23 @@ __global__ void k_outside_rows(const IdWork* work, int a, int b, int level, int 
 // ---------------------------------------------------------------------------
 
 struct IdLds {
-  size_t a, rows, act, vn1, vn2, perm, red, total;
+  size_t a, rgeo, cgeo, rows, act, vn1, vn2, perm, red, total;
 };
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
+
+// Geometry of a DOF staged in LDS: point x, y, normal x, y, weight, curvature.
+constexpr int kGeo = 6;
 
 __host__ __device__ inline IdLds id_lds_layout(int m, int n, int q, bool a_in_lds, size_t tsz) {
   IdLds L;
   size_t o = 0;
   L.a = o;
   if (a_in_lds) o = align16(o + (size_t)m * n * tsz);
+  L.rgeo = o;
+  o = align16(o + sizeof(double) * kGeo * (size_t)q);
+  L.cgeo = o;
+  o = align16(o + sizeof(double) * kGeo * (size_t)n);
   L.rows = o;
   o = align16(o + sizeof(int) * (size_t)q);
   L.act = o;
@@ -207,25 +214,88 @@ __host__ __device__ inline IdLds id_lds_layout(int m, int n, int q, bool a_in_ld
   L.perm = o;
   o = align16(o + sizeof(int) * (size_t)n);
   L.red = o;
-  o = align16(o + sizeof(double) * 16);
+  o = align16(o + sizeof(double) * 32);
   L.total = o;
   return L;
 }
 
+__device__ __forceinline__ void stage_geo(const GeomView& g, int dof, double* dst) {
+  const int p = dof / g.sd;
+  dst[0] = g.px[p];
+  dst[1] = g.py[p];
+  dst[2] = g.nx[p];
+  dst[3] = g.ny[p];
+  dst[4] = g.w[p];
+  dst[5] = g.kappa[p];
+}
+
+// K(t, s) from staged geometry (same operations as kernel_entry)
+__device__ __forceinline__ double staged_entry(int pde, int tdof, const double* tg, int sdof,
+                                               const double* sg) {
+#pragma clang fp contract(off)
+  if (pde == KIGPU_STOKES)
+    return stokes_entry(tdof, sdof, tg[0], tg[1], tg[2], tg[3], sg[0], sg[1], sg[2], sg[3], sg[4],
+                        sg[5]);
+  return laplace_entry(pde, tg[0] - sg[0], tg[1] - sg[1], sg[4], sg[5], sg[2], sg[3], tg[2],
+                       tg[3], false);
+}
+
+// proxy-block entry against a box DOF with staged geometry bg (see proxy_entry)
+__device__ __forceinline__ double staged_proxy_entry(int pde, int row, int bdof, const double* bg,
+                                                     double cx, double cy, double rad, double pw,
+                                                     const double* ring_c, const double* ring_s) {
+#pragma clang fp contract(off)
+  if (pde != KIGPU_STOKES) {
+    int half = row >= kProxyPoints;
+    int i = half ? row - kProxyPoints : row;
+    double pc = ring_c[i], ps = ring_s[i];
+    double r1 = (cx + rad * pc) - bg[0];
+    double r2 = (cy + rad * ps) - bg[1];
+    if (!half) return laplace_entry(pde, r1, r2, bg[4], 0.0, bg[2], bg[3], pc, ps, false);
+    return laplace_entry(pde, -r1, -r2, pw, 0.0, pc, ps, bg[2], bg[3], false);
+  }
+  int half = row >= 2 * kProxyPoints;
+  int rr_ = half ? row - 2 * kProxyPoints : row;
+  int i = rr_ >> 1, comp = rr_ & 1;
+  double pc = ring_c[i], ps = ring_s[i];
+  const double r[2] = {(cx + rad * pc) - bg[0], (cy + rad * ps) - bg[1]};
+  const double sn[2] = {bg[2], bg[3]};
+  const double tn[2] = {pc, ps};
+  const double sw = bg[4];
+  double rr = r[0] * r[0] + r[1] * r[1];
+  int sc = bdof % 2;
+  if (!half) {
+    double potential = sw * kInvPi * (r[0] * sn[0] + r[1] * sn[1]) / (rr * rr);
+    return potential * r[comp] * r[sc] + sw * tn[comp] * sn[sc];
+  }
+  double potential = -pw * kInvPi * (r[0] * tn[0] + r[1] * tn[1]) / (rr * rr);
+  return potential * r[comp] * r[sc] + pw * sn[sc] * tn[comp];
+}
+
+constexpr int kIdTB = 1024;
+constexpr int kIdNW = kIdTB / kWave;
+
+// One workgroup per box: generate the ID matrix, Householder QR with column
+// pivoting (the steps of LAPACK dlaqp2: first-maximum pivot on the partial
+// column norms, norm downdating with recomputation when
+// (1 - (|a_ij|/vn1)^2)(vn1/vn2)^2 <= sqrt(eps)) until the first i >= 1 with
+// |R_ii| < |tol R_00| (ki_mat.cpp:516-528), then T = R11^{-1} R12.
 // T: arithmetic of the QR (FP64: an ID at tol ~1e-6 needs it, since
 // T = R11^{-1} R12 inherits cond(R11) ~ 1/tol); TS: storage type of the
 // interpolation matrix in the factor slab.
 template <typename T, typename TS, bool LDS>
-__global__ __launch_bounds__(kTB) void k_id(const IdWork* work, GeomView g, int pde, double tol,
-                                            const int* rows_pool, const int* act_pool,
-                                            const int64_t* act_off, const double* ring, T* ws,
-                                            int* rank, int* sr_pool, int* pos_pool,
-                                            const uint64_t* fac_ptr, int* status) {
+__global__ __launch_bounds__(kIdTB) void k_id(const IdWork* work, GeomView g, int pde, double tol,
+                                              const int* rows_pool, const int* act_pool,
+                                              const int64_t* act_off, const double* ring, T* ws,
+                                              int* rank, int* sr_pool, int* pos_pool,
+                                              const uint64_t* fac_ptr, int* status) {
   extern __shared__ __align__(16) unsigned char smem[];
   const IdWork w = work[blockIdx.x];
   const int n = w.n, q = w.q, m = w.m;
   const IdLds Ly = id_lds_layout(m, n, q, LDS, sizeof(T));
   T* A = LDS ? reinterpret_cast<T*>(smem + Ly.a) : ws + w.a_off;
+  double* rgeo = reinterpret_cast<double*>(smem + Ly.rgeo);
+  double* cgeo = reinterpret_cast<double*>(smem + Ly.cgeo);
   int* s_rows = reinterpret_cast<int*>(smem + Ly.rows);
   int* s_act = reinterpret_cast<int*>(smem + Ly.act);
   double* vn1 = reinterpret_cast<double*>(smem + Ly.vn1);
@@ -237,31 +307,37 @@ __global__ __launch_bounds__(kTB) void k_id(const IdWork* work, GeomView g, int 
   const int lda = m;
   const int64_t aoff = act_off[w.node];
 
-  for (int i = tid; i < q; i += kTB) s_rows[i] = rows_pool[w.rows_off + i];
-  for (int j = tid; j < n; j += kTB) {
-    s_act[j] = act_pool[aoff + j];
+  for (int i = tid; i < q; i += kIdTB) {
+    const int d = rows_pool[w.rows_off + i];
+    s_rows[i] = d;
+    stage_geo(g, d, rgeo + kGeo * i);
+  }
+  for (int j = tid; j < n; j += kIdTB) {
+    const int d = act_pool[aoff + j];
+    s_act[j] = d;
     perm[j] = j;
+    stage_geo(g, d, cgeo + kGeo * j);
   }
   __syncthreads();
 
   // ---- ID matrix [K(rows, B); K(B, rows)^T; proxy] (kernel.cpp:153-235)
   const double* ring_c = ring;
   const double* ring_s = ring + kProxyPoints;
-  for (int j = 0; j < n; ++j) {
+  const int mn = m * n;
+  for (int e = tid; e < mn; e += kIdTB) {
+    const int j = e / m, i = e - j * m;
     const int bd = s_act[j];
-    T* col = A + (size_t)j * lda;
-    for (int i = tid; i < m; i += kTB) {
-      double v;
-      if (i < q) v = kernel_entry(g, pde, s_rows[i], bd);
-      else if (i < 2 * q) v = kernel_entry(g, pde, bd, s_rows[i - q]);
-      else v = proxy_entry(g, pde, i - 2 * q, bd, w.cx, w.cy, w.rad, w.pw, ring_c, ring_s);
-      col[i] = (T)v;
-    }
+    const double* bg = cgeo + kGeo * j;
+    double v;
+    if (i < q) v = staged_entry(pde, s_rows[i], rgeo + kGeo * i, bd, bg);
+    else if (i < 2 * q) v = staged_entry(pde, bd, bg, s_rows[i - q], rgeo + kGeo * (i - q));
+    else v = staged_proxy_entry(pde, i - 2 * q, bd, bg, w.cx, w.cy, w.rad, w.pw, ring_c, ring_s);
+    A[e] = (T)v;
   }
   __syncthreads();
 
   // ---- column norms (dgeqp3: DNRM2 of each column), FP64
-  for (int j = wv; j < n; j += kNW) {
+  for (int j = wv; j < n; j += kIdNW) {
     const T* col = A + (size_t)j * lda;
     double s = 0;
     for (int i = lane; i < m; i += kWave) {
@@ -276,8 +352,6 @@ __global__ __launch_bounds__(kTB) void k_id(const IdWork* work, GeomView g, int 
   }
   __syncthreads();
 
-  // ---- Householder QR with column pivoting (LAPACK dlaqp2 steps), stopping
-  // at the first i >= 1 with |R_ii| < |tol R_00| (ki_mat.cpp:516-528)
   const double tol3z = sqrt(2.220446049250313e-16);
   double thresh = 0;
   int k = 0;
@@ -305,7 +379,7 @@ __global__ __launch_bounds__(kTB) void k_id(const IdWork* work, GeomView g, int 
     if (p != i) {
       T* ci = A + (size_t)i * lda;
       T* cp = A + (size_t)p * lda;
-      for (int r = tid; r < m; r += kTB) {
+      for (int r = tid; r < m; r += kIdTB) {
         T t0 = ci[r];
         ci[r] = cp[r];
         cp[r] = t0;
@@ -322,11 +396,11 @@ __global__ __launch_bounds__(kTB) void k_id(const IdWork* work, GeomView g, int 
     // dlarfg on A(i:m, i)
     T* ci = A + (size_t)i * lda;
     double part = 0;
-    for (int r = i + 1 + tid; r < m; r += kTB) {
+    for (int r = i + 1 + tid; r < m; r += kIdTB) {
       double v = (double)ci[r];
       part += v * v;
     }
-    const double xnorm = sqrt(block_sum<kNW>(part, red));
+    const double xnorm = sqrt(block_sum<kIdNW>(part, red));
     const double alpha = (double)ci[i];
     double beta, tau;
     if (xnorm == 0.0) {
@@ -347,12 +421,12 @@ __global__ __launch_bounds__(kTB) void k_id(const IdWork* work, GeomView g, int 
       break;
     }
     const T scal = xnorm == 0.0 ? T(0) : (T)(1.0 / (alpha - beta));
-    for (int r = i + 1 + tid; r < m; r += kTB) ci[r] *= scal;
+    for (int r = i + 1 + tid; r < m; r += kIdTB) ci[r] *= scal;
     __syncthreads();
     if (tid == 0) ci[i] = (T)beta;
     // apply H(i)^T to A(i:m, i+1:n) and downdate the partial column norms
     const T ttau = (T)tau;
-    for (int j = i + 1 + wv; j < n; j += kNW) {
+    for (int j = i + 1 + wv; j < n; j += kIdNW) {
       T* cj = A + (size_t)j * lda;
       const T aij = cj[i];
       T d = lane == 0 ? aij : T(0);
@@ -369,14 +443,14 @@ __global__ __launch_bounds__(kTB) void k_id(const IdWork* work, GeomView g, int 
         double rr = v1 / vn2[j];
         double temp2 = temp * rr * rr;
         if (temp2 <= tol3z) {
-          double s = 0;
+          double s2 = 0;
           for (int r = i + 1 + lane; r < m; r += kWave) {
             double v = (double)cj[r];
-            s += v * v;
+            s2 += v * v;
           }
-          s = wave_sum(s);
+          s2 = wave_sum(s2);
           if (lane == 0) {
-            double nv = i + 1 < m ? sqrt(s) : 0.0;
+            double nv = i + 1 < m ? sqrt(s2) : 0.0;
             vn1[j] = nv;
             vn2[j] = nv;
           }
@@ -393,18 +467,18 @@ __global__ __launch_bounds__(kTB) void k_id(const IdWork* work, GeomView g, int 
     return;
   }
   // ---- outputs: pivots, permuted active list, T = R11^{-1} R12 (dtrtrs)
-  for (int j = tid; j < n; j += kTB) {
+  for (int j = tid; j < n; j += kIdTB) {
     pos_pool[aoff + j] = perm[j];
     sr_pool[aoff + j] = s_act[perm[j]];
   }
   const int r = n - k;
   TS* Tm = reinterpret_cast<TS*>(fac_ptr[w.node]);
-  for (int c = tid; c < r; c += kTB) {  // back substitution in place of R12's column
+  for (int c = tid; c < r; c += kIdTB) {  // back substitution in place of R12's column
     T* rc = A + (size_t)(k + c) * lda;
     for (int row = k - 1; row >= 0; --row) {
-      T s = rc[row];
-      for (int j = row + 1; j < k; ++j) s -= A[row + (size_t)j * lda] * rc[j];
-      rc[row] = s / A[row + (size_t)row * lda];
+      T s2 = rc[row];
+      for (int j = row + 1; j < k; ++j) s2 -= A[row + (size_t)j * lda] * rc[j];
+      rc[row] = s2 / A[row + (size_t)row * lda];
     }
     TS* tc = Tm + (size_t)c * k;
     for (int row = 0; row < k; ++row) tc[row] = (TS)rc[row];
@@ -417,18 +491,20 @@ __global__ __launch_bounds__(kTB) void k_id(const IdWork* work, GeomView g, int 
 // ---------------------------------------------------------------------------
 
 struct DecLds {
-  size_t uact, kp, x, krsn, ksnr, sr, pos, piv, total;
+  size_t uact, kp, x, krsn, ksnr, tm, lm, sr, pos, piv, total;
 };
 
 __host__ __device__ inline DecLds dec_layout(int n, size_t tsz) {
   DecLds L;
   size_t o = 0;
-  size_t nn = (size_t)n * n;
+  const size_t nn = (size_t)n * n, q4 = nn / 4 + 1;
   L.uact = o; o = align16(o + nn * tsz);
   L.kp = o;   o = align16(o + nn * tsz);
   L.x = o;    o = align16(o + nn * tsz);
-  L.krsn = o; o = align16(o + nn * tsz / 4 + tsz);
-  L.ksnr = o; o = align16(o + nn * tsz / 4 + tsz);
+  L.krsn = o; o = align16(o + q4 * tsz);
+  L.ksnr = o; o = align16(o + q4 * tsz);
+  L.tm = o;   o = align16(o + q4 * tsz);
+  L.lm = o;   o = align16(o + q4 * tsz);
   L.sr = o;   o = align16(o + sizeof(int) * (size_t)n);
   L.pos = o;  o = align16(o + sizeof(int) * (size_t)n);
   L.piv = o;  o = align16(o + sizeof(int) * (size_t)n);
@@ -437,20 +513,24 @@ __host__ __device__ inline DecLds dec_layout(int n, size_t tsz) {
 }
 
 // Factor slab of a compressed node (k skeleton, r redundant DOFs):
-//   T (k x r) | L (k x r) | U (r x k) | X_rr LU (r x r) | Acc (k x k)
+//   T (k x r) | L (k x r) | U (r x k) | X_rr^{-1} (r x r) | Acc (k x k)
 // all column-major with leading dimension = height.
 __host__ __device__ inline size_t slab_elems(int n) { return ((size_t)n * n * 5 + 3) / 4 + 4; }
 
+constexpr int kDecTB = 1024;
+constexpr int kDecNW = kDecTB / kWave;
+
 template <typename T, bool LDS>
-__global__ __launch_bounds__(kTB) void k_decouple(int a, GeomView g, int pde, const int* is_leaf,
-                                                  const int* child, const int* osize_in,
-                                                  const uint64_t* acc_ptr_in, const int* acc_ld_in,
-                                                  const int* nact, const int* rank,
-                                                  const int64_t* act_off, const int* act_pool,
-                                                  const int* sr_pool, const int* pos_pool,
-                                                  int* piv_pool, const uint64_t* fac_ptr,
-                                                  int* osize, uint64_t* acc_ptr, int* acc_ld,
-                                                  T* ws, size_t ws_stride, int* status) {
+__global__ __launch_bounds__(kDecTB) void k_decouple(int a, GeomView g, int pde, const int* is_leaf,
+                                                     const int* child, const int* osize_in,
+                                                     const uint64_t* acc_ptr_in,
+                                                     const int* acc_ld_in, const int* nact,
+                                                     const int* rank, const int64_t* act_off,
+                                                     const int* act_pool, const int* sr_pool,
+                                                     const int* pos_pool, int* piv_pool,
+                                                     const uint64_t* fac_ptr, int* osize,
+                                                     uint64_t* acc_ptr, int* acc_ld, T* ws,
+                                                     size_t ws_stride, int* status) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int x = a + blockIdx.x;
   const int n = nact[x];
@@ -472,7 +552,7 @@ __global__ __launch_bounds__(kTB) void k_decouple(int a, GeomView g, int pde, co
 
   // accumulated updates of the children, block diagonal in active order
   if (!leaf) {
-    for (int i = tid; i < n * n; i += kTB) Uact[i] = T(0);
+    for (int i = tid; i < n * n; i += kDecTB) Uact[i] = T(0);
     __syncthreads();
     int o = 0;
     for (int s = 0; s < 4; ++s) {
@@ -481,7 +561,7 @@ __global__ __launch_bounds__(kTB) void k_decouple(int a, GeomView g, int pde, co
       const T* acc = reinterpret_cast<const T*>(acc_ptr_in[y]);
       if (acc) {
         int ld = acc_ld_in[y];
-        for (int e = tid; e < sz * sz; e += kTB) {
+        for (int e = tid; e < sz * sz; e += kDecTB) {
           int i = e % sz, j = e / sz;
           Uact[(o + i) + (size_t)(o + j) * n] = acc[i + (size_t)j * ld];
         }
@@ -492,7 +572,7 @@ __global__ __launch_bounds__(kTB) void k_decouple(int a, GeomView g, int pde, co
   }
   if (k < 0) {  // not compressed: hand the whole active list and its update up
     if (!leaf) {
-      for (int i = tid; i < n * n; i += kTB) slab[i] = Uact[i];
+      for (int i = tid; i < n * n; i += kDecTB) slab[i] = Uact[i];
     }
     if (tid == 0) {
       osize[x] = n;
@@ -506,113 +586,81 @@ __global__ __launch_bounds__(kTB) void k_decouple(int a, GeomView g, int pde, co
   T* X = reinterpret_cast<T*>(base + Ly.x);
   T* Krsn = reinterpret_cast<T*>(base + Ly.krsn);
   T* Ksnr = reinterpret_cast<T*>(base + Ly.ksnr);
+  T* Ts = reinterpret_cast<T*>(base + Ly.tm);
+  T* Ls = reinterpret_cast<T*>(base + Ly.lm);
   int* sr = reinterpret_cast<int*>(base + Ly.sr);
   int* pos = reinterpret_cast<int*>(base + Ly.pos);
   int* piv = reinterpret_cast<int*>(base + Ly.piv);
   const int64_t aoff = act_off[x];
-  for (int j = tid; j < n; j += kTB) {
+  for (int j = tid; j < n; j += kDecTB) {
     sr[j] = sr_pool[aoff + j];
     pos[j] = pos_pool[aoff + j];
   }
+  for (int e = tid; e < k * r; e += kDecTB) Ts[e] = slab[e];  // T from the ID
   __syncthreads();
   // K_BN = K(BN, BN) - update, permuted to [s r] (skel_factorization.cpp:66-78)
-  for (int e = tid; e < n * n; e += kTB) {
+  for (int e = tid; e < n * n; e += kDecTB) {
     int i = e % n, j = e / n;
     double v = kernel_entry(g, pde, sr[i], sr[j]);
     if (!leaf) v -= (double)Uact[pos[i] + (size_t)pos[j] * n];
     Kp[e] = (T)v;
   }
   __syncthreads();
-  const T* Tm = slab;                       // k x r from the ID
   T* Lm = slab + (size_t)k * r;             // k x r
   T* Um = slab + 2 * (size_t)k * r;         // r x k
-  T* Xlu = slab + 3 * (size_t)k * r;        // r x r
-  T* Acc = Xlu + (size_t)r * r;             // k x k
+  T* Xinv = slab + 3 * (size_t)k * r;       // r x r
+  T* Acc = Xinv + (size_t)r * r;            // k x k
   auto KP = [&](int i, int j) -> T { return Kp[i + (size_t)j * n]; };
   // K_r_sn = K_rs - T^T K_ss (r x k); K_sn_r = K_sr - K_ss T (k x r)
-  for (int e = tid; e < r * k; e += kTB) {
+  for (int e = tid; e < r * k; e += kDecTB) {
     int i = e % r, j = e / r;
     T s = KP(k + i, j);
-    for (int t = 0; t < k; ++t) s -= Tm[t + (size_t)i * k] * KP(t, j);
+    for (int t = 0; t < k; ++t) s -= Ts[t + (size_t)i * k] * KP(t, j);
     Krsn[i + (size_t)j * r] = s;
   }
-  for (int e = tid; e < k * r; e += kTB) {
+  for (int e = tid; e < k * r; e += kDecTB) {
     int i = e % k, j = e / k;
     T s = KP(i, k + j);
-    for (int t = 0; t < k; ++t) s -= KP(i, t) * Tm[t + (size_t)j * k];
+    for (int t = 0; t < k; ++t) s -= KP(i, t) * Ts[t + (size_t)j * k];
     Ksnr[i + (size_t)j * k] = s;
   }
   __syncthreads();
   // X_rr = K_rr - T^T K_sr - K_r_sn T (r x r)
-  for (int e = tid; e < r * r; e += kTB) {
+  for (int e = tid; e < r * r; e += kDecTB) {
     int i = e % r, j = e / r;
     T s = KP(k + i, k + j);
-    for (int t = 0; t < k; ++t) s -= Tm[t + (size_t)i * k] * KP(t, k + j);
-    for (int t = 0; t < k; ++t) s -= Krsn[i + (size_t)t * r] * Tm[t + (size_t)j * k];
+    for (int t = 0; t < k; ++t) s -= Ts[t + (size_t)i * k] * KP(t, k + j);
+    for (int t = 0; t < k; ++t) s -= Krsn[i + (size_t)t * r] * Ts[t + (size_t)j * k];
     X[i + (size_t)j * r] = s;
   }
   __syncthreads();
-  // LU with partial pivoting (dgetrf): first largest |.| in the column
-  const int lane = tid & (kWave - 1), wv = tid / kWave;
+  // X_rr^{-1} in place (Gauss-Jordan with dgetrf's pivot order)
+  T* Z = X;
   __shared__ int s_p;
-  for (int c = 0; c < r; ++c) {
-    if (wv == 0) {
-      double best = -1.0;
-      int bi = r;
-      for (int i = c + lane; i < r; i += kWave) {
-        double v = fabs((double)X[i + (size_t)c * r]);
-        if (v > best) {
-          best = v;
-          bi = i;
-        }
-      }
-      wave_argmax(best, bi);
-      if (lane == 0) {
-        s_p = bi;
-        piv[c] = bi;
-      }
-    }
-    __syncthreads();
-    const int p = s_p;
-    if (p != c) {
-      for (int j = tid; j < r; j += kTB) {
-        T t0 = X[c + (size_t)j * r];
-        X[c + (size_t)j * r] = X[p + (size_t)j * r];
-        X[p + (size_t)j * r] = t0;
-      }
-    }
-    __syncthreads();
-    const T d = X[c + (size_t)c * r];
-    if (d == T(0)) {
-      if (tid == 0) flag(status, 4);  // singular X_rr: the reference leaves it unchecked
-    } else {
-      const T inv = T(1) / d;
-      for (int i = c + 1 + tid; i < r; i += kTB) X[i + (size_t)c * r] *= inv;
-    }
-    __syncthreads();
-    const int rem = r - c - 1;
-    for (int e = tid; e < rem * rem; e += kTB) {
-      int i = c + 1 + e % rem, j = c + 1 + e / rem;
-      X[i + (size_t)j * r] -= X[i + (size_t)c * r] * X[c + (size_t)j * r];
-    }
-    __syncthreads();
-  }
-  // L = K_sn_r X_rr^{-1}: rows z with X^T z^T = K_sn_r(row, :)^T (dgetrs 'T')
-  for (int e = tid; e < k * r; e += kTB) Lm[e] = Ksnr[e];
-  // U = X_rr^{-1} K_r_sn (dgetrs 'N')
-  for (int e = tid; e < r * k; e += kTB) Um[e] = Krsn[e];
+  if (!block_gj_inverse(X, r, piv, Kp, Kp + n, &s_p) && tid == 0) flag(status, 4);
   __syncthreads();
-  block_lu_solve<T, kNW>(X, r, piv, r, Lm, 1, k, k, true);
-  block_lu_solve<T, kNW>(X, r, piv, r, Um, r, 1, k, false);
-  __syncthreads();
-  for (int e = tid; e < r * r; e += kTB) Xlu[e] = X[e];
-  for (int c = tid; c < r; c += kTB) piv_pool[aoff + c] = piv[c];
-  __syncthreads();
-  // schur_update = L K_r_sn (k x k), plus the descendants' update on the skeleton
-  for (int e = tid; e < k * k; e += kTB) {
+  // L = K_sn_r X_rr^{-1} (k x r), U = X_rr^{-1} K_r_sn (r x k)
+  for (int e = tid; e < k * r; e += kDecTB) {
     int i = e % k, j = e / k;
     T s = T(0);
-    for (int t = 0; t < r; ++t) s += Lm[i + (size_t)t * k] * Krsn[t + (size_t)j * r];
+    for (int t = 0; t < r; ++t) s += Ksnr[i + (size_t)t * k] * Z[t + (size_t)j * r];
+    Ls[e] = s;
+    Lm[e] = s;
+  }
+  for (int e = tid; e < r * k; e += kDecTB) {
+    int i = e % r, j = e / r;
+    T s = T(0);
+    for (int t = 0; t < r; ++t) s += Z[i + (size_t)t * r] * Krsn[t + (size_t)j * r];
+    Um[e] = s;
+  }
+  for (int e = tid; e < r * r; e += kDecTB) Xinv[e] = Z[e];
+  for (int c = tid; c < r; c += kDecTB) piv_pool[aoff + c] = piv[c];
+  __syncthreads();
+  // schur_update = L K_r_sn (k x k), plus the descendants' update on the skeleton
+  for (int e = tid; e < k * k; e += kDecTB) {
+    int i = e % k, j = e / k;
+    T s = T(0);
+    for (int t = 0; t < r; ++t) s += Ls[i + (size_t)t * k] * Krsn[t + (size_t)j * r];
     if (!leaf) s += Uact[pos[i] + (size_t)pos[j] * n];
     Acc[e] = s;
   }
@@ -626,11 +674,15 @@ __global__ __launch_bounds__(kTB) void k_decouple(int a, GeomView g, int pde, co
 // ---------------------------------------------------------------------------
 // root: allskel_mat = K(allskel, allskel) - updates, FP64 LU (skel_factorization.cpp:116-128)
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, bool LDS>
 __global__ __launch_bounds__(1024) void k_root(int x, GeomView g, int pde, const int* child,
                                                const int* osize, const uint64_t* acc_ptr,
                                                const int* acc_ld, const int* dofs, int n,
-                                               double* A, int* piv, int* status) {
+                                               double* Aws, double* Ainv, int* piv_out,
+                                               int* status) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* A = LDS ? reinterpret_cast<double*>(smem) : Aws;
+  __shared__ int piv[1024];
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int e = tid; e < n * n; e += nt) {
     int i = e % n, j = e / n;
@@ -653,49 +705,12 @@ __global__ __launch_bounds__(1024) void k_root(int x, GeomView g, int pde, const
     o += sz;
   }
   __syncthreads();
+  // allskel_mat^{-1} in place (Gauss-Jordan, dgetrf's pivot order), then out
   __shared__ int s_p;
-  const int lane = tid & (kWave - 1), wv = tid / kWave;
-  for (int c = 0; c < n; ++c) {
-    if (wv == 0) {
-      double best = -1.0;
-      int bi = n;
-      for (int i = c + lane; i < n; i += kWave) {
-        double v = fabs(A[i + (size_t)c * n]);
-        if (v > best) {
-          best = v;
-          bi = i;
-        }
-      }
-      wave_argmax(best, bi);
-      if (lane == 0) {
-        s_p = bi;
-        piv[c] = bi;
-      }
-    }
-    __syncthreads();
-    const int p = s_p;
-    if (p != c)
-      for (int j = tid; j < n; j += nt) {
-        double t0 = A[c + (size_t)j * n];
-        A[c + (size_t)j * n] = A[p + (size_t)j * n];
-        A[p + (size_t)j * n] = t0;
-      }
-    __syncthreads();
-    const double d = A[c + (size_t)c * n];
-    if (d == 0.0) {
-      if (tid == 0) flag(status, 8);
-    } else {
-      const double inv = 1.0 / d;
-      for (int i = c + 1 + tid; i < n; i += nt) A[i + (size_t)c * n] *= inv;
-    }
-    __syncthreads();
-    const int rem = n - c - 1;
-    for (int e = tid; e < rem * rem; e += nt) {
-      int i = c + 1 + e % rem, j = c + 1 + e / rem;
-      A[i + (size_t)j * n] -= A[i + (size_t)c * n] * A[c + (size_t)j * n];
-    }
-    __syncthreads();
-  }
+  __shared__ double prow[1024], fcol[1024];
+  if (!block_gj_inverse(A, n, piv, prow, fcol, &s_p) && tid == 0) flag(status, 8);
+  for (int e = tid; e < n * n; e += nt) Ainv[e] = A[e];
+  for (int i = tid; i < n; i += nt) piv_out[i] = piv[i];
 }
 
 __global__ void k_add_i64(int64_t* p, int n, int64_t add) {
@@ -722,6 +737,16 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
   const GeomView g{t.px.p, t.py.p, t.nx.p, t.ny.p, t.w.p, t.kappa.p, t.npts, t.sd};
   const int pde = o.pde;
   const int prow = pde == KIGPU_STOKES ? 4 * kProxyPoints : 2 * kProxyPoints;
+  const size_t lds_cap = c.lds_max;
+  // at most 6 markers per level (lists, ID LDS start/end, ID end, elimination
+  // end) plus 2 for the root
+  c.ensure_events(8 * (size_t)t.nlevels + 8);
+  int ev = 0;
+  auto mark = [&]() {
+    if ((size_t)ev >= c.evpool.size()) KI_FAIL(KIGPU_ERR_STATE, "event pool exhausted");
+    KI_HIP(hipEventRecord(c.evpool[ev], s));
+    return ev++;
+  };
 
   // leaves' original_box DOF lists
   DBuf<int> cnt;
@@ -754,13 +779,12 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
   t.act_pool_size = pool;
   t.work.clear();
   t.work.resize(t.nlevels);
-  DBuf<int64_t> loc;
-  loc.alloc(nn + 1);
-  DBuf<IdWork> dwork;
   std::vector<int> h_n, h_q;
-  std::vector<int64_t> h_off;
-  const size_t lds_cap = c.lds_max;
   int64_t pool_base = 0;
+  struct LevelEv {
+    int l0, l1, id0, id1, dec0, dec1, idl0, idl1;
+  };
+  std::vector<LevelEv> lev;
 
   for (int l = t.nlevels - 1; l >= 0; --l) {
     const int a = (int)t.level_off[l], b = (int)t.level_off[l + 1], cn = b - a;
@@ -768,8 +792,10 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     lw->level = l;
     lw->a = a;
     lw->b = b;
-    hipEventRecord(c.ev[0], s);
-    // ---- active lists
+    LevelEv E{};
+    E.idl0 = E.idl1 = -1;
+    E.l0 = mark();
+    // ---- active lists (remove_inactive_dofs_at_level, quadtree.cpp:664-691)
     k_active_size<<<ceil_div(cn, kTB), kTB, 0, s>>>(a, b, t.is_leaf.p, t.child.p, t.pcount.p,
                                                    t.osize.p, t.sd, t.nact.p, t.rank.p);
     exclusive_scan_i32_to_i64(t.nact.p + a, t.act_off.p + a, cn, s);
@@ -778,7 +804,9 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
                                     t.osize.p, t.rank.p, t.act_off.p, t.nact.p, t.act_pool.p,
                                     t.sr_pool.p);
     if (l == 0) {
-      hipEventRecord(c.ev[1], s);
+      E.l1 = mark();
+      lev.push_back(E);
+      t.work[l] = std::move(lw);
       break;
     }
     if (l >= 3)
@@ -795,23 +823,26 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
                             s));
     KI_HIP(hipStreamSynchronize(s));
     ++c.tim.syncs;
+    int64_t level_dofs = 0;
+    for (int i = 0; i < cn; ++i) level_dofs += h_n[i];
     if (l <= 2) {
-      int64_t tot = 0;
-      for (int i = 0; i < cn; ++i) tot += h_n[i];
       int64_t extra = 0;
       if (l == 2) {
         for (int y = (int)t.level_off[1]; y < (int)t.level_off[2]; ++y)
           if (t.h_is_leaf[y]) extra += (int64_t)t.h_pcount[y] * t.sd;
       }
-      for (int i = 0; i < cn; ++i) h_q[i] = (int)(tot - h_n[i] + extra);
+      for (int i = 0; i < cn; ++i) h_q[i] = (int)(level_dofs - h_n[i] + extra);
     }
     // ---- work list: boxes with >= min_dofs active DOFs and a non-empty ID matrix
-    std::vector<IdWork> wl;
-    std::vector<int64_t> rows_off(cn, 0);
-    int64_t rtot = 0, atot = 0;
+    std::vector<IdWork> wl_lds, wl_glb;
+    std::vector<int64_t> rows_off(cn, 0), soff(cn);
+    int64_t rtot = 0, atot = 0, stot = 0;
     int nmax = 0;
+    size_t lds_bytes = 0, glds = 0;
     for (int i = 0; i < cn; ++i) {
       nmax = std::max(nmax, h_n[i]);
+      soff[i] = stot;
+      stot += (int64_t)slab_elems(h_n[i]);
       if (h_n[i] < o.min_dofs) continue;
       const int x = a + i;
       IdWork w{};
@@ -829,108 +860,85 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
       w.cy = cc[1] + t.h_side[x] / 2.0;
       w.rad = t.h_side[x] * kRadiusRatio;
       w.pw = 2.0 * M_PI * w.rad / kProxyPoints;
-      IdLds L = id_lds_layout(w.m, w.n, w.q, true, sizeof(double));
-      w.in_lds = L.total <= lds_cap;
+      const size_t in_l = id_lds_layout(w.m, w.n, w.q, true, sizeof(double)).total;
+      w.in_lds = in_l <= lds_cap;
       w.a_off = atot;
-      if (!w.in_lds) atot += (int64_t)w.m * w.n;
-      wl.push_back(w);
+      if (w.in_lds) {
+        lds_bytes = std::max(lds_bytes, in_l);
+        wl_lds.push_back(w);
+      } else {
+        atot += (int64_t)w.m * w.n;
+        glds = std::max(glds, id_lds_layout(w.m, w.n, w.q, false, sizeof(double)).total);
+        wl_glb.push_back(w);
+      }
     }
     lw->nmax = nmax;
     // ---- factor slabs for every node of the level
-    std::vector<int64_t> soff(cn);
-    int64_t stot = 0;
-    for (int i = 0; i < cn; ++i) {
-      soff[i] = stot;
-      stot += (int64_t)slab_elems(h_n[i]);
-    }
-    DBuf<int64_t> dsoff;
-    dsoff.upload(soff, s);
+    lw->soff.upload(soff, s);
+    T* slab;
     if (sizeof(T) == 8) {
       lw->slab64.alloc(stot);
-      k_set_ptr<<<ceil_div(cn, kTB), kTB, 0, s>>>(t.fac_ptr.p, a, b, t.nact.p,
-                                                 reinterpret_cast<uint64_t>(lw->slab64.p),
-                                                 dsoff.p, sizeof(T));
+      slab = reinterpret_cast<T*>(lw->slab64.p);
     } else {
       lw->slab32.alloc(stot);
-      k_set_ptr<<<ceil_div(cn, kTB), kTB, 0, s>>>(t.fac_ptr.p, a, b, t.nact.p,
-                                                 reinterpret_cast<uint64_t>(lw->slab32.p),
-                                                 dsoff.p, sizeof(T));
+      slab = reinterpret_cast<T*>(lw->slab32.p);
     }
+    k_set_ptr<<<ceil_div(cn, kTB), kTB, 0, s>>>(t.fac_ptr.p, a, b, t.nact.p,
+                                               reinterpret_cast<uint64_t>(slab), lw->soff.p,
+                                               sizeof(T));
     // ---- ID rows
-    t.rows_pool.alloc(std::max<int64_t>(rtot, 1));
-    if (!wl.empty()) {
-      dwork.upload(wl, s);
-      if (l >= 3) {
-        DBuf<int64_t> droff;
-        droff.upload(rows_off, s);
-        // rows_off is indexed by node - a inside the kernel through this shifted pointer
-        k_inner_rows<<<cn, kWave, 0, s>>>(a, b, o.min_dofs, g, t.is_leaf.p, t.obox_off.p,
-                                          t.obox_pool.p, t.pcount.p, t.act_off.p, t.nact.p,
-                                          t.act_pool.p, t.sl_nbr.p, t.sl_cnt.p, t.cp_off.p,
-                                          t.cp_nbr.p, t.corners.p, t.side.p, t.nrows.p,
-                                          droff.p - a, t.rows_pool.p, 1);
-        KI_HIP(hipStreamSynchronize(s));
-      } else {
-        k_outside_rows<<<(int)wl.size(), kTB, 0, s>>>(
-            dwork.p, a, b, l, (int)t.level_off[1], (int)t.level_off[2], t.is_leaf.p,
+    lw->rows.alloc(std::max<int64_t>(rtot, 1));
+    if (l >= 3) {
+      lw->rows_off.upload(rows_off, s);
+      k_inner_rows<<<cn, kWave, 0, s>>>(a, b, o.min_dofs, g, t.is_leaf.p, t.obox_off.p,
+                                        t.obox_pool.p, t.pcount.p, t.act_off.p, t.nact.p,
+                                        t.act_pool.p, t.sl_nbr.p, t.sl_cnt.p, t.cp_off.p,
+                                        t.cp_nbr.p, t.corners.p, t.side.p, t.nrows.p,
+                                        lw->rows_off.p - a, lw->rows.p, 1);
+    }
+    if (!wl_lds.empty()) lw->wl_lds.upload(wl_lds, s);
+    if (!wl_glb.empty()) lw->wl_glb.upload(wl_glb, s);
+    if (l <= 2) {
+      if (!wl_lds.empty())
+        k_outside_rows<<<(int)wl_lds.size(), kTB, 0, s>>>(
+            lw->wl_lds.p, a, b, l, (int)t.level_off[1], (int)t.level_off[2], t.is_leaf.p,
             t.obox_off.p, t.obox_pool.p, t.pcount.p, t.sd, t.act_off.p, t.nact.p, t.act_pool.p,
-            t.rows_pool.p);
-      }
+            lw->rows.p);
+      if (!wl_glb.empty())
+        k_outside_rows<<<(int)wl_glb.size(), kTB, 0, s>>>(
+            lw->wl_glb.p, a, b, l, (int)t.level_off[1], (int)t.level_off[2], t.is_leaf.p,
+            t.obox_off.p, t.obox_pool.p, t.pcount.p, t.sd, t.act_off.p, t.nact.p, t.act_pool.p,
+            lw->rows.p);
     }
-    hipEventRecord(c.ev[1], s);
+    E.l1 = mark();
     // ---- ID
-    if (!wl.empty()) {
-      std::vector<IdWork> wl_lds, wl_glb;
-      size_t lds_bytes = 0, glds = 0;
-      for (auto& w : wl) {
-        if (w.in_lds) {
-          wl_lds.push_back(w);
-          lds_bytes = std::max(lds_bytes, id_lds_layout(w.m, w.n, w.q, true, sizeof(double)).total);
-        } else {
-          wl_glb.push_back(w);
-          glds = std::max(glds, id_lds_layout(w.m, w.n, w.q, false, sizeof(double)).total);
-        }
-      }
-      double* ws = nullptr;
-      if (atot) {
-        t.ws64.alloc(atot);
-        ws = t.ws64.p;
-      }
-      DBuf<IdWork> d1, d2;
-      if (!wl_lds.empty()) {
-        d1.upload(wl_lds, s);
-        allow_lds(k_id<double, T, true>, lds_bytes);
-        KI_HIP(hipEventRecord(c.ev[12], s));
-        k_id<double, T, true><<<(int)wl_lds.size(), kTB, lds_bytes, s>>>(
-            d1.p, g, pde, o.id_tol, t.rows_pool.p, t.act_pool.p, t.act_off.p, c.ring.p, ws,
-            t.rank.p, t.sr_pool.p, t.pos_pool.p, t.fac_ptr.p, t.status.p);
-        KI_HIP(hipGetLastError());
-        KI_HIP(hipEventRecord(c.ev[13], s));
-        KI_HIP(hipEventSynchronize(c.ev[13]));
-        float kms = 0;
-        KI_HIP(hipEventElapsedTime(&kms, c.ev[12], c.ev[13]));
-        c.tim.id_kernel_s += kms * 1e-3;
-        c.tim.id_launches += 1;
-        lw->id_lds = wl_lds;
-      }
-      if (!wl_glb.empty()) {
-        d2.upload(wl_glb, s);
-        allow_lds(k_id<double, T, false>, glds);
-        k_id<double, T, false><<<(int)wl_glb.size(), kTB, glds, s>>>(
-            d2.p, g, pde, o.id_tol, t.rows_pool.p, t.act_pool.p, t.act_off.p, c.ring.p, ws,
-            t.rank.p, t.sr_pool.p, t.pos_pool.p, t.fac_ptr.p, t.status.p);
-        KI_HIP(hipGetLastError());
-      }
-      KI_HIP(hipStreamSynchronize(s));
+    E.id0 = E.l1;
+    if (!wl_lds.empty()) {
+      allow_lds(k_id<double, T, true>, lds_bytes);
+      E.idl0 = mark();
+      k_id<double, T, true><<<(int)wl_lds.size(), kIdTB, lds_bytes, s>>>(
+          lw->wl_lds.p, g, pde, o.id_tol, lw->rows.p, t.act_pool.p, t.act_off.p, c.ring.p,
+          (double*)nullptr, t.rank.p, t.sr_pool.p, t.pos_pool.p, t.fac_ptr.p, t.status.p);
+      KI_HIP(hipGetLastError());
+      E.idl1 = mark();
+      lw->id_lds = wl_lds;
     }
-    hipEventRecord(c.ev[2], s);
+    if (!wl_glb.empty()) {
+      lw->ws.alloc(atot);
+      allow_lds(k_id<double, T, false>, glds);
+      k_id<double, T, false><<<(int)wl_glb.size(), kIdTB, glds, s>>>(
+          lw->wl_glb.p, g, pde, o.id_tol, lw->rows.p, t.act_pool.p, t.act_off.p, c.ring.p,
+          lw->ws.p, t.rank.p, t.sr_pool.p, t.pos_pool.p, t.fac_ptr.p, t.status.p);
+      KI_HIP(hipGetLastError());
+    }
+    E.id1 = mark();
     // ---- elimination for every node of the level
+    E.dec0 = E.id1;
     {
       DecLds L = dec_layout(std::max(nmax, 1), sizeof(T));
-      KI_HIP(hipEventRecord(c.ev[14], s));
       if (L.total <= lds_cap) {
         allow_lds(k_decouple<T, true>, L.total);
-        k_decouple<T, true><<<cn, kTB, L.total, s>>>(
+        k_decouple<T, true><<<cn, kDecTB, L.total, s>>>(
             a, g, pde, t.is_leaf.p, t.child.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, t.nact.p,
             t.rank.p, t.act_off.p, t.act_pool.p, t.sr_pool.p, t.pos_pool.p, t.piv_pool.p,
             t.fac_ptr.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, (T*)nullptr, 0, t.status.p);
@@ -938,70 +946,80 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
         size_t stride = (L.total + sizeof(T) - 1) / sizeof(T);
         T* ws;
         if (sizeof(T) == 8) {
-          t.ws64.alloc(stride * cn);
-          ws = reinterpret_cast<T*>(t.ws64.p);
+          lw->dws64.alloc(stride * cn);
+          ws = reinterpret_cast<T*>(lw->dws64.p);
         } else {
-          t.ws32.alloc(stride * cn);
-          ws = reinterpret_cast<T*>(t.ws32.p);
+          lw->dws32.alloc(stride * cn);
+          ws = reinterpret_cast<T*>(lw->dws32.p);
         }
-        k_decouple<T, false><<<cn, kTB, 0, s>>>(
+        k_decouple<T, false><<<cn, kDecTB, 0, s>>>(
             a, g, pde, t.is_leaf.p, t.child.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, t.nact.p,
             t.rank.p, t.act_off.p, t.act_pool.p, t.sr_pool.p, t.pos_pool.p, t.piv_pool.p,
             t.fac_ptr.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, ws, stride, t.status.p);
       }
       KI_HIP(hipGetLastError());
-      KI_HIP(hipEventRecord(c.ev[15], s));
-      KI_HIP(hipEventSynchronize(c.ev[15]));
-      float kms = 0;
-      KI_HIP(hipEventElapsedTime(&kms, c.ev[14], c.ev[15]));
-      c.tim.dec_kernel_s += kms * 1e-3;
-      c.tim.dec_launches += 1;
     }
-    hipEventRecord(c.ev[3], s);
-    KI_HIP(hipStreamSynchronize(s));
-    float ms = 0;
-    hipEventElapsedTime(&ms, c.ev[0], c.ev[1]);
-    c.tim.t_lists += ms * 1e-3;
-    hipEventElapsedTime(&ms, c.ev[1], c.ev[2]);
-    c.tim.t_id += ms * 1e-3;
-    hipEventElapsedTime(&ms, c.ev[2], c.ev[3]);
-    c.tim.t_decouple += ms * 1e-3;
-    pool_base += 0;
-    (void)h_off;
+    E.dec1 = mark();
+    lev.push_back(E);
     t.work[l] = std::move(lw);
-    // next level's lists start after this level's
-    int64_t lastoff = 0;
-    KI_HIP(hipMemcpyAsync(&lastoff, t.act_off.p + b - 1, sizeof(int64_t), hipMemcpyDeviceToHost,
-                          s));
-    KI_HIP(hipStreamSynchronize(s));
-    pool_base = lastoff + h_n[cn - 1];
+    pool_base += level_dofs;  // the next level's lists start after this level's
   }
-  (void)loc;
 
   // ---- root (level 0)
-  hipEventRecord(c.ev[4], s);
+  const int e_root0 = mark();
   const int root = 0;
   int n0 = 0;
   int64_t r_off = 0;
   KI_HIP(hipMemcpyAsync(&n0, t.nact.p + root, sizeof(int), hipMemcpyDeviceToHost, s));
   KI_HIP(hipMemcpyAsync(&r_off, t.act_off.p + root, sizeof(int64_t), hipMemcpyDeviceToHost, s));
   KI_HIP(hipStreamSynchronize(s));
+  ++c.tim.syncs;
   t.nroot = n0;
   if (n0 > 0) {
+    if (n0 > 1024) KI_FAIL(KIGPU_ERR_NOTSUPP, "root skeleton larger than 1024 DOFs");
     t.root_dofs.alloc(n0);
     KI_HIP(hipMemcpyAsync(t.root_dofs.p, t.act_pool.p + r_off, sizeof(int) * n0,
                           hipMemcpyDeviceToDevice, s));
-    t.root_lu.alloc((size_t)n0 * n0);
+    t.root_lu.alloc((size_t)n0 * n0);   // allskel_mat^{-1}
     t.root_piv.alloc(n0);
-    k_root<T><<<1, 1024, 0, s>>>(root, g, pde, t.child.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p,
-                                 t.root_dofs.p, n0, t.root_lu.p, t.root_piv.p, t.status.p);
+    const size_t lbytes = sizeof(double) * (size_t)n0 * n0;
+    if (lbytes + 24576 <= lds_cap) {  // static LDS of k_root: pivots + 2 scratch rows
+      allow_lds(k_root<T, true>, lbytes);
+      k_root<T, true><<<1, 1024, lbytes, s>>>(root, g, pde, t.child.p, t.osize.p, t.acc_ptr.p,
+                                              t.acc_ld.p, t.root_dofs.p, n0, nullptr, t.root_lu.p,
+                                              t.root_piv.p, t.status.p);
+    } else {
+      t.ws64.alloc((size_t)n0 * n0);
+      k_root<T, false><<<1, 1024, 0, s>>>(root, g, pde, t.child.p, t.osize.p, t.acc_ptr.p,
+                                          t.acc_ld.p, t.root_dofs.p, n0, t.ws64.p, t.root_lu.p,
+                                          t.root_piv.p, t.status.p);
+    }
     KI_HIP(hipGetLastError());
   }
-  hipEventRecord(c.ev[5], s);
+  const int e_root1 = mark();
   KI_HIP(hipStreamSynchronize(s));
-  float ms = 0;
-  hipEventElapsedTime(&ms, c.ev[4], c.ev[5]);
-  c.tim.t_root += ms * 1e-3;
+  ++c.tim.syncs;
+  auto el = [&](int e0, int e1) {
+    float ms = 0;
+    KI_HIP(hipEventElapsedTime(&ms, c.evpool[e0], c.evpool[e1]));
+    return ms * 1e-3;
+  };
+  for (const LevelEv& E : lev) {
+    c.tim.t_lists += el(E.l0, E.l1);
+    if (E.id1 > E.id0) {
+      c.tim.t_id += el(E.id0, E.id1);
+      c.tim.t_decouple += el(E.dec0, E.dec1);
+    }
+    if (E.idl0 >= 0) {
+      c.tim.id_kernel_s += el(E.idl0, E.idl1);
+      c.tim.id_launches += 1;
+    }
+    if (E.dec1 > E.dec0) {
+      c.tim.dec_kernel_s += el(E.dec0, E.dec1);
+      c.tim.dec_launches += 1;
+    }
+  }
+  c.tim.t_root += el(e_root0, e_root1);
 }
 
 // Algorithmic flops of the ID kernel for one box, as executed: entry

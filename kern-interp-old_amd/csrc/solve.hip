@@ -26,8 +26,7 @@ constexpr int kRC = 16;  // right-hand sides per block
 template <typename T>
 __global__ __launch_bounds__(kTB) void k_up(int a, const int* rank, const int* nact,
                                             const int64_t* act_off, const int* sr_pool,
-                                            const int* piv_pool, const uint64_t* fac_ptr, T* x,
-                                            int ldx, int nrhs) {
+                                            const uint64_t* fac_ptr, T* x, int ldx, int nrhs) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int node = a + blockIdx.x;
   const int k = rank[node];
@@ -35,11 +34,11 @@ __global__ __launch_bounds__(kTB) void k_up(int a, const int* rank, const int* n
   const int n = nact[node], r = n - k;
   const int c0 = blockIdx.y * kRC, nc = min(kRC, nrhs - c0);
   const int* dofs = sr_pool + act_off[node];
-  const int* piv = piv_pool + act_off[node];
   const T* Tm = reinterpret_cast<const T*>(fac_ptr[node]);
   const T* Lm = Tm + (size_t)k * r;
-  const T* Xlu = Tm + 3 * (size_t)k * r;
+  const T* Xinv = Tm + 3 * (size_t)k * r;
   T* v = reinterpret_cast<T*>(smem);
+  T* tmp = v + (size_t)n * nc;
   const int tid = threadIdx.x;
   for (int e = tid; e < n * nc; e += kTB) {
     int i = e % n, c = e / n;
@@ -61,12 +60,21 @@ __global__ __launch_bounds__(kTB) void k_up(int a, const int* rank, const int* n
     for (int i = 0; i < r; ++i) s -= Lm[t + (size_t)i * k] * vs[k + i];
     v[t + (size_t)c * n] = s;
   }
+  for (int e = tid; e < r * nc; e += kTB) {  // x_r <- X_rr^{-1} x_r
+    int i = e % r, c = e / r;
+    const T* vs = v + (size_t)c * n + k;
+    T s = T(0);
+    for (int j = 0; j < r; ++j) s += Xinv[i + (size_t)j * r] * vs[j];
+    tmp[e] = s;
+  }
   __syncthreads();
-  block_lu_solve<T, kNW>(Xlu, r, piv, r, v + k, n, 1, nc, false);  // x_r <- X_rr^{-1} x_r
-  __syncthreads();
-  for (int e = tid; e < n * nc; e += kTB) {
-    int i = e % n, c = e / n;
-    x[dofs[i] + (size_t)(c0 + c) * ldx] = v[e];
+  for (int e = tid; e < k * nc; e += kTB) {
+    int t = e % k, c = e / k;
+    x[dofs[t] + (size_t)(c0 + c) * ldx] = v[t + (size_t)c * n];
+  }
+  for (int e = tid; e < r * nc; e += kTB) {
+    int i = e % r, c = e / r;
+    x[dofs[k + i] + (size_t)(c0 + c) * ldx] = tmp[e];
   }
 }
 
@@ -116,60 +124,20 @@ __global__ __launch_bounds__(kTB) void k_down(int a, const int* rank, const int*
   }
 }
 
-// x[allskel] <- allskel_mat^{-1} x[allskel] in FP64, one wave per column.
-// Lane l holds elements l, l + 64, ... (E per lane, n <= 64 E).
-template <typename T, int E>
-__global__ __launch_bounds__(64) void k_root_solve(const int* dofs, int n, const double* lu,
-                                                   const int* piv, T* x, int ldx) {
-  __shared__ double buf[64 * E];
-  const int lane = threadIdx.x;
+// x[allskel] <- allskel_mat^{-1} x[allskel] in FP64 with the explicit inverse;
+// one block per right-hand side, one thread per row.
+template <typename T>
+__global__ __launch_bounds__(kTB) void k_root_apply(const int* dofs, int n, const double* Ainv,
+                                                    T* x, int ldx) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* v = reinterpret_cast<double*>(smem);
   T* xc = x + (size_t)blockIdx.x * ldx;
-  for (int i = lane; i < n; i += 64) buf[i] = (double)xc[dofs[i]];
+  for (int i = threadIdx.x; i < n; i += kTB) v[i] = (double)xc[dofs[i]];
   __syncthreads();
-  if (lane == 0)
-    for (int c = 0; c < n; ++c) {
-      int p = piv[c];
-      if (p != c) {
-        double t0 = buf[c];
-        buf[c] = buf[p];
-        buf[p] = t0;
-      }
-    }
-  __syncthreads();
-  double u[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) u[e] = e * 64 + lane < n ? buf[e * 64 + lane] : 0.0;
-#pragma unroll
-  for (int tb = 0; tb < E; ++tb) {
-    for (int tl = 0; tl < 64; ++tl) {
-      const int t = tb * 64 + tl;
-      if (t >= n - 1) break;
-      double ut = __shfl(u[tb], tl, 64);
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        int i = e * 64 + lane;
-        if (i > t && i < n) u[e] -= lu[i + (size_t)t * n] * ut;
-      }
-    }
-  }
-#pragma unroll
-  for (int tb = E - 1; tb >= 0; --tb) {
-    for (int tl = 63; tl >= 0; --tl) {
-      const int t = tb * 64 + tl;
-      if (t >= n) continue;
-      if (lane == tl) u[tb] /= lu[t + (size_t)t * n];
-      double ut = __shfl(u[tb], tl, 64);
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        int i = e * 64 + lane;
-        if (i < t) u[e] -= lu[i + (size_t)t * n] * ut;
-      }
-    }
-  }
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    int i = e * 64 + lane;
-    if (i < n) xc[dofs[i]] = (T)u[e];
+  for (int i = threadIdx.x; i < n; i += kTB) {
+    double s = 0;
+    for (int j = 0; j < n; ++j) s += Ainv[i + (size_t)j * n] * v[j];
+    xc[dofs[i]] = (T)s;
   }
 }
 
@@ -319,25 +287,14 @@ void sweeps(Context& c, TreeData& t, T* x, int ldx, int nrhs) {
   for (int l = t.nlevels - 1; l >= 1; --l) {
     LevelWork& w = *t.work[l];
     if (w.nmax == 0) continue;
-    size_t lds = sizeof(T) * (size_t)w.nmax * rcol;
+    size_t lds = 2 * sizeof(T) * (size_t)w.nmax * rcol;
     allow_lds(k_up<T>, lds);
     k_up<T><<<dim3(w.b - w.a, ncb), kTB, lds, s>>>(w.a, t.rank.p, t.nact.p, t.act_off.p,
-                                                   t.sr_pool.p, t.piv_pool.p, t.fac_ptr.p, x,
-                                                   ldx, nrhs);
+                                                   t.sr_pool.p, t.fac_ptr.p, x, ldx, nrhs);
   }
-  if (t.nroot > 0) {
-    const int n = t.nroot;
-    if (n <= 64)
-      k_root_solve<T, 1><<<nrhs, 64, 0, s>>>(t.root_dofs.p, n, t.root_lu.p, t.root_piv.p, x, ldx);
-    else if (n <= 128)
-      k_root_solve<T, 2><<<nrhs, 64, 0, s>>>(t.root_dofs.p, n, t.root_lu.p, t.root_piv.p, x, ldx);
-    else if (n <= 256)
-      k_root_solve<T, 4><<<nrhs, 64, 0, s>>>(t.root_dofs.p, n, t.root_lu.p, t.root_piv.p, x, ldx);
-    else if (n <= 512)
-      k_root_solve<T, 8><<<nrhs, 64, 0, s>>>(t.root_dofs.p, n, t.root_lu.p, t.root_piv.p, x, ldx);
-    else
-      KI_FAIL(KIGPU_ERR_NOTSUPP, "root skeleton larger than 512 DOFs");
-  }
+  if (t.nroot > 0)
+    k_root_apply<T><<<nrhs, kTB, sizeof(double) * t.nroot, s>>>(t.root_dofs.p, t.nroot,
+                                                                t.root_lu.p, x, ldx);
   for (int l = 1; l < t.nlevels; ++l) {
     LevelWork& w = *t.work[l];
     if (w.nmax == 0) continue;

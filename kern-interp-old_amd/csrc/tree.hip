@@ -29,10 +29,13 @@ namespace {
 
 constexpr int kTB = 256;
 
-__global__ void k_minmax(const double* px, const double* py, int n, double* out) {
+// ---- root box: min/max pooled over x and y (quadtree.cpp:80-88) ----------
+constexpr int kMMBlocks = 120;
+
+__global__ void k_minmax_part(const double* px, const double* py, int n, double* part) {
   __shared__ double smin[kTB], smax[kTB];
   double mn = px[0], mx = px[0];
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+  for (int i = blockIdx.x * kTB + threadIdx.x; i < n; i += gridDim.x * kTB) {
     double a = px[i], b = py[i];
     mn = fmin(mn, fmin(a, b));
     mx = fmax(mx, fmax(a, b));
@@ -40,22 +43,38 @@ __global__ void k_minmax(const double* px, const double* py, int n, double* out)
   smin[threadIdx.x] = mn;
   smax[threadIdx.x] = mx;
   __syncthreads();
-  for (int s = kTB / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + s]);
-      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+  for (int st = kTB / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + st]);
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + st]);
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    out[0] = smin[0];
-    out[1] = smax[0];
+    part[2 * blockIdx.x] = smin[0];
+    part[2 * blockIdx.x + 1] = smax[0];
   }
+}
+
+__global__ void k_minmax_final(const double* part, int np, double* out) {
+  if (threadIdx.x != 0) return;
+  double mn = part[0], mx = part[1];
+  for (int i = 1; i < np; ++i) {
+    mn = fmin(mn, part[2 * i]);
+    mx = fmax(mx, part[2 * i + 1]);
+  }
+  out[0] = mn;
+  out[1] = mx;
 }
 
 __global__ void k_iota(int* p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = i;
+}
+
+__global__ void k_fill_i32(int* p, int n, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 __device__ __forceinline__ void node_mid(const double* c, double* midx, double* midy) {
@@ -88,8 +107,57 @@ __global__ void k_split_info(int c, const int* pcount, const int64_t* pstart, co
   st[i] = t;
 }
 
+// Rank of each splitting node by (split point, child-slot path): bitonic sort
+// of (st << 40 | path) keys in one workgroup.  Needs c <= 4096, paths of at
+// most 20 levels and split points below 2^24; k_rank_pairs covers the rest.
+constexpr int kSortMax = 4096;
+
+__global__ __launch_bounds__(1024) void k_rank_sort(int c, int P, const int* split, const int* st,
+                                                    const uint64_t* path, int* rank, int* nsplit) {
+  __shared__ uint64_t key[kSortMax];
+  __shared__ int val[kSortMax];
+  for (int i = threadIdx.x; i < P; i += 1024) {
+    if (i < c && split[i]) {
+      key[i] = ((uint64_t)st[i] << 40) | path[i];
+    } else {
+      key[i] = ~0ull;
+    }
+    val[i] = i;
+  }
+  __syncthreads();
+  for (int size = 2; size <= P; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = threadIdx.x; i < P; i += 1024) {
+        int j = i ^ stride;
+        if (j > i) {
+          bool up = (i & size) == 0;
+          uint64_t a = key[i], b = key[j];
+          if ((a > b) == up) {
+            key[i] = b;
+            key[j] = a;
+            int t = val[i];
+            val[i] = val[j];
+            val[j] = t;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  int cnt = 0;
+  for (int i = threadIdx.x; i < P; i += 1024) {
+    if (key[i] != ~0ull) {
+      rank[val[i]] = i;
+      ++cnt;
+    } else if (val[i] < c) {
+      rank[val[i]] = -1;
+    }
+  }
+  if (cnt) atomicAdd(nsplit, cnt);
+}
+
 // rank of each splitting node among the splitting nodes by (st, path)
-__global__ void k_rank(int c, const int* split, const int* st, const uint64_t* path, int* rank,
+__global__ void k_rank_pairs(int c, const int* split, const int* st, const uint64_t* path, int* rank,
                        int* nsplit) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c) return;
@@ -141,75 +209,98 @@ __global__ void k_children(int c, const int* rank, const double* corners, const 
   }
 }
 
-// quadrant counts of each splitting node's points -> cnt[4*rank + q]
-__global__ void k_quad_count(const int* rank, const int* pcount, const int64_t* pstart,
-                             const int* perm, const double* corners, const double* px,
-                             const double* py, int* cnt) {
-  int i = blockIdx.x;
-  if (rank[i] < 0) return;
-  __shared__ int sc[4];
-  if (threadIdx.x < 4) sc[threadIdx.x] = 0;
-  __syncthreads();
-  double midx, midy;
-  node_mid(corners + 8 * i, &midx, &midy);
-  int loc[4] = {0, 0, 0, 0};
-  const int* pp = perm + pstart[i];
-  for (int j = threadIdx.x; j < pcount[i]; j += blockDim.x) {
-    int p = pp[j];
-    loc[quadrant(px[p], py[p], midx, midy)]++;
-  }
-  for (int q = 0; q < 4; ++q)
-    if (loc[q]) atomicAdd(&sc[q], loc[q]);
-  __syncthreads();
-  if (threadIdx.x < 4) cnt[4 * rank[i] + threadIdx.x] = sc[threadIdx.x];
+// ---- stable 4-way partition of the points of splitting nodes -----------
+struct Q4 {
+  int v[4];
+};
+
+__device__ __forceinline__ Q4 q4_add(const Q4& a, const Q4& b) {
+  Q4 r;
+  for (int i = 0; i < 4; ++i) r.v[i] = a.v[i] + b.v[i];
+  return r;
 }
 
-// stable 4-way partition of each splitting node's points into its children
-__global__ __launch_bounds__(kTB) void k_partition(const int* rank, const int* pcount,
-                                                   const int64_t* pstart, const int* perm,
-                                                   const double* corners, const double* px,
-                                                   const double* py, const int64_t* child_start,
-                                                   int* next_perm) {
-  int i = blockIdx.x;
-  if (rank[i] < 0) return;
-  __shared__ int wtot[kTB / kWave][4];
-  __shared__ int running[4];
-  if (threadIdx.x < 4) running[threadIdx.x] = 0;
-  double midx, midy;
-  node_mid(corners + 8 * i, &midx, &midy);
-  int64_t base[4];
-  for (int q = 0; q < 4; ++q) base[q] = child_start[4 * rank[i] + q];
-  const int* pp = perm + pstart[i];
-  int cnt = pcount[i];
-  int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+// quadrant of each point position of the level (-1: its node does not split)
+__global__ void k_classify(int np, const int* perm, const int* pnode, const int* rank,
+                           const double* corners, const double* px, const double* py, int* qcode) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  int nd = pnode[p];
+  int q = -1;
+  if (rank[nd] >= 0) {
+    double midx, midy;
+    node_mid(corners + 8 * nd, &midx, &midy);
+    int pt = perm[p];
+    q = quadrant(px[pt], py[pt], midx, midy);
+  }
+  qcode[p] = q;
+}
+
+constexpr int kScanTB = 1024;
+
+// block-level inclusive scan of the one-hot quadrant flags; block totals out
+__global__ __launch_bounds__(kScanTB) void k_q4_block(int np, const int* qcode, Q4* pre,
+                                                      Q4* btot) {
+  __shared__ Q4 buf[kScanTB];
+  int p = blockIdx.x * kScanTB + threadIdx.x;
+  Q4 v = {{0, 0, 0, 0}};
+  if (p < np && qcode[p] >= 0) v.v[qcode[p]] = 1;
+  buf[threadIdx.x] = v;
   __syncthreads();
-  for (int c0 = 0; c0 < cnt; c0 += kTB) {
-    int j = c0 + threadIdx.x;
-    int p = -1, q = -1;
-    if (j < cnt) {
-      p = pp[j];
-      q = quadrant(px[p], py[p], midx, midy);
-    }
-    int pre = 0;
-    for (int qq = 0; qq < 4; ++qq) {
-      unsigned long long m = __ballot(q == qq);
-      if (q == qq) pre = __popcll(m & ((1ull << lane) - 1));
-      if (lane == 0) wtot[wv][qq] = __popcll(m);
-    }
+  for (int off = 1; off < kScanTB; off <<= 1) {
+    Q4 t = {{0, 0, 0, 0}};
+    if (threadIdx.x >= off) t = buf[threadIdx.x - off];
     __syncthreads();
-    if (q >= 0) {
-      int off = running[q] + pre;
-      for (int w = 0; w < wv; ++w) off += wtot[w][q];
-      next_perm[base[q] + off] = p;
-    }
-    __syncthreads();
-    if (threadIdx.x < 4) {
-      int s = 0;
-      for (int w = 0; w < kTB / kWave; ++w) s += wtot[w][threadIdx.x];
-      running[threadIdx.x] += s;
-    }
+    buf[threadIdx.x] = q4_add(buf[threadIdx.x], t);
     __syncthreads();
   }
+  if (p < np) {  // exclusive within the block
+    Q4 e = buf[threadIdx.x];
+    for (int i = 0; i < 4; ++i) e.v[i] -= v.v[i];
+    pre[p] = e;
+  }
+  if (threadIdx.x == kScanTB - 1) btot[blockIdx.x] = buf[kScanTB - 1];
+}
+
+// exclusive scan of the block totals (one thread; few blocks); total at [nb]
+__global__ void k_q4_top(int nb, Q4* btot, Q4* total) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  Q4 run = {{0, 0, 0, 0}};
+  for (int b = 0; b < nb; ++b) {
+    Q4 t = btot[b];
+    btot[b] = run;
+    run = q4_add(run, t);
+  }
+  *total = run;
+}
+
+__global__ void k_q4_apply(int np, Q4* pre, const Q4* btot, const Q4* total) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < np) pre[p] = q4_add(pre[p], btot[p / kScanTB]);
+  if (p == np) pre[np] = *total;
+}
+
+// points per child: counts between the node's first and one-past-last position
+__global__ void k_child_counts(int c, const int* rank, const int64_t* pstart, const int* pcount,
+                               const Q4* pre, int* ccnt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c || rank[i] < 0) return;
+  const Q4 a = pre[pstart[i]], b = pre[pstart[i] + pcount[i]];
+  for (int q = 0; q < 4; ++q) ccnt[4 * rank[i] + q] = b.v[q] - a.v[q];
+}
+
+__global__ void k_scatter(int np, const int* perm, const int* pnode, const int* qcode,
+                          const int* rank, const int64_t* pstart, const Q4* pre,
+                          const int64_t* cstart, int* nperm_out, int* npnode_out) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const int q = qcode[p];
+  if (q < 0) return;
+  const int nd = pnode[p];
+  const int ch = 4 * rank[nd] + q;
+  const int64_t dst = cstart[ch] + (pre[p].v[q] - pre[pstart[nd]].v[q]);
+  nperm_out[dst] = perm[p];
+  npnode_out[dst] = ch;
 }
 
 // same-level neighbor lists of level [a, b): siblings, then cousins
@@ -312,52 +403,53 @@ __global__ void k_coarse(int nnodes, const int* parent, const int* level, const 
   if (!pass) cp_cnt[d] = cnt;
 }
 
-template <typename T>
-__global__ void k_gather_copy(const T* src, T* dst, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = src[i];
-}
-
-__global__ void k_level_fill(int* level, int a, int b, int l) {
-  int i = a + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < b) level[i] = l;
-}
-
-__global__ void k_offset_i32(int* p, int n, int add) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && p[i] >= 0) p[i] += add;
-}
-
-__global__ void k_offset_i64(int64_t* p, int n, int64_t add) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] += add;
-}
-
-// children ids of level-l nodes: child[4*x + s] = next_base + 4*rank + s
-__global__ void k_child_ids(int c, const int* rank, int next_base, int* child) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= c) return;
-  for (int s = 0; s < 4; ++s) child[4 * i + s] = rank[i] < 0 ? -1 : next_base + 4 * rank[i] + s;
-}
-
 struct LevelBuf {
   int count = 0;
-  DBuf<int> parent_local, slot, split, st, rank, pcount, child;
+  DBuf<int> parent_local, slot, split, st, rank, pcount;
   DBuf<int64_t> pstart;
   DBuf<uint64_t> path;
   DBuf<double> corners, side;
-  DBuf<int> perm;
+  DBuf<int> perm, pnode;
   int nperm = 0;
 };
+
+// one launch per level: copy the level's records into the flat node arrays
+struct FlatArgs {
+  int c, base, parent_base, next_base, level, has_next;
+  int64_t pbase;
+  const int *parent_local, *slot, *split, *rank, *pcount;
+  const int64_t* pstart;
+  const double *corners, *side;
+  int *parent, *child, *is_leaf, *levelv, *slotv, *pcountv;
+  int64_t* pstartv;
+  double *cornersv, *sidev;
+};
+
+__global__ void k_flatten(FlatArgs f) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= f.c) return;
+  const int x = f.base + i;
+  f.parent[x] = f.parent_local[i] < 0 ? -1 : f.parent_base + f.parent_local[i];
+  for (int s = 0; s < 4; ++s)
+    f.child[4 * x + s] = (f.has_next && f.rank[i] >= 0) ? f.next_base + 4 * f.rank[i] + s : -1;
+  f.is_leaf[x] = f.split[i] ? 0 : 1;
+  f.levelv[x] = f.level;
+  f.slotv[x] = f.slot[i];
+  f.pcountv[x] = f.pcount[i];
+  f.pstartv[x] = f.pstart[i] + f.pbase;
+  for (int j = 0; j < 8; ++j) f.cornersv[8 * x + j] = f.corners[8 * i + j];
+  f.sidev[x] = f.side[i];
+}
 
 }  // namespace
 
 void build_tree(TreeData& t, hipStream_t s, int* syncs) {
   const int n = t.npts;
-  // root box from min/max pooled over x and y (quadtree.cpp:80-105)
-  DBuf<double> mm;
+  DBuf<double> mm, mpart;
   mm.alloc(2);
-  k_minmax<<<1, kTB, 0, s>>>(t.px.p, t.py.p, n, mm.p);
+  mpart.alloc(2 * kMMBlocks);
+  k_minmax_part<<<kMMBlocks, kTB, 0, s>>>(t.px.p, t.py.p, n, mpart.p);
+  k_minmax_final<<<1, 64, 0, s>>>(mpart.p, kMMBlocks, mm.p);
   double hmm[2];
   mm.download(hmm, 2, s);
   KI_HIP(hipStreamSynchronize(s));
@@ -383,29 +475,50 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs) {
     r->corners.upload(h_c, 8, s);
     r->side.upload(&h_side, 1, s);
     r->perm.alloc(n);
+    r->pnode.alloc(n);
     r->nperm = n;
     k_iota<<<ceil_div(n, kTB), kTB, 0, s>>>(r->perm.p, n);
+    k_fill_i32<<<ceil_div(n, kTB), kTB, 0, s>>>(r->pnode.p, n, 0);
     lv.push_back(std::move(r));
   }
-  DBuf<int> nsplit;
-  nsplit.alloc(1);
-  DBuf<int> cnt;
-  DBuf<int64_t> cstart, cnt64;
+  DBuf<int> small, qcode, ccnt;
+  small.alloc(8);
+  DBuf<Q4> pre, btot, qtot;
+  qtot.alloc(1);
   for (int l = 0;; ++l) {
     LevelBuf& L = *lv[l];
     const int c = L.count;
+    const int np = L.nperm;
     L.split.alloc(c);
     L.st.alloc(c);
     L.rank.alloc(c);
-    L.child.alloc(4 * (size_t)c);
     const int* pst = l ? lv[l - 1]->st.p : nullptr;
     k_split_info<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.pcount.p, L.pstart.p, L.perm.p,
                                                   L.parent_local.p, pst, t.sd, t.leaf_dofs,
                                                   L.split.p, L.st.p);
-    nsplit.zero(s);
-    k_rank<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.split.p, L.st.p, L.path.p, L.rank.p, nsplit.p);
+    small.zero(s);
+    if (c <= kSortMax && l <= 20 && (int64_t)n < (1ll << 24)) {
+      int P = 1;
+      while (P < c) P <<= 1;
+      k_rank_sort<<<1, 1024, 0, s>>>(c, P, L.split.p, L.st.p, L.path.p, L.rank.p, small.p);
+    } else {
+      k_rank_pairs<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.split.p, L.st.p, L.path.p, L.rank.p,
+                                                    small.p);
+    }
+    // classify and scan the points of splitting nodes
+    const int nb = ceil_div(np, kScanTB);
+    qcode.alloc(np);
+    pre.alloc((size_t)np + 1);
+    btot.alloc(nb);
+    k_classify<<<ceil_div(np, kTB), kTB, 0, s>>>(np, L.perm.p, L.pnode.p, L.rank.p, L.corners.p,
+                                                 t.px.p, t.py.p, qcode.p);
+    k_q4_block<<<nb, kScanTB, 0, s>>>(np, qcode.p, pre.p, btot.p);
+    k_q4_top<<<1, 64, 0, s>>>(nb, btot.p, qtot.p);
+    k_q4_apply<<<ceil_div(np + 1, kTB), kTB, 0, s>>>(np, pre.p, btot.p, qtot.p);
     int S = 0;
-    nsplit.download(&S, 1, s);
+    Q4 tot;
+    small.download(&S, 1, s);
+    qtot.download(&tot, 1, s);
     KI_HIP(hipStreamSynchronize(s));
     ++*syncs;
     if (S == 0) break;
@@ -423,23 +536,15 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs) {
     k_children<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.rank.p, L.corners.p, L.side.p, L.path.p,
                                                 N.parent_local.p, N.slot.p, N.path.p,
                                                 N.corners.p, N.side.p);
-    cnt.alloc(N.count);
-    k_quad_count<<<c, kTB, 0, s>>>(L.rank.p, L.pcount.p, L.pstart.p, L.perm.p, L.corners.p,
-                                   t.px.p, t.py.p, cnt.p);
-    exclusive_scan_i32_to_i64(cnt.p, N.pstart.p, N.count, s);
-    KI_HIP(hipMemcpyAsync(N.pcount.p, cnt.p, sizeof(int) * N.count, hipMemcpyDeviceToDevice, s));
-    // every point of a splitting node goes to the next level
-    int64_t last_start = 0;
-    int last_cnt = 0;
-    KI_HIP(hipMemcpyAsync(&last_start, N.pstart.p + N.count - 1, sizeof(int64_t),
-                          hipMemcpyDeviceToHost, s));
-    KI_HIP(hipMemcpyAsync(&last_cnt, cnt.p + N.count - 1, sizeof(int), hipMemcpyDeviceToHost, s));
-    KI_HIP(hipStreamSynchronize(s));
-    ++*syncs;
-    N.nperm = (int)(last_start + last_cnt);
+    k_child_counts<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.rank.p, L.pstart.p, L.pcount.p, pre.p,
+                                                    N.pcount.p);
+    exclusive_scan_i32_to_i64(N.pcount.p, N.pstart.p, N.count, s);
+    N.nperm = tot.v[0] + tot.v[1] + tot.v[2] + tot.v[3];
     N.perm.alloc(std::max(N.nperm, 1));
-    k_partition<<<c, kTB, 0, s>>>(L.rank.p, L.pcount.p, L.pstart.p, L.perm.p, L.corners.p,
-                                  t.px.p, t.py.p, N.pstart.p, N.perm.p);
+    N.pnode.alloc(std::max(N.nperm, 1));
+    k_scatter<<<ceil_div(np, kTB), kTB, 0, s>>>(np, L.perm.p, L.pnode.p, qcode.p, L.rank.p,
+                                                L.pstart.p, pre.p, N.pstart.p, N.perm.p,
+                                                N.pnode.p);
     lv.push_back(std::move(nx));
   }
 
@@ -464,45 +569,37 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs) {
   int64_t pbase = 0;
   for (int l = 0; l < t.nlevels; ++l) {
     LevelBuf& L = *lv[l];
-    const int c = L.count;
-    const int64_t b = t.level_off[l];
-    const int g = ceil_div(c, kTB);
-    KI_HIP(hipMemcpyAsync(t.parent.p + b, L.parent_local.p, sizeof(int) * c,
-                          hipMemcpyDeviceToDevice, s));
-    if (l > 0) k_offset_i32<<<g, kTB, 0, s>>>(t.parent.p + b, c, (int)t.level_off[l - 1]);
-    if (l + 1 < t.nlevels) {
-      k_child_ids<<<g, kTB, 0, s>>>(c, L.rank.p, (int)t.level_off[l + 1], t.child.p + 4 * b);
-    } else {
-      KI_HIP(hipMemsetAsync(t.child.p + 4 * b, 0xff, sizeof(int) * 4 * c, s));
-    }
-    KI_HIP(hipMemcpyAsync(t.slot.p + b, L.slot.p, sizeof(int) * c, hipMemcpyDeviceToDevice, s));
-    KI_HIP(hipMemcpyAsync(t.corners.p + 8 * b, L.corners.p, sizeof(double) * 8 * c,
-                          hipMemcpyDeviceToDevice, s));
-    KI_HIP(hipMemcpyAsync(t.side.p + b, L.side.p, sizeof(double) * c, hipMemcpyDeviceToDevice, s));
-    KI_HIP(hipMemcpyAsync(t.pcount.p + b, L.pcount.p, sizeof(int) * c, hipMemcpyDeviceToDevice,
-                          s));
-    KI_HIP(hipMemcpyAsync(t.pstart.p + b, L.pstart.p, sizeof(int64_t) * c,
-                          hipMemcpyDeviceToDevice, s));
-    k_offset_i64<<<g, kTB, 0, s>>>(t.pstart.p + b, c, pbase);
+    FlatArgs f;
+    f.c = L.count;
+    f.base = (int)t.level_off[l];
+    f.parent_base = l ? (int)t.level_off[l - 1] : 0;
+    f.next_base = l + 1 < t.nlevels ? (int)t.level_off[l + 1] : 0;
+    f.level = l;
+    f.has_next = l + 1 < t.nlevels;
+    f.pbase = pbase;
+    f.parent_local = L.parent_local.p;
+    f.slot = L.slot.p;
+    f.split = L.split.p;
+    f.rank = L.rank.p;
+    f.pcount = L.pcount.p;
+    f.pstart = L.pstart.p;
+    f.corners = L.corners.p;
+    f.side = L.side.p;
+    f.parent = t.parent.p;
+    f.child = t.child.p;
+    f.is_leaf = t.is_leaf.p;
+    f.levelv = t.level.p;
+    f.slotv = t.slot.p;
+    f.pcountv = t.pcount.p;
+    f.pstartv = t.pstart.p;
+    f.cornersv = t.corners.p;
+    f.sidev = t.side.p;
+    k_flatten<<<ceil_div(L.count, kTB), kTB, 0, s>>>(f);
     if (L.nperm)
       KI_HIP(hipMemcpyAsync(t.perm_all.p + pbase, L.perm.p, sizeof(int) * L.nperm,
                             hipMemcpyDeviceToDevice, s));
     pbase += L.nperm;
-    // a node is a leaf iff it did not split (the last level never splits)
-    KI_HIP(hipMemcpyAsync(t.is_leaf.p + b, L.split.p ? L.split.p : L.pcount.p, sizeof(int) * c,
-                          hipMemcpyDeviceToDevice, s));
-    k_level_fill<<<g, kTB, 0, s>>>(t.level.p, (int)b, (int)(b + c), l);
   }
-  // is_leaf currently holds the split flag
-  {
-    std::vector<int> sp(nn);
-    t.is_leaf.download(sp.data(), nn, s);
-    KI_HIP(hipStreamSynchronize(s));
-    ++*syncs;
-    for (int i = 0; i < nn; ++i) sp[i] = sp[i] ? 0 : 1;
-    t.is_leaf.upload(sp, s);
-  }
-
   // ---- neighbor lists ----
   t.sl_nbr.alloc(kMaxNbrs * (size_t)nn);
   t.sl_cnt.alloc(nn);
