@@ -250,6 +250,8 @@ kigpu_status kigpu_tree_get_info(const kigpu_tree* tree, kigpu_tree_info* info) 
 kigpu_status kigpu_tree_export_nodes(const kigpu_tree* tree, int64_t* level_offsets, int* parent,
                                      int* children, int* is_leaf, double* corners, double* side) {
   KI_REQUIRE(tree, "null tree");
+  kigpu_status st = guard([&] { ensure_tree_mirrors(const_cast<kigpu_tree*>(tree)->t); });
+  if (st != KIGPU_OK) return st;
   const TreeData& t = tree->t;
   const size_t nn = t.nnodes;
   if (level_offsets) std::memcpy(level_offsets, t.level_off.data(), sizeof(int64_t) * (t.nlevels + 1));
@@ -263,6 +265,8 @@ kigpu_status kigpu_tree_export_nodes(const kigpu_tree* tree, int64_t* level_offs
 
 kigpu_status kigpu_tree_export_boxes(const kigpu_tree* tree, int64_t* offsets, int* dofs) {
   KI_REQUIRE(tree && offsets, "null argument");
+  kigpu_status st = guard([&] { ensure_tree_mirrors(const_cast<kigpu_tree*>(tree)->t); });
+  if (st != KIGPU_OK) return st;
   const TreeData& t = tree->t;
   int64_t o = 0;
   for (int x = 0; x < t.nnodes; ++x) {
@@ -281,6 +285,8 @@ kigpu_status kigpu_tree_export_boxes(const kigpu_tree* tree, int64_t* offsets, i
 
 kigpu_status kigpu_tree_export_neighbors(const kigpu_tree* tree, int64_t* offsets, int* nbrs) {
   KI_REQUIRE(tree && offsets, "null argument");
+  kigpu_status st = guard([&] { ensure_tree_mirrors(const_cast<kigpu_tree*>(tree)->t); });
+  if (st != KIGPU_OK) return st;
   const TreeData& t = tree->t;
   int64_t o = 0;
   for (int x = 0; x < t.nnodes; ++x) {
@@ -316,10 +322,6 @@ kigpu_status kigpu_factor(kigpu_ctx* ctx, kigpu_tree* tree, const kigpu_factor_o
              "solution dimension does not match the pde");
   KI_REQUIRE(opts->precision == KIGPU_FP64 || opts->precision == KIGPU_FP32_REFINE,
              "unknown precision");
-  if (opts->pde == KIGPU_LAPLACE_NEUMANN) {
-    set_error("the Neumann kernel is not supported yet");
-    return KIGPU_ERR_NOTSUPP;
-  }
   return guard([&] {
     kigpu_factor_opts o = *opts;
     if (o.min_dofs <= 0) o.min_dofs = 16;
@@ -348,6 +350,8 @@ kigpu_status kigpu_factor_export_lists(const kigpu_tree* tree, int64_t* skel_off
     set_error("tree is not factored");
     return KIGPU_ERR_STATE;
   }
+  kigpu_status st = guard([&] { ensure_factor_mirrors(const_cast<kigpu_tree*>(tree)->t); });
+  if (st != KIGPU_OK) return st;
   int64_t os = 0, orr = 0;
   for (int x = 0; x < t.nnodes; ++x) {
     skel_off[x] = os;
@@ -374,6 +378,8 @@ kigpu_status kigpu_factor_allskel(const kigpu_tree* tree, int* n, int* allskel) 
     return KIGPU_ERR_STATE;
   }
   *n = t.nroot;
+  kigpu_status st = guard([&] { ensure_factor_mirrors(const_cast<kigpu_tree*>(tree)->t); });
+  if (st != KIGPU_OK) return st;
   if (allskel && t.nroot) std::memcpy(allskel, &t.h_act[t.h_act_off[0]], sizeof(int) * t.nroot);
   return KIGPU_OK;
 }

@@ -1062,10 +1062,7 @@ void factor_tree(Context& c, TreeData& t, const kigpu_factor_opts& o) {
   t.rank.download(t.h_rank.data(), nn, c.stream);
   t.nact.download(t.h_nact.data(), nn, c.stream);
   t.act_off.download(t.h_act_off.data(), nn, c.stream);
-  t.h_sr.resize(t.act_pool_size);
-  t.h_act.resize(t.act_pool_size);
-  t.sr_pool.download(t.h_sr.data(), t.act_pool_size, c.stream);
-  t.act_pool.download(t.h_act.data(), t.act_pool_size, c.stream);
+  t.fmirrors_ready = false;
   KI_HIP(hipStreamSynchronize(c.stream));
   const double c_gen = o.pde == KIGPU_STOKES ? 20.0 : 11.0;
   for (auto& lw : t.work) {

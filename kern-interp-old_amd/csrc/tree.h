@@ -26,7 +26,9 @@ struct TreeData {
   DBuf<int64_t> cp_off;            // coarse leaves pushed into a node (CSR)
   DBuf<int> cp_nbr;
 
-  // host mirrors
+  // host mirrors (the export-only ones are filled on demand)
+  bool mirrors_ready = false, fmirrors_ready = false;
+  int64_t nperm_total = 0, ncp = 0;
   std::vector<int> h_parent, h_child, h_is_leaf, h_level, h_pcount, h_perm, h_sl_nbr, h_sl_cnt,
       h_cp_nbr;
   std::vector<int64_t> h_pstart, h_cp_off;
@@ -81,5 +83,7 @@ struct TreeData {
 
 // tree.hip
 void build_tree(TreeData& t, hipStream_t s, int* syncs);
+void ensure_tree_mirrors(TreeData& t);    // host copies used by the export calls
+void ensure_factor_mirrors(TreeData& t);
 
 }  // namespace kigpu

@@ -636,35 +636,60 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs) {
                                               t.corners.p, t.sl_nbr.p, t.sl_cnt.p, t.cp_off.p,
                                               cpc.p, t.cp_nbr.p, 1);
 
-  // ---- host mirrors ----
-  t.h_parent.resize(nn);
-  t.h_child.resize(4 * (size_t)nn);
+  // ---- host mirrors the factorization needs; the rest on export ----
+  t.nperm_total = ptot;
+  t.ncp = ncp;
+  t.mirrors_ready = false;
   t.h_is_leaf.resize(nn);
-  t.h_level.resize(nn);
   t.h_pcount.resize(nn);
-  t.h_pstart.resize(nn);
   t.h_corners.resize(8 * (size_t)nn);
   t.h_side.resize(nn);
-  t.h_perm.resize(ptot);
-  t.h_sl_nbr.resize(kMaxNbrs * (size_t)nn);
-  t.h_sl_cnt.resize(nn);
-  t.h_cp_off.resize(nn + 1);
-  t.h_cp_nbr.resize(ncp);
-  t.parent.download(t.h_parent.data(), nn, s);
-  t.child.download(t.h_child.data(), 4 * (size_t)nn, s);
   t.is_leaf.download(t.h_is_leaf.data(), nn, s);
-  t.level.download(t.h_level.data(), nn, s);
   t.pcount.download(t.h_pcount.data(), nn, s);
-  t.pstart.download(t.h_pstart.data(), nn, s);
   t.corners.download(t.h_corners.data(), 8 * (size_t)nn, s);
   t.side.download(t.h_side.data(), nn, s);
-  t.perm_all.download(t.h_perm.data(), ptot, s);
-  t.sl_nbr.download(t.h_sl_nbr.data(), kMaxNbrs * (size_t)nn, s);
-  t.sl_cnt.download(t.h_sl_cnt.data(), nn, s);
-  t.cp_off.download(t.h_cp_off.data(), nn + 1, s);
-  t.cp_nbr.download(t.h_cp_nbr.data(), ncp, s);
   KI_HIP(hipStreamSynchronize(s));
   ++*syncs;
+}
+
+void ensure_tree_mirrors(TreeData& t) {
+  if (t.mirrors_ready) return;
+  KI_HIP(hipDeviceSynchronize());
+  const size_t nn = t.nnodes;
+  t.h_parent.resize(nn);
+  t.h_child.resize(4 * nn);
+  t.h_level.resize(nn);
+  t.h_pstart.resize(nn);
+  t.h_perm.resize(t.nperm_total);
+  t.h_sl_nbr.resize(kMaxNbrs * nn);
+  t.h_sl_cnt.resize(nn);
+  t.h_cp_off.resize(nn + 1);
+  t.h_cp_nbr.resize(t.ncp);
+  auto get = [](void* dst, const void* src, size_t bytes) {
+    if (bytes) KI_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  };
+  get(t.h_parent.data(), t.parent.p, sizeof(int) * nn);
+  get(t.h_child.data(), t.child.p, sizeof(int) * 4 * nn);
+  get(t.h_level.data(), t.level.p, sizeof(int) * nn);
+  get(t.h_pstart.data(), t.pstart.p, sizeof(int64_t) * nn);
+  get(t.h_perm.data(), t.perm_all.p, sizeof(int) * t.nperm_total);
+  get(t.h_sl_nbr.data(), t.sl_nbr.p, sizeof(int) * kMaxNbrs * nn);
+  get(t.h_sl_cnt.data(), t.sl_cnt.p, sizeof(int) * nn);
+  get(t.h_cp_off.data(), t.cp_off.p, sizeof(int64_t) * (nn + 1));
+  get(t.h_cp_nbr.data(), t.cp_nbr.p, sizeof(int) * t.ncp);
+  t.mirrors_ready = true;
+}
+
+void ensure_factor_mirrors(TreeData& t) {
+  if (t.fmirrors_ready) return;
+  KI_HIP(hipDeviceSynchronize());
+  t.h_sr.resize(t.act_pool_size);
+  t.h_act.resize(t.act_pool_size);
+  KI_HIP(hipMemcpy(t.h_sr.data(), t.sr_pool.p, sizeof(int) * t.act_pool_size,
+                   hipMemcpyDeviceToHost));
+  KI_HIP(hipMemcpy(t.h_act.data(), t.act_pool.p, sizeof(int) * t.act_pool_size,
+                   hipMemcpyDeviceToHost));
+  t.fmirrors_ready = true;
 }
 
 }  // namespace kigpu
