@@ -310,8 +310,12 @@ kigpu_status kigpu_set_border(kigpu_tree* tree, const double* U, const double* P
     return KIGPU_OK;
   }
   KI_REQUIRE(U && Psi, "null border matrix");
-  set_error("bordered (multiply connected) systems are not supported yet");
-  return KIGPU_ERR_NOTSUPP;
+  const size_t len = (size_t)t.ndof * nborder;
+  t.h_U.assign(U, U + len);
+  t.h_Psi.assign(Psi, Psi + len);
+  t.nborder = nborder;
+  t.factored = false;
+  return KIGPU_OK;
 }
 
 kigpu_status kigpu_factor(kigpu_ctx* ctx, kigpu_tree* tree, const kigpu_factor_opts* opts) {
@@ -407,17 +411,19 @@ kigpu_status kigpu_solve(kigpu_ctx* ctx, kigpu_tree* tree, const double* b, int 
     set_error("solve before factor");
     return KIGPU_ERR_STATE;
   }
-  (void)alpha;
   return guard([&] {
     Context& c = ctx->c;
-    DBuf<double> db, dx;
+    DBuf<double> db, dx, da;
     db.alloc((size_t)nd * nrhs);
     dx.alloc((size_t)nd * nrhs);
+    const int h = tree->t.nborder;
+    if (h) da.alloc((size_t)h * nrhs);
     KI_HIP(hipMemcpy2DAsync(db.p, sizeof(double) * nd, b, sizeof(double) * ldb,
                             sizeof(double) * nd, nrhs, hipMemcpyHostToDevice, c.stream));
-    solve_tree(c, tree->t, db.p, nrhs, dx.p, nullptr, refine, info);
+    solve_tree(c, tree->t, db.p, nrhs, dx.p, h ? da.p : nullptr, refine, info);
     KI_HIP(hipMemcpy2DAsync(x, sizeof(double) * ldx, dx.p, sizeof(double) * nd,
                             sizeof(double) * nd, nrhs, hipMemcpyDeviceToHost, c.stream));
+    if (h && alpha) da.download(alpha, (size_t)h * nrhs, c.stream);
     KI_HIP(hipStreamSynchronize(c.stream));
   });
 }

@@ -265,11 +265,15 @@ __device__ bool block_gj_inverse(T* A, int n, int* piv, T* prow, T* fcol, int* s
     for (int j = tid; j < n; j += nt) prow[j] = j == c ? inv : A[c + (size_t)j * n] * inv;
     for (int i = tid; i < n; i += nt) fcol[i] = A[i + (size_t)c * n];
     __syncthreads();
-    for (int e = tid; e < n * n; e += nt) {
-      const int i = e % n, j = e / n;
-      if (i == c) A[e] = prow[j];
-      else if (j == c) A[e] = -fcol[i] * inv;
-      else A[e] -= fcol[i] * prow[j];
+    const int nw = nt >> 6;
+    for (int j = wv; j < n; j += nw) {
+      const T pj = prow[j];
+      T* col = A + (size_t)j * n;
+      for (int i = lane; i < n; i += 64) {
+        if (i == c) col[i] = pj;
+        else if (j == c) col[i] = -fcol[i] * inv;
+        else col[i] -= fcol[i] * pj;
+      }
     }
     __syncthreads();
   }
@@ -284,6 +288,20 @@ __device__ bool block_gj_inverse(T* A, int n, int* piv, T* prow, T* fcol, int* s
     __syncthreads();
   }
   return ok;
+}
+
+}  // namespace kigpu
+
+namespace kigpu {
+
+// f(i, j) over a rows x cols index space: wavefront w takes columns
+// w, w + NW, ...; its lanes take rows.  Avoids integer division by runtime
+// sizes in the hot loops.
+template <int NW, typename F>
+__device__ __forceinline__ void for2d(int rows, int cols, F&& f) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int j = wv; j < cols; j += NW)
+    for (int i = lane; i < rows; i += 64) f(i, j);
 }
 
 }  // namespace kigpu

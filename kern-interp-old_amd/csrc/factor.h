@@ -41,6 +41,7 @@ struct Context {
   std::vector<hipEvent_t> evpool;  // per-level phase markers of a factorization
   kigpu_timings tim{};
   DBuf<double> ring;        // proxy unit vectors cos | sin (host libm)
+  DBuf<double> rsum;        // residual partial sums
   size_t lds_max = 65536;
   void ensure_events(size_t n) {
     while (evpool.size() < n) {
@@ -55,6 +56,11 @@ struct Context {
 };
 
 void factor_tree(Context& c, TreeData& t, const kigpu_factor_opts& o);
+// bordered system (border.hip)
+template <typename T>
+void border_factor(Context& c, TreeData& t);
+template <typename T>
+void border_step(Context& c, TreeData& t, T* x, int ldx, int nrhs, const double* g);
 void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_x,
                 double* d_alpha, const kigpu_refine_opts* ro, kigpu_solve_info* info);
 void kernel_apply(Context& c, const TreeData& t, int pde, const double* d_mu, int nrhs,

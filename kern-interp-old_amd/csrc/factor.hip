@@ -323,17 +323,15 @@ __global__ __launch_bounds__(kIdTB) void k_id(const IdWork* work, GeomView g, in
   // ---- ID matrix [K(rows, B); K(B, rows)^T; proxy] (kernel.cpp:153-235)
   const double* ring_c = ring;
   const double* ring_s = ring + kProxyPoints;
-  const int mn = m * n;
-  for (int e = tid; e < mn; e += kIdTB) {
-    const int j = e / m, i = e - j * m;
+  for2d<kIdNW>(m, n, [&](int i, int j) {
     const int bd = s_act[j];
     const double* bg = cgeo + kGeo * j;
     double v;
     if (i < q) v = staged_entry(pde, s_rows[i], rgeo + kGeo * i, bd, bg);
     else if (i < 2 * q) v = staged_entry(pde, bd, bg, s_rows[i - q], rgeo + kGeo * (i - q));
     else v = staged_proxy_entry(pde, i - 2 * q, bd, bg, w.cx, w.cy, w.rad, w.pw, ring_c, ring_s);
-    A[e] = (T)v;
-  }
+    A[i + (size_t)j * lda] = (T)v;
+  });
   __syncthreads();
 
   // ---- column norms (dgeqp3: DNRM2 of each column), FP64
@@ -561,10 +559,9 @@ __global__ __launch_bounds__(kDecTB) void k_decouple(int a, GeomView g, int pde,
       const T* acc = reinterpret_cast<const T*>(acc_ptr_in[y]);
       if (acc) {
         int ld = acc_ld_in[y];
-        for (int e = tid; e < sz * sz; e += kDecTB) {
-          int i = e % sz, j = e / sz;
+        for2d<kDecNW>(sz, sz, [&](int i, int j) {
           Uact[(o + i) + (size_t)(o + j) * n] = acc[i + (size_t)j * ld];
-        }
+        });
       }
       o += sz;
     }
@@ -599,12 +596,11 @@ __global__ __launch_bounds__(kDecTB) void k_decouple(int a, GeomView g, int pde,
   for (int e = tid; e < k * r; e += kDecTB) Ts[e] = slab[e];  // T from the ID
   __syncthreads();
   // K_BN = K(BN, BN) - update, permuted to [s r] (skel_factorization.cpp:66-78)
-  for (int e = tid; e < n * n; e += kDecTB) {
-    int i = e % n, j = e / n;
+  for2d<kDecNW>(n, n, [&](int i, int j) {
     double v = kernel_entry(g, pde, sr[i], sr[j]);
     if (!leaf) v -= (double)Uact[pos[i] + (size_t)pos[j] * n];
-    Kp[e] = (T)v;
-  }
+    Kp[i + (size_t)j * n] = (T)v;
+  });
   __syncthreads();
   T* Lm = slab + (size_t)k * r;             // k x r
   T* Um = slab + 2 * (size_t)k * r;         // r x k
@@ -612,27 +608,24 @@ __global__ __launch_bounds__(kDecTB) void k_decouple(int a, GeomView g, int pde,
   T* Acc = Xinv + (size_t)r * r;            // k x k
   auto KP = [&](int i, int j) -> T { return Kp[i + (size_t)j * n]; };
   // K_r_sn = K_rs - T^T K_ss (r x k); K_sn_r = K_sr - K_ss T (k x r)
-  for (int e = tid; e < r * k; e += kDecTB) {
-    int i = e % r, j = e / r;
+  for2d<kDecNW>(r, k, [&](int i, int j) {
     T s = KP(k + i, j);
     for (int t = 0; t < k; ++t) s -= Ts[t + (size_t)i * k] * KP(t, j);
     Krsn[i + (size_t)j * r] = s;
-  }
-  for (int e = tid; e < k * r; e += kDecTB) {
-    int i = e % k, j = e / k;
+  });
+  for2d<kDecNW>(k, r, [&](int i, int j) {
     T s = KP(i, k + j);
     for (int t = 0; t < k; ++t) s -= KP(i, t) * Ts[t + (size_t)j * k];
     Ksnr[i + (size_t)j * k] = s;
-  }
+  });
   __syncthreads();
   // X_rr = K_rr - T^T K_sr - K_r_sn T (r x r)
-  for (int e = tid; e < r * r; e += kDecTB) {
-    int i = e % r, j = e / r;
+  for2d<kDecNW>(r, r, [&](int i, int j) {
     T s = KP(k + i, k + j);
     for (int t = 0; t < k; ++t) s -= Ts[t + (size_t)i * k] * KP(t, k + j);
     for (int t = 0; t < k; ++t) s -= Krsn[i + (size_t)t * r] * Ts[t + (size_t)j * k];
     X[i + (size_t)j * r] = s;
-  }
+  });
   __syncthreads();
   // X_rr^{-1} in place (Gauss-Jordan with dgetrf's pivot order)
   T* Z = X;
@@ -640,30 +633,27 @@ __global__ __launch_bounds__(kDecTB) void k_decouple(int a, GeomView g, int pde,
   if (!block_gj_inverse(X, r, piv, Kp, Kp + n, &s_p) && tid == 0) flag(status, 4);
   __syncthreads();
   // L = K_sn_r X_rr^{-1} (k x r), U = X_rr^{-1} K_r_sn (r x k)
-  for (int e = tid; e < k * r; e += kDecTB) {
-    int i = e % k, j = e / k;
+  for2d<kDecNW>(k, r, [&](int i, int j) {
     T s = T(0);
     for (int t = 0; t < r; ++t) s += Ksnr[i + (size_t)t * k] * Z[t + (size_t)j * r];
-    Ls[e] = s;
-    Lm[e] = s;
-  }
-  for (int e = tid; e < r * k; e += kDecTB) {
-    int i = e % r, j = e / r;
+    Ls[i + (size_t)j * k] = s;
+    Lm[i + (size_t)j * k] = s;
+  });
+  for2d<kDecNW>(r, k, [&](int i, int j) {
     T s = T(0);
     for (int t = 0; t < r; ++t) s += Z[i + (size_t)t * r] * Krsn[t + (size_t)j * r];
-    Um[e] = s;
-  }
+    Um[i + (size_t)j * r] = s;
+  });
   for (int e = tid; e < r * r; e += kDecTB) Xinv[e] = Z[e];
   for (int c = tid; c < r; c += kDecTB) piv_pool[aoff + c] = piv[c];
   __syncthreads();
   // schur_update = L K_r_sn (k x k), plus the descendants' update on the skeleton
-  for (int e = tid; e < k * k; e += kDecTB) {
-    int i = e % k, j = e / k;
+  for2d<kDecNW>(k, k, [&](int i, int j) {
     T s = T(0);
     for (int t = 0; t < r; ++t) s += Ls[i + (size_t)t * k] * Krsn[t + (size_t)j * r];
     if (!leaf) s += Uact[pos[i] + (size_t)pos[j] * n];
-    Acc[e] = s;
-  }
+    Acc[i + (size_t)j * k] = s;
+  });
   if (tid == 0) {
     osize[x] = k;
     acc_ptr[x] = reinterpret_cast<uint64_t>(Acc);
@@ -679,15 +669,12 @@ __global__ __launch_bounds__(1024) void k_root(int x, GeomView g, int pde, const
                                                const int* osize, const uint64_t* acc_ptr,
                                                const int* acc_ld, const int* dofs, int n,
                                                double* Aws, double* Ainv, int* piv_out,
-                                               int* status) {
+                                               double* Amat, int* status) {
   extern __shared__ __align__(16) unsigned char smem[];
   double* A = LDS ? reinterpret_cast<double*>(smem) : Aws;
   __shared__ int piv[1024];
   const int tid = threadIdx.x, nt = blockDim.x;
-  for (int e = tid; e < n * n; e += nt) {
-    int i = e % n, j = e / n;
-    A[e] = kernel_entry(g, pde, dofs[i], dofs[j]);
-  }
+  for2d<16>(n, n, [&](int i, int j) { A[i + (size_t)j * n] = kernel_entry(g, pde, dofs[i], dofs[j]); });
   __syncthreads();
   int o = 0;
   for (int s = 0; s < 4; ++s) {
@@ -697,14 +684,14 @@ __global__ __launch_bounds__(1024) void k_root(int x, GeomView g, int pde, const
     const T* acc = reinterpret_cast<const T*>(acc_ptr[y]);
     if (acc) {
       int ld = acc_ld[y];
-      for (int e = tid; e < sz * sz; e += nt) {
-        int i = e % sz, j = e / sz;
+      for2d<16>(sz, sz, [&](int i, int j) {
         A[(o + i) + (size_t)(o + j) * n] -= (double)acc[i + (size_t)j * ld];
-      }
+      });
     }
     o += sz;
   }
   __syncthreads();
+  for (int e = tid; e < n * n; e += nt) Amat[e] = A[e];
   // allskel_mat^{-1} in place (Gauss-Jordan, dgetrf's pivot order), then out
   __shared__ int s_p;
   __shared__ double prow[1024], fcol[1024];
@@ -981,18 +968,19 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     KI_HIP(hipMemcpyAsync(t.root_dofs.p, t.act_pool.p + r_off, sizeof(int) * n0,
                           hipMemcpyDeviceToDevice, s));
     t.root_lu.alloc((size_t)n0 * n0);   // allskel_mat^{-1}
+    t.root_mat.alloc((size_t)n0 * n0);  // allskel_mat
     t.root_piv.alloc(n0);
     const size_t lbytes = sizeof(double) * (size_t)n0 * n0;
     if (lbytes + 24576 <= lds_cap) {  // static LDS of k_root: pivots + 2 scratch rows
       allow_lds(k_root<T, true>, lbytes);
       k_root<T, true><<<1, 1024, lbytes, s>>>(root, g, pde, t.child.p, t.osize.p, t.acc_ptr.p,
                                               t.acc_ld.p, t.root_dofs.p, n0, nullptr, t.root_lu.p,
-                                              t.root_piv.p, t.status.p);
+                                              t.root_piv.p, t.root_mat.p, t.status.p);
     } else {
       t.ws64.alloc((size_t)n0 * n0);
       k_root<T, false><<<1, 1024, 0, s>>>(root, g, pde, t.child.p, t.osize.p, t.acc_ptr.p,
                                           t.acc_ld.p, t.root_dofs.p, n0, t.ws64.p, t.root_lu.p,
-                                          t.root_piv.p, t.status.p);
+                                          t.root_piv.p, t.root_mat.p, t.status.p);
     }
     KI_HIP(hipGetLastError());
   }
@@ -1050,6 +1038,10 @@ void factor_tree(Context& c, TreeData& t, const kigpu_factor_opts& o) {
   c.tim.id_kernel_s = c.tim.id_flops = c.tim.dec_kernel_s = c.tim.dec_flops = 0;
   if (o.precision == KIGPU_FP32_REFINE) factor_levels<float>(c, t, o);
   else factor_levels<double>(c, t, o);
+  if (t.nborder > 0) {
+    if (o.precision == KIGPU_FP32_REFINE) border_factor<float>(c, t);
+    else border_factor<double>(c, t);
+  }
   int st = 0;
   t.status.download(&st, 1, c.stream);
   KI_HIP(hipStreamSynchronize(c.stream));

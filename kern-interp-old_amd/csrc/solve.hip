@@ -23,10 +23,14 @@ constexpr int kTB = 256;
 constexpr int kNW = kTB / kWave;
 constexpr int kRC = 16;  // right-hand sides per block
 
+// mode 0: x_r -= T^T x_s; x_s -= L x_r        (sweeps of U in skeletonize, :169-185)
+// mode 1: the same, then x_r <- X_rr^{-1} x_r   (solve's upward pass with D^{-1})
+// mode 2: x_r -= T^T x_s; x_s -= U^T x_r       (sweeps of Psi^T, :187-204)
 template <typename T>
 __global__ __launch_bounds__(kTB) void k_up(int a, const int* rank, const int* nact,
                                             const int64_t* act_off, const int* sr_pool,
-                                            const uint64_t* fac_ptr, T* x, int ldx, int nrhs) {
+                                            const uint64_t* fac_ptr, T* x, int ldx, int nrhs,
+                                            int mode) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int node = a + blockIdx.x;
   const int k = rank[node];
@@ -53,18 +57,34 @@ __global__ __launch_bounds__(kTB) void k_up(int a, const int* rank, const int* n
     v[k + i + (size_t)c * n] = s;
   }
   __syncthreads();
-  for (int e = tid; e < k * nc; e += kTB) {  // x_s -= L x_r
-    int t = e % k, c = e / k;
-    const T* vs = v + (size_t)c * n;
-    T s = vs[t];
-    for (int i = 0; i < r; ++i) s -= Lm[t + (size_t)i * k] * vs[k + i];
-    v[t + (size_t)c * n] = s;
+  if (mode == 2) {
+    const T* Um = Tm + 2 * (size_t)k * r;  // r x k
+    for (int e = tid; e < k * nc; e += kTB) {  // x_s -= U^T x_r
+      int t = e % k, c = e / k;
+      const T* vs = v + (size_t)c * n;
+      T s = vs[t];
+      for (int i = 0; i < r; ++i) s -= Um[i + (size_t)t * r] * vs[k + i];
+      v[t + (size_t)c * n] = s;
+    }
+  } else {
+    for (int e = tid; e < k * nc; e += kTB) {  // x_s -= L x_r
+      int t = e % k, c = e / k;
+      const T* vs = v + (size_t)c * n;
+      T s = vs[t];
+      for (int i = 0; i < r; ++i) s -= Lm[t + (size_t)i * k] * vs[k + i];
+      v[t + (size_t)c * n] = s;
+    }
   }
-  for (int e = tid; e < r * nc; e += kTB) {  // x_r <- X_rr^{-1} x_r
+  for (int e = tid; e < r * nc; e += kTB) {  // x_r <- X_rr^{-1} x_r (mode 1)
     int i = e % r, c = e / r;
     const T* vs = v + (size_t)c * n + k;
-    T s = T(0);
-    for (int j = 0; j < r; ++j) s += Xinv[i + (size_t)j * r] * vs[j];
+    T s;
+    if (mode == 1) {
+      s = T(0);
+      for (int j = 0; j < r; ++j) s += Xinv[i + (size_t)j * r] * vs[j];
+    } else {
+      s = vs[i];
+    }
     tmp[e] = s;
   }
   __syncthreads();
@@ -279,8 +299,11 @@ __global__ void k_axpy_f32(double* x, const float* d, size_t n) {
   if (i < n) x[i] += (double)d[i];
 }
 
+}  // namespace
+
+// upward pass over all levels, finest first (see k_up for the modes)
 template <typename T>
-void sweeps(Context& c, TreeData& t, T* x, int ldx, int nrhs) {
+void up_sweeps(Context& c, TreeData& t, T* x, int ldx, int nrhs, int mode) {
   hipStream_t s = c.stream;
   const int ncb = ceil_div(nrhs, kRC);
   const int rcol = std::min(nrhs, kRC);
@@ -290,11 +313,18 @@ void sweeps(Context& c, TreeData& t, T* x, int ldx, int nrhs) {
     size_t lds = 2 * sizeof(T) * (size_t)w.nmax * rcol;
     allow_lds(k_up<T>, lds);
     k_up<T><<<dim3(w.b - w.a, ncb), kTB, lds, s>>>(w.a, t.rank.p, t.nact.p, t.act_off.p,
-                                                   t.sr_pool.p, t.fac_ptr.p, x, ldx, nrhs);
+                                                   t.sr_pool.p, t.fac_ptr.p, x, ldx, nrhs, mode);
   }
-  if (t.nroot > 0)
-    k_root_apply<T><<<nrhs, kTB, sizeof(double) * t.nroot, s>>>(t.root_dofs.p, t.nroot,
-                                                                t.root_lu.p, x, ldx);
+}
+template void up_sweeps<double>(Context&, TreeData&, double*, int, int, int);
+template void up_sweeps<float>(Context&, TreeData&, float*, int, int, int);
+
+// downward pass, coarsest first
+template <typename T>
+void down_sweeps(Context& c, TreeData& t, T* x, int ldx, int nrhs) {
+  hipStream_t s = c.stream;
+  const int ncb = ceil_div(nrhs, kRC);
+  const int rcol = std::min(nrhs, kRC);
   for (int l = 1; l < t.nlevels; ++l) {
     LevelWork& w = *t.work[l];
     if (w.nmax == 0) continue;
@@ -304,6 +334,47 @@ void sweeps(Context& c, TreeData& t, T* x, int ldx, int nrhs) {
                                                      t.sr_pool.p, t.fac_ptr.p, x, ldx, nrhs);
   }
   KI_HIP(hipGetLastError());
+}
+
+namespace {
+
+// y(:, c) += U alpha(:, c)  (U: nd x h, FP64)
+__global__ void k_add_ualpha(int nd, int h, const double* U, const double* alpha, double* y) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+  if (d >= nd) return;
+  double s = 0;
+  for (int j = 0; j < h; ++j) s += U[d + (size_t)j * nd] * alpha[j + (size_t)c * h];
+  y[d + (size_t)c * nd] += s;
+}
+
+// rb(i, c) = alpha(i, c) - sum_d Psi^T(d, i) x(d, c)
+__global__ __launch_bounds__(kTB) void k_border_resid(int nd, int h, const double* PsiT,
+                                                      const double* x, const double* alpha,
+                                                      double* rb) {
+  __shared__ double red[kNW];
+  const int i = blockIdx.x, c = blockIdx.y;
+  double s = 0;
+  for (int d = threadIdx.x; d < nd; d += kTB) s += PsiT[d + (size_t)i * nd] * x[d + (size_t)c * nd];
+  s = block_sum<kNW>(s, red);
+  if (threadIdx.x == 0) rb[i + (size_t)c * h] = alpha[i + (size_t)c * h] - s;
+}
+
+__global__ void k_axpy_f64(double* x, const double* d, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += d[i];
+}
+
+template <typename T>
+void sweeps(Context& c, TreeData& t, T* x, int ldx, int nrhs, const double* g = nullptr) {
+  hipStream_t s = c.stream;
+  up_sweeps<T>(c, t, x, ldx, nrhs, 1);
+  if (t.nborder > 0) {
+    border_step<T>(c, t, x, ldx, nrhs, g);
+  } else if (t.nroot > 0) {
+    k_root_apply<T><<<nrhs, kTB, sizeof(double) * t.nroot, s>>>(t.root_dofs.p, t.nroot,
+                                                                t.root_lu.p, x, ldx);
+  }
+  down_sweeps<T>(c, t, x, ldx, nrhs);
 }
 
 }  // namespace
@@ -374,89 +445,117 @@ void forward_apply(Context& c, const TreeData& t, int pde, const double* d_tgt, 
   KI_HIP(hipStreamSynchronize(s));
 }
 
-static void residual_norms(Context& c, const double* f, const double* y, double* r, int n,
-                           int nrhs, double* rel) {
+// per-column sums of r^2 and f^2 with r = f - y written out
+static void residual_sums(Context& c, const double* f, const double* y, double* r, int n, int nrhs,
+                          std::vector<double>* rr, std::vector<double>* ff) {
   hipStream_t s = c.stream;
   const int nb = 64;
-  DBuf<double> sums;
-  sums.alloc(2 * (size_t)nb * nrhs);
-  k_residual<<<dim3(nb, nrhs), kTB, 0, s>>>(f, y, r, n, nrhs, sums.p);
+  c.rsum.alloc(2 * (size_t)nb * nrhs);
+  k_residual<<<dim3(nb, nrhs), kTB, 0, s>>>(f, y, r, n, nrhs, c.rsum.p);
   std::vector<double> h(2 * (size_t)nb * nrhs);
-  sums.download(h.data(), h.size(), s);
+  c.rsum.download(h.data(), h.size(), s);
   KI_HIP(hipStreamSynchronize(s));
-  double worst = 0;
-  for (int col = 0; col < nrhs; ++col) {
-    double a = 0, b = 0;
+  rr->assign(nrhs, 0);
+  ff->assign(nrhs, 0);
+  for (int col = 0; col < nrhs; ++col)
     for (int i = 0; i < nb; ++i) {
-      a += h[2 * ((size_t)col * nb + i)];
-      b += h[2 * ((size_t)col * nb + i) + 1];
+      (*rr)[col] += h[2 * ((size_t)col * nb + i)];
+      (*ff)[col] += h[2 * ((size_t)col * nb + i) + 1];
     }
-    worst = std::max(worst, b > 0 ? std::sqrt(a / b) : std::sqrt(a));
-  }
-  *rel = worst;
 }
 
 void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_x, double* d_alpha,
                 const kigpu_refine_opts* ro, kigpu_solve_info* info) {
-  (void)d_alpha;
   hipStream_t s = c.stream;
   const size_t len = (size_t)t.ndof * nrhs;
+  const int h = t.nborder;
+  const size_t alen = (size_t)h * nrhs;
   if (info) {
     info->iterations = 0;
     info->rel_residual = -1;
   }
-  hipEventRecord(c.ev[6], s);
+  KI_HIP(hipEventRecord(c.ev[6], s));
   if (t.opts.precision == KIGPU_FP64) {
     if (d_x != d_b)
       KI_HIP(hipMemcpyAsync(d_x, d_b, sizeof(double) * len, hipMemcpyDeviceToDevice, s));
     sweeps<double>(c, t, d_x, t.ndof, nrhs);
-    hipEventRecord(c.ev[7], s);
+    if (h && d_alpha)
+      KI_HIP(hipMemcpyAsync(d_alpha, t.alpha_dev.p, sizeof(double) * alen,
+                            hipMemcpyDeviceToDevice, s));
+    KI_HIP(hipEventRecord(c.ev[7], s));
     KI_HIP(hipStreamSynchronize(s));
     float ms = 0;
-    hipEventElapsedTime(&ms, c.ev[6], c.ev[7]);
+    KI_HIP(hipEventElapsedTime(&ms, c.ev[6], c.ev[7]));
     c.tim.t_solve = ms * 1e-3;
     c.tim.t_refine_apply = 0;
     return;
   }
-  // FP32 factor: solve in FP32, then FP64 refinement mu += F^{-1}(f - K mu)
+  // FP32 factor: solve in FP32, then FP64 refinement
+  //   r = [f - K mu - U alpha; alpha - Psi mu],  [mu; alpha] += F^{-1} r
   t.x32.alloc(len);
   const int g = ceil_div((long)len, 256);
   k_convert<double, float><<<g, 256, 0, s>>>(d_b, t.x32.p, len);
   sweeps<float>(c, t, t.x32.p, t.ndof, nrhs);
   k_convert<float, double><<<g, 256, 0, s>>>(t.x32.p, d_x, len);
-  hipEventRecord(c.ev[7], s);
+  if (h) {
+    t.alpha64.alloc(alen);
+    KI_HIP(hipMemcpyAsync(t.alpha64.p, t.alpha_dev.p, sizeof(double) * alen,
+                          hipMemcpyDeviceToDevice, s));
+  }
+  KI_HIP(hipEventRecord(c.ev[7], s));
   KI_HIP(hipStreamSynchronize(s));
   float ms = 0;
-  hipEventElapsedTime(&ms, c.ev[6], c.ev[7]);
+  KI_HIP(hipEventElapsedTime(&ms, c.ev[6], c.ev[7]));
   c.tim.t_solve = ms * 1e-3;
   c.tim.t_refine_apply = 0;
   const int iters = ro ? ro->max_iters : 0;
-  if (iters <= 0) return;
-  t.r64.alloc(len);
-  t.f64.alloc(len);
-  double rel = 0;
-  for (int it = 0; it <= iters; ++it) {
-    hipEventRecord(c.ev[8], s);
-    kernel_apply(c, t, t.opts.pde, d_x, nrhs, t.f64.p);
-    residual_norms(c, d_b, t.f64.p, t.r64.p, t.ndof, nrhs, &rel);
-    hipEventRecord(c.ev[9], s);
-    KI_HIP(hipStreamSynchronize(s));
-    hipEventElapsedTime(&ms, c.ev[8], c.ev[9]);
-    c.tim.t_refine_apply += ms * 1e-3;
-    if (info) {
-      info->rel_residual = rel;
-      info->iterations = it;
+  if (iters > 0) {
+    t.r64.alloc(len);
+    t.f64.alloc(len);
+    if (h) t.rb64.alloc(alen);
+    std::vector<double> rr, ff, hb(alen);
+    for (int it = 0; it <= iters; ++it) {
+      KI_HIP(hipEventRecord(c.ev[8], s));
+      kernel_apply(c, t, t.opts.pde, d_x, nrhs, t.f64.p);
+      if (h) {
+        k_add_ualpha<<<dim3(ceil_div(t.ndof, 256), nrhs), 256, 0, s>>>(t.ndof, h, t.bU64in.p,
+                                                                      t.alpha64.p, t.f64.p);
+        k_border_resid<<<dim3(h, nrhs), kTB, 0, s>>>(t.ndof, h, t.bP64in.p, d_x, t.alpha64.p,
+                                                     t.rb64.p);
+      }
+      residual_sums(c, d_b, t.f64.p, t.r64.p, t.ndof, nrhs, &rr, &ff);
+      if (h) {
+        t.rb64.download(hb.data(), alen, s);
+        KI_HIP(hipStreamSynchronize(s));
+        for (int col = 0; col < nrhs; ++col)
+          for (int i = 0; i < h; ++i) rr[col] += hb[i + (size_t)col * h] * hb[i + (size_t)col * h];
+      }
+      double rel = 0;
+      for (int col = 0; col < nrhs; ++col)
+        rel = std::max(rel, ff[col] > 0 ? std::sqrt(rr[col] / ff[col]) : std::sqrt(rr[col]));
+      KI_HIP(hipEventRecord(c.ev[9], s));
+      KI_HIP(hipStreamSynchronize(s));
+      KI_HIP(hipEventElapsedTime(&ms, c.ev[8], c.ev[9]));
+      c.tim.t_refine_apply += ms * 1e-3;
+      if (info) {
+        info->rel_residual = rel;
+        info->iterations = it;
+      }
+      if (rel <= ro->rel_residual || it == iters) break;
+      KI_HIP(hipEventRecord(c.ev[8], s));
+      k_convert<double, float><<<g, 256, 0, s>>>(t.r64.p, t.x32.p, len);
+      sweeps<float>(c, t, t.x32.p, t.ndof, nrhs, h ? t.rb64.p : nullptr);
+      k_axpy_f32<<<g, 256, 0, s>>>(d_x, t.x32.p, len);
+      if (h) k_axpy_f64<<<ceil_div((long)alen, 256), 256, 0, s>>>(t.alpha64.p, t.alpha_dev.p, alen);
+      KI_HIP(hipEventRecord(c.ev[9], s));
+      KI_HIP(hipStreamSynchronize(s));
+      KI_HIP(hipEventElapsedTime(&ms, c.ev[8], c.ev[9]));
+      c.tim.t_solve += ms * 1e-3;
     }
-    if (rel <= ro->rel_residual || it == iters) break;
-    hipEventRecord(c.ev[8], s);
-    k_convert<double, float><<<g, 256, 0, s>>>(t.r64.p, t.x32.p, len);
-    sweeps<float>(c, t, t.x32.p, t.ndof, nrhs);
-    k_axpy_f32<<<g, 256, 0, s>>>(d_x, t.x32.p, len);
-    hipEventRecord(c.ev[9], s);
-    KI_HIP(hipStreamSynchronize(s));
-    hipEventElapsedTime(&ms, c.ev[8], c.ev[9]);
-    c.tim.t_solve += ms * 1e-3;
   }
+  if (h && d_alpha)
+    KI_HIP(hipMemcpyAsync(d_alpha, t.alpha64.p, sizeof(double) * alen, hipMemcpyDeviceToDevice, s));
+  KI_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace kigpu

@@ -71,12 +71,19 @@ struct TreeData {
   // per-level compressed-node lists for the solve sweeps (device), by level
   std::vector<std::unique_ptr<LevelWork>> work;
 
-  // border (holes)
-  int nborder = 0;
+  // border (holes): U (ndof x h), Psi (h x ndof) as given; their swept forms
+  int nborder = 0, nS = 0;
   std::vector<double> h_U, h_Psi;
+  DBuf<double> bU64in, bP64in;         // U and Psi^T in FP64 (residuals)
+  DBuf<double> bU64, bD64, bP64;       // swept U, D^{-1} C, swept Psi^T (FP64 factor)
+  DBuf<float> bU32, bD32, bP32;        // the same for the FP32 factor
+  DBuf<int> is_skel;                   // 1 on the root skeleton DOFs
+  DBuf<double> root_mat;               // allskel_mat before inversion
+  DBuf<double> Sinv;                   // inverse of the bordered matrix S
+  DBuf<double> bgram, bM;              // solve scratch
 
   // solve scratch
-  DBuf<double> x64, r64, f64;
+  DBuf<double> x64, r64, f64, rb64, alpha64;
   DBuf<float> x32;
   DBuf<double> alpha_dev;
 };

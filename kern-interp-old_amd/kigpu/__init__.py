@@ -215,6 +215,20 @@ class Tree:
     def export_neighbors(self):
         return self._csr(self.lib.kigpu_tree_export_neighbors, "kigpu_tree_export_neighbors")
 
+    def set_border(self, U, Psi):
+        """QuadTree::U / ::Psi for holes: U (ndof x h), Psi (h x ndof)."""
+        U = np.asfortranarray(U, dtype=np.float64)
+        Psi = np.asfortranarray(Psi, dtype=np.float64)
+        h = U.shape[1] if U.size else 0
+        self.nborder = h
+        if h == 0:
+            _check(self.lib.kigpu_set_border(self.h, None, None, 0), "kigpu_set_border")
+            return
+        assert U.shape == (self.ndof, h) and Psi.shape == (h, self.ndof)
+        self._U, self._Psi = U, Psi
+        _check(self.lib.kigpu_set_border(self.h, U.ctypes.data_as(C.c_void_p),
+                                         Psi.ctypes.data_as(C.c_void_p), h), "kigpu_set_border")
+
     def factor(self, pde=LAPLACE, id_tol=1e-6, precision=FP64, min_dofs=16):
         o = FactorOpts(pde, id_tol, precision, min_dofs)
         _check(self.lib.kigpu_factor(self.ctx.h, self.h, C.byref(o)), "kigpu_factor")
@@ -253,13 +267,20 @@ class Tree:
         nrhs = B.shape[1]
         bflat = np.ascontiguousarray(B.T).reshape(-1)   # column-major
         x = np.zeros_like(bflat)
+        h = getattr(self, "nborder", 0)
+        alpha = np.zeros(max(h * nrhs, 1))
         ro = RefineOpts(refine_iters, rel_residual)
         info = SolveInfo()
         _check(self.lib.kigpu_solve(self.ctx.h, self.h, bflat, self.ndof, nrhs, x, self.ndof,
-                                    None, C.byref(ro), C.byref(info)), "kigpu_solve")
+                                    alpha.ctypes.data_as(C.c_void_p) if h else None,
+                                    C.byref(ro), C.byref(info)), "kigpu_solve")
         X = x.reshape(nrhs, self.ndof).T
         self.last_info = {"iterations": info.iterations, "rel_residual": info.rel_residual}
-        return X[:, 0].copy() if one else np.array(X)
+        mu = X[:, 0].copy() if one else np.array(X)
+        if h:
+            A = alpha[: h * nrhs].reshape(nrhs, h).T
+            self.last_alpha = A[:, 0].copy() if one else np.array(A)
+        return mu
 
     def apply(self, mu, pde=None):
         pde = self.pde if pde is None else pde
