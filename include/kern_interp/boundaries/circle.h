@@ -1,0 +1,22 @@
+// Drop-in for kern_interp/boundaries/circle.h: the reference's circle test geometry.
+#ifndef KERN_INTERP_BOUNDARIES_CIRCLE_H_
+#define KERN_INTERP_BOUNDARIES_CIRCLE_H_
+
+#include <memory>
+
+#include "kern_interp/boundaries/boundary.h"
+
+namespace kern_interp {
+
+class Circle : public Boundary {
+ public:
+  void initialize(int n, BoundaryCondition bc) override;
+  bool is_in_domain(const Vec2& a) const override;
+  std::unique_ptr<Boundary> clone() const override {
+    return std::unique_ptr<Boundary>(new Circle(*this));
+  }
+};
+
+}  // namespace kern_interp
+
+#endif  // KERN_INTERP_BOUNDARIES_CIRCLE_H_

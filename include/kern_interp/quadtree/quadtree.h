@@ -1,0 +1,93 @@
+// Drop-in for kern_interp/quadtree/quadtree.h (reference quadtree.h:14-102).
+// initialize_tree builds the tree on the GPU (kigpu_tree_build) and mirrors
+// nodes, DOF lists and neighbor lists into these host structs; the factor
+// data stays device-resident in the handle (gpu()).
+#ifndef KERN_INTERP_QUADTREE_QUADTREE_H_
+#define KERN_INTERP_QUADTREE_QUADTREE_H_
+
+#include <cassert>
+#include <vector>
+
+#include "kern_interp/boundaries/boundary.h"
+#include "kern_interp/ki_mat.h"
+
+#ifndef MAX_LEAF_DOFS
+#define MAX_LEAF_DOFS 32
+#endif
+
+struct kigpu_tree;
+
+namespace kern_interp {
+
+struct InteractionLists {
+  std::vector<int> original_box, active_box, redundant, skel, near, skelnear, permutation;
+
+  void set_rs_ranges(const std::vector<int>& prm, int sk, int rd);
+  void set_skelnear_range();
+};
+
+struct QuadTreeNode {
+  int level, dofs_below;
+  bool is_leaf, X_rr_is_LU_factored = false, compressed = false;
+  double side_length;
+  QuadTreeNode* parent;
+  QuadTreeNode* children[4];
+  std::vector<QuadTreeNode*> neighbors;
+  InteractionLists dof_lists;
+  // factor blocks live on the device; these stay empty in this build
+  ki_Mat T, L, U, X_rr, schur_update, X_rr_lu;
+  std::vector<lapack_int> X_rr_piv;
+  // format is {BL, TL, TR, BR}
+  double corners[8];
+  QuadTreeNode() {
+    is_leaf = true;
+    for (int i = 0; i < 4; i++) children[i] = nullptr;
+  }
+};
+
+struct QuadTreeLevel {
+  std::vector<QuadTreeNode*> nodes;
+  ~QuadTreeLevel() {
+    for (QuadTreeNode* node : nodes) delete node;
+  }
+};
+
+class QuadTree {
+ public:
+  int solution_dimension, domain_dimension;
+  int no_proxy_level = 0;
+  // leaf capacity in DOFs (the reference fixes it with MAX_LEAF_DOFS)
+  int leaf_dofs = MAX_LEAF_DOFS;
+  double min, max;
+  std::vector<double> boundary_points;
+  QuadTreeNode* root = nullptr;
+  ki_Mat allskel_mat, allskel_mat_lu, U, Psi, S_LU;
+  std::vector<lapack_int> allskel_mat_piv, S_piv;
+  std::vector<QuadTreeLevel*> levels;
+
+  QuadTree() = default;
+  QuadTree(const QuadTree&) = delete;
+  QuadTree& operator=(const QuadTree&) = delete;
+  ~QuadTree();
+  void initialize_tree(Boundary* boundary, int solution_dimension_, int domain_dimension_);
+  void reset();
+  void reset(Boundary* boundary_);
+
+  void remove_inactive_dofs_at_level(int level);
+  void remove_inactive_dofs_at_all_boxes();
+  void remove_inactive_dofs_at_box(QuadTreeNode* node);
+
+  // device handle (geometry, tree, factorization)
+  kigpu_tree* gpu() const { return gpu_; }
+  // node ids in level order, as the C ABI numbers them
+  const std::vector<QuadTreeNode*>& nodes_by_id() const { return by_id_; }
+
+ private:
+  void clear();
+  kigpu_tree* gpu_ = nullptr;
+  std::vector<QuadTreeNode*> by_id_;
+};
+
+}  // namespace kern_interp
+
+#endif  // KERN_INTERP_QUADTREE_QUADTREE_H_

@@ -146,6 +146,12 @@ kigpu_status kigpu_factor_ranks(const kigpu_tree* tree, int* rank, int* active);
  * ::redundant, quadtree.h:17-18; filled by set_rs_ranges, quadtree.h:23-35). */
 kigpu_status kigpu_factor_export_lists(const kigpu_tree* tree, int64_t* skel_off, int* skel,
                                        int64_t* red_off, int* red);
+/* InteractionLists::active_box of every node as used by the factorization
+ * (CSR like above), and ::permutation (the ID pivots, positions in the
+ * active list; empty for nodes that were not compressed).  perm_off may be
+ * NULL when perm is NULL. */
+kigpu_status kigpu_factor_export_active(const kigpu_tree* tree, int64_t* act_off, int* active,
+                                        int64_t* perm_off, int* perm);
 /* the root skeleton QuadTree::root->dof_lists.active_box after factoring */
 kigpu_status kigpu_factor_allskel(const kigpu_tree* tree, int* n, int* allskel);
 
@@ -162,6 +168,22 @@ kigpu_status kigpu_solve(kigpu_ctx* ctx, kigpu_tree* tree, const double* b, int 
 kigpu_status kigpu_solve_device(kigpu_ctx* ctx, kigpu_tree* tree, const double* d_b,
                                 int nrhs, double* d_x, double* d_alpha,
                                 const kigpu_refine_opts* refine, kigpu_solve_info* info);
+
+/* ---- dense kernel blocks ------------------------------------------------ */
+typedef struct kigpu_geom kigpu_geom;
+/* Device copies of a Kernel's arrays (kern_interp/kernel/kernel.h:22-37):
+ * boundary points/normals (2*npts interleaved), weights/curvatures (npts),
+ * domain points (2*ndomain interleaved, may be 0). */
+kigpu_status kigpu_geometry_create(kigpu_ctx* ctx, const double* points, const double* normals,
+                                   const double* weights, const double* curvatures, int npts,
+                                   const double* domain_points, int ndomain, kigpu_geom** geom);
+kigpu_status kigpu_geometry_destroy(kigpu_geom* geom);
+/* out (ntgt x nsrc, column-major) = Kernel::operator()(tgt, src, forward)
+ * (kernel.cpp:99-150 Laplace/Neumann, :47-96 Stokes); sd = 2 for Stokes.
+ * forward = 1 takes targets from the domain points (Kernel::forward, :337-357). */
+kigpu_status kigpu_kernel_block(kigpu_ctx* ctx, const kigpu_geom* geom, int pde, int sd,
+                                const int* tgt, int ntgt, const int* src, int nsrc, int forward,
+                                double* out);
 
 /* ---- matrix-free kernel apply ----------------------------------------- */
 /* y = K(all, all) mu in FP64 on the fly, Kernel::operator() over all DOFs

@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libkigpu.so")
 
-SOURCES = ["alloc.cpp", "scan.hip", "tree.hip", "factor.hip", "border.hip", "solve.hip", "capi.cpp"]
+SOURCES = ["alloc.cpp", "scan.hip", "tree.hip", "factor.hip", "border.hip", "solve.hip", "blocks.hip", "capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-std=c++17", "-O3", "-fPIC", "--offload-arch=gfx950",
@@ -38,6 +38,43 @@ def _compile(src: str) -> str:
     return obj
 
 
+API_SOURCES = ["runtime.cpp", "ki_mat.cpp", "boundary.cpp", "quadtree.cpp", "kernel.cpp",
+               "skel_factorization.cpp", "linear_solve.cpp"]
+API_LIB = os.path.join(HERE, "libkern_interp.so")
+DROPIN_TEST = os.path.join(BUILD, "dropin_test")
+CXX = os.environ.get("CXX", "g++")
+CXXFLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-I" + os.path.join(ROOT, "include")]
+
+
+def _compile_api(src: str) -> str:
+    obj = os.path.join(BUILD, "api_" + os.path.splitext(src)[0] + ".o")
+    cmd = [CXX, *CXXFLAGS, "-c", os.path.join(HERE, "src", src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed on {src}:\n{r.stderr[-8000:]}")
+    return obj
+
+
+def build_api(jobs: int = 4) -> str:
+    """libkern_interp.so (the drop-in C++ API over libkigpu.so) and the
+    C++ drop-in test program."""
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_compile_api, API_SOURCES))
+    tmp = API_LIB + ".tmp"
+    cmd = [CXX, "-shared", "-o", tmp, *objs, "-L" + HERE, "-lkigpu", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link of libkern_interp.so failed:\n{r.stderr[-8000:]}")
+    os.replace(tmp, API_LIB)
+    cmd = [CXX, *CXXFLAGS, os.path.join(ROOT, "tests", "cpp", "dropin_test.cpp"), "-o",
+           DROPIN_TEST, "-L" + HERE, "-lkern_interp", "-lkigpu",
+           "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link," + HERE]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"dropin_test link failed:\n{r.stderr[-8000:]}")
+    return API_LIB
+
+
 def build(jobs: int = 4) -> str:
     os.makedirs(BUILD, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
@@ -48,6 +85,7 @@ def build(jobs: int = 4) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-8000:]}")
     os.replace(tmp, LIB)
+    build_api(jobs)
     return LIB
 
 

@@ -18,6 +18,11 @@ struct kigpu_tree {
   TreeData t;
 };
 
+struct kigpu_geom {
+  int npts = 0, ndomain = 0;
+  DBuf<double> px, py, nx, ny, w, kappa, dx, dy;
+};
+
 namespace kigpu {
 static thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
@@ -374,6 +379,33 @@ kigpu_status kigpu_factor_export_lists(const kigpu_tree* tree, int64_t* skel_off
   return KIGPU_OK;
 }
 
+kigpu_status kigpu_factor_export_active(const kigpu_tree* tree, int64_t* act_off, int* active,
+                                        int64_t* perm_off, int* perm) {
+  KI_REQUIRE(tree && act_off, "null argument");
+  const TreeData& t = tree->t;
+  if (!t.factored) {
+    set_error("tree is not factored");
+    return KIGPU_ERR_STATE;
+  }
+  kigpu_status st = guard([&] { ensure_factor_mirrors(const_cast<kigpu_tree*>(tree)->t); });
+  if (st != KIGPU_OK) return st;
+  int64_t oa = 0, op = 0;
+  for (int x = 0; x < t.nnodes; ++x) {
+    act_off[x] = oa;
+    if (perm_off) perm_off[x] = op;
+    const int n = t.h_nact[x];
+    if (active && n) std::memcpy(active + oa, &t.h_act[t.h_act_off[x]], sizeof(int) * n);
+    oa += n;
+    if (t.h_rank[x] >= 0) {
+      if (perm) std::memcpy(perm + op, &t.h_pos[t.h_act_off[x]], sizeof(int) * n);
+      op += n;
+    }
+  }
+  act_off[t.nnodes] = oa;
+  if (perm_off) perm_off[t.nnodes] = op;
+  return KIGPU_OK;
+}
+
 kigpu_status kigpu_factor_allskel(const kigpu_tree* tree, int* n, int* allskel) {
   KI_REQUIRE(tree && n, "null argument");
   const TreeData& t = tree->t;
@@ -425,6 +457,66 @@ kigpu_status kigpu_solve(kigpu_ctx* ctx, kigpu_tree* tree, const double* b, int 
                             sizeof(double) * nd, nrhs, hipMemcpyDeviceToHost, c.stream));
     if (h && alpha) da.download(alpha, (size_t)h * nrhs, c.stream);
     KI_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+kigpu_status kigpu_geometry_create(kigpu_ctx* ctx, const double* points, const double* normals,
+                                   const double* weights, const double* curvatures, int npts,
+                                   const double* domain_points, int ndomain, kigpu_geom** out) {
+  KI_REQUIRE(ctx && out && npts >= 0 && ndomain >= 0, "invalid argument");
+  KI_REQUIRE(npts == 0 || (points && normals && weights && curvatures), "null geometry array");
+  KI_REQUIRE(ndomain == 0 || domain_points, "null domain points");
+  *out = nullptr;
+  auto g = std::make_unique<kigpu_geom>();
+  kigpu_status st = guard([&] {
+    hipStream_t s = ctx->c.stream;
+    g->npts = npts;
+    g->ndomain = ndomain;
+    std::vector<double> a(npts), b(npts), na(npts), nb(npts), da(ndomain), db(ndomain);
+    for (int i = 0; i < npts; ++i) {
+      a[i] = points[2 * i];
+      b[i] = points[2 * i + 1];
+      na[i] = normals[2 * i];
+      nb[i] = normals[2 * i + 1];
+    }
+    for (int i = 0; i < ndomain; ++i) {
+      da[i] = domain_points[2 * i];
+      db[i] = domain_points[2 * i + 1];
+    }
+    g->px.upload(a, s);
+    g->py.upload(b, s);
+    g->nx.upload(na, s);
+    g->ny.upload(nb, s);
+    g->w.upload(weights, npts, s);
+    g->kappa.upload(curvatures, npts, s);
+    g->dx.upload(da, s);
+    g->dy.upload(db, s);
+    KI_HIP(hipStreamSynchronize(s));
+  });
+  if (st == KIGPU_OK) *out = g.release();
+  return st;
+}
+
+kigpu_status kigpu_geometry_destroy(kigpu_geom* geom) {
+  (void)hipDeviceSynchronize();
+  delete geom;
+  return KIGPU_OK;
+}
+
+kigpu_status kigpu_kernel_block(kigpu_ctx* ctx, const kigpu_geom* geom, int pde, int sd,
+                                const int* tgt, int ntgt, const int* src, int nsrc, int forward,
+                                double* out) {
+  KI_REQUIRE(ctx && geom && out && ntgt >= 0 && nsrc >= 0, "invalid argument");
+  KI_REQUIRE((ntgt == 0 || tgt) && (nsrc == 0 || src), "null index list");
+  KI_REQUIRE(sd == 1 || sd == 2, "solution dimension must be 1 or 2");
+  const int lim_t = (forward ? geom->ndomain : geom->npts) * sd;
+  for (int i = 0; i < ntgt; ++i) KI_REQUIRE(tgt[i] >= 0 && tgt[i] < lim_t, "target index out of range");
+  for (int j = 0; j < nsrc; ++j)
+    KI_REQUIRE(src[j] >= 0 && src[j] < geom->npts * sd, "source index out of range");
+  return guard([&] {
+    const GeomView g{geom->px.p, geom->py.p, geom->nx.p, geom->ny.p, geom->w.p,
+                     geom->kappa.p, geom->npts, sd};
+    kernel_block(ctx->c, g, geom->dx.p, geom->dy.p, pde, sd, tgt, ntgt, src, nsrc, forward, out);
   });
 }
 

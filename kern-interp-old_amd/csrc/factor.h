@@ -65,6 +65,9 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
                 double* d_alpha, const kigpu_refine_opts* ro, kigpu_solve_info* info);
 void kernel_apply(Context& c, const TreeData& t, int pde, const double* d_mu, int nrhs,
                   double* d_y);
+void kernel_block(Context& c, const GeomView& g, const double* dpx, const double* dpy, int pde,
+                  int sd, const int* h_tgt, int ntgt, const int* h_src, int nsrc, int forward,
+                  double* h_out);
 void forward_apply(Context& c, const TreeData& t, int pde, const double* d_tgt, int ntgt,
                    const double* d_mu, int nrhs, double* d_u);
 

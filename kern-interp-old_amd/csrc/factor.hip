@@ -627,17 +627,27 @@ __global__ __launch_bounds__(kDecTB) void k_decouple(int a, GeomView g, int pde,
     X[i + (size_t)j * r] = s;
   });
   __syncthreads();
+  // K_BN and T are dead from here on: the K_BN area holds a copy of X_rr and
+  // the two pivot scratch vectors (r^2 + 2r <= n^2 since k >= 1)
+  T* Xc = Kp;
+  for (int e = tid; e < r * r; e += kDecTB) Xc[e] = X[e];
+  __syncthreads();
   // X_rr^{-1} in place (Gauss-Jordan with dgetrf's pivot order)
   T* Z = X;
   __shared__ int s_p;
-  if (!block_gj_inverse(X, r, piv, Kp, Kp + n, &s_p) && tid == 0) flag(status, 4);
+  if (!block_gj_inverse(X, r, piv, Xc + (size_t)r * r, Xc + (size_t)r * r + r, &s_p) && tid == 0)
+    flag(status, 4);
   __syncthreads();
-  // L = K_sn_r X_rr^{-1} (k x r), U = X_rr^{-1} K_r_sn (r x k)
+  // L0 = K_sn_r X_rr^{-1} (k x r), U = X_rr^{-1} K_r_sn (r x k).  The right
+  // residual of an explicit inverse is small but its left residual is not:
+  // multiplied from the right, X_rr^{-1} loses digits when X_rr is
+  // ill-conditioned (tight id_tol), so L takes one refinement step against
+  // X_rr below, which brings it to the accuracy of the dgetrs solve of
+  // skel_factorization.cpp:98-103.
   for2d<kDecNW>(k, r, [&](int i, int j) {
     T s = T(0);
     for (int t = 0; t < r; ++t) s += Ksnr[i + (size_t)t * k] * Z[t + (size_t)j * r];
     Ls[i + (size_t)j * k] = s;
-    Lm[i + (size_t)j * k] = s;
   });
   for2d<kDecNW>(r, k, [&](int i, int j) {
     T s = T(0);
@@ -646,6 +656,21 @@ __global__ __launch_bounds__(kDecTB) void k_decouple(int a, GeomView g, int pde,
   });
   for (int e = tid; e < r * r; e += kDecTB) Xinv[e] = Z[e];
   for (int c = tid; c < r; c += kDecTB) piv_pool[aoff + c] = piv[c];
+  __syncthreads();
+  // R = K_sn_r - L0 X_rr in place of K_sn_r (each entry reads only itself)
+  for2d<kDecNW>(k, r, [&](int i, int j) {
+    T s = Ksnr[i + (size_t)j * k];
+    for (int t = 0; t < r; ++t) s -= Ls[i + (size_t)t * k] * Xc[t + (size_t)j * r];
+    Ksnr[i + (size_t)j * k] = s;
+  });
+  __syncthreads();
+  // L = L0 + R X_rr^{-1}
+  for2d<kDecNW>(k, r, [&](int i, int j) {
+    T s = Ls[i + (size_t)j * k];
+    for (int t = 0; t < r; ++t) s += Ksnr[i + (size_t)t * k] * Z[t + (size_t)j * r];
+    Ls[i + (size_t)j * k] = s;
+    Lm[i + (size_t)j * k] = s;
+  });
   __syncthreads();
   // schur_update = L K_r_sn (k x k), plus the descendants' update on the skeleton
   for2d<kDecNW>(k, k, [&](int i, int j) {

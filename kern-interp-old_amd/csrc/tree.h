@@ -61,6 +61,7 @@ struct TreeData {
   std::vector<int64_t> h_act_off;
   std::vector<int> h_sr;   // host copy of sr_pool after factoring
   std::vector<int> h_act;  // host copy of act_pool after factoring
+  std::vector<int> h_pos;  // host copy of pos_pool after factoring
 
   // root
   int nroot = 0;

@@ -685,6 +685,9 @@ void ensure_factor_mirrors(TreeData& t) {
   KI_HIP(hipDeviceSynchronize());
   t.h_sr.resize(t.act_pool_size);
   t.h_act.resize(t.act_pool_size);
+  t.h_pos.resize(t.act_pool_size);
+  KI_HIP(hipMemcpy(t.h_pos.data(), t.pos_pool.p, sizeof(int) * t.act_pool_size,
+                   hipMemcpyDeviceToHost));
   KI_HIP(hipMemcpy(t.h_sr.data(), t.sr_pool.p, sizeof(int) * t.act_pool_size,
                    hipMemcpyDeviceToHost));
   KI_HIP(hipMemcpy(t.h_act.data(), t.act_pool.p, sizeof(int) * t.act_pool_size,

@@ -56,7 +56,8 @@ EXPORTS = [
     "kigpu_tree_export_nodes", "kigpu_tree_export_boxes", "kigpu_tree_export_neighbors",
     "kigpu_set_border", "kigpu_factor", "kigpu_factor_ranks", "kigpu_factor_export_lists",
     "kigpu_factor_allskel", "kigpu_solve", "kigpu_solve_device", "kigpu_kernel_apply",
-    "kigpu_kernel_apply_device", "kigpu_forward_apply",
+    "kigpu_kernel_apply_device", "kigpu_forward_apply", "kigpu_geometry_create",
+    "kigpu_geometry_destroy", "kigpu_kernel_block", "kigpu_factor_export_active",
 ]
 
 _lib = None

@@ -1,0 +1,406 @@
+// ki_Mat: host value type of the drop-in API (interface of the reference's
+// kern_interp/ki_mat.h:19-101).  Dense helpers are written out here (no
+// BLAS/LAPACK dependency); the factor/solve path runs on the GPU.
+#include "kern_interp/ki_mat.h"
+
+#include <algorithm>
+#include <cassert>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+namespace kern_interp {
+
+ki_Mat::ki_Mat() : mat(nullptr), lda_(0), height_(0), width_(0) {}
+
+ki_Mat::~ki_Mat() { delete[] mat; }
+
+ki_Mat::ki_Mat(int h, int w) : lda_(h), height_(h), width_(w) {
+  mat = new double[(size_t)h * w]();
+}
+
+ki_Mat::ki_Mat(const ki_Mat& o) : lda_(o.lda_), height_(o.height_), width_(o.width_) {
+  mat = new double[(size_t)height_ * width_];
+  if (height_ * width_) std::memcpy(mat, o.mat, sizeof(double) * height_ * width_);
+}
+
+ki_Mat& ki_Mat::operator=(const ki_Mat& o) {
+  if (this == &o) return *this;
+  if (height_ * width_ != o.height_ * o.width_ || !mat) {
+    delete[] mat;
+    mat = new double[(size_t)o.height_ * o.width_];
+  }
+  lda_ = o.lda_;
+  height_ = o.height_;
+  width_ = o.width_;
+  if (height_ * width_) std::memcpy(mat, o.mat, sizeof(double) * height_ * width_);
+  return *this;
+}
+
+ki_Mat::ki_Mat(ki_Mat&& o) : mat(o.mat), lda_(o.lda_), height_(o.height_), width_(o.width_) {
+  o.mat = nullptr;
+  o.lda_ = o.height_ = o.width_ = 0;
+}
+
+ki_Mat& ki_Mat::operator=(ki_Mat&& o) {
+  if (this == &o) return *this;
+  delete[] mat;
+  mat = o.mat;
+  lda_ = o.lda_;
+  height_ = o.height_;
+  width_ = o.width_;
+  o.mat = nullptr;
+  o.lda_ = o.height_ = o.width_ = 0;
+  return *this;
+}
+
+double ki_Mat::get(int i, int j) const {
+  assert(i < height_ && j < width_ && mat);
+  return mat[i + (size_t)lda_ * j];
+}
+
+void ki_Mat::set(int i, int j, double a) {
+  assert(i < height_ && j < width_ && mat);
+  mat[i + (size_t)lda_ * j] = a;
+}
+
+void ki_Mat::addset(int i, int j, double a) {
+  assert(i < height_ && j < width_ && mat);
+  mat[i + (size_t)lda_ * j] += a;
+}
+
+void ki_Mat::set_submatrix(const std::vector<int>& I_, const std::vector<int>& J_,
+                           const ki_Mat& A, bool transpose_A) {
+  for (size_t j = 0; j < J_.size(); ++j)
+    for (size_t i = 0; i < I_.size(); ++i)
+      set(I_[i], J_[j], transpose_A ? A.get(j, i) : A.get(i, j));
+}
+
+void ki_Mat::set_submatrix(int row_s, int row_e, int col_s, int col_e, const ki_Mat& A,
+                           bool transpose_A, bool) {
+  for (int j = 0; j < col_e - col_s; ++j)
+    for (int i = 0; i < row_e - row_s; ++i)
+      set(row_s + i, col_s + j, transpose_A ? A.get(j, i) : A.get(i, j));
+}
+
+void ki_Mat::set_submatrix(const std::vector<int>& I_, int col_s, int col_e, const ki_Mat& A,
+                           bool transpose_A) {
+  for (int j = 0; j < col_e - col_s; ++j)
+    for (size_t i = 0; i < I_.size(); ++i)
+      set(I_[i], col_s + j, transpose_A ? A.get(j, i) : A.get(i, j));
+}
+
+void ki_Mat::set_submatrix(int row_s, int row_e, const std::vector<int>& J_, const ki_Mat& A,
+                           bool transpose_A) {
+  for (size_t j = 0; j < J_.size(); ++j)
+    for (int i = 0; i < row_e - row_s; ++i)
+      set(row_s + i, J_[j], transpose_A ? A.get(j, i) : A.get(i, j));
+}
+
+void ki_Mat::transpose_into(ki_Mat* t) const {
+  if (t->height_ != width_ || t->width_ != height_) *t = ki_Mat(width_, height_);
+  for (int j = 0; j < width_; ++j)
+    for (int i = 0; i < height_; ++i) t->mat[j + (size_t)t->lda_ * i] = get(i, j);
+}
+
+void ki_Mat::eye(int n) {
+  if (height_ != n || width_ != n) *this = ki_Mat(n, n);
+  for (int i = 0; i < n; ++i) set(i, i, 1.0);
+}
+
+ki_Mat ki_Mat::transpose() const {
+  ki_Mat t(width_, height_);
+  transpose_into(&t);
+  return t;
+}
+
+int ki_Mat::height() const { return height_; }
+int ki_Mat::width() const { return width_; }
+
+ki_Mat& ki_Mat::operator-=(const ki_Mat& o) {
+  assert(o.height_ == height_ && o.width_ == width_);
+  for (int j = 0; j < width_; ++j)
+    for (int i = 0; i < height_; ++i) mat[i + (size_t)lda_ * j] -= o.get(i, j);
+  return *this;
+}
+
+ki_Mat& ki_Mat::operator+=(const ki_Mat& o) {
+  assert(o.height_ == height_ && o.width_ == width_);
+  for (int j = 0; j < width_; ++j)
+    for (int i = 0; i < height_; ++i) mat[i + (size_t)lda_ * j] += o.get(i, j);
+  return *this;
+}
+
+ki_Mat& ki_Mat::operator*=(double o) {
+  for (size_t e = 0; e < (size_t)height_ * width_; ++e) mat[e] *= o;
+  return *this;
+}
+
+ki_Mat& ki_Mat::operator/=(double o) {
+  assert(std::abs(o) > 1e-8 && "Error: divide matrix by 0.");
+  for (size_t e = 0; e < (size_t)height_ * width_; ++e) mat[e] /= o;
+  return *this;
+}
+
+ki_Mat ki_Mat::operator-() const { return (*this) * -1.0; }
+
+ki_Mat ki_Mat::operator-(const ki_Mat& o) const {
+  ki_Mat r(*this);
+  r -= o;
+  return r;
+}
+
+ki_Mat ki_Mat::operator+(const ki_Mat& o) const {
+  ki_Mat r(*this);
+  r += o;
+  return r;
+}
+
+ki_Mat ki_Mat::operator*(const ki_Mat& o) const {
+  assert(width_ == o.height_);
+  ki_Mat r(height_, o.width_);
+  for (int j = 0; j < o.width_; ++j)
+    for (int t = 0; t < width_; ++t) {
+      const double b = o.mat[t + (size_t)o.lda_ * j];
+      if (b == 0.0) continue;
+      const double* a = mat + (size_t)lda_ * t;
+      double* c = r.mat + (size_t)r.lda_ * j;
+      for (int i = 0; i < height_; ++i) c[i] += a[i] * b;
+    }
+  return r;
+}
+
+ki_Mat ki_Mat::operator*(double o) const {
+  ki_Mat r(*this);
+  r *= o;
+  return r;
+}
+
+ki_Mat ki_Mat::operator/(double o) const {
+  ki_Mat r(*this);
+  r /= o;
+  return r;
+}
+
+ki_Mat ki_Mat::operator()(const std::vector<int>& I_, const std::vector<int>& J_) const {
+  ki_Mat r(I_.size(), J_.size());
+  for (size_t j = 0; j < J_.size(); ++j)
+    for (size_t i = 0; i < I_.size(); ++i) r.mat[i + r.lda_ * j] = get(I_[i], J_[j]);
+  return r;
+}
+
+ki_Mat ki_Mat::operator()(const std::vector<int>& I_, int col_s, int col_e) const {
+  ki_Mat r(I_.size(), col_e - col_s);
+  for (int j = 0; j < col_e - col_s; ++j)
+    for (size_t i = 0; i < I_.size(); ++i) r.mat[i + r.lda_ * j] = get(I_[i], col_s + j);
+  return r;
+}
+
+ki_Mat ki_Mat::operator()(int row_s, int row_e, const std::vector<int>& J_) const {
+  ki_Mat r(row_e - row_s, J_.size());
+  for (size_t j = 0; j < J_.size(); ++j)
+    for (int i = 0; i < row_e - row_s; ++i) r.mat[i + r.lda_ * j] = get(row_s + i, J_[j]);
+  return r;
+}
+
+ki_Mat ki_Mat::operator()(int row_s, int row_e, int col_s, int col_e) const {
+  ki_Mat r(row_e - row_s, col_e - col_s);
+  for (int j = 0; j < col_e - col_s; ++j)
+    for (int i = 0; i < row_e - row_s; ++i) r.mat[i + r.lda_ * j] = get(row_s + i, col_s + j);
+  return r;
+}
+
+double ki_Mat::frob_norm() const {
+  double s = 0;
+  for (int j = 0; j < width_; ++j)
+    for (int i = 0; i < height_; ++i) s += get(i, j) * get(i, j);
+  return std::sqrt(s);
+}
+
+double ki_Mat::max_entry_magnitude() const {
+  double m = 0;
+  for (int j = 0; j < width_; ++j)
+    for (int i = 0; i < height_; ++i) m = std::max(m, std::abs(get(i, j)));
+  return m;
+}
+
+double ki_Mat::one_norm() const {
+  double top = 0;
+  for (int j = 0; j < width_; ++j) {
+    double s = 0;
+    for (int i = 0; i < height_; ++i) s += std::fabs(get(i, j));
+    top = std::max(top, s);
+  }
+  return top;
+}
+
+double ki_Mat::vec_two_norm() const {
+  assert(width() == 1);
+  return frob_norm();
+}
+
+void ki_Mat::LU_factorize(ki_Mat* K_LU, std::vector<lapack_int>* piv) const {
+  *K_LU = *this;
+  const int n = std::min(height_, width_);
+  piv->assign(n, 0);
+  ki_Mat& A = *K_LU;
+  for (int c = 0; c < n; ++c) {
+    int p = c;
+    double best = std::fabs(A.get(c, c));
+    for (int i = c + 1; i < height_; ++i)
+      if (std::fabs(A.get(i, c)) > best) {
+        best = std::fabs(A.get(i, c));
+        p = i;
+      }
+    (*piv)[c] = p + 1;
+    if (p != c)
+      for (int j = 0; j < width_; ++j) std::swap(A.mat[c + (size_t)A.lda_ * j],
+                                                 A.mat[p + (size_t)A.lda_ * j]);
+    const double d = A.get(c, c);
+    if (d == 0.0) continue;
+    for (int i = c + 1; i < height_; ++i) A.mat[i + (size_t)A.lda_ * c] /= d;
+    for (int j = c + 1; j < width_; ++j) {
+      const double u = A.get(c, j);
+      for (int i = c + 1; i < height_; ++i)
+        A.mat[i + (size_t)A.lda_ * j] -= A.get(i, c) * u;
+    }
+  }
+}
+
+// in place: B <- op(LU)^{-1} B
+static void lu_solve(const ki_Mat& LU, const std::vector<lapack_int>& piv, ki_Mat* B, bool trans) {
+  const int n = LU.height();
+  for (int col = 0; col < B->width(); ++col) {
+    double* b = B->mat + (size_t)B->lda_ * col;
+    if (!trans) {
+      for (int c = 0; c < n; ++c) std::swap(b[c], b[piv[c] - 1]);
+      for (int i = 0; i < n; ++i)
+        for (int t = 0; t < i; ++t) b[i] -= LU.get(i, t) * b[t];
+      for (int i = n - 1; i >= 0; --i) {
+        for (int t = i + 1; t < n; ++t) b[i] -= LU.get(i, t) * b[t];
+        b[i] /= LU.get(i, i);
+      }
+    } else {
+      for (int i = 0; i < n; ++i) {
+        for (int t = 0; t < i; ++t) b[i] -= LU.get(t, i) * b[t];
+        b[i] /= LU.get(i, i);
+      }
+      for (int i = n - 1; i >= 0; --i)
+        for (int t = i + 1; t < n; ++t) b[i] -= LU.get(t, i) * b[t];
+      for (int c = n - 1; c >= 0; --c) std::swap(b[c], b[piv[c] - 1]);
+    }
+  }
+}
+
+void ki_Mat::left_multiply_inverse(const ki_Mat& K, ki_Mat* U) const {
+  ki_Mat lu;
+  std::vector<lapack_int> piv;
+  LU_factorize(&lu, &piv);
+  *U = K;
+  lu_solve(lu, piv, U, false);
+}
+
+void ki_Mat::right_multiply_inverse(const ki_Mat& K, ki_Mat* L) const {
+  ki_Mat lu;
+  std::vector<lapack_int> piv;
+  LU_factorize(&lu, &piv);
+  ki_Mat kt = K.transpose();
+  lu_solve(lu, piv, &kt, true);
+  *L = kt.transpose();
+}
+
+void ki_Mat::left_multiply_inverse(const ki_Mat& K, const std::vector<lapack_int>& piv,
+                                   ki_Mat* U) const {
+  *U = K;
+  lu_solve(*this, piv, U, false);
+}
+
+void ki_Mat::right_multiply_inverse(const ki_Mat& K, const std::vector<lapack_int>& piv,
+                                    ki_Mat* L) const {
+  ki_Mat kt = K.transpose();
+  lu_solve(*this, piv, &kt, true);
+  *L = kt.transpose();
+}
+
+int ki_Mat::id(std::vector<int>* p, ki_Mat* Z, double tol) const {
+  assert(height() != 0 && width() != 0);
+  const int m = height_, n = width_;
+  ki_Mat A(*this);
+  std::vector<int> perm(n);
+  std::vector<double> vn1(n), vn2(n);
+  for (int j = 0; j < n; ++j) {
+    perm[j] = j;
+    double s = 0;
+    for (int i = 0; i < m; ++i) s += A.get(i, j) * A.get(i, j);
+    vn1[j] = vn2[j] = std::sqrt(s);
+  }
+  const double tol3z = std::sqrt(2.220446049250313e-16);
+  std::vector<double> rdiag(std::min(m, n));
+  for (int i = 0; i < std::min(m, n); ++i) {
+    int pv = i;
+    for (int j = i + 1; j < n; ++j)
+      if (vn1[j] > vn1[pv]) pv = j;
+    if (pv != i) {
+      for (int r = 0; r < m; ++r) std::swap(A.mat[r + (size_t)m * i], A.mat[r + (size_t)m * pv]);
+      std::swap(perm[i], perm[pv]);
+      vn1[pv] = vn1[i];
+      vn2[pv] = vn2[i];
+    }
+    double* v = A.mat + (size_t)m * i;
+    double xn = 0;
+    for (int r = i + 1; r < m; ++r) xn += v[r] * v[r];
+    xn = std::sqrt(xn);
+    const double alpha = v[i];
+    double beta = alpha, tau = 0;
+    if (xn != 0.0) {
+      beta = -std::copysign(std::hypot(alpha, xn), alpha);
+      tau = (beta - alpha) / beta;
+      const double sc = 1.0 / (alpha - beta);
+      for (int r = i + 1; r < m; ++r) v[r] *= sc;
+    }
+    v[i] = beta;
+    rdiag[i] = beta;
+    for (int j = i + 1; j < n; ++j) {
+      double* cj = A.mat + (size_t)m * j;
+      double d = cj[i];
+      for (int r = i + 1; r < m; ++r) d += v[r] * cj[r];
+      d *= tau;
+      cj[i] -= d;
+      for (int r = i + 1; r < m; ++r) cj[r] -= d * v[r];
+      if (vn1[j] != 0.0) {
+        double tt = std::fabs(cj[i]) / vn1[j];
+        double temp = std::max(1.0 - tt * tt, 0.0);
+        double rr = vn1[j] / vn2[j];
+        if (temp * rr * rr <= tol3z) {
+          double s = 0;
+          for (int r = i + 1; r < m; ++r) s += cj[r] * cj[r];
+          vn1[j] = vn2[j] = i + 1 < m ? std::sqrt(s) : 0.0;
+        } else {
+          vn1[j] *= std::sqrt(temp);
+        }
+      }
+    }
+  }
+  const double thresh = std::fabs(tol * rdiag[0]);
+  int skel = 0;
+  for (int i = 1; i < n; ++i) {
+    if (i >= m) throw std::runtime_error("ki_Mat::id: fewer rows than columns");
+    if (std::fabs(rdiag[i]) < thresh) {
+      skel = i;
+      break;
+    }
+  }
+  if (skel == 0) return 0;
+  p->assign(perm.begin(), perm.end());
+  const int red = n - skel;
+  *Z = ki_Mat(skel, red);
+  for (int c = 0; c < red; ++c)
+    for (int row = skel - 1; row >= 0; --row) {
+      double s = A.get(row, skel + c);
+      for (int j = row + 1; j < skel; ++j) s -= A.get(row, j) * Z->get(j, c);
+      Z->set(row, c, s / A.get(row, row));
+    }
+  return skel;
+}
+
+}  // namespace kern_interp

@@ -9,7 +9,12 @@ nearly singular (cond(allskel_mat) 1.8e9 for the annulus, 8.9e9 for the
 space) and cond(S) reaches 1.3e5, so last-bit differences in the order of
 floating-point operations show up at ~1e-8 (measured 7e-9 and 6e-9).  The
 backward error of the full bordered system and the analytic interior
-solution pin the result tighter."""
+solution pin the result tighter.
+
+alpha is compared on the scale of the whole solution [mu; alpha]: for
+boundary data from exterior sources the hole fluxes are zero in exact
+arithmetic (|alpha| ~ 1e-9 here), so a relative bound on alpha alone would
+measure rounding noise."""
 import numpy as np
 import pytest
 
@@ -32,6 +37,15 @@ def border_case(ctx, b, pde, leaf, tol, precision=kigpu.FP64, refine=0):
     return t, mu, t.last_alpha
 
 
+def check_parity(mu, alpha, mu_ref, alpha_ref, bar=1e-7):
+    scale = np.linalg.norm(np.concatenate([mu_ref.ravel(), alpha_ref.ravel()]))
+    emu = np.linalg.norm(mu - mu_ref) / scale
+    eal = np.linalg.norm(alpha - alpha_ref) / scale
+    print(f"  mu diff {emu:.3e}  alpha diff {eal:.3e}  |alpha| {np.linalg.norm(alpha_ref):.3e}",
+          flush=True)
+    assert emu < bar and eal < bar
+
+
 @pytest.mark.gpu
 def test_annulus_laplace_parity(ctx):
     print("[border] annulus laplace N=1024 leaf 32", flush=True)
@@ -43,8 +57,7 @@ def test_annulus_laplace_parity(ctx):
     rank, _ = t.ranks()
     orank = np.array([len(n.skel) if n.compressed else -1 for n in otree.all_nodes()])
     assert np.array_equal(rank, orank)
-    assert np.linalg.norm(mu - mu_ref) / np.linalg.norm(mu_ref) < 1e-7
-    assert np.linalg.norm(alpha - alpha_ref) / np.linalg.norm(alpha_ref) < 1e-7
+    check_parity(mu, alpha, mu_ref, alpha_ref)
     assert LS.backward_error(b, K.LAPLACE, mu, alpha) <= 10 * 1e-6
 
 
@@ -81,8 +94,7 @@ def test_stokes_holes(ctx, precision, refine):
     mu_ref, alpha_ref = ofac.multiply_connected_solve(otree, b.boundary_values)
     t, mu, alpha = border_case(ctx, b, K.STOKES, 64, 1e-6, precision, refine)
     if precision == kigpu.FP64:
-        assert np.linalg.norm(mu - mu_ref) / np.linalg.norm(mu_ref) < 1e-7
-        assert np.linalg.norm(alpha - alpha_ref) / np.linalg.norm(alpha_ref) < 1e-7
+        check_parity(mu, alpha, mu_ref, alpha_ref)
         assert LS.backward_error(b, K.STOKES, mu, alpha) <= 10 * 1e-6
     else:
         assert t.last_info["rel_residual"] <= 1e-12
