@@ -198,6 +198,40 @@ kigpu_status kigpu_forward_apply(kigpu_ctx* ctx, const kigpu_tree* tree, int pde
                                  const double* targets, int ntgt, const double* mu,
                                  int nrhs, double* u);
 
+/* ---- adjoint ------------------------------------------------------------ */
+/* No counterpart in the reference, whose experiments differentiate by finite
+ * differences of whole solves (experiments/experiment_three.cpp:22-56,
+ * 160-171).  x = F^{-T} b for the factorization F of kigpu_factor: the
+ * transposes of the sweeps of SkelFactorization::solve
+ * (skel_factorization.cpp:341-403) in reverse order.  With FP32_REFINE and
+ * refine->max_iters > 0 it is refined against the FP64 K^T apply.  Trees
+ * with a border (holes) return KIGPU_ERR_NOTSUPP. */
+kigpu_status kigpu_solve_transpose(kigpu_ctx* ctx, kigpu_tree* tree, const double* b, int ldb,
+                                   int nrhs, double* x, int ldx,
+                                   const kigpu_refine_opts* refine, kigpu_solve_info* info);
+kigpu_status kigpu_solve_transpose_device(kigpu_ctx* ctx, kigpu_tree* tree, const double* d_b,
+                                          int nrhs, double* d_x,
+                                          const kigpu_refine_opts* refine,
+                                          kigpu_solve_info* info);
+/* y = K(all, all)^T lam in FP64 on the fly (entries of kernel.cpp:11-96). */
+kigpu_status kigpu_kernel_apply_transpose(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
+                                          const double* lam, int nrhs, double* y);
+/* y = Kernel::forward()^T v (kernel.cpp:337-357): v is (sd*ntgt) x nrhs,
+ * y is ndof x nrhs; targets: 2*ntgt interleaved. */
+kigpu_status kigpu_forward_apply_transpose(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
+                                          const double* targets, int ntgt, const double* v,
+                                          int nrhs, double* y);
+/* Shape sensitivities (Laplace double layer, KIGPU_LAPLACE only): for every
+ * boundary point p, sens[6p .. 6p+5] = d/d(x_p, y_p, nx_p, ny_p, w_p, kappa_p)
+ * of lam^T K mu (kernel_sensitivity) or of v^T F mu with F the forward
+ * matrix at fixed targets (forward_sensitivity).  lam, mu: npts; v: ntgt.
+ * Other PDEs return KIGPU_ERR_NOTSUPP. */
+kigpu_status kigpu_kernel_sensitivity(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
+                                      const double* lam, const double* mu, double* sens);
+kigpu_status kigpu_forward_sensitivity(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
+                                       const double* targets, int ntgt, const double* v,
+                                       const double* mu, double* sens);
+
 #ifdef __cplusplus
 }
 #endif

@@ -98,6 +98,11 @@ std::vector<int> full_neighbors(const TreeData& t, int x) {
 
 }  // namespace
 
+static bool pde_matches(const kigpu_tree* tree, int pde) {
+  if (pde != KIGPU_LAPLACE && pde != KIGPU_LAPLACE_NEUMANN && pde != KIGPU_STOKES) return false;
+  return (pde == KIGPU_STOKES) == (tree->t.sd == 2);
+}
+
 extern "C" {
 
 const char* kigpu_status_string(kigpu_status s) {
@@ -523,12 +528,14 @@ kigpu_status kigpu_kernel_block(kigpu_ctx* ctx, const kigpu_geom* geom, int pde,
 kigpu_status kigpu_kernel_apply_device(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
                                        const double* d_mu, int nrhs, double* d_y) {
   KI_REQUIRE(ctx && tree && d_mu && d_y && nrhs > 0, "invalid argument");
+  KI_REQUIRE(pde_matches(tree, pde), "pde does not match the tree's solution dimension");
   return guard([&] { kernel_apply(ctx->c, tree->t, pde, d_mu, nrhs, d_y); });
 }
 
 kigpu_status kigpu_kernel_apply(kigpu_ctx* ctx, const kigpu_tree* tree, int pde, const double* mu,
                                 int nrhs, double* y) {
   KI_REQUIRE(ctx && tree && mu && y && nrhs > 0, "invalid argument");
+  KI_REQUIRE(pde_matches(tree, pde), "pde does not match the tree's solution dimension");
   return guard([&] {
     Context& c = ctx->c;
     const size_t len = (size_t)tree->t.ndof * nrhs;
@@ -545,6 +552,7 @@ kigpu_status kigpu_forward_apply(kigpu_ctx* ctx, const kigpu_tree* tree, int pde
                                  const double* targets, int ntgt, const double* mu, int nrhs,
                                  double* u) {
   KI_REQUIRE(ctx && tree && targets && mu && u && ntgt > 0 && nrhs > 0, "invalid argument");
+  KI_REQUIRE(pde_matches(tree, pde), "pde does not match the tree's solution dimension");
   return guard([&] {
     Context& c = ctx->c;
     const TreeData& t = tree->t;
@@ -555,6 +563,121 @@ kigpu_status kigpu_forward_apply(kigpu_ctx* ctx, const kigpu_tree* tree, int pde
     du.alloc((size_t)sd * ntgt * nrhs);
     forward_apply(c, t, pde, dt.p, ntgt, dm.p, nrhs, du.p);
     du.download(u, (size_t)sd * ntgt * nrhs, c.stream);
+    KI_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+kigpu_status kigpu_solve_transpose_device(kigpu_ctx* ctx, kigpu_tree* tree, const double* d_b,
+                                          int nrhs, double* d_x, const kigpu_refine_opts* refine,
+                                          kigpu_solve_info* info) {
+  KI_REQUIRE(ctx && tree && d_b && d_x, "null argument");
+  KI_REQUIRE(nrhs > 0, "nrhs must be positive");
+  if (!tree->t.factored) {
+    set_error("solve before factor");
+    return KIGPU_ERR_STATE;
+  }
+  return guard(
+      [&] { solve_tree(ctx->c, tree->t, d_b, nrhs, d_x, nullptr, refine, info, true); });
+}
+
+kigpu_status kigpu_solve_transpose(kigpu_ctx* ctx, kigpu_tree* tree, const double* b, int ldb,
+                                   int nrhs, double* x, int ldx, const kigpu_refine_opts* refine,
+                                   kigpu_solve_info* info) {
+  KI_REQUIRE(ctx && tree && b && x, "null argument");
+  KI_REQUIRE(nrhs > 0, "nrhs must be positive");
+  const int nd = tree->t.ndof;
+  KI_REQUIRE(ldb >= nd && ldx >= nd, "leading dimension smaller than the number of DOFs");
+  if (!tree->t.factored) {
+    set_error("solve before factor");
+    return KIGPU_ERR_STATE;
+  }
+  return guard([&] {
+    Context& c = ctx->c;
+    DBuf<double> db, dx;
+    db.alloc((size_t)nd * nrhs);
+    dx.alloc((size_t)nd * nrhs);
+    KI_HIP(hipMemcpy2DAsync(db.p, sizeof(double) * nd, b, sizeof(double) * ldb,
+                            sizeof(double) * nd, nrhs, hipMemcpyHostToDevice, c.stream));
+    solve_tree(c, tree->t, db.p, nrhs, dx.p, nullptr, refine, info, true);
+    KI_HIP(hipMemcpy2DAsync(x, sizeof(double) * ldx, dx.p, sizeof(double) * nd,
+                            sizeof(double) * nd, nrhs, hipMemcpyDeviceToHost, c.stream));
+    KI_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+kigpu_status kigpu_kernel_apply_transpose(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
+                                          const double* lam, int nrhs, double* y) {
+  KI_REQUIRE(ctx && tree && lam && y && nrhs > 0, "invalid argument");
+  KI_REQUIRE(pde_matches(tree, pde), "pde does not match the tree's solution dimension");
+  return guard([&] {
+    Context& c = ctx->c;
+    const size_t len = (size_t)tree->t.ndof * nrhs;
+    DBuf<double> dl, dy;
+    dl.upload(lam, len, c.stream);
+    dy.alloc(len);
+    kernel_apply_transpose(c, tree->t, pde, dl.p, nrhs, dy.p);
+    dy.download(y, len, c.stream);
+    KI_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+kigpu_status kigpu_forward_apply_transpose(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
+                                          const double* targets, int ntgt, const double* v,
+                                          int nrhs, double* y) {
+  KI_REQUIRE(ctx && tree && targets && v && y && ntgt > 0 && nrhs > 0, "invalid argument");
+  KI_REQUIRE(pde_matches(tree, pde), "pde does not match the tree's solution dimension");
+  return guard([&] {
+    Context& c = ctx->c;
+    const TreeData& t = tree->t;
+    const int sd = pde == KIGPU_STOKES ? 2 : 1;
+    DBuf<double> dt, dv, dy;
+    dt.upload(targets, 2 * (size_t)ntgt, c.stream);
+    dv.upload(v, (size_t)sd * ntgt * nrhs, c.stream);
+    dy.alloc((size_t)t.ndof * nrhs);
+    forward_apply_transpose(c, t, pde, dt.p, ntgt, dv.p, nrhs, dy.p);
+    dy.download(y, (size_t)t.ndof * nrhs, c.stream);
+    KI_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+kigpu_status kigpu_kernel_sensitivity(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
+                                      const double* lam, const double* mu, double* sens) {
+  KI_REQUIRE(ctx && tree && lam && mu && sens, "invalid argument");
+  if (pde != KIGPU_LAPLACE || tree->t.sd != 1) {
+    set_error("shape sensitivities are implemented for the Laplace double layer only");
+    return KIGPU_ERR_NOTSUPP;
+  }
+  return guard([&] {
+    Context& c = ctx->c;
+    const TreeData& t = tree->t;
+    DBuf<double> dl, dm, ds;
+    dl.upload(lam, t.npts, c.stream);
+    dm.upload(mu, t.npts, c.stream);
+    ds.alloc(6 * (size_t)t.npts);
+    kernel_sensitivity(c, t, dl.p, dm.p, ds.p);
+    ds.download(sens, 6 * (size_t)t.npts, c.stream);
+    KI_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+kigpu_status kigpu_forward_sensitivity(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
+                                       const double* targets, int ntgt, const double* v,
+                                       const double* mu, double* sens) {
+  KI_REQUIRE(ctx && tree && targets && v && mu && sens && ntgt > 0, "invalid argument");
+  if (pde != KIGPU_LAPLACE || tree->t.sd != 1) {
+    set_error("shape sensitivities are implemented for the Laplace double layer only");
+    return KIGPU_ERR_NOTSUPP;
+  }
+  return guard([&] {
+    Context& c = ctx->c;
+    const TreeData& t = tree->t;
+    DBuf<double> dt, dv, dm, ds;
+    dt.upload(targets, 2 * (size_t)ntgt, c.stream);
+    dv.upload(v, ntgt, c.stream);
+    dm.upload(mu, t.npts, c.stream);
+    ds.alloc(6 * (size_t)t.npts);
+    forward_sensitivity(c, t, dt.p, ntgt, dv.p, dm.p, ds.p);
+    ds.download(sens, 6 * (size_t)t.npts, c.stream);
     KI_HIP(hipStreamSynchronize(c.stream));
   });
 }

@@ -62,9 +62,20 @@ void border_factor(Context& c, TreeData& t);
 template <typename T>
 void border_step(Context& c, TreeData& t, T* x, int ldx, int nrhs, const double* g);
 void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_x,
-                double* d_alpha, const kigpu_refine_opts* ro, kigpu_solve_info* info);
+                double* d_alpha, const kigpu_refine_opts* ro, kigpu_solve_info* info,
+                bool trans = false);
 void kernel_apply(Context& c, const TreeData& t, int pde, const double* d_mu, int nrhs,
                   double* d_y);
+void kernel_apply_transpose(Context& c, const TreeData& t, int pde, const double* d_lam, int nrhs,
+                            double* d_y);
+void forward_apply_transpose(Context& c, const TreeData& t, int pde, const double* d_tgt, int ntgt,
+                             const double* d_v, int nrhs, double* d_y);
+void split_points(Context& c, const double* d_xy, int n, DBuf<double>* x, DBuf<double>* y);
+// shape sensitivities of the Laplace double layer (adjoint.hip)
+void kernel_sensitivity(Context& c, const TreeData& t, const double* d_lam, const double* d_mu,
+                        double* d_sens);
+void forward_sensitivity(Context& c, const TreeData& t, const double* d_tgt, int ntgt,
+                         const double* d_v, const double* d_mu, double* d_sens);
 void kernel_block(Context& c, const GeomView& g, const double* dpx, const double* dpy, int pde,
                   int sd, const int* h_tgt, int ntgt, const int* h_src, int nsrc, int forward,
                   double* h_out);

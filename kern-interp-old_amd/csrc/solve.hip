@@ -8,6 +8,12 @@
 // box's redundant DOFs are touched by no later (coarser) box, so the order of
 // the operations on each DOF is unchanged.  Then the root solve and the
 // downward pass, levels coarse to fine:  x_r -= U x_s ;  x_s -= T x_r.
+//
+// The transposed solve (adjoint, x = F^{-T} b for the same factorization F)
+// reverses the product: per box, finest level first,
+//   x_r -= T^T x_s ;  x_s -= U^T x_r ;  x_r <- X_rr^{-T} x_r
+// then allskel_mat^{-T} on the root skeleton, then coarse to fine
+//   x_r -= L^T x_s ;  x_s -= T x_r.
 #include <algorithm>
 #include <cmath>
 
@@ -26,6 +32,7 @@ constexpr int kRC = 16;  // right-hand sides per block
 // mode 0: x_r -= T^T x_s; x_s -= L x_r        (sweeps of U in skeletonize, :169-185)
 // mode 1: the same, then x_r <- X_rr^{-1} x_r   (solve's upward pass with D^{-1})
 // mode 2: x_r -= T^T x_s; x_s -= U^T x_r       (sweeps of Psi^T, :187-204)
+// mode 3: mode 2, then x_r <- X_rr^{-T} x_r     (transposed solve's upward pass)
 template <typename T>
 __global__ __launch_bounds__(kTB) void k_up(int a, const int* rank, const int* nact,
                                             const int64_t* act_off, const int* sr_pool,
@@ -57,7 +64,7 @@ __global__ __launch_bounds__(kTB) void k_up(int a, const int* rank, const int* n
     v[k + i + (size_t)c * n] = s;
   }
   __syncthreads();
-  if (mode == 2) {
+  if (mode >= 2) {
     const T* Um = Tm + 2 * (size_t)k * r;  // r x k
     for (int e = tid; e < k * nc; e += kTB) {  // x_s -= U^T x_r
       int t = e % k, c = e / k;
@@ -75,13 +82,16 @@ __global__ __launch_bounds__(kTB) void k_up(int a, const int* rank, const int* n
       v[t + (size_t)c * n] = s;
     }
   }
-  for (int e = tid; e < r * nc; e += kTB) {  // x_r <- X_rr^{-1} x_r (mode 1)
+  for (int e = tid; e < r * nc; e += kTB) {  // x_r <- X_rr^{-1} x_r (mode 1, ^{-T} mode 3)
     int i = e % r, c = e / r;
     const T* vs = v + (size_t)c * n + k;
     T s;
     if (mode == 1) {
       s = T(0);
       for (int j = 0; j < r; ++j) s += Xinv[i + (size_t)j * r] * vs[j];
+    } else if (mode == 3) {
+      s = T(0);
+      for (int j = 0; j < r; ++j) s += Xinv[j + (size_t)i * r] * vs[j];
     } else {
       s = vs[i];
     }
@@ -98,10 +108,12 @@ __global__ __launch_bounds__(kTB) void k_up(int a, const int* rank, const int* n
   }
 }
 
+// trans = 0: x_r -= U x_s; x_s -= T x_r.  trans = 1: x_r -= L^T x_s; x_s -= T x_r.
 template <typename T>
 __global__ __launch_bounds__(kTB) void k_down(int a, const int* rank, const int* nact,
                                               const int64_t* act_off, const int* sr_pool,
-                                              const uint64_t* fac_ptr, T* x, int ldx, int nrhs) {
+                                              const uint64_t* fac_ptr, T* x, int ldx, int nrhs,
+                                              int trans) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int node = a + blockIdx.x;
   const int k = rank[node];
@@ -110,6 +122,7 @@ __global__ __launch_bounds__(kTB) void k_down(int a, const int* rank, const int*
   const int c0 = blockIdx.y * kRC, nc = min(kRC, nrhs - c0);
   const int* dofs = sr_pool + act_off[node];
   const T* Tm = reinterpret_cast<const T*>(fac_ptr[node]);
+  const T* Lm = Tm + (size_t)k * r;
   const T* Um = Tm + 2 * (size_t)k * r;
   T* v = reinterpret_cast<T*>(smem);
   const int tid = threadIdx.x;
@@ -118,11 +131,14 @@ __global__ __launch_bounds__(kTB) void k_down(int a, const int* rank, const int*
     v[e] = x[dofs[i] + (size_t)(c0 + c) * ldx];
   }
   __syncthreads();
-  for (int e = tid; e < r * nc; e += kTB) {  // x_r -= U x_s
+  for (int e = tid; e < r * nc; e += kTB) {  // x_r -= U x_s  (or L^T x_s)
     int i = e % r, c = e / r;
     const T* vs = v + (size_t)c * n;
     T s = vs[k + i];
-    for (int t = 0; t < k; ++t) s -= Um[i + (size_t)t * r] * vs[t];
+    if (trans)
+      for (int t = 0; t < k; ++t) s -= Lm[t + (size_t)i * k] * vs[t];
+    else
+      for (int t = 0; t < k; ++t) s -= Um[i + (size_t)t * r] * vs[t];
     v[k + i + (size_t)c * n] = s;
   }
   __syncthreads();
@@ -144,11 +160,11 @@ __global__ __launch_bounds__(kTB) void k_down(int a, const int* rank, const int*
   }
 }
 
-// x[allskel] <- allskel_mat^{-1} x[allskel] in FP64 with the explicit inverse;
-// one block per right-hand side, one thread per row.
+// x[allskel] <- allskel_mat^{-1} x[allskel] (trans: ^{-T}) in FP64 with the
+// explicit inverse; one block per right-hand side, one thread per row.
 template <typename T>
 __global__ __launch_bounds__(kTB) void k_root_apply(const int* dofs, int n, const double* Ainv,
-                                                    T* x, int ldx) {
+                                                    T* x, int ldx, int trans) {
   extern __shared__ __align__(16) unsigned char smem[];
   double* v = reinterpret_cast<double*>(smem);
   T* xc = x + (size_t)blockIdx.x * ldx;
@@ -156,7 +172,10 @@ __global__ __launch_bounds__(kTB) void k_root_apply(const int* dofs, int n, cons
   __syncthreads();
   for (int i = threadIdx.x; i < n; i += kTB) {
     double s = 0;
-    for (int j = 0; j < n; ++j) s += Ainv[i + (size_t)j * n] * v[j];
+    if (trans)
+      for (int j = 0; j < n; ++j) s += Ainv[j + (size_t)i * n] * v[j];
+    else
+      for (int j = 0; j < n; ++j) s += Ainv[i + (size_t)j * n] * v[j];
     xc[dofs[i]] = (T)s;
   }
 }
@@ -248,6 +267,81 @@ __global__ __launch_bounds__(kAT) void k_apply(Targets tg, GeomView g, const dou
   }
 }
 
+// Transposed apply: y_part[split](i) = sum over this split's rows j of
+// K(j, i) lam(j), one thread per column (source point) i.  Rows are the
+// boundary points (K^T) or domain targets without normals (forward^T).
+template <int PDE, bool FWD>
+__global__ __launch_bounds__(kAT) void k_apply_t(Targets rows, GeomView g, const double* lam,
+                                                 int ldl, int nrhs, double* part, int ldp,
+                                                 int rows_per_split) {
+#pragma clang fp contract(off)
+  __shared__ double sx[kAT], sy[kAT], snx[kAT], sny[kAT];
+  __shared__ double slam[kAT * 2 * kAR];
+  constexpr int SD = PDE == KIGPU_STOKES ? 2 : 1;
+  const int si = blockIdx.x * kAT + threadIdx.x;
+  const bool valid = si < g.npts;
+  const double ox = valid ? g.px[si] : 0, oy = valid ? g.py[si] : 0;
+  const double onx = valid ? g.nx[si] : 0, ony = valid ? g.ny[si] : 0;
+  const double ow = valid ? g.w[si] : 0, okap = valid ? g.kappa[si] : 0;
+  const int j0 = blockIdx.y * rows_per_split;
+  const int j1 = min(rows.n, j0 + rows_per_split);
+  for (int c0 = 0; c0 < nrhs; c0 += kAR) {
+    const int nc = min(kAR, nrhs - c0);
+    double acc[kAR][SD];
+#pragma unroll
+    for (int c = 0; c < kAR; ++c)
+#pragma unroll
+      for (int a = 0; a < SD; ++a) acc[c][a] = 0;
+    for (int b0 = j0; b0 < j1; b0 += kAT) {
+      const int nb = min(kAT, j1 - b0);
+      __syncthreads();
+      if (threadIdx.x < nb) {
+        int rp = b0 + threadIdx.x;
+        sx[threadIdx.x] = rows.x[rp];
+        sy[threadIdx.x] = rows.y[rp];
+        snx[threadIdx.x] = rows.nx ? rows.nx[rp] : 0.0;
+        sny[threadIdx.x] = rows.ny ? rows.ny[rp] : 0.0;
+        for (int c = 0; c < nc; ++c)
+          for (int b = 0; b < SD; ++b)
+            slam[(threadIdx.x * SD + b) * kAR + c] = lam[SD * rp + b + (size_t)(c0 + c) * ldl];
+      }
+      __syncthreads();
+      if (!valid) continue;
+      for (int j = 0; j < nb; ++j) {
+        const double r1 = sx[j] - ox, r2 = sy[j] - oy;  // target (row) - source (own)
+        if (PDE != KIGPU_STOKES) {
+          double kv;
+          if (FWD && PDE == KIGPU_LAPLACE_NEUMANN) {
+            kv = ow * kInv2Pi * log(sqrt(r1 * r1 + r2 * r2));
+          } else {
+            kv = laplace_entry(PDE, r1, r2, ow, okap, onx, ony, snx[j], sny[j], FWD);
+          }
+#pragma unroll
+          for (int c = 0; c < kAR; ++c)
+            if (c < nc) acc[c][0] += kv * slam[j * kAR + c];
+        } else {
+#pragma unroll
+          for (int ta = 0; ta < SD; ++ta)
+#pragma unroll
+            for (int sb = 0; sb < SD; ++sb) {
+              double kv = stokes_entry(ta, sb, sx[j], sy[j], snx[j], sny[j], ox, oy, onx, ony, ow,
+                                       okap);
+#pragma unroll
+              for (int c = 0; c < kAR; ++c)
+                if (c < nc) acc[c][sb] += kv * slam[(j * SD + ta) * kAR + c];
+            }
+        }
+      }
+    }
+    if (valid) {
+      for (int c = 0; c < nc; ++c)
+        for (int a = 0; a < SD; ++a)
+          part[(size_t)blockIdx.y * ldp * nrhs + SD * si + a + (size_t)(c0 + c) * ldp] =
+              acc[c][a];
+    }
+  }
+}
+
 __global__ void k_sum_parts(const double* part, int nsplit, size_t len, double* y) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= len) return;
@@ -319,9 +413,9 @@ void up_sweeps(Context& c, TreeData& t, T* x, int ldx, int nrhs, int mode) {
 template void up_sweeps<double>(Context&, TreeData&, double*, int, int, int);
 template void up_sweeps<float>(Context&, TreeData&, float*, int, int, int);
 
-// downward pass, coarsest first
+// downward pass, coarsest first (trans: with L^T in place of U)
 template <typename T>
-void down_sweeps(Context& c, TreeData& t, T* x, int ldx, int nrhs) {
+void down_sweeps(Context& c, TreeData& t, T* x, int ldx, int nrhs, int trans = 0) {
   hipStream_t s = c.stream;
   const int ncb = ceil_div(nrhs, kRC);
   const int rcol = std::min(nrhs, kRC);
@@ -331,7 +425,8 @@ void down_sweeps(Context& c, TreeData& t, T* x, int ldx, int nrhs) {
     size_t lds = sizeof(T) * (size_t)w.nmax * rcol;
     allow_lds(k_down<T>, lds);
     k_down<T><<<dim3(w.b - w.a, ncb), kTB, lds, s>>>(w.a, t.rank.p, t.nact.p, t.act_off.p,
-                                                     t.sr_pool.p, t.fac_ptr.p, x, ldx, nrhs);
+                                                     t.sr_pool.p, t.fac_ptr.p, x, ldx, nrhs,
+                                                     trans);
   }
   KI_HIP(hipGetLastError());
 }
@@ -364,17 +459,19 @@ __global__ void k_axpy_f64(double* x, const double* d, size_t n) {
   if (i < n) x[i] += d[i];
 }
 
+// x <- F^{-1} x, or F^{-T} x with trans (no border then)
 template <typename T>
-void sweeps(Context& c, TreeData& t, T* x, int ldx, int nrhs, const double* g = nullptr) {
+void sweeps(Context& c, TreeData& t, T* x, int ldx, int nrhs, const double* g = nullptr,
+            bool trans = false) {
   hipStream_t s = c.stream;
-  up_sweeps<T>(c, t, x, ldx, nrhs, 1);
+  up_sweeps<T>(c, t, x, ldx, nrhs, trans ? 3 : 1);
   if (t.nborder > 0) {
     border_step<T>(c, t, x, ldx, nrhs, g);
   } else if (t.nroot > 0) {
     k_root_apply<T><<<nrhs, kTB, sizeof(double) * t.nroot, s>>>(t.root_dofs.p, t.nroot,
-                                                                t.root_lu.p, x, ldx);
+                                                                t.root_lu.p, x, ldx, trans);
   }
-  down_sweeps<T>(c, t, x, ldx, nrhs);
+  down_sweeps<T>(c, t, x, ldx, nrhs, trans);
 }
 
 }  // namespace
@@ -405,22 +502,30 @@ void kernel_apply(Context& c, const TreeData& t, int pde, const double* d_mu, in
   KI_HIP(hipStreamSynchronize(s));
 }
 
+namespace {
+__global__ void k_split_xy(const double* xy, int n, double* x, double* y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    x[i] = xy[2 * i];
+    y[i] = xy[2 * i + 1];
+  }
+}
+}  // namespace
+
+// interleaved [x0, y0, x1, ...] device points -> separate x and y arrays
+void split_points(Context& c, const double* d_xy, int n, DBuf<double>* x, DBuf<double>* y) {
+  x->alloc(n);
+  y->alloc(n);
+  if (n > 0) k_split_xy<<<ceil_div(n, 256), 256, 0, c.stream>>>(d_xy, n, x->p, y->p);
+  KI_HIP(hipGetLastError());
+}
+
 void forward_apply(Context& c, const TreeData& t, int pde, const double* d_tgt, int ntgt,
                    const double* d_mu, int nrhs, double* d_u) {
   hipStream_t s = c.stream;
   const GeomView g{t.px.p, t.py.p, t.nx.p, t.ny.p, t.w.p, t.kappa.p, t.npts, t.sd};
   DBuf<double> tx, ty;
-  tx.alloc(ntgt);
-  ty.alloc(ntgt);
-  std::vector<double> h(2 * (size_t)ntgt), hx(ntgt), hy(ntgt);
-  KI_HIP(hipMemcpyAsync(h.data(), d_tgt, sizeof(double) * 2 * ntgt, hipMemcpyDeviceToHost, s));
-  KI_HIP(hipStreamSynchronize(s));
-  for (int i = 0; i < ntgt; ++i) {
-    hx[i] = h[2 * i];
-    hy[i] = h[2 * i + 1];
-  }
-  tx.upload(hx, s);
-  ty.upload(hy, s);
+  split_points(c, d_tgt, ntgt, &tx, &ty);
   Targets tg{tx.p, ty.p, nullptr, nullptr, ntgt};
   const int sd = pde == KIGPU_STOKES ? 2 : 1;
   const int tb = ceil_div(ntgt, kAT);
@@ -445,6 +550,52 @@ void forward_apply(Context& c, const TreeData& t, int pde, const double* d_tgt, 
   KI_HIP(hipStreamSynchronize(s));
 }
 
+// y = K(all, all)^T lam, or forward^T v with domain targets (fwd_tgt != null)
+static void apply_transpose(Context& c, const TreeData& t, int pde, const Targets& rows, bool fwd,
+                            const double* d_lam, int ldl, int nrhs, double* d_y) {
+  hipStream_t s = c.stream;
+  const GeomView g{t.px.p, t.py.p, t.nx.p, t.ny.p, t.w.p, t.kappa.p, t.npts, t.sd};
+  const int tb = ceil_div(t.npts, kAT);
+  int split = std::max(1, std::min(256, ceil_div(2048, tb)));
+  int per = ceil_div(rows.n, split);
+  per = ceil_div(per, kAT) * kAT;
+  split = ceil_div(rows.n, per);
+  const size_t len = (size_t)t.ndof * nrhs;
+  DBuf<double> part;
+  part.alloc(len * split);
+  dim3 grid(tb, split);
+#define KI_APPLY_T(P, F) \
+  k_apply_t<P, F><<<grid, kAT, 0, s>>>(rows, g, d_lam, ldl, nrhs, part.p, t.ndof, per)
+  if (pde == KIGPU_LAPLACE) {
+    if (fwd) KI_APPLY_T(KIGPU_LAPLACE, true); else KI_APPLY_T(KIGPU_LAPLACE, false);
+  } else if (pde == KIGPU_LAPLACE_NEUMANN) {
+    if (fwd) KI_APPLY_T(KIGPU_LAPLACE_NEUMANN, true); else KI_APPLY_T(KIGPU_LAPLACE_NEUMANN, false);
+  } else {
+    if (fwd) KI_APPLY_T(KIGPU_STOKES, true); else KI_APPLY_T(KIGPU_STOKES, false);
+  }
+#undef KI_APPLY_T
+  k_sum_parts<<<ceil_div((long)len, 256), 256, 0, s>>>(part.p, split, len, d_y);
+  KI_HIP(hipGetLastError());
+  KI_HIP(hipStreamSynchronize(s));
+}
+
+void kernel_apply_transpose(Context& c, const TreeData& t, int pde, const double* d_lam, int nrhs,
+                            double* d_y) {
+  Targets rows{t.px.p, t.py.p, t.nx.p, t.ny.p, t.npts};
+  apply_transpose(c, t, pde, rows, false, d_lam, t.ndof, nrhs, d_y);
+}
+
+void forward_apply_transpose(Context& c, const TreeData& t, int pde, const double* d_tgt, int ntgt,
+                             const double* d_v, int nrhs, double* d_y) {
+  hipStream_t s = c.stream;
+  DBuf<double> tx, ty;
+  split_points(c, d_tgt, ntgt, &tx, &ty);
+  Targets rows{tx.p, ty.p, nullptr, nullptr, ntgt};
+  const int sd = pde == KIGPU_STOKES ? 2 : 1;
+  apply_transpose(c, t, pde, rows, true, d_v, sd * ntgt, nrhs, d_y);
+  KI_HIP(hipStreamSynchronize(s));
+}
+
 // per-column sums of r^2 and f^2 with r = f - y written out
 static void residual_sums(Context& c, const double* f, const double* y, double* r, int n, int nrhs,
                           std::vector<double>* rr, std::vector<double>* ff) {
@@ -465,8 +616,10 @@ static void residual_sums(Context& c, const double* f, const double* y, double* 
 }
 
 void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_x, double* d_alpha,
-                const kigpu_refine_opts* ro, kigpu_solve_info* info) {
+                const kigpu_refine_opts* ro, kigpu_solve_info* info, bool trans) {
   hipStream_t s = c.stream;
+  if (trans && t.nborder > 0)
+    KI_FAIL(KIGPU_ERR_NOTSUPP, "transposed solve of a bordered (multiply connected) system");
   const size_t len = (size_t)t.ndof * nrhs;
   const int h = t.nborder;
   const size_t alen = (size_t)h * nrhs;
@@ -478,7 +631,7 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
   if (t.opts.precision == KIGPU_FP64) {
     if (d_x != d_b)
       KI_HIP(hipMemcpyAsync(d_x, d_b, sizeof(double) * len, hipMemcpyDeviceToDevice, s));
-    sweeps<double>(c, t, d_x, t.ndof, nrhs);
+    sweeps<double>(c, t, d_x, t.ndof, nrhs, nullptr, trans);
     if (h && d_alpha)
       KI_HIP(hipMemcpyAsync(d_alpha, t.alpha_dev.p, sizeof(double) * alen,
                             hipMemcpyDeviceToDevice, s));
@@ -495,7 +648,7 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
   t.x32.alloc(len);
   const int g = ceil_div((long)len, 256);
   k_convert<double, float><<<g, 256, 0, s>>>(d_b, t.x32.p, len);
-  sweeps<float>(c, t, t.x32.p, t.ndof, nrhs);
+  sweeps<float>(c, t, t.x32.p, t.ndof, nrhs, nullptr, trans);
   k_convert<float, double><<<g, 256, 0, s>>>(t.x32.p, d_x, len);
   if (h) {
     t.alpha64.alloc(alen);
@@ -516,7 +669,8 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
     std::vector<double> rr, ff, hb(alen);
     for (int it = 0; it <= iters; ++it) {
       KI_HIP(hipEventRecord(c.ev[8], s));
-      kernel_apply(c, t, t.opts.pde, d_x, nrhs, t.f64.p);
+      if (trans) kernel_apply_transpose(c, t, t.opts.pde, d_x, nrhs, t.f64.p);
+      else kernel_apply(c, t, t.opts.pde, d_x, nrhs, t.f64.p);
       if (h) {
         k_add_ualpha<<<dim3(ceil_div(t.ndof, 256), nrhs), 256, 0, s>>>(t.ndof, h, t.bU64in.p,
                                                                       t.alpha64.p, t.f64.p);
@@ -544,7 +698,7 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
       if (rel <= ro->rel_residual || it == iters) break;
       KI_HIP(hipEventRecord(c.ev[8], s));
       k_convert<double, float><<<g, 256, 0, s>>>(t.r64.p, t.x32.p, len);
-      sweeps<float>(c, t, t.x32.p, t.ndof, nrhs, h ? t.rb64.p : nullptr);
+      sweeps<float>(c, t, t.x32.p, t.ndof, nrhs, h ? t.rb64.p : nullptr, trans);
       k_axpy_f32<<<g, 256, 0, s>>>(d_x, t.x32.p, len);
       if (h) k_axpy_f64<<<ceil_div((long)alen, 256), 256, 0, s>>>(t.alpha64.p, t.alpha_dev.p, alen);
       KI_HIP(hipEventRecord(c.ev[9], s));

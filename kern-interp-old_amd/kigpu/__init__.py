@@ -58,6 +58,8 @@ EXPORTS = [
     "kigpu_factor_allskel", "kigpu_solve", "kigpu_solve_device", "kigpu_kernel_apply",
     "kigpu_kernel_apply_device", "kigpu_forward_apply", "kigpu_geometry_create",
     "kigpu_geometry_destroy", "kigpu_kernel_block", "kigpu_factor_export_active",
+    "kigpu_solve_transpose", "kigpu_solve_transpose_device", "kigpu_kernel_apply_transpose",
+    "kigpu_forward_apply_transpose", "kigpu_kernel_sensitivity", "kigpu_forward_sensitivity",
 ]
 
 _lib = None
@@ -100,6 +102,16 @@ def load() -> C.CDLL:
         "kigpu_kernel_apply": (C.c_int, [P, P, C.c_int, _f64p, C.c_int, _f64p]),
         "kigpu_kernel_apply_device": (C.c_int, [P, P, C.c_int, P, C.c_int, P]),
         "kigpu_forward_apply": (C.c_int, [P, P, C.c_int, _f64p, C.c_int, _f64p, C.c_int, _f64p]),
+        "kigpu_solve_transpose": (C.c_int, [P, P, _f64p, C.c_int, C.c_int, _f64p, C.c_int,
+                                            C.POINTER(RefineOpts), C.POINTER(SolveInfo)]),
+        "kigpu_solve_transpose_device": (C.c_int, [P, P, P, C.c_int, P, C.POINTER(RefineOpts),
+                                                   C.POINTER(SolveInfo)]),
+        "kigpu_kernel_apply_transpose": (C.c_int, [P, P, C.c_int, _f64p, C.c_int, _f64p]),
+        "kigpu_forward_apply_transpose": (C.c_int, [P, P, C.c_int, _f64p, C.c_int, _f64p,
+                                                    C.c_int, _f64p]),
+        "kigpu_kernel_sensitivity": (C.c_int, [P, P, C.c_int, _f64p, _f64p, _f64p]),
+        "kigpu_forward_sensitivity": (C.c_int, [P, P, C.c_int, _f64p, C.c_int, _f64p, _f64p,
+                                                _f64p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -282,6 +294,68 @@ class Tree:
             A = alpha[: h * nrhs].reshape(nrhs, h).T
             self.last_alpha = A[:, 0].copy() if one else np.array(A)
         return mu
+
+    def solve_transpose(self, b, refine_iters=0, rel_residual=1e-10):
+        """x = F^{-T} b (adjoint solve with the same factorization)."""
+        b = np.asarray(b, np.float64)
+        one = b.ndim == 1
+        B = b.reshape(self.ndof, -1)
+        nrhs = B.shape[1]
+        bflat = np.ascontiguousarray(B.T).reshape(-1)
+        x = np.zeros_like(bflat)
+        ro = RefineOpts(refine_iters, rel_residual)
+        info = SolveInfo()
+        _check(self.lib.kigpu_solve_transpose(self.ctx.h, self.h, bflat, self.ndof, nrhs, x,
+                                              self.ndof, C.byref(ro), C.byref(info)),
+               "kigpu_solve_transpose")
+        self.last_info = {"iterations": info.iterations, "rel_residual": info.rel_residual}
+        X = x.reshape(nrhs, self.ndof).T
+        return X[:, 0].copy() if one else np.array(X)
+
+    def apply_transpose(self, lam, pde=None):
+        pde = self.pde if pde is None else pde
+        lam = np.asarray(lam, np.float64)
+        one = lam.ndim == 1
+        M = lam.reshape(self.ndof, -1)
+        nrhs = M.shape[1]
+        flat = np.ascontiguousarray(M.T).reshape(-1)
+        y = np.zeros_like(flat)
+        _check(self.lib.kigpu_kernel_apply_transpose(self.ctx.h, self.h, pde, flat, nrhs, y),
+               "kigpu_kernel_apply_transpose")
+        Y = y.reshape(nrhs, self.ndof).T
+        return Y[:, 0].copy() if one else np.array(Y)
+
+    def forward_transpose(self, targets, v, pde=None):
+        pde = self.pde if pde is None else pde
+        t = _f64(targets).reshape(-1)
+        ntgt = t.size // 2
+        sd = 2 if pde == STOKES else 1
+        v = np.asarray(v, np.float64)
+        one = v.ndim == 1
+        V = v.reshape(sd * ntgt, -1)
+        nrhs = V.shape[1]
+        flat = np.ascontiguousarray(V.T).reshape(-1)
+        y = np.zeros(self.ndof * nrhs)
+        _check(self.lib.kigpu_forward_apply_transpose(self.ctx.h, self.h, pde, t, ntgt, flat,
+                                                      nrhs, y), "kigpu_forward_apply_transpose")
+        Y = y.reshape(nrhs, self.ndof).T
+        return Y[:, 0].copy() if one else np.array(Y)
+
+    def kernel_sensitivity(self, lam, mu):
+        """(npts, 6): d(lam^T K mu)/d(x, y, nx, ny, w, kappa) per boundary point."""
+        out = np.zeros(6 * self.ndof)
+        _check(self.lib.kigpu_kernel_sensitivity(self.ctx.h, self.h, LAPLACE, _f64(lam),
+                                                 _f64(mu), out), "kigpu_kernel_sensitivity")
+        return out.reshape(-1, 6)
+
+    def forward_sensitivity(self, targets, v, mu):
+        """(npts, 6): d(v^T F mu)/d(x, y, nx, ny, w, kappa) per boundary point."""
+        t = _f64(targets).reshape(-1)
+        out = np.zeros(6 * self.ndof)
+        _check(self.lib.kigpu_forward_sensitivity(self.ctx.h, self.h, LAPLACE, t, t.size // 2,
+                                                  _f64(v), _f64(mu), out),
+               "kigpu_forward_sensitivity")
+        return out.reshape(-1, 6)
 
     def apply(self, mu, pde=None):
         pde = self.pde if pde is None else pde

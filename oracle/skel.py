@@ -229,6 +229,42 @@ class SkelFactorization:
                 sweep(-nd.T, x, nd.redundant, nd.skel, False)
         return x[:, 0] if squeeze else x
 
+    def solve_transpose(self, tree, b):
+        """x = F^{-T} b for the factorization F that :meth:`solve` inverts.
+
+        No reference counterpart (the reference only solves with F).  The
+        sweeps of ``solve`` (skel_factorization.cpp:341-403) transposed and
+        taken in reverse order: the down pass's steps, finest level first,
+        as x_r -= T^T x_s, x_s -= U^T x_r; then X_rr^{-T} and
+        allskel_mat^{-T} (``dgetrs`` with 'T'); then the up pass's steps,
+        coarsest level first, as x_r -= L^T x_s, x_s -= T x_r.  Pinned by the
+        adjoint identity <F^{-T} g, b> = <g, F^{-1} b> and, at tight id_tol,
+        by the dense K^{-T} g.
+        """
+        x = np.array(b, dtype=np.float64, copy=True)
+        squeeze = x.ndim == 1
+        if squeeze:
+            x = x[:, None]
+        for nodes in self._compressed_by_level(tree):
+            for nd in nodes:
+                sweep(-nd.T, x, nd.skel, nd.redundant, True)
+                sweep(-nd.U, x, nd.redundant, nd.skel, True)
+        for nodes in self._compressed_by_level(tree):
+            for nd in nodes:
+                if nd.redundant:
+                    x[nd.redundant, :] = lu_solve(nd.X_lu, nd.X_piv, x[nd.redundant, :],
+                                                  trans=1)
+        if tree.allskel:
+            x[tree.allskel, :] = lu_solve(tree.allskel_lu, tree.allskel_piv,
+                                          x[tree.allskel, :], trans=1)
+        for level in range(len(tree.levels)):
+            for nd in tree.levels[level]:
+                if not nd.compressed:
+                    continue
+                sweep(-nd.L, x, nd.skel, nd.redundant, True)
+                sweep(-nd.T, x, nd.redundant, nd.skel, False)
+        return x[:, 0] if squeeze else x
+
     def multiply_connected_solve(self, tree, b):
         """``multiply_connected_solve`` (skel_factorization.cpp:406-551). Returns (mu, alpha)."""
         allskel = tree.allskel
