@@ -113,6 +113,26 @@ kigpu_status kigpu_tree_build_device(kigpu_ctx* ctx, const double* d_px, const d
                                      const double* d_kappa, int npts, int sd, int leaf_dofs,
                                      kigpu_tree** tree);
 kigpu_status kigpu_tree_destroy(kigpu_tree* tree);
+/* A batch of independent boundaries (instance i: npts[i] points, stored one
+ * after another in the arrays, interleaved like kigpu_tree_build) built as
+ * one forest.  No counterpart in the reference, which would loop over
+ * QuadTree::initialize_tree (quadtree.cpp:72-114); each instance's quadtree
+ * is the one kigpu_tree_build gives it.  kigpu_factor then factors every
+ * instance with one launch per tree level over the boxes of all instances,
+ * kigpu_solve / kigpu_solve_transpose take the instances' right-hand sides
+ * stacked (sum of sd*npts rows), kigpu_kernel_apply is block diagonal and
+ * FP32 refinement stops when every instance meets rel_residual.  Holes,
+ * domain targets and sensitivities need a single instance
+ * (KIGPU_ERR_NOTSUPP). */
+kigpu_status kigpu_tree_build_batch(kigpu_ctx* ctx, int ninst, const int* npts,
+                                    const double* points, const double* normals,
+                                    const double* weights, const double* curvatures, int sd,
+                                    int leaf_dofs, kigpu_tree** tree);
+/* ninst; point_off[ninst + 1]; bounds[2 * ninst] = root-box min/max
+ * (QuadTree::min/max per instance); root_skel[ninst] = |allskel| per
+ * instance after kigpu_factor (-1 before).  Output pointers may be NULL. */
+kigpu_status kigpu_tree_instances(const kigpu_tree* tree, int* ninst, int64_t* point_off,
+                                  double* bounds, int* root_skel);
 kigpu_status kigpu_tree_get_info(const kigpu_tree* tree, kigpu_tree_info* info);
 
 /* Node arrays in level order (node id = position in levels[0], levels[1], ...):
@@ -152,7 +172,8 @@ kigpu_status kigpu_factor_export_lists(const kigpu_tree* tree, int64_t* skel_off
  * NULL when perm is NULL. */
 kigpu_status kigpu_factor_export_active(const kigpu_tree* tree, int64_t* act_off, int* active,
                                         int64_t* perm_off, int* perm);
-/* the root skeleton QuadTree::root->dof_lists.active_box after factoring */
+/* the root skeleton QuadTree::root->dof_lists.active_box after factoring
+ * (for a batch: every instance's, one after another) */
 kigpu_status kigpu_factor_allskel(const kigpu_tree* tree, int* n, int* allskel);
 
 /* ---- solve ------------------------------------------------------------ */

@@ -171,11 +171,13 @@ kigpu_status kigpu_last_timings(kigpu_ctx* ctx, kigpu_timings* out) {
 
 static kigpu_status tree_build_common(kigpu_ctx* ctx, int npts, int sd, int leaf_dofs,
                                       kigpu_tree** out,
-                                      const std::function<void(TreeData&, hipStream_t)>& load) {
+                                      const std::function<void(TreeData&, hipStream_t)>& load,
+                                      const std::vector<int64_t>* inst_pt = nullptr) {
   KI_REQUIRE(ctx && out, "null argument");
   KI_REQUIRE(npts > 0, "number of boundary points to init tree cannot be 0");
   KI_REQUIRE(sd == 1 || sd == 2, "solution dimension must be 1 or 2");
   KI_REQUIRE(leaf_dofs >= sd, "leaf_dofs must hold at least one point");
+  KI_REQUIRE((int64_t)npts * sd < (1ll << 31), "too many DOFs for 32-bit indices");
   *out = nullptr;
   auto tr = std::make_unique<kigpu_tree>();
   Context& c = ctx->c;
@@ -185,6 +187,13 @@ static kigpu_status tree_build_common(kigpu_ctx* ctx, int npts, int sd, int leaf
     t.sd = sd;
     t.leaf_dofs = leaf_dofs;
     t.ndof = npts * sd;
+    if (inst_pt) {
+      t.inst_pt = *inst_pt;
+      t.ninst = (int)inst_pt->size() - 1;
+    } else {
+      t.inst_pt = {0, (int64_t)npts};
+      t.ninst = 1;
+    }
     KI_HIP(hipEventRecord(c.ev[10], c.stream));
     load(t, c.stream);
     c.tim.syncs = 0;
@@ -235,6 +244,57 @@ kigpu_status kigpu_tree_build_device(kigpu_ctx* ctx, const double* d_px, const d
       KI_HIP(hipMemcpyAsync(dst[i]->p, src[i], b, hipMemcpyDeviceToDevice, s));
     }
   });
+}
+
+kigpu_status kigpu_tree_build_batch(kigpu_ctx* ctx, int ninst, const int* npts,
+                                    const double* points, const double* normals,
+                                    const double* weights, const double* curvatures, int sd,
+                                    int leaf_dofs, kigpu_tree** out) {
+  KI_REQUIRE(ninst > 0 && npts, "invalid instance count");
+  KI_REQUIRE(points && normals && weights && curvatures, "null geometry array");
+  std::vector<int64_t> off(ninst + 1, 0);
+  for (int i = 0; i < ninst; ++i) {
+    KI_REQUIRE(npts[i] > 0, "number of boundary points to init tree cannot be 0");
+    off[i + 1] = off[i] + npts[i];
+  }
+  KI_REQUIRE(off[ninst] < (1ll << 30), "too many points in the batch");
+  const int total = (int)off[ninst];
+  return tree_build_common(
+      ctx, total, sd, leaf_dofs, out,
+      [&](TreeData& t, hipStream_t s) {
+        std::vector<double> a(total), b(total), na(total), nb(total);
+        for (int i = 0; i < total; ++i) {
+          a[i] = points[2 * i];
+          b[i] = points[2 * i + 1];
+          na[i] = normals[2 * i];
+          nb[i] = normals[2 * i + 1];
+        }
+        t.px.upload(a, s);
+        t.py.upload(b, s);
+        t.nx.upload(na, s);
+        t.ny.upload(nb, s);
+        t.w.upload(weights, total, s);
+        t.kappa.upload(curvatures, total, s);
+        KI_HIP(hipStreamSynchronize(s));
+      },
+      &off);
+}
+
+kigpu_status kigpu_tree_instances(const kigpu_tree* tree, int* ninst, int64_t* point_off,
+                                  double* bounds, int* root_skel) {
+  KI_REQUIRE(tree && ninst, "null argument");
+  const TreeData& t = tree->t;
+  *ninst = t.ninst;
+  for (int i = 0; i < t.ninst; ++i) {
+    if (point_off) point_off[i] = t.inst_pt[i];
+    if (bounds) {
+      bounds[2 * i] = t.inst_min[i];
+      bounds[2 * i + 1] = t.inst_max[i];
+    }
+    if (root_skel) root_skel[i] = t.factored ? t.h_root_n[i] : -1;
+  }
+  if (point_off) point_off[t.ninst] = t.inst_pt[t.ninst];
+  return KIGPU_OK;
 }
 
 kigpu_status kigpu_tree_destroy(kigpu_tree* tree) {
@@ -313,6 +373,10 @@ kigpu_status kigpu_set_border(kigpu_tree* tree, const double* U, const double* P
   KI_REQUIRE(tree, "null tree");
   KI_REQUIRE(nborder >= 0, "negative border size");
   TreeData& t = tree->t;
+  if (t.ninst > 1 && nborder > 0) {
+    set_error("holes are not supported for a batch of instances");
+    return KIGPU_ERR_NOTSUPP;
+  }
   if (nborder == 0) {
     t.nborder = 0;
     t.h_U.clear();
@@ -553,6 +617,10 @@ kigpu_status kigpu_forward_apply(kigpu_ctx* ctx, const kigpu_tree* tree, int pde
                                  double* u) {
   KI_REQUIRE(ctx && tree && targets && mu && u && ntgt > 0 && nrhs > 0, "invalid argument");
   KI_REQUIRE(pde_matches(tree, pde), "pde does not match the tree's solution dimension");
+  if (tree->t.ninst > 1) {
+    set_error("domain targets and sensitivities need a single instance");
+    return KIGPU_ERR_NOTSUPP;
+  }
   return guard([&] {
     Context& c = ctx->c;
     const TreeData& t = tree->t;
@@ -626,6 +694,10 @@ kigpu_status kigpu_forward_apply_transpose(kigpu_ctx* ctx, const kigpu_tree* tre
                                           int nrhs, double* y) {
   KI_REQUIRE(ctx && tree && targets && v && y && ntgt > 0 && nrhs > 0, "invalid argument");
   KI_REQUIRE(pde_matches(tree, pde), "pde does not match the tree's solution dimension");
+  if (tree->t.ninst > 1) {
+    set_error("domain targets and sensitivities need a single instance");
+    return KIGPU_ERR_NOTSUPP;
+  }
   return guard([&] {
     Context& c = ctx->c;
     const TreeData& t = tree->t;
@@ -647,6 +719,10 @@ kigpu_status kigpu_kernel_sensitivity(kigpu_ctx* ctx, const kigpu_tree* tree, in
     set_error("shape sensitivities are implemented for the Laplace double layer only");
     return KIGPU_ERR_NOTSUPP;
   }
+  if (tree->t.ninst > 1) {
+    set_error("domain targets and sensitivities need a single instance");
+    return KIGPU_ERR_NOTSUPP;
+  }
   return guard([&] {
     Context& c = ctx->c;
     const TreeData& t = tree->t;
@@ -666,6 +742,10 @@ kigpu_status kigpu_forward_sensitivity(kigpu_ctx* ctx, const kigpu_tree* tree, i
   KI_REQUIRE(ctx && tree && targets && v && mu && sens && ntgt > 0, "invalid argument");
   if (pde != KIGPU_LAPLACE || tree->t.sd != 1) {
     set_error("shape sensitivities are implemented for the Laplace double layer only");
+    return KIGPU_ERR_NOTSUPP;
+  }
+  if (tree->t.ninst > 1) {
+    set_error("domain targets and sensitivities need a single instance");
     return KIGPU_ERR_NOTSUPP;
   }
   return guard([&] {

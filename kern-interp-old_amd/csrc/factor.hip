@@ -157,22 +157,24 @@ __global__ void k_inner_rows(int a, int b, int min_dofs, GeomView g, const int* 
 
 // outside rows for levels 1 and 2 (kernel.cpp:164-207): every other node's
 // active_box at this level, then (level 2) the level-1 leaves' original_box.
+// With several instances only the box's own instance counts.
 __global__ void k_outside_rows(const IdWork* work, int a, int b, int level, int l1a, int l1b,
                                const int* is_leaf, const int64_t* obox_off, const int* obox,
                                const int* pcount, int sd, const int64_t* act_off, const int* nact,
-                               const int* act_pool, int* rows_pool) {
+                               const int* act_pool, const int* inst, int* rows_pool) {
   const IdWork w = work[blockIdx.x];
+  const int me = inst[w.node];
   int* out = rows_pool + w.rows_off;
   int o = 0;
   for (int y = a; y < b; ++y) {
-    if (y == w.node) continue;
+    if (y == w.node || inst[y] != me) continue;
     const int* src = act_pool + act_off[y];
     for (int j = threadIdx.x; j < nact[y]; j += blockDim.x) out[o + j] = src[j];
     o += nact[y];
   }
   if (level == 2) {
     for (int y = l1a; y < l1b; ++y) {
-      if (!is_leaf[y]) continue;
+      if (!is_leaf[y] || inst[y] != me) continue;
       const int* src = obox + obox_off[y];
       int n = pcount[y] * sd;
       for (int j = threadIdx.x; j < n; j += blockDim.x) out[o + j] = src[j];
@@ -689,14 +691,24 @@ __global__ __launch_bounds__(kDecTB) void k_decouple(int a, GeomView g, int pde,
 // ---------------------------------------------------------------------------
 // root: allskel_mat = K(allskel, allskel) - updates, FP64 LU (skel_factorization.cpp:116-128)
 // ---------------------------------------------------------------------------
+// One block per instance; the root of instance b is node b.
 template <typename T, bool LDS>
-__global__ __launch_bounds__(1024) void k_root(int x, GeomView g, int pde, const int* child,
+__global__ __launch_bounds__(1024) void k_root(GeomView g, int pde, const int* child,
                                                const int* osize, const uint64_t* acc_ptr,
-                                               const int* acc_ld, const int* dofs, int n,
-                                               double* Aws, double* Ainv, int* piv_out,
-                                               double* Amat, int* status) {
+                                               const int* acc_ld, const int* root_n,
+                                               const int64_t* root_doff, const int64_t* root_moff,
+                                               const int* dofs_all, double* Aws_all,
+                                               double* Ainv_all, int* piv_all, double* Amat_all,
+                                               int* status) {
   extern __shared__ __align__(16) unsigned char smem[];
-  double* A = LDS ? reinterpret_cast<double*>(smem) : Aws;
+  const int x = blockIdx.x;
+  const int n = root_n[x];
+  if (n == 0) return;
+  const int* dofs = dofs_all + root_doff[x];
+  double* Ainv = Ainv_all + root_moff[x];
+  double* Amat = Amat_all + root_moff[x];
+  int* piv_out = piv_all + root_doff[x];
+  double* A = LDS ? reinterpret_cast<double*>(smem) : Aws_all + root_moff[x];
   __shared__ int piv[1024];
   const int tid = threadIdx.x, nt = blockDim.x;
   for2d<16>(n, n, [&](int i, int j) { A[i + (size_t)j * n] = kernel_entry(g, pde, dofs[i], dofs[j]); });
@@ -838,12 +850,18 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     int64_t level_dofs = 0;
     for (int i = 0; i < cn; ++i) level_dofs += h_n[i];
     if (l <= 2) {
-      int64_t extra = 0;
+      // rows of a box: the other boxes of its instance on this level, plus
+      // (level 2) its instance's level-1 leaves
+      std::vector<int64_t> inst_dofs(t.ninst, 0), extra(t.ninst, 0);
+      for (int i = 0; i < cn; ++i) inst_dofs[t.h_inst[a + i]] += h_n[i];
       if (l == 2) {
         for (int y = (int)t.level_off[1]; y < (int)t.level_off[2]; ++y)
-          if (t.h_is_leaf[y]) extra += (int64_t)t.h_pcount[y] * t.sd;
+          if (t.h_is_leaf[y]) extra[t.h_inst[y]] += (int64_t)t.h_pcount[y] * t.sd;
       }
-      for (int i = 0; i < cn; ++i) h_q[i] = (int)(level_dofs - h_n[i] + extra);
+      for (int i = 0; i < cn; ++i) {
+        const int ii = t.h_inst[a + i];
+        h_q[i] = (int)(inst_dofs[ii] - h_n[i] + extra[ii]);
+      }
     }
     // ---- work list: boxes with >= min_dofs active DOFs and a non-empty ID matrix
     std::vector<IdWork> wl_lds, wl_glb;
@@ -915,12 +933,12 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
         k_outside_rows<<<(int)wl_lds.size(), kTB, 0, s>>>(
             lw->wl_lds.p, a, b, l, (int)t.level_off[1], (int)t.level_off[2], t.is_leaf.p,
             t.obox_off.p, t.obox_pool.p, t.pcount.p, t.sd, t.act_off.p, t.nact.p, t.act_pool.p,
-            lw->rows.p);
+            t.inst.p, lw->rows.p);
       if (!wl_glb.empty())
         k_outside_rows<<<(int)wl_glb.size(), kTB, 0, s>>>(
             lw->wl_glb.p, a, b, l, (int)t.level_off[1], (int)t.level_off[2], t.is_leaf.p,
             t.obox_off.p, t.obox_pool.p, t.pcount.p, t.sd, t.act_off.p, t.nact.p, t.act_pool.p,
-            lw->rows.p);
+            t.inst.p, lw->rows.p);
     }
     E.l1 = mark();
     // ---- ID
@@ -977,35 +995,52 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     pool_base += level_dofs;  // the next level's lists start after this level's
   }
 
-  // ---- root (level 0)
+  // ---- roots (level 0), one per instance
   const int e_root0 = mark();
-  const int root = 0;
-  int n0 = 0;
-  int64_t r_off = 0;
-  KI_HIP(hipMemcpyAsync(&n0, t.nact.p + root, sizeof(int), hipMemcpyDeviceToHost, s));
-  KI_HIP(hipMemcpyAsync(&r_off, t.act_off.p + root, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  const int ni = t.ninst;
+  t.h_root_n.assign(ni, 0);
+  std::vector<int64_t> r_off(ni);
+  KI_HIP(hipMemcpyAsync(t.h_root_n.data(), t.nact.p, sizeof(int) * ni, hipMemcpyDeviceToHost, s));
+  KI_HIP(hipMemcpyAsync(r_off.data(), t.act_off.p, sizeof(int64_t) * ni, hipMemcpyDeviceToHost,
+                        s));
   KI_HIP(hipStreamSynchronize(s));
   ++c.tim.syncs;
-  t.nroot = n0;
-  if (n0 > 0) {
-    if (n0 > 1024) KI_FAIL(KIGPU_ERR_NOTSUPP, "root skeleton larger than 1024 DOFs");
-    t.root_dofs.alloc(n0);
-    KI_HIP(hipMemcpyAsync(t.root_dofs.p, t.act_pool.p + r_off, sizeof(int) * n0,
+  t.h_root_doff.assign(ni + 1, 0);
+  t.h_root_moff.assign(ni + 1, 0);
+  t.nroot_max = 0;
+  for (int i = 0; i < ni; ++i) {
+    const int64_t n0 = t.h_root_n[i];
+    t.h_root_doff[i + 1] = t.h_root_doff[i] + n0;
+    t.h_root_moff[i + 1] = t.h_root_moff[i] + n0 * n0;
+    t.nroot_max = std::max(t.nroot_max, (int)n0);
+  }
+  t.nroot = (int)t.h_root_doff[ni];
+  if (t.nroot_max > 1024) KI_FAIL(KIGPU_ERR_NOTSUPP, "root skeleton larger than 1024 DOFs");
+  if (t.nroot > 0) {
+    // the roots' active lists are contiguous in act_pool (level 0 is last)
+    t.root_dofs.alloc(t.nroot);
+    KI_HIP(hipMemcpyAsync(t.root_dofs.p, t.act_pool.p + r_off[0], sizeof(int) * t.nroot,
                           hipMemcpyDeviceToDevice, s));
-    t.root_lu.alloc((size_t)n0 * n0);   // allskel_mat^{-1}
-    t.root_mat.alloc((size_t)n0 * n0);  // allskel_mat
-    t.root_piv.alloc(n0);
-    const size_t lbytes = sizeof(double) * (size_t)n0 * n0;
+    t.root_n.upload(t.h_root_n, s);
+    t.root_doff.upload(t.h_root_doff, s);
+    t.root_moff.upload(t.h_root_moff, s);
+    const size_t msz = std::max<int64_t>(t.h_root_moff[ni], 1);
+    t.root_lu.alloc(msz);   // allskel_mat^{-1}
+    t.root_mat.alloc(msz);  // allskel_mat
+    t.root_piv.alloc(t.nroot);
+    const size_t lbytes = sizeof(double) * (size_t)t.nroot_max * t.nroot_max;
     if (lbytes + 24576 <= lds_cap) {  // static LDS of k_root: pivots + 2 scratch rows
       allow_lds(k_root<T, true>, lbytes);
-      k_root<T, true><<<1, 1024, lbytes, s>>>(root, g, pde, t.child.p, t.osize.p, t.acc_ptr.p,
-                                              t.acc_ld.p, t.root_dofs.p, n0, nullptr, t.root_lu.p,
-                                              t.root_piv.p, t.root_mat.p, t.status.p);
+      k_root<T, true><<<ni, 1024, lbytes, s>>>(g, pde, t.child.p, t.osize.p, t.acc_ptr.p,
+                                               t.acc_ld.p, t.root_n.p, t.root_doff.p,
+                                               t.root_moff.p, t.root_dofs.p, nullptr, t.root_lu.p,
+                                               t.root_piv.p, t.root_mat.p, t.status.p);
     } else {
-      t.ws64.alloc((size_t)n0 * n0);
-      k_root<T, false><<<1, 1024, 0, s>>>(root, g, pde, t.child.p, t.osize.p, t.acc_ptr.p,
-                                          t.acc_ld.p, t.root_dofs.p, n0, t.ws64.p, t.root_lu.p,
-                                          t.root_piv.p, t.root_mat.p, t.status.p);
+      t.ws64.alloc(msz);
+      k_root<T, false><<<ni, 1024, 0, s>>>(g, pde, t.child.p, t.osize.p, t.acc_ptr.p,
+                                           t.acc_ld.p, t.root_n.p, t.root_doff.p, t.root_moff.p,
+                                           t.root_dofs.p, t.ws64.p, t.root_lu.p, t.root_piv.p,
+                                           t.root_mat.p, t.status.p);
     }
     KI_HIP(hipGetLastError());
   }

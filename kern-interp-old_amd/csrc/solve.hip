@@ -161,11 +161,19 @@ __global__ __launch_bounds__(kTB) void k_down(int a, const int* rank, const int*
 }
 
 // x[allskel] <- allskel_mat^{-1} x[allskel] (trans: ^{-T}) in FP64 with the
-// explicit inverse; one block per right-hand side, one thread per row.
+// explicit inverse; one block per (right-hand side, instance), one thread
+// per row.
 template <typename T>
-__global__ __launch_bounds__(kTB) void k_root_apply(const int* dofs, int n, const double* Ainv,
-                                                    T* x, int ldx, int trans) {
+__global__ __launch_bounds__(kTB) void k_root_apply(const int* root_n, const int64_t* root_doff,
+                                                    const int64_t* root_moff, const int* dofs_all,
+                                                    const double* Ainv_all, T* x, int ldx,
+                                                    int trans) {
   extern __shared__ __align__(16) unsigned char smem[];
+  const int b = blockIdx.y;
+  const int n = root_n[b];
+  if (n == 0) return;
+  const int* dofs = dofs_all + root_doff[b];
+  const double* Ainv = Ainv_all + root_moff[b];
   double* v = reinterpret_cast<double*>(smem);
   T* xc = x + (size_t)blockIdx.x * ldx;
   for (int i = threadIdx.x; i < n; i += kTB) v[i] = (double)xc[dofs[i]];
@@ -192,21 +200,40 @@ struct Targets {
   int n;             // target points
 };
 
+// Block-diagonal interaction of independent instances.  Tile b of kAT "own"
+// points is [t0[b], t1[b]) of instance inst[b]; the "other" points it meets
+// are [opt[inst[b]], opt[inst[b] + 1]), cut into gridDim.y equal splits
+// (rounded to kAT).  A single instance is one segment.
+struct Tiles {
+  const int* inst;
+  const int* t0;
+  const int* t1;
+  const int64_t* opt;
+};
+
+__device__ __forceinline__ void split_range(const Tiles& tl, int* o0, int* o1) {
+  const int b = tl.inst[blockIdx.x];
+  const int a = (int)tl.opt[b], e = (int)tl.opt[b + 1];
+  int per = (e - a + (int)gridDim.y - 1) / (int)gridDim.y;
+  per = (per + kAT - 1) / kAT * kAT;
+  *o0 = min(e, a + (int)blockIdx.y * per);
+  *o1 = min(e, *o0 + per);
+}
+
 // y_part[split] = sum over this split's sources; one thread per target point
 template <int PDE, bool FWD>
-__global__ __launch_bounds__(kAT) void k_apply(Targets tg, GeomView g, const double* mu,
-                                               int ldm, int nrhs, double* part, int ldp,
-                                               int src_per_split) {
+__global__ __launch_bounds__(kAT) void k_apply(Targets tg, Tiles tl, GeomView g, const double* mu,
+                                               int ldm, int nrhs, double* part, int ldp) {
 #pragma clang fp contract(off)
   __shared__ double sx[kAT], sy[kAT], snx[kAT], sny[kAT], sw[kAT], sk[kAT];
   __shared__ double smu[kAT * 2 * kAR];
   constexpr int SD = PDE == KIGPU_STOKES ? 2 : 1;
-  const int ti = blockIdx.x * kAT + threadIdx.x;
-  const bool valid = ti < tg.n;
+  const int ti = tl.t0[blockIdx.x] + threadIdx.x;
+  const bool valid = ti < tl.t1[blockIdx.x];
   const double tx = valid ? tg.x[ti] : 0, ty = valid ? tg.y[ti] : 0;
   const double tnx = (valid && !FWD) ? tg.nx[ti] : 0, tny = (valid && !FWD) ? tg.ny[ti] : 0;
-  const int s0 = blockIdx.y * src_per_split;
-  const int s1 = min(g.npts, s0 + src_per_split);
+  int s0, s1;
+  split_range(tl, &s0, &s1);
   for (int c0 = 0; c0 < nrhs; c0 += kAR) {
     const int nc = min(kAR, nrhs - c0);
     double acc[kAR][SD];
@@ -271,20 +298,20 @@ __global__ __launch_bounds__(kAT) void k_apply(Targets tg, GeomView g, const dou
 // K(j, i) lam(j), one thread per column (source point) i.  Rows are the
 // boundary points (K^T) or domain targets without normals (forward^T).
 template <int PDE, bool FWD>
-__global__ __launch_bounds__(kAT) void k_apply_t(Targets rows, GeomView g, const double* lam,
-                                                 int ldl, int nrhs, double* part, int ldp,
-                                                 int rows_per_split) {
+__global__ __launch_bounds__(kAT) void k_apply_t(Targets rows, Tiles tl, GeomView g,
+                                                 const double* lam, int ldl, int nrhs,
+                                                 double* part, int ldp) {
 #pragma clang fp contract(off)
   __shared__ double sx[kAT], sy[kAT], snx[kAT], sny[kAT];
   __shared__ double slam[kAT * 2 * kAR];
   constexpr int SD = PDE == KIGPU_STOKES ? 2 : 1;
-  const int si = blockIdx.x * kAT + threadIdx.x;
-  const bool valid = si < g.npts;
+  const int si = tl.t0[blockIdx.x] + threadIdx.x;
+  const bool valid = si < tl.t1[blockIdx.x];
   const double ox = valid ? g.px[si] : 0, oy = valid ? g.py[si] : 0;
   const double onx = valid ? g.nx[si] : 0, ony = valid ? g.ny[si] : 0;
   const double ow = valid ? g.w[si] : 0, okap = valid ? g.kappa[si] : 0;
-  const int j0 = blockIdx.y * rows_per_split;
-  const int j1 = min(rows.n, j0 + rows_per_split);
+  int j0, j1;
+  split_range(tl, &j0, &j1);
   for (int c0 = 0; c0 < nrhs; c0 += kAR) {
     const int nc = min(kAR, nrhs - c0);
     double acc[kAR][SD];
@@ -350,13 +377,15 @@ __global__ void k_sum_parts(const double* part, int nsplit, size_t len, double* 
   y[i] = s;
 }
 
-// r = f - y (FP64); per-column partial sums of squares of r and f
+// r = f - y (FP64); partial sums of squares of r and f per (instance
+// blockIdx.z, column blockIdx.y) over the instance's DOFs [doff[z], doff[z+1])
 __global__ void k_residual(const double* f, const double* y, double* r, int n, int nrhs,
-                           double* sums) {
+                           const int64_t* doff, double* sums) {
   __shared__ double red[2 * kNW];
   const int c = blockIdx.y;
+  const int i0 = (int)doff[blockIdx.z], i1 = (int)doff[blockIdx.z + 1];
   double sr = 0, sf = 0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  for (int i = i0 + blockIdx.x * blockDim.x + threadIdx.x; i < i1; i += gridDim.x * blockDim.x) {
     double fv = f[i + (size_t)c * n];
     double rv = fv - y[i + (size_t)c * n];
     r[i + (size_t)c * n] = rv;
@@ -377,8 +406,9 @@ __global__ void k_residual(const double* f, const double* y, double* r, int n, i
       a += red[i];
       b += red[kNW + i];
     }
-    sums[2 * ((size_t)c * gridDim.x + blockIdx.x)] = a;
-    sums[2 * ((size_t)c * gridDim.x + blockIdx.x) + 1] = b;
+    const size_t o = ((size_t)blockIdx.z * nrhs + c) * gridDim.x + blockIdx.x;
+    sums[2 * o] = a;
+    sums[2 * o + 1] = b;
   }
 }
 
@@ -468,39 +498,13 @@ void sweeps(Context& c, TreeData& t, T* x, int ldx, int nrhs, const double* g = 
   if (t.nborder > 0) {
     border_step<T>(c, t, x, ldx, nrhs, g);
   } else if (t.nroot > 0) {
-    k_root_apply<T><<<nrhs, kTB, sizeof(double) * t.nroot, s>>>(t.root_dofs.p, t.nroot,
-                                                                t.root_lu.p, x, ldx, trans);
+    k_root_apply<T><<<dim3(nrhs, t.ninst), kTB, sizeof(double) * t.nroot_max, s>>>(
+        t.root_n.p, t.root_doff.p, t.root_moff.p, t.root_dofs.p, t.root_lu.p, x, ldx, trans);
   }
   down_sweeps<T>(c, t, x, ldx, nrhs, trans);
 }
 
 }  // namespace
-
-void kernel_apply(Context& c, const TreeData& t, int pde, const double* d_mu, int nrhs,
-                  double* d_y) {
-  hipStream_t s = c.stream;
-  const GeomView g{t.px.p, t.py.p, t.nx.p, t.ny.p, t.w.p, t.kappa.p, t.npts, t.sd};
-  Targets tg{t.px.p, t.py.p, t.nx.p, t.ny.p, t.npts};
-  const int tb = ceil_div(t.npts, kAT);
-  int split = std::max(1, std::min(64, ceil_div(2048, tb)));
-  int per = ceil_div(t.npts, split);
-  per = ceil_div(per, kAT) * kAT;
-  split = ceil_div(t.npts, per);
-  const size_t len = (size_t)t.ndof * nrhs;
-  DBuf<double> part;
-  part.alloc(len * split);
-  dim3 grid(tb, split);
-  if (pde == KIGPU_LAPLACE)
-    k_apply<KIGPU_LAPLACE, false><<<grid, kAT, 0, s>>>(tg, g, d_mu, t.ndof, nrhs, part.p, t.ndof, per);
-  else if (pde == KIGPU_LAPLACE_NEUMANN)
-    k_apply<KIGPU_LAPLACE_NEUMANN, false><<<grid, kAT, 0, s>>>(tg, g, d_mu, t.ndof, nrhs, part.p,
-                                                               t.ndof, per);
-  else
-    k_apply<KIGPU_STOKES, false><<<grid, kAT, 0, s>>>(tg, g, d_mu, t.ndof, nrhs, part.p, t.ndof, per);
-  k_sum_parts<<<ceil_div((long)len, 256), 256, 0, s>>>(part.p, split, len, d_y);
-  KI_HIP(hipGetLastError());
-  KI_HIP(hipStreamSynchronize(s));
-}
 
 namespace {
 __global__ void k_split_xy(const double* xy, int n, double* x, double* y) {
@@ -510,6 +514,38 @@ __global__ void k_split_xy(const double* xy, int n, double* x, double* y) {
     y[i] = xy[2 * i + 1];
   }
 }
+
+// tiles of kAT own points per segment [own[i], own[i+1]); other segments opt
+struct TileBuf {
+  DBuf<int> inst, t0, t1;
+  DBuf<int64_t> opt;
+  int ntiles = 0;
+  int maxlen = 0;  // longest other-segment
+  void build(const std::vector<int64_t>& own, const std::vector<int64_t>& other, hipStream_t s) {
+    std::vector<int> hi, h0, h1;
+    maxlen = 0;
+    for (size_t b = 0; b + 1 < own.size(); ++b) {
+      for (int64_t p = own[b]; p < own[b + 1]; p += kAT) {
+        hi.push_back((int)b);
+        h0.push_back((int)p);
+        h1.push_back((int)std::min<int64_t>(own[b + 1], p + kAT));
+      }
+      maxlen = std::max(maxlen, (int)(other[b + 1] - other[b]));
+    }
+    ntiles = (int)hi.size();
+    inst.upload(hi, s);
+    t0.upload(h0, s);
+    t1.upload(h1, s);
+    opt.upload(other, s);
+  }
+  Tiles view() const { return Tiles{inst.p, t0.p, t1.p, opt.p}; }
+  // splits of the other range: enough blocks to fill the chip, each split at
+  // least kAT points
+  int splits(int cap) const {
+    const int by_len = std::max(1, ceil_div(maxlen, kAT));
+    return std::max(1, std::min({cap, ceil_div(2048, std::max(ntiles, 1)), by_len}));
+  }
+};
 }  // namespace
 
 // interleaved [x0, y0, x1, ...] device points -> separate x and y arrays
@@ -520,98 +556,105 @@ void split_points(Context& c, const double* d_xy, int n, DBuf<double>* x, DBuf<d
   KI_HIP(hipGetLastError());
 }
 
-void forward_apply(Context& c, const TreeData& t, int pde, const double* d_tgt, int ntgt,
-                   const double* d_mu, int nrhs, double* d_u) {
+namespace {
+
+// K(all, all) restricted to each instance's diagonal block (FWD = false), or
+// the forward matrix at domain targets (FWD = true, one instance)
+template <bool TRANS>
+void run_apply(Context& c, const TreeData& t, int pde, bool fwd, const Targets& other,
+               const TileBuf& tb, const double* d_in, int ldi, int nrhs, double* d_out, int ldo) {
   hipStream_t s = c.stream;
   const GeomView g{t.px.p, t.py.p, t.nx.p, t.ny.p, t.w.p, t.kappa.p, t.npts, t.sd};
-  DBuf<double> tx, ty;
-  split_points(c, d_tgt, ntgt, &tx, &ty);
-  Targets tg{tx.p, ty.p, nullptr, nullptr, ntgt};
-  const int sd = pde == KIGPU_STOKES ? 2 : 1;
-  const int tb = ceil_div(ntgt, kAT);
-  int split = std::max(1, std::min(256, ceil_div(2048, tb)));
-  int per = ceil_div(t.npts, split);
-  per = ceil_div(per, kAT) * kAT;
-  split = ceil_div(t.npts, per);
-  const int ldp = sd * ntgt;
-  const size_t len = (size_t)ldp * nrhs;
+  const int split = tb.splits(TRANS ? 256 : (fwd ? 256 : 64));
+  const size_t len = (size_t)ldo * nrhs;
   DBuf<double> part;
   part.alloc(len * split);
-  dim3 grid(tb, split);
-  if (pde == KIGPU_LAPLACE)
-    k_apply<KIGPU_LAPLACE, true><<<grid, kAT, 0, s>>>(tg, g, d_mu, t.ndof, nrhs, part.p, ldp, per);
-  else if (pde == KIGPU_LAPLACE_NEUMANN)
-    k_apply<KIGPU_LAPLACE_NEUMANN, true><<<grid, kAT, 0, s>>>(tg, g, d_mu, t.ndof, nrhs, part.p,
-                                                              ldp, per);
-  else
-    k_apply<KIGPU_STOKES, true><<<grid, kAT, 0, s>>>(tg, g, d_mu, t.ndof, nrhs, part.p, ldp, per);
-  k_sum_parts<<<ceil_div((long)len, 256), 256, 0, s>>>(part.p, split, len, d_u);
+  dim3 grid(tb.ntiles, split);
+  const Tiles tl = tb.view();
+#define KI_RUN(P, F)                                                                         \
+  do {                                                                                       \
+    if (TRANS)                                                                               \
+      k_apply_t<P, F><<<grid, kAT, 0, s>>>(other, tl, g, d_in, ldi, nrhs, part.p, ldo);      \
+    else                                                                                     \
+      k_apply<P, F><<<grid, kAT, 0, s>>>(other, tl, g, d_in, ldi, nrhs, part.p, ldo);        \
+  } while (0)
+  if (pde == KIGPU_LAPLACE) {
+    if (fwd) KI_RUN(KIGPU_LAPLACE, true); else KI_RUN(KIGPU_LAPLACE, false);
+  } else if (pde == KIGPU_LAPLACE_NEUMANN) {
+    if (fwd) KI_RUN(KIGPU_LAPLACE_NEUMANN, true); else KI_RUN(KIGPU_LAPLACE_NEUMANN, false);
+  } else {
+    if (fwd) KI_RUN(KIGPU_STOKES, true); else KI_RUN(KIGPU_STOKES, false);
+  }
+#undef KI_RUN
+  k_sum_parts<<<ceil_div((long)len, 256), 256, 0, s>>>(part.p, split, len, d_out);
   KI_HIP(hipGetLastError());
-  KI_HIP(hipStreamSynchronize(s));
 }
 
-// y = K(all, all)^T lam, or forward^T v with domain targets (fwd_tgt != null)
-static void apply_transpose(Context& c, const TreeData& t, int pde, const Targets& rows, bool fwd,
-                            const double* d_lam, int ldl, int nrhs, double* d_y) {
-  hipStream_t s = c.stream;
-  const GeomView g{t.px.p, t.py.p, t.nx.p, t.ny.p, t.w.p, t.kappa.p, t.npts, t.sd};
-  const int tb = ceil_div(t.npts, kAT);
-  int split = std::max(1, std::min(256, ceil_div(2048, tb)));
-  int per = ceil_div(rows.n, split);
-  per = ceil_div(per, kAT) * kAT;
-  split = ceil_div(rows.n, per);
-  const size_t len = (size_t)t.ndof * nrhs;
-  DBuf<double> part;
-  part.alloc(len * split);
-  dim3 grid(tb, split);
-#define KI_APPLY_T(P, F) \
-  k_apply_t<P, F><<<grid, kAT, 0, s>>>(rows, g, d_lam, ldl, nrhs, part.p, t.ndof, per)
-  if (pde == KIGPU_LAPLACE) {
-    if (fwd) KI_APPLY_T(KIGPU_LAPLACE, true); else KI_APPLY_T(KIGPU_LAPLACE, false);
-  } else if (pde == KIGPU_LAPLACE_NEUMANN) {
-    if (fwd) KI_APPLY_T(KIGPU_LAPLACE_NEUMANN, true); else KI_APPLY_T(KIGPU_LAPLACE_NEUMANN, false);
-  } else {
-    if (fwd) KI_APPLY_T(KIGPU_STOKES, true); else KI_APPLY_T(KIGPU_STOKES, false);
-  }
-#undef KI_APPLY_T
-  k_sum_parts<<<ceil_div((long)len, 256), 256, 0, s>>>(part.p, split, len, d_y);
-  KI_HIP(hipGetLastError());
-  KI_HIP(hipStreamSynchronize(s));
+}  // namespace
+
+void kernel_apply(Context& c, const TreeData& t, int pde, const double* d_mu, int nrhs,
+                  double* d_y) {
+  Targets tg{t.px.p, t.py.p, t.nx.p, t.ny.p, t.npts};
+  TileBuf tb;
+  tb.build(t.inst_pt, t.inst_pt, c.stream);
+  run_apply<false>(c, t, pde, false, tg, tb, d_mu, t.ndof, nrhs, d_y, t.ndof);
+  KI_HIP(hipStreamSynchronize(c.stream));
 }
 
 void kernel_apply_transpose(Context& c, const TreeData& t, int pde, const double* d_lam, int nrhs,
                             double* d_y) {
   Targets rows{t.px.p, t.py.p, t.nx.p, t.ny.p, t.npts};
-  apply_transpose(c, t, pde, rows, false, d_lam, t.ndof, nrhs, d_y);
+  TileBuf tb;
+  tb.build(t.inst_pt, t.inst_pt, c.stream);
+  run_apply<true>(c, t, pde, false, rows, tb, d_lam, t.ndof, nrhs, d_y, t.ndof);
+  KI_HIP(hipStreamSynchronize(c.stream));
+}
+
+void forward_apply(Context& c, const TreeData& t, int pde, const double* d_tgt, int ntgt,
+                   const double* d_mu, int nrhs, double* d_u) {
+  DBuf<double> tx, ty;
+  split_points(c, d_tgt, ntgt, &tx, &ty);
+  Targets tg{tx.p, ty.p, nullptr, nullptr, ntgt};
+  const int sd = pde == KIGPU_STOKES ? 2 : 1;
+  TileBuf tb;  // own = targets, other = all boundary points
+  tb.build({0, (int64_t)ntgt}, {0, (int64_t)t.npts}, c.stream);
+  run_apply<false>(c, t, pde, true, tg, tb, d_mu, t.ndof, nrhs, d_u, sd * ntgt);
+  KI_HIP(hipStreamSynchronize(c.stream));
 }
 
 void forward_apply_transpose(Context& c, const TreeData& t, int pde, const double* d_tgt, int ntgt,
                              const double* d_v, int nrhs, double* d_y) {
-  hipStream_t s = c.stream;
   DBuf<double> tx, ty;
   split_points(c, d_tgt, ntgt, &tx, &ty);
   Targets rows{tx.p, ty.p, nullptr, nullptr, ntgt};
   const int sd = pde == KIGPU_STOKES ? 2 : 1;
-  apply_transpose(c, t, pde, rows, true, d_v, sd * ntgt, nrhs, d_y);
-  KI_HIP(hipStreamSynchronize(s));
+  TileBuf tb;  // own = boundary points, other = targets
+  tb.build({0, (int64_t)t.npts}, {0, (int64_t)ntgt}, c.stream);
+  run_apply<true>(c, t, pde, true, rows, tb, d_v, sd * ntgt, nrhs, d_y, t.ndof);
+  KI_HIP(hipStreamSynchronize(c.stream));
 }
 
-// per-column sums of r^2 and f^2 with r = f - y written out
-static void residual_sums(Context& c, const double* f, const double* y, double* r, int n, int nrhs,
-                          std::vector<double>* rr, std::vector<double>* ff) {
+// r = f - y written out; per (instance, column) sums of r^2 and f^2
+static void residual_sums(Context& c, const TreeData& t, const double* f, const double* y,
+                          double* r, int nrhs, std::vector<double>* rr, std::vector<double>* ff) {
   hipStream_t s = c.stream;
-  const int nb = 64;
-  c.rsum.alloc(2 * (size_t)nb * nrhs);
-  k_residual<<<dim3(nb, nrhs), kTB, 0, s>>>(f, y, r, n, nrhs, c.rsum.p);
-  std::vector<double> h(2 * (size_t)nb * nrhs);
+  const int ni = t.ninst, n = t.ndof;
+  const int nb = ni == 1 ? 64 : 4;
+  std::vector<int64_t> doff(ni + 1);
+  for (int i = 0; i <= ni; ++i) doff[i] = t.inst_pt[i] * t.sd;
+  DBuf<int64_t> ddoff;
+  ddoff.upload(doff, s);
+  c.rsum.alloc(2 * (size_t)nb * nrhs * ni);
+  k_residual<<<dim3(nb, nrhs, ni), kTB, 0, s>>>(f, y, r, n, nrhs, ddoff.p, c.rsum.p);
+  std::vector<double> h(2 * (size_t)nb * nrhs * ni);
   c.rsum.download(h.data(), h.size(), s);
   KI_HIP(hipStreamSynchronize(s));
-  rr->assign(nrhs, 0);
-  ff->assign(nrhs, 0);
-  for (int col = 0; col < nrhs; ++col)
+  rr->assign((size_t)nrhs * ni, 0);
+  ff->assign((size_t)nrhs * ni, 0);
+  for (size_t q = 0; q < (size_t)nrhs * ni; ++q)
     for (int i = 0; i < nb; ++i) {
-      (*rr)[col] += h[2 * ((size_t)col * nb + i)];
-      (*ff)[col] += h[2 * ((size_t)col * nb + i) + 1];
+      (*rr)[q] += h[2 * (q * nb + i)];
+      (*ff)[q] += h[2 * (q * nb + i) + 1];
     }
 }
 
@@ -677,16 +720,16 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
         k_border_resid<<<dim3(h, nrhs), kTB, 0, s>>>(t.ndof, h, t.bP64in.p, d_x, t.alpha64.p,
                                                      t.rb64.p);
       }
-      residual_sums(c, d_b, t.f64.p, t.r64.p, t.ndof, nrhs, &rr, &ff);
+      residual_sums(c, t, d_b, t.f64.p, t.r64.p, nrhs, &rr, &ff);
       if (h) {
         t.rb64.download(hb.data(), alen, s);
         KI_HIP(hipStreamSynchronize(s));
         for (int col = 0; col < nrhs; ++col)
           for (int i = 0; i < h; ++i) rr[col] += hb[i + (size_t)col * h] * hb[i + (size_t)col * h];
       }
-      double rel = 0;
-      for (int col = 0; col < nrhs; ++col)
-        rel = std::max(rel, ff[col] > 0 ? std::sqrt(rr[col] / ff[col]) : std::sqrt(rr[col]));
+      double rel = 0;  // worst column of the worst instance
+      for (size_t q = 0; q < rr.size(); ++q)
+        rel = std::max(rel, ff[q] > 0 ? std::sqrt(rr[q] / ff[q]) : std::sqrt(rr[q]));
       KI_HIP(hipEventRecord(c.ev[9], s));
       KI_HIP(hipStreamSynchronize(s));
       KI_HIP(hipEventElapsedTime(&ms, c.ev[8], c.ev[9]));

@@ -12,8 +12,19 @@ struct LevelWork;  // per-level factor work lists (factor.cpp)
 struct TreeData {
   // ---- geometry (Kernel's copies of the boundary arrays) ----
   int npts = 0, sd = 1, leaf_dofs = 32, ndof = 0;
-  double min = 0, max = 0;
+  double min = 0, max = 0;  // root box bounds of instance 0
   DBuf<double> px, py, nx, ny, w, kappa;
+
+  // ---- instances: a batch of independent boundaries factored together ----
+  // Instance i owns points [inst_pt[i], inst_pt[i+1]) and its own quadtree;
+  // every tree level holds the nodes of all instances, instance by instance,
+  // so each level is one launch over all of them.  A single tree is ninst = 1.
+  int ninst = 1;
+  std::vector<int64_t> inst_pt;        // ninst + 1 point offsets
+  std::vector<double> inst_min, inst_max;
+  DBuf<int64_t> d_inst_pt;
+  DBuf<int> inst;                      // node -> instance
+  std::vector<int> h_inst;
 
   // ---- tree, node ids in level order ----
   int nlevels = 0, nnodes = 0;
@@ -63,8 +74,14 @@ struct TreeData {
   std::vector<int> h_act;  // host copy of act_pool after factoring
   std::vector<int> h_pos;  // host copy of pos_pool after factoring
 
-  // root
-  int nroot = 0;
+  // roots, one per instance: instance i's allskel is root_dofs[root_doff[i] ..
+  // + root_n[i]), its matrices start at root_moff[i] (n^2 entries each)
+  int nroot = 0;                       // total root skeleton DOFs
+  int nroot_max = 0;
+  std::vector<int> h_root_n;
+  std::vector<int64_t> h_root_doff, h_root_moff;
+  DBuf<int> root_n;
+  DBuf<int64_t> root_doff, root_moff;
   DBuf<int> root_dofs;
   DBuf<double> root_lu;
   DBuf<int> root_piv;

@@ -77,6 +77,39 @@ __global__ void k_fill_i32(int* p, int n, int v) {
   if (i < n) p[i] = v;
 }
 
+// per-instance root boxes: block i reduces points [pt_off[i], pt_off[i+1])
+__global__ void k_minmax_seg(const double* px, const double* py, const int64_t* pt_off,
+                             double* out) {
+  __shared__ double smin[kTB], smax[kTB];
+  const int64_t a = pt_off[blockIdx.x], b = pt_off[blockIdx.x + 1];
+  double mn = px[a], mx = px[a];
+  for (int64_t i = a + threadIdx.x; i < b; i += kTB) {
+    double u = px[i], v = py[i];
+    mn = fmin(mn, fmin(u, v));
+    mx = fmax(mx, fmax(u, v));
+  }
+  smin[threadIdx.x] = mn;
+  smax[threadIdx.x] = mx;
+  __syncthreads();
+  for (int st = kTB / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + st]);
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + st]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = smin[0];
+    out[2 * blockIdx.x + 1] = smax[0];
+  }
+}
+
+// pnode of every point = its instance (the instance's root is node i)
+__global__ void k_fill_inst(const int64_t* pt_off, int* pnode) {
+  const int64_t a = pt_off[blockIdx.x], b = pt_off[blockIdx.x + 1];
+  for (int64_t i = a + threadIdx.x; i < b; i += blockDim.x) pnode[i] = blockIdx.x;
+}
+
 __device__ __forceinline__ void node_mid(const double* c, double* midx, double* midy) {
 #pragma clang fp contract(off)
   *midx = ((c[6] - c[0]) / 2.0) + c[0];
@@ -107,18 +140,29 @@ __global__ void k_split_info(int c, const int* pcount, const int64_t* pstart, co
   st[i] = t;
 }
 
-// Rank of each splitting node by (split point, child-slot path): bitonic sort
-// of (st << 40 | path) keys in one workgroup.  Needs c <= 4096, paths of at
-// most 20 levels and split points below 2^24; k_rank_pairs covers the rest.
+// Rank of each splitting node among the splitting nodes of its instance by
+// (split point, child-slot path).  The nodes of a level are grouped by
+// instance (segment b = nodes [seg[b], seg[b+1])); block b sorts its
+// segment with a bitonic sort of (st << 40 | path) keys.  Needs segments of
+// at most 4096 nodes, paths of at most 20 levels and split points below
+// 2^24; k_rank_pairs covers the rest.  seg_cnt[b] receives the segment's
+// number of splitting nodes.
 constexpr int kSortMax = 4096;
 
-__global__ __launch_bounds__(1024) void k_rank_sort(int c, int P, const int* split, const int* st,
-                                                    const uint64_t* path, int* rank, int* nsplit) {
+__global__ __launch_bounds__(1024) void k_rank_sort(const int64_t* seg, const int* split,
+                                                    const int* st, const uint64_t* path,
+                                                    int* rank, int* seg_cnt) {
   __shared__ uint64_t key[kSortMax];
   __shared__ int val[kSortMax];
+  __shared__ int s_cnt;
+  const int base = (int)seg[blockIdx.x];
+  const int c = (int)seg[blockIdx.x + 1] - base;
+  int P = 1;
+  while (P < c) P <<= 1;
+  if (threadIdx.x == 0) s_cnt = 0;
   for (int i = threadIdx.x; i < P; i += 1024) {
-    if (i < c && split[i]) {
-      key[i] = ((uint64_t)st[i] << 40) | path[i];
+    if (i < c && split[base + i]) {
+      key[i] = ((uint64_t)st[base + i] << 40) | path[base + i];
     } else {
       key[i] = ~0ull;
     }
@@ -147,40 +191,61 @@ __global__ __launch_bounds__(1024) void k_rank_sort(int c, int P, const int* spl
   int cnt = 0;
   for (int i = threadIdx.x; i < P; i += 1024) {
     if (key[i] != ~0ull) {
-      rank[val[i]] = i;
+      rank[base + val[i]] = i;
       ++cnt;
     } else if (val[i] < c) {
-      rank[val[i]] = -1;
+      rank[base + val[i]] = -1;
     }
   }
-  if (cnt) atomicAdd(nsplit, cnt);
+  if (cnt) atomicAdd(&s_cnt, cnt);
+  __syncthreads();
+  if (threadIdx.x == 0) seg_cnt[blockIdx.x] = s_cnt;
 }
 
-// rank of each splitting node among the splitting nodes by (st, path)
-__global__ void k_rank_pairs(int c, const int* split, const int* st, const uint64_t* path, int* rank,
-                       int* nsplit) {
+// the same by counting, one thread per node, for segments the sort cannot take
+__global__ void k_rank_pairs(int c, const int* inst, const int64_t* seg, const int* split,
+                             const int* st, const uint64_t* path, int* rank, int* seg_cnt) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c) return;
   if (!split[i]) {
     rank[i] = -1;
     return;
   }
+  const int b = inst[i];
+  const int a0 = (int)seg[b], a1 = (int)seg[b + 1];
   int si = st[i];
   uint64_t pi = path[i];
   int r = 0;
-  for (int j = 0; j < c; ++j) {
+  for (int j = a0; j < a1; ++j) {
     if (!split[j]) continue;
     int sj = st[j];
     if (sj < si || (sj == si && path[j] < pi)) ++r;
   }
   rank[i] = r;
-  atomicAdd(nsplit, 1);
+  atomicAdd(seg_cnt + b, 1);
+}
+
+// exclusive scan of the per-instance split counts (ninst + 1 entries)
+__global__ void k_seg_scan(int ninst, const int* seg_cnt, int* seg_split) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int run = 0;
+  for (int b = 0; b < ninst; ++b) {
+    seg_split[b] = run;
+    run += seg_cnt[b];
+  }
+  seg_split[ninst] = run;
+}
+
+// instance-local rank -> rank within the level
+__global__ void k_rank_offset(int c, const int* inst, const int* seg_split, int* rank) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < c && rank[i] >= 0) rank[i] += seg_split[inst[i]];
 }
 
 // records of the 4 children of every splitting node (node_subdivide, quadtree.cpp:203-253)
 __global__ void k_children(int c, const int* rank, const double* corners, const double* side,
-                           const uint64_t* path, int* n_parent, int* n_slot, uint64_t* n_path,
-                           double* n_corners, double* n_side) {
+                           const uint64_t* path, const int* inst, int* n_parent, int* n_slot,
+                           uint64_t* n_path, double* n_corners, double* n_side, int* n_inst) {
 #pragma clang fp contract(off)
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c || rank[i] < 0) return;
@@ -206,6 +271,7 @@ __global__ void k_children(int c, const int* rank, const double* corners, const 
     n_parent[o] = i;
     n_slot[o] = s;
     n_path[o] = path[i] * 4 + s;
+    n_inst[o] = inst[i];
   }
 }
 
@@ -405,6 +471,9 @@ __global__ void k_coarse(int nnodes, const int* parent, const int* level, const 
 
 struct LevelBuf {
   int count = 0;
+  std::vector<int64_t> seg;  // node offsets of the instances (ninst + 1)
+  DBuf<int64_t> dseg;
+  DBuf<int> inst;
   DBuf<int> parent_local, slot, split, st, rank, pcount;
   DBuf<int64_t> pstart;
   DBuf<uint64_t> path;
@@ -417,10 +486,10 @@ struct LevelBuf {
 struct FlatArgs {
   int c, base, parent_base, next_base, level, has_next;
   int64_t pbase;
-  const int *parent_local, *slot, *split, *rank, *pcount;
+  const int *parent_local, *slot, *split, *rank, *pcount, *inst;
   const int64_t* pstart;
   const double *corners, *side;
-  int *parent, *child, *is_leaf, *levelv, *slotv, *pcountv;
+  int *parent, *child, *is_leaf, *levelv, *slotv, *pcountv, *instv;
   int64_t* pstartv;
   double *cornersv, *sidev;
 };
@@ -435,6 +504,7 @@ __global__ void k_flatten(FlatArgs f) {
   f.is_leaf[x] = f.split[i] ? 0 : 1;
   f.levelv[x] = f.level;
   f.slotv[x] = f.slot[i];
+  f.instv[x] = f.inst[i];
   f.pcountv[x] = f.pcount[i];
   f.pstartv[x] = f.pstart[i] + f.pbase;
   for (int j = 0; j < 8; ++j) f.cornersv[8 * x + j] = f.corners[8 * i + j];
@@ -445,44 +515,74 @@ __global__ void k_flatten(FlatArgs f) {
 
 void build_tree(TreeData& t, hipStream_t s, int* syncs) {
   const int n = t.npts;
-  DBuf<double> mm, mpart;
-  mm.alloc(2);
-  mpart.alloc(2 * kMMBlocks);
-  k_minmax_part<<<kMMBlocks, kTB, 0, s>>>(t.px.p, t.py.p, n, mpart.p);
-  k_minmax_final<<<1, 64, 0, s>>>(mpart.p, kMMBlocks, mm.p);
-  double hmm[2];
-  mm.download(hmm, 2, s);
+  const int ni = t.ninst;
+  if (t.inst_pt.empty()) t.inst_pt = {0, (int64_t)n};
+  if ((int)t.inst_pt.size() != ni + 1 || t.inst_pt[ni] != n)
+    KI_FAIL(KIGPU_ERR_ARG, "instance point offsets do not cover the points");
+  t.d_inst_pt.upload(t.inst_pt, s);
+  // root boxes: min/max pooled over x and y (quadtree.cpp:80-88), per instance
+  std::vector<double> hmm(2 * (size_t)ni);
+  DBuf<double> mm;
+  mm.alloc(2 * (size_t)ni);
+  if (ni == 1) {
+    DBuf<double> mpart;
+    mpart.alloc(2 * kMMBlocks);
+    k_minmax_part<<<kMMBlocks, kTB, 0, s>>>(t.px.p, t.py.p, n, mpart.p);
+    k_minmax_final<<<1, 64, 0, s>>>(mpart.p, kMMBlocks, mm.p);
+  } else {
+    k_minmax_seg<<<ni, kTB, 0, s>>>(t.px.p, t.py.p, t.d_inst_pt.p, mm.p);
+  }
+  mm.download(hmm.data(), hmm.size(), s);
   KI_HIP(hipStreamSynchronize(s));
   ++*syncs;
-  t.min = hmm[0];
-  t.max = hmm[1];
-  const double lo = t.min - 0.01, hi = t.max + 0.01;
+  t.inst_min.resize(ni);
+  t.inst_max.resize(ni);
+  for (int i = 0; i < ni; ++i) {
+    t.inst_min[i] = hmm[2 * i];
+    t.inst_max[i] = hmm[2 * i + 1];
+  }
+  t.min = t.inst_min[0];
+  t.max = t.inst_max[0];
 
   std::vector<std::unique_ptr<LevelBuf>> lv;
   {
     auto r = std::make_unique<LevelBuf>();
-    r->count = 1;
-    int h_pl = -1, h_slot = 0, h_pc = n;
-    int64_t h_ps = 0;
-    uint64_t h_path = 0;
-    double h_c[8] = {lo, lo, lo, hi, hi, hi, hi, lo};
-    double h_side = hi - lo;
-    r->parent_local.upload(&h_pl, 1, s);
-    r->slot.upload(&h_slot, 1, s);
-    r->pcount.upload(&h_pc, 1, s);
-    r->pstart.upload(&h_ps, 1, s);
-    r->path.upload(&h_path, 1, s);
-    r->corners.upload(h_c, 8, s);
-    r->side.upload(&h_side, 1, s);
+    r->count = ni;
+    std::vector<int> h_pl(ni, -1), h_slot(ni, 0), h_pc(ni), h_inst(ni);
+    std::vector<int64_t> h_ps(ni);
+    std::vector<uint64_t> h_path(ni, 0);
+    std::vector<double> h_c(8 * (size_t)ni), h_side(ni);
+    r->seg.resize(ni + 1);
+    for (int i = 0; i < ni; ++i) {
+      const double lo = t.inst_min[i] - 0.01, hi = t.inst_max[i] + 0.01;
+      const double c8[8] = {lo, lo, lo, hi, hi, hi, hi, lo};
+      for (int j = 0; j < 8; ++j) h_c[8 * i + j] = c8[j];
+      h_side[i] = hi - lo;
+      h_pc[i] = (int)(t.inst_pt[i + 1] - t.inst_pt[i]);
+      h_ps[i] = t.inst_pt[i];
+      h_inst[i] = i;
+      r->seg[i] = i;
+    }
+    r->seg[ni] = ni;
+    r->parent_local.upload(h_pl, s);
+    r->slot.upload(h_slot, s);
+    r->pcount.upload(h_pc, s);
+    r->pstart.upload(h_ps, s);
+    r->path.upload(h_path, s);
+    r->corners.upload(h_c, s);
+    r->side.upload(h_side, s);
+    r->inst.upload(h_inst, s);
     r->perm.alloc(n);
     r->pnode.alloc(n);
     r->nperm = n;
     k_iota<<<ceil_div(n, kTB), kTB, 0, s>>>(r->perm.p, n);
-    k_fill_i32<<<ceil_div(n, kTB), kTB, 0, s>>>(r->pnode.p, n, 0);
+    k_fill_inst<<<ni, kTB, 0, s>>>(t.d_inst_pt.p, r->pnode.p);
     lv.push_back(std::move(r));
   }
-  DBuf<int> small, qcode, ccnt;
-  small.alloc(8);
+  DBuf<int> seg_cnt, seg_split, qcode, ccnt;
+  seg_cnt.alloc(ni);
+  seg_split.alloc(ni + 1);
+  std::vector<int> h_split(ni + 1);
   DBuf<Q4> pre, btot, qtot;
   qtot.alloc(1);
   for (int l = 0;; ++l) {
@@ -492,19 +592,23 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs) {
     L.split.alloc(c);
     L.st.alloc(c);
     L.rank.alloc(c);
+    L.dseg.upload(L.seg, s);
     const int* pst = l ? lv[l - 1]->st.p : nullptr;
     k_split_info<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.pcount.p, L.pstart.p, L.perm.p,
                                                   L.parent_local.p, pst, t.sd, t.leaf_dofs,
                                                   L.split.p, L.st.p);
-    small.zero(s);
-    if (c <= kSortMax && l <= 20 && (int64_t)n < (1ll << 24)) {
-      int P = 1;
-      while (P < c) P <<= 1;
-      k_rank_sort<<<1, 1024, 0, s>>>(c, P, L.split.p, L.st.p, L.path.p, L.rank.p, small.p);
+    int64_t segmax = 0;
+    for (int i = 0; i < ni; ++i) segmax = std::max(segmax, L.seg[i + 1] - L.seg[i]);
+    seg_cnt.zero(s);
+    if (segmax <= kSortMax && l <= 20 && (int64_t)n < (1ll << 24)) {
+      k_rank_sort<<<ni, 1024, 0, s>>>(L.dseg.p, L.split.p, L.st.p, L.path.p, L.rank.p,
+                                      seg_cnt.p);
     } else {
-      k_rank_pairs<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.split.p, L.st.p, L.path.p, L.rank.p,
-                                                    small.p);
+      k_rank_pairs<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.inst.p, L.dseg.p, L.split.p, L.st.p,
+                                                    L.path.p, L.rank.p, seg_cnt.p);
     }
+    k_seg_scan<<<1, 64, 0, s>>>(ni, seg_cnt.p, seg_split.p);
+    if (ni > 1) k_rank_offset<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.inst.p, seg_split.p, L.rank.p);
     // classify and scan the points of splitting nodes
     const int nb = ceil_div(np, kScanTB);
     qcode.alloc(np);
@@ -515,17 +619,19 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs) {
     k_q4_block<<<nb, kScanTB, 0, s>>>(np, qcode.p, pre.p, btot.p);
     k_q4_top<<<1, 64, 0, s>>>(nb, btot.p, qtot.p);
     k_q4_apply<<<ceil_div(np + 1, kTB), kTB, 0, s>>>(np, pre.p, btot.p, qtot.p);
-    int S = 0;
     Q4 tot;
-    small.download(&S, 1, s);
+    seg_split.download(h_split.data(), ni + 1, s);
     qtot.download(&tot, 1, s);
     KI_HIP(hipStreamSynchronize(s));
     ++*syncs;
+    const int S = h_split[ni];
     if (S == 0) break;
     if (l >= 30) KI_FAIL(KIGPU_ERR_NOTSUPP, "quadtree deeper than 31 levels");
     auto nx = std::make_unique<LevelBuf>();
     LevelBuf& N = *nx;
     N.count = 4 * S;
+    N.seg.resize(ni + 1);
+    for (int i = 0; i <= ni; ++i) N.seg[i] = 4 * (int64_t)h_split[i];
     N.parent_local.alloc(N.count);
     N.slot.alloc(N.count);
     N.path.alloc(N.count);
@@ -533,9 +639,10 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs) {
     N.side.alloc(N.count);
     N.pcount.alloc(N.count);
     N.pstart.alloc(N.count);
+    N.inst.alloc(N.count);
     k_children<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.rank.p, L.corners.p, L.side.p, L.path.p,
-                                                N.parent_local.p, N.slot.p, N.path.p,
-                                                N.corners.p, N.side.p);
+                                                L.inst.p, N.parent_local.p, N.slot.p, N.path.p,
+                                                N.corners.p, N.side.p, N.inst.p);
     k_child_counts<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.rank.p, L.pstart.p, L.pcount.p, pre.p,
                                                     N.pcount.p);
     exclusive_scan_i32_to_i64(N.pcount.p, N.pstart.p, N.count, s);
@@ -563,6 +670,7 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs) {
   t.side.alloc(nn);
   t.pstart.alloc(nn);
   t.pcount.alloc(nn);
+  t.inst.alloc(nn);
   int64_t ptot = 0;
   for (auto& L : lv) ptot += L->nperm;
   t.perm_all.alloc(std::max<int64_t>(ptot, 1));
@@ -582,6 +690,8 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs) {
     f.split = L.split.p;
     f.rank = L.rank.p;
     f.pcount = L.pcount.p;
+    f.inst = L.inst.p;
+    f.instv = t.inst.p;
     f.pstart = L.pstart.p;
     f.corners = L.corners.p;
     f.side = L.side.p;
@@ -642,6 +752,8 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs) {
   t.mirrors_ready = false;
   t.h_is_leaf.resize(nn);
   t.h_pcount.resize(nn);
+  t.h_inst.resize(nn);
+  t.inst.download(t.h_inst.data(), nn, s);
   t.h_corners.resize(8 * (size_t)nn);
   t.h_side.resize(nn);
   t.is_leaf.download(t.h_is_leaf.data(), nn, s);

@@ -60,6 +60,7 @@ EXPORTS = [
     "kigpu_geometry_destroy", "kigpu_kernel_block", "kigpu_factor_export_active",
     "kigpu_solve_transpose", "kigpu_solve_transpose_device", "kigpu_kernel_apply_transpose",
     "kigpu_forward_apply_transpose", "kigpu_kernel_sensitivity", "kigpu_forward_sensitivity",
+    "kigpu_tree_build_batch", "kigpu_tree_instances",
 ]
 
 _lib = None
@@ -86,6 +87,9 @@ def load() -> C.CDLL:
         "kigpu_tree_build_device": (C.c_int, [P, P, P, P, P, P, P, C.c_int, C.c_int, C.c_int,
                                               C.POINTER(P)]),
         "kigpu_tree_destroy": (C.c_int, [P]),
+        "kigpu_tree_build_batch": (C.c_int, [P, C.c_int, _i32p, _f64p, _f64p, _f64p, _f64p,
+                                             C.c_int, C.c_int, C.POINTER(P)]),
+        "kigpu_tree_instances": (C.c_int, [P, C.POINTER(C.c_int), P, P, P]),
         "kigpu_tree_get_info": (C.c_int, [P, C.POINTER(TreeInfo)]),
         "kigpu_tree_export_nodes": (C.c_int, [P, _i64p, _i32p, _i32p, _i32p, _f64p, _f64p]),
         "kigpu_tree_export_boxes": (C.c_int, [P, _i64p, P]),
@@ -167,23 +171,53 @@ class Context:
     def build_tree(self, points, normals, weights, curvatures, sd=1, leaf_dofs=32):
         return Tree(self, points, normals, weights, curvatures, sd, leaf_dofs)
 
+    def build_batch(self, boundaries, sd=1, leaf_dofs=32):
+        """One forest over independent boundaries: a sequence of
+        (points, normals, weights, curvatures) tuples, points/normals
+        interleaved (2n,) or (n, 2)."""
+        return Tree(self, None, None, None, None, sd, leaf_dofs, batch=boundaries)
+
 
 class Tree:
     """Geometry + quadtree + factorization handle (kigpu_tree)."""
 
-    def __init__(self, ctx, points, normals, weights, curvatures, sd, leaf_dofs):
+    def __init__(self, ctx, points, normals, weights, curvatures, sd, leaf_dofs, batch=None):
         self.ctx = ctx
         self.lib = ctx.lib
-        pts, nrm = _f64(points).reshape(-1), _f64(normals).reshape(-1)
-        w, k = _f64(weights), _f64(curvatures)
-        self.npts = w.size
+        self.h = None
         self.sd = sd
-        self.ndof = sd * self.npts
         h = C.c_void_p()
-        _check(self.lib.kigpu_tree_build(ctx.h, pts, nrm, w, k, self.npts, sd, leaf_dofs,
-                                         C.byref(h)), "kigpu_tree_build")
+        if batch is None:
+            pts, nrm = _f64(points).reshape(-1), _f64(normals).reshape(-1)
+            w, k = _f64(weights), _f64(curvatures)
+            self.npts = w.size
+            _check(self.lib.kigpu_tree_build(ctx.h, pts, nrm, w, k, self.npts, sd, leaf_dofs,
+                                             C.byref(h)), "kigpu_tree_build")
+        else:
+            parts = [[_f64(a).reshape(-1) for a in b] for b in batch]
+            counts = np.array([p[2].size for p in parts], np.int32)
+            pts, nrm, w, k = (np.concatenate([p[i] for p in parts]) for i in range(4))
+            self.npts = int(counts.sum())
+            _check(self.lib.kigpu_tree_build_batch(ctx.h, len(parts), counts, pts, nrm, w, k, sd,
+                                                   leaf_dofs, C.byref(h)),
+                   "kigpu_tree_build_batch")
+        self.ndof = sd * self.npts
         self.h = h
         self.pde = None
+
+    def instances(self) -> dict:
+        """Per-instance point offsets, root-box bounds and root skeleton sizes."""
+        n = C.c_int()
+        _check(self.lib.kigpu_tree_instances(self.h, C.byref(n), None, None, None),
+               "kigpu_tree_instances")
+        off = np.zeros(n.value + 1, np.int64)
+        bounds = np.zeros(2 * n.value)
+        rs = np.zeros(n.value, np.int32)
+        _check(self.lib.kigpu_tree_instances(self.h, C.byref(n), off.ctypes.data_as(C.c_void_p),
+                                             bounds.ctypes.data_as(C.c_void_p),
+                                             rs.ctypes.data_as(C.c_void_p)),
+               "kigpu_tree_instances")
+        return {"point_offsets": off, "bounds": bounds.reshape(-1, 2), "root_skel": rs}
 
     def close(self):
         if self.h:
