@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--leaf", type=int, default=64)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the batched-instances and adjoint measurements")
     ap.add_argument("--cpu-n", type=int, default=65536)
     return ap.parse_args()
 
@@ -52,15 +54,16 @@ class DeviceProblem:
     """Geometry and right-hand sides resident in device memory (torch is only
     the allocator here; the library runs on its own HIP stream)."""
 
-    def __init__(self, torch, b, F):
+    def __init__(self, torch, geom, F):
         dev = torch.device("cuda:0")
-        xy = b.points.reshape(-1, 2)
-        nm = b.normals.reshape(-1, 2)
+        pts, nrm, w, kap = geom
+        xy = pts.reshape(-1, 2)
+        nm = nrm.reshape(-1, 2)
         self.arrays = [torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
-                       for a in (xy[:, 0], xy[:, 1], nm[:, 0], nm[:, 1], b.weights, b.curvatures)]
+                       for a in (xy[:, 0], xy[:, 1], nm[:, 0], nm[:, 1], w, kap)]
         self.F = torch.tensor(np.ascontiguousarray(F.T), dtype=torch.float64, device=dev)
         self.X = torch.zeros_like(self.F)
-        self.n = b.num_points
+        self.n = w.size
         self.nrhs = F.shape[1]
         torch.cuda.synchronize()
 
@@ -106,10 +109,9 @@ def time_path(torch, kigpu, ctx, prob, precision, args):
     return dt, tims, info, h, mu
 
 
-def interior_error(kigpu, ctx, h, mu, b):
-    from oracle import geometry as G
-    import ctypes as C
-    tgt = G.interior_disc_targets(1000, 1234)
+def interior_error(kigpu, ctx, h, mu):
+    from kigpu import workloads as W
+    tgt = W.interior_disc_targets(1000, 1234)
     xy = tgt.reshape(-1, 2)
     ex = np.log(np.sqrt((xy[:, 0] - 3) ** 2 + (xy[:, 1] - 3) ** 2)) / (2 * np.pi)
     u = np.zeros(1000)
@@ -117,6 +119,75 @@ def interior_error(kigpu, ctx, h, mu, b):
     kigpu._check(ctx.lib.kigpu_forward_apply(ctx.h, h, kigpu.LAPLACE, tgt, 1000, m0, 1, u),
                  "kigpu_forward_apply")
     return float(np.linalg.norm(u - ex) / np.linalg.norm(ex))
+
+
+def bench_batch(torch, kigpu, W, ctx, steps, warmup):
+    """256 stars r = 1 + sum_{k=2..6} a_k cos(k t + phi_k), N ~ U{1024..16384}
+    rounded to 64, Laplace, tol 1e-6, FP32 factor + FP64 refinement to 1e-10:
+    one forest, one launch per tree level over every instance."""
+    import ctypes as C
+    sizes = W.batch_sizes(256, 4)
+    geoms = [W.random_star(int(n), 4000 + i) for i, n in enumerate(sizes)]
+    pts = np.concatenate([g[0] for g in geoms]).reshape(-1, 2)
+    nrm = np.concatenate([g[1] for g in geoms]).reshape(-1, 2)
+    w = np.concatenate([g[2] for g in geoms])
+    kap = np.concatenate([g[3] for g in geoms])
+    f = np.concatenate([W.point_source_bc(g[0]) for g in geoms])
+    dev = torch.device("cuda:0")
+    arrs = [torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
+            for a in (pts[:, 0], pts[:, 1], nrm[:, 0], nrm[:, 1], w, kap)]
+    dF = torch.tensor(f, dtype=torch.float64, device=dev)
+    dX = torch.zeros_like(dF)
+    counts = sizes.astype(np.int32)
+    lib = ctx.lib
+
+    def step():
+        h = C.c_void_p()
+        kigpu._check(lib.kigpu_tree_build_batch_device(
+            ctx.h, len(counts), counts, *[C.c_void_p(a.data_ptr()) for a in arrs], 1, 64,
+            C.byref(h)), "kigpu_tree_build_batch_device")
+        opts = kigpu.FactorOpts(kigpu.LAPLACE, 1e-6, kigpu.FP32_REFINE, 16)
+        kigpu._check(lib.kigpu_factor(ctx.h, h, C.byref(opts)), "kigpu_factor")
+        ro = kigpu.RefineOpts(20, 1e-10)
+        info = kigpu.SolveInfo()
+        kigpu._check(lib.kigpu_solve_device(ctx.h, h, C.c_void_p(dF.data_ptr()), 1,
+                                            C.c_void_p(dX.data_ptr()), None, C.byref(ro),
+                                            C.byref(info)), "kigpu_solve_device")
+        lib.kigpu_tree_destroy(h)
+        return info
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        info = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    ndof = int(sizes.sum())
+    return {"workload": "256 uneven random stars, N~U{1024..16384}/64, FP32+refine to 1e-10",
+            "dofs": ndof, "s_per_step": dt, "dof_per_s": ndof / dt,
+            "refine": {"iterations": info.iterations, "rel_residual": info.rel_residual}}
+
+
+def bench_adjoint(kigpu, ctx, evals):
+    """Shape gradient of L = sum u(x_j)^2 (1024 targets) in 16 Fourier
+    coefficients of a star with N = 32768: factor, solve, transposed solve,
+    sensitivities (FP32 factor refined to 1e-12)."""
+    from kigpu import shape
+    from kigpu import workloads as W
+    c = 0.02 * (2 * W.uniform01(3, 16) - 1)
+    tgt = W.interior_disc_targets(1024, 33, radius=0.3)
+    star = shape.FourierStar(32768, 16)
+    kw = dict(id_tol=1e-10, precision=kigpu.FP32_REFINE, refine_iters=10, rel_residual=1e-12)
+    shape.loss_and_gradient(ctx, star, c, tgt, **kw)
+    t0 = time.perf_counter()
+    for _ in range(evals):
+        L, g, info = shape.loss_and_gradient(ctx, star, c, tgt, **kw)
+    dt = (time.perf_counter() - t0) / evals
+    return {"workload": "star N=32768, 16 coefficients, 1024 targets, id_tol 1e-10",
+            "s_per_gradient": dt, "loss": L, "grad_norm": float(np.linalg.norm(g)),
+            "note": "host arrays in and out (includes PCIe copies)"}
 
 
 def cpu_baseline(n, leaf, tol):
@@ -143,18 +214,18 @@ def main():
         sys.exit(2)
     import torch
     import kigpu
-    from oracle import geometry as G
+    from kigpu import workloads as W
 
-    b = G.star(args.n)
-    srcs = [(3.0, 3.0)] + [tuple(s) for s in G.point_sources(max(args.nrhs - 1, 0), 7)]
-    F = np.stack([G.bc_point_source(b, s) for s in srcs[: args.nrhs]], 1)
+    pts, nrm, w, kap = W.star(args.n)
+    srcs = [(3.0, 3.0)] + [tuple(s) for s in W.point_sources(max(args.nrhs - 1, 0), 7)]
+    F = np.stack([W.point_source_bc(pts, s) for s in srcs[: args.nrhs]], 1)
     ctx = kigpu.Context(0)
-    prob = DeviceProblem(torch, b, F)
+    prob = DeviceProblem(torch, (pts, nrm, w, kap), F)
 
     res = {}
     for name, prec in (("fp64", kigpu.FP64), ("fp32_refine", kigpu.FP32_REFINE)):
         dt, tims, info, h, mu = time_path(torch, kigpu, ctx, prob, prec, args)
-        err = interior_error(kigpu, ctx, h, mu, b)
+        err = interior_error(kigpu, ctx, h, mu)
         ctx.lib.kigpu_tree_destroy(h)
         tm = {k: float(np.median([t[k] for t in tims])) for k in tims[0]}
         res[name] = {"s_per_step": dt, "dof_per_s": args.n / dt, "interior_rel_err": err,
@@ -190,6 +261,9 @@ def main():
         "speedup_fp32_refine_vs_fp64": res["fp64"]["s_per_step"] / fast["s_per_step"],
         "roofline": roofline,
     }
+    if not args.no_extra:
+        out["batch"] = bench_batch(torch, kigpu, W, ctx, 2, 1)
+        out["adjoint"] = bench_adjoint(kigpu, ctx, 2)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.leaf, args.tol)
     print(json.dumps(out))

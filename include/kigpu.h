@@ -128,6 +128,13 @@ kigpu_status kigpu_tree_build_batch(kigpu_ctx* ctx, int ninst, const int* npts,
                                     const double* points, const double* normals,
                                     const double* weights, const double* curvatures, int sd,
                                     int leaf_dofs, kigpu_tree** tree);
+/* the same from device arrays (SoA, resident in HBM, instances one after
+ * another); npts is a host array */
+kigpu_status kigpu_tree_build_batch_device(kigpu_ctx* ctx, int ninst, const int* npts,
+                                           const double* d_px, const double* d_py,
+                                           const double* d_nx, const double* d_ny,
+                                           const double* d_w, const double* d_kappa, int sd,
+                                           int leaf_dofs, kigpu_tree** tree);
 /* ninst; point_off[ninst + 1]; bounds[2 * ninst] = root-box min/max
  * (QuadTree::min/max per instance); root_skel[ninst] = |allskel| per
  * instance after kigpu_factor (-1 before).  Output pointers may be NULL. */

@@ -280,6 +280,34 @@ kigpu_status kigpu_tree_build_batch(kigpu_ctx* ctx, int ninst, const int* npts,
       &off);
 }
 
+kigpu_status kigpu_tree_build_batch_device(kigpu_ctx* ctx, int ninst, const int* npts,
+                                           const double* d_px, const double* d_py,
+                                           const double* d_nx, const double* d_ny,
+                                           const double* d_w, const double* d_kappa, int sd,
+                                           int leaf_dofs, kigpu_tree** out) {
+  KI_REQUIRE(ninst > 0 && npts, "invalid instance count");
+  KI_REQUIRE(d_px && d_py && d_nx && d_ny && d_w && d_kappa, "null geometry array");
+  std::vector<int64_t> off(ninst + 1, 0);
+  for (int i = 0; i < ninst; ++i) {
+    KI_REQUIRE(npts[i] > 0, "number of boundary points to init tree cannot be 0");
+    off[i + 1] = off[i] + npts[i];
+  }
+  KI_REQUIRE(off[ninst] < (1ll << 30), "too many points in the batch");
+  const int total = (int)off[ninst];
+  return tree_build_common(
+      ctx, total, sd, leaf_dofs, out,
+      [&](TreeData& t, hipStream_t s) {
+        const size_t b = sizeof(double) * total;
+        DBuf<double>* dst[6] = {&t.px, &t.py, &t.nx, &t.ny, &t.w, &t.kappa};
+        const double* src[6] = {d_px, d_py, d_nx, d_ny, d_w, d_kappa};
+        for (int i = 0; i < 6; ++i) {
+          dst[i]->alloc(total);
+          KI_HIP(hipMemcpyAsync(dst[i]->p, src[i], b, hipMemcpyDeviceToDevice, s));
+        }
+      },
+      &off);
+}
+
 kigpu_status kigpu_tree_instances(const kigpu_tree* tree, int* ninst, int64_t* point_off,
                                   double* bounds, int* root_skel) {
   KI_REQUIRE(tree && ninst, "null argument");

@@ -369,6 +369,95 @@ __global__ __launch_bounds__(kAT) void k_apply_t(Targets rows, Tiles tl, GeomVie
   }
 }
 
+// 1/d to full FP64 precision for d > 0: the hardware reciprocal (about 2^-23
+// relative error) and two Newton steps r <- r + r (1 - d r), each squaring the
+// relative error.
+__device__ __forceinline__ double recip(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+
+// Laplace double layer, one right-hand side (the refinement residual and the
+// interior evaluation).  Per source s the LDS tile holds (x, y), the scaled
+// normal a = -w n / (2 pi) and (diagonal value, density), so a pair costs
+// three 16-byte LDS reads and
+//   q = (r . a) / |r|^2,  or the diagonal value where r is exactly zero
+// (the entries of kernel.cpp:11-36 up to rounding).
+// TR = false: own points are targets, other points sources, in = mu.
+// TR = true:  own points are sources, other points targets, in = lam (rows).
+template <bool TR>
+__global__ __launch_bounds__(kAT) void k_apply_dl(Targets other_or_tg, Tiles tl, GeomView g,
+                                                  const double* in, double* part, int ldp) {
+  __shared__ double2 s_p[kAT], s_a[kAT], s_d[kAT];
+  const int oi = tl.t0[blockIdx.x] + threadIdx.x;
+  const bool valid = oi < tl.t1[blockIdx.x];
+  // own point
+  double ox = 0, oy = 0, oax = 0, oay = 0, od = 0;
+  if (valid) {
+    if (TR) {
+      const double cw = -g.w[oi] * kInv2Pi;
+      ox = g.px[oi];
+      oy = g.py[oi];
+      oax = cw * g.nx[oi];
+      oay = cw * g.ny[oi];
+      od = 0.5 + 0.5 * g.kappa[oi] * g.w[oi] * kInv2Pi;
+    } else {
+      ox = other_or_tg.x[oi];
+      oy = other_or_tg.y[oi];
+    }
+  }
+  int j0, j1;
+  split_range(tl, &j0, &j1);
+  double acc = 0;
+  for (int b0 = j0; b0 < j1; b0 += kAT) {
+    const int nb = min(kAT, j1 - b0);
+    __syncthreads();
+    if (threadIdx.x < nb) {
+      const int j = b0 + threadIdx.x;
+      if (TR) {
+        s_p[threadIdx.x] = make_double2(other_or_tg.x[j], other_or_tg.y[j]);
+        s_d[threadIdx.x] = make_double2(0.0, in[j]);
+      } else {
+        const double w = g.w[j];
+        const double cw = -w * kInv2Pi;
+        s_p[threadIdx.x] = make_double2(g.px[j], g.py[j]);
+        s_a[threadIdx.x] = make_double2(cw * g.nx[j], cw * g.ny[j]);
+        s_d[threadIdx.x] = make_double2(0.5 + 0.5 * g.kappa[j] * w * kInv2Pi, in[j]);
+      }
+    }
+    __syncthreads();
+    if (!valid) continue;
+#pragma unroll 4
+    for (int jj = 0; jj < nb; ++jj) {
+      const double2 p = s_p[jj];
+      const double2 dm = s_d[jj];
+      double dx, dy, ax, ay, dv;
+      if (TR) {  // target = row point jj, source = own
+        dx = p.x - ox;
+        dy = p.y - oy;
+        ax = oax;
+        ay = oay;
+        dv = od;
+      } else {   // target = own, source = jj
+        const double2 a = s_a[jj];
+        dx = ox - p.x;
+        dy = oy - p.y;
+        ax = a.x;
+        ay = a.y;
+        dv = dm.x;
+      }
+      const double den = fma(dx, dx, dy * dy);
+      const double num = fma(dx, ax, dy * ay);
+      const double q = den != 0.0 ? num * recip(den) : dv;
+      acc = fma(q, dm.y, acc);
+    }
+  }
+  if (valid) part[(size_t)blockIdx.y * ldp + oi] = acc;
+}
+
 __global__ void k_sum_parts(const double* part, int nsplit, size_t len, double* y) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= len) return;
@@ -571,6 +660,15 @@ void run_apply(Context& c, const TreeData& t, int pde, bool fwd, const Targets& 
   part.alloc(len * split);
   dim3 grid(tb.ntiles, split);
   const Tiles tl = tb.view();
+  if (pde == KIGPU_LAPLACE && nrhs == 1) {
+    if (TRANS)
+      k_apply_dl<true><<<grid, kAT, 0, s>>>(other, tl, g, d_in, part.p, ldo);
+    else
+      k_apply_dl<false><<<grid, kAT, 0, s>>>(other, tl, g, d_in, part.p, ldo);
+    k_sum_parts<<<ceil_div((long)len, 256), 256, 0, s>>>(part.p, split, len, d_out);
+    KI_HIP(hipGetLastError());
+    return;
+  }
 #define KI_RUN(P, F)                                                                         \
   do {                                                                                       \
     if (TRANS)                                                                               \

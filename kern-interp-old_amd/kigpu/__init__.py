@@ -60,7 +60,7 @@ EXPORTS = [
     "kigpu_geometry_destroy", "kigpu_kernel_block", "kigpu_factor_export_active",
     "kigpu_solve_transpose", "kigpu_solve_transpose_device", "kigpu_kernel_apply_transpose",
     "kigpu_forward_apply_transpose", "kigpu_kernel_sensitivity", "kigpu_forward_sensitivity",
-    "kigpu_tree_build_batch", "kigpu_tree_instances",
+    "kigpu_tree_build_batch", "kigpu_tree_build_batch_device", "kigpu_tree_instances",
 ]
 
 _lib = None
@@ -90,6 +90,8 @@ def load() -> C.CDLL:
         "kigpu_tree_build_batch": (C.c_int, [P, C.c_int, _i32p, _f64p, _f64p, _f64p, _f64p,
                                              C.c_int, C.c_int, C.POINTER(P)]),
         "kigpu_tree_instances": (C.c_int, [P, C.POINTER(C.c_int), P, P, P]),
+        "kigpu_tree_build_batch_device": (C.c_int, [P, C.c_int, _i32p, P, P, P, P, P, P, C.c_int,
+                                                    C.c_int, C.POINTER(P)]),
         "kigpu_tree_get_info": (C.c_int, [P, C.POINTER(TreeInfo)]),
         "kigpu_tree_export_nodes": (C.c_int, [P, _i64p, _i32p, _i32p, _i32p, _f64p, _f64p]),
         "kigpu_tree_export_boxes": (C.c_int, [P, _i64p, P]),
