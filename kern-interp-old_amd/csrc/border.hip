@@ -80,15 +80,19 @@ template <bool LDS>
 __global__ __launch_bounds__(1024) void k_dense_inverse(const double* A, int n, double* Ainv,
                                                         int* status) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ int piv[1024];
-  __shared__ double prow[1024], fcol[1024];
+  __shared__ int piv[1024], sig[1024];
+  __shared__ double prow[1024], fcol[1024], orow[1024];
   __shared__ int s_p;
   double* W = LDS ? reinterpret_cast<double*>(smem) : Ainv;
   for (int e = threadIdx.x; e < n * n; e += blockDim.x) W[e] = A[e];
   __syncthreads();
-  if (!block_gj_inverse(W, n, piv, prow, fcol, &s_p) && threadIdx.x == 0) atomicOr(status, 16);
-  if (LDS)
-    for (int e = threadIdx.x; e < n * n; e += blockDim.x) Ainv[e] = W[e];
+  if (LDS) {  // columns left permuted; applied on the way out
+    if (!block_gj_inverse_perm(W, n, piv, sig, prow, orow, fcol, &s_p) && threadIdx.x == 0)
+      atomicOr(status, 16);
+    for2d<16>(n, n, [&](int i, int j) { Ainv[i + (size_t)j * n] = W[i + (size_t)sig[j] * n]; });
+  } else if (!block_gj_inverse(W, n, piv, prow, fcol, &s_p) && threadIdx.x == 0) {
+    atomicOr(status, 16);
+  }
 }
 
 // M = [x(allskel, :); g - Psi'(:, red) x(red, :)] with gram = Psi'(:, red) x(red, :)
@@ -177,7 +181,7 @@ void border_factor(Context& c, TreeData& t) {
   k_assemble_s<T><<<nS, 256, 0, s>>>(ns, h, nd, t.root_dofs.p, t.root_mat.p, sU, sP, G.p, S.p);
   t.Sinv.alloc((size_t)nS * nS);
   const size_t lb = sizeof(double) * (size_t)nS * nS;
-  if (lb + 24576 <= c.lds_max) {
+  if (lb + 40960 <= c.lds_max) {
     allow_lds(k_dense_inverse<true>, lb);
     k_dense_inverse<true><<<1, 1024, lb, s>>>(S.p, nS, t.Sinv.p, t.status.p);
   } else {

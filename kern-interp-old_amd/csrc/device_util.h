@@ -5,24 +5,72 @@
 
 namespace kigpu {
 
+// Cross-lane steps with DPP (no LDS round trip): quad_perm xor 1 and xor 2,
+// then row_ror 4 and 8 give every lane the result of its 16-lane row; the
+// four rows are combined through readlane, so every lane gets the same,
+// deterministically ordered result.  All 64 lanes must be active.
+namespace dpp {
+constexpr int kXor1 = 0xB1;   // quad_perm [1, 0, 3, 2]
+constexpr int kXor2 = 0x4E;   // quad_perm [2, 3, 0, 1]
+constexpr int kRor4 = 0x124;  // row_ror:4
+constexpr int kRor8 = 0x128;  // row_ror:8
+
+template <int CTRL>
+__device__ __forceinline__ int mov(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float mov(float v) {
+  return __int_as_float(mov<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ double mov(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = mov<CTRL>((int)b), hi = mov<CTRL>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ int lane_value(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float lane_value(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+}  // namespace dpp
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v += dpp::mov<dpp::kXor1>(v);
+  v += dpp::mov<dpp::kXor2>(v);
+  v += dpp::mov<dpp::kRor4>(v);
+  v += dpp::mov<dpp::kRor8>(v);
+  return (dpp::lane_value(v, 0) + dpp::lane_value(v, 16)) +
+         (dpp::lane_value(v, 32) + dpp::lane_value(v, 48));
 }
 
 // argmax over (value, index): larger value wins, ties go to the smaller index
-__device__ __forceinline__ void wave_argmax(double& v, int& idx) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    double v2 = __shfl_xor(v, o, 64);
-    int i2 = __shfl_xor(idx, o, 64);
-    if (v2 > v || (v2 == v && i2 < idx)) {
-      v = v2;
-      idx = i2;
-    }
+__device__ __forceinline__ void argmax_step(double& v, int& idx, double v2, int i2) {
+  if (v2 > v || (v2 == v && i2 < idx)) {
+    v = v2;
+    idx = i2;
   }
+}
+
+__device__ __forceinline__ void wave_argmax(double& v, int& idx) {
+  argmax_step(v, idx, dpp::mov<dpp::kXor1>(v), dpp::mov<dpp::kXor1>(idx));
+  argmax_step(v, idx, dpp::mov<dpp::kXor2>(v), dpp::mov<dpp::kXor2>(idx));
+  argmax_step(v, idx, dpp::mov<dpp::kRor4>(v), dpp::mov<dpp::kRor4>(idx));
+  argmax_step(v, idx, dpp::mov<dpp::kRor8>(v), dpp::mov<dpp::kRor8>(idx));
+  double bv = dpp::lane_value(v, 0);
+  int bi = dpp::lane_value(idx, 0);
+  argmax_step(bv, bi, dpp::lane_value(v, 16), dpp::lane_value(idx, 16));
+  argmax_step(bv, bi, dpp::lane_value(v, 32), dpp::lane_value(idx, 32));
+  argmax_step(bv, bi, dpp::lane_value(v, 48), dpp::lane_value(idx, 48));
+  v = bv;
+  idx = bi;
 }
 
 // sum over a workgroup of NW waves; `red` holds NW doubles of LDS.  Every
@@ -287,6 +335,108 @@ __device__ bool block_gj_inverse(T* A, int n, int* piv, T* prow, T* fcol, int* s
       }
     __syncthreads();
   }
+  return ok;
+}
+
+// Gauss-Jordan inverse with two barriers per column.  The row interchange
+// of step c is folded into the elimination (the new pivot row is read from
+// row p, the old row c is captured in orow and becomes row p), and the wave
+// that eliminates column c + 1 also finds the next pivot, so a step is:
+// capture [barrier] eliminate + next pivot [barrier].  Pivots follow
+// dgetrf: first largest |A(i, c)|, i >= c, of the current matrix; piv[c]
+// records them.  The column interchanges that finish the inverse are not
+// applied: on return inv(A)(i, j) = A[i + sigma[j] * n].  prow, orow, fcol
+// are n-element LDS scratch vectors, piv and sigma n-element LDS int arrays.
+// Returns false on a zero pivot.
+template <typename T>
+__device__ bool block_gj_inverse_perm(T* __restrict__ A, int n, int* __restrict__ piv,
+                                      int* __restrict__ sigma, T* __restrict__ prow,
+                                      T* __restrict__ orow, T* __restrict__ fcol, int* s_p) {
+  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
+  const int nw = nt >> 6;
+  bool ok = true;
+  if (wv == 0) {  // pivot of column 0
+    double best = -1.0;
+    int bi = n;
+    for (int i = lane; i < n; i += 64) {
+      double v = fabs((double)A[i]);
+      if (v > best) {
+        best = v;
+        bi = i;
+      }
+    }
+    wave_argmax(best, bi);
+    if (lane == 0) *s_p = bi;
+  }
+  __syncthreads();
+  for (int c = 0; c < n; ++c) {
+    const int p = *s_p;
+    const T d = A[p + (size_t)c * n];
+    if (d == T(0)) ok = false;
+    const T inv = T(1) / d;
+    for (int j = tid; j < n; j += nt) {
+      prow[j] = j == c ? inv : A[p + (size_t)j * n] * inv;
+      orow[j] = A[c + (size_t)j * n];
+    }
+    for (int i = tid; i < n; i += nt) fcol[i] = A[i + (size_t)c * n];
+    if (tid == 0) piv[c] = p;
+    __syncthreads();
+    const T fp = orow[c];  // A(c, c) before the interchange: row p's multiplier
+    // Eliminate.  Wave w takes columns j(jj) = (c + 1 + jj) mod n for
+    // jj = w, w + nw, ...; its lanes take rows.  Loads of a batch of four
+    // columns are issued before any store so that they overlap.  The wave
+    // with jj = 0 (column c + 1) also finds the next pivot.
+    double best = -1.0;
+    int bi = n;
+    for (int i = lane; i < n; i += 64) {
+      const bool rp = i == p, rc = i == c;
+      const T f = rp ? fp : fcol[i];
+      for (int jb = wv; jb < n; jb += 4 * nw) {
+        T src[4], pj[4];
+        int jx[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int jj = jb + u * nw;
+          int j = c + 1 + jj;
+          if (j >= n) j -= n;
+          jx[u] = jj < n ? j : -1;
+          if (jj < n) {
+            pj[u] = prow[j];
+            src[u] = rp ? orow[j] : A[i + (size_t)j * n];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int j = jx[u];
+          if (j < 0) continue;
+          const T v = rc ? pj[u] : (j == c ? -f * inv : src[u] - f * pj[u]);
+          A[i + (size_t)j * n] = v;
+          if (jb + u * nw == 0 && i > c) {  // column c + 1
+            const double a = fabs((double)v);
+            if (a > best) {
+              best = a;
+              bi = i;
+            }
+          }
+        }
+      }
+    }
+    if (wv == 0 && c + 1 < n) {
+      wave_argmax(best, bi);
+      if (lane == 0) *s_p = bi;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {  // the column interchanges, in reverse order, as a permutation
+    for (int j = 0; j < n; ++j) sigma[j] = j;
+    for (int c = n - 1; c >= 0; --c) {
+      const int q = piv[c];
+      const int t0 = sigma[c];
+      sigma[c] = sigma[q];
+      sigma[q] = t0;
+    }
+  }
+  __syncthreads();
   return ok;
 }
 

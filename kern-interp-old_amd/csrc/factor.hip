@@ -491,7 +491,7 @@ __global__ __launch_bounds__(kIdTB) void k_id(const IdWork* work, GeomView g, in
 // ---------------------------------------------------------------------------
 
 struct DecLds {
-  size_t uact, kp, x, krsn, ksnr, tm, lm, sr, pos, piv, total;
+  size_t uact, kp, x, krsn, ksnr, tm, lm, sr, pos, piv, sig, total;
 };
 
 __host__ __device__ inline DecLds dec_layout(int n, size_t tsz) {
@@ -508,6 +508,7 @@ __host__ __device__ inline DecLds dec_layout(int n, size_t tsz) {
   L.sr = o;   o = align16(o + sizeof(int) * (size_t)n);
   L.pos = o;  o = align16(o + sizeof(int) * (size_t)n);
   L.piv = o;  o = align16(o + sizeof(int) * (size_t)n);
+  L.sig = o;  o = align16(o + sizeof(int) * (size_t)n);
   L.total = o;
   return L;
 }
@@ -590,6 +591,7 @@ __global__ __launch_bounds__(kDecTB) void k_decouple(int a, GeomView g, int pde,
   int* sr = reinterpret_cast<int*>(base + Ly.sr);
   int* pos = reinterpret_cast<int*>(base + Ly.pos);
   int* piv = reinterpret_cast<int*>(base + Ly.piv);
+  int* sig = reinterpret_cast<int*>(base + Ly.sig);
   const int64_t aoff = act_off[x];
   for (int j = tid; j < n; j += kDecTB) {
     sr[j] = sr_pool[aoff + j];
@@ -634,12 +636,15 @@ __global__ __launch_bounds__(kDecTB) void k_decouple(int a, GeomView g, int pde,
   T* Xc = Kp;
   for (int e = tid; e < r * r; e += kDecTB) Xc[e] = X[e];
   __syncthreads();
-  // X_rr^{-1} in place (Gauss-Jordan with dgetrf's pivot order)
+  // X_rr^{-1} in place (Gauss-Jordan with dgetrf's pivot order), columns
+  // left permuted: X_rr^{-1}(i, j) = Z[i + sig[j] r].  Scratch: two vectors
+  // after the copy of X_rr, one after X_rr (r^2 + r <= n^2).
   T* Z = X;
   __shared__ int s_p;
-  if (!block_gj_inverse(X, r, piv, Xc + (size_t)r * r, Xc + (size_t)r * r + r, &s_p) && tid == 0)
+  if (!block_gj_inverse_perm(X, r, piv, sig, Xc + (size_t)r * r, X + (size_t)r * r,
+                             Xc + (size_t)r * r + r, &s_p) &&
+      tid == 0)
     flag(status, 4);
-  __syncthreads();
   // L0 = K_sn_r X_rr^{-1} (k x r), U = X_rr^{-1} K_r_sn (r x k).  The right
   // residual of an explicit inverse is small but its left residual is not:
   // multiplied from the right, X_rr^{-1} loses digits when X_rr is
@@ -647,16 +652,17 @@ __global__ __launch_bounds__(kDecTB) void k_decouple(int a, GeomView g, int pde,
   // X_rr below, which brings it to the accuracy of the dgetrs solve of
   // skel_factorization.cpp:98-103.
   for2d<kDecNW>(k, r, [&](int i, int j) {
+    const T* zj = Z + (size_t)sig[j] * r;
     T s = T(0);
-    for (int t = 0; t < r; ++t) s += Ksnr[i + (size_t)t * k] * Z[t + (size_t)j * r];
+    for (int t = 0; t < r; ++t) s += Ksnr[i + (size_t)t * k] * zj[t];
     Ls[i + (size_t)j * k] = s;
   });
   for2d<kDecNW>(r, k, [&](int i, int j) {
     T s = T(0);
-    for (int t = 0; t < r; ++t) s += Z[i + (size_t)t * r] * Krsn[t + (size_t)j * r];
+    for (int t = 0; t < r; ++t) s += Z[i + (size_t)sig[t] * r] * Krsn[t + (size_t)j * r];
     Um[i + (size_t)j * r] = s;
   });
-  for (int e = tid; e < r * r; e += kDecTB) Xinv[e] = Z[e];
+  for2d<kDecNW>(r, r, [&](int i, int j) { Xinv[i + (size_t)j * r] = Z[i + (size_t)sig[j] * r]; });
   for (int c = tid; c < r; c += kDecTB) piv_pool[aoff + c] = piv[c];
   __syncthreads();
   // R = K_sn_r - L0 X_rr in place of K_sn_r (each entry reads only itself)
@@ -668,8 +674,9 @@ __global__ __launch_bounds__(kDecTB) void k_decouple(int a, GeomView g, int pde,
   __syncthreads();
   // L = L0 + R X_rr^{-1}
   for2d<kDecNW>(k, r, [&](int i, int j) {
+    const T* zj = Z + (size_t)sig[j] * r;
     T s = Ls[i + (size_t)j * k];
-    for (int t = 0; t < r; ++t) s += Ksnr[i + (size_t)t * k] * Z[t + (size_t)j * r];
+    for (int t = 0; t < r; ++t) s += Ksnr[i + (size_t)t * k] * zj[t];
     Ls[i + (size_t)j * k] = s;
     Lm[i + (size_t)j * k] = s;
   });
@@ -730,10 +737,12 @@ __global__ __launch_bounds__(1024) void k_root(GeomView g, int pde, const int* c
   __syncthreads();
   for (int e = tid; e < n * n; e += nt) Amat[e] = A[e];
   // allskel_mat^{-1} in place (Gauss-Jordan, dgetrf's pivot order), then out
+  // with the column permutation applied
   __shared__ int s_p;
-  __shared__ double prow[1024], fcol[1024];
-  if (!block_gj_inverse(A, n, piv, prow, fcol, &s_p) && tid == 0) flag(status, 8);
-  for (int e = tid; e < n * n; e += nt) Ainv[e] = A[e];
+  __shared__ int sig[1024];
+  __shared__ double prow[1024], orow[1024], fcol[1024];
+  if (!block_gj_inverse_perm(A, n, piv, sig, prow, orow, fcol, &s_p) && tid == 0) flag(status, 8);
+  for2d<16>(n, n, [&](int i, int j) { Ainv[i + (size_t)j * n] = A[i + (size_t)sig[j] * n]; });
   for (int i = tid; i < n; i += nt) piv_out[i] = piv[i];
 }
 
@@ -1029,7 +1038,7 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     t.root_mat.alloc(msz);  // allskel_mat
     t.root_piv.alloc(t.nroot);
     const size_t lbytes = sizeof(double) * (size_t)t.nroot_max * t.nroot_max;
-    if (lbytes + 24576 <= lds_cap) {  // static LDS of k_root: pivots + 2 scratch rows
+    if (lbytes + 40960 <= lds_cap) {  // static LDS of k_root: 2 int + 3 scratch vectors
       allow_lds(k_root<T, true>, lbytes);
       k_root<T, true><<<ni, 1024, lbytes, s>>>(g, pde, t.child.p, t.osize.p, t.acc_ptr.p,
                                                t.acc_ld.p, t.root_n.p, t.root_doff.p,
