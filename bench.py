@@ -190,6 +190,21 @@ def bench_adjoint(kigpu, ctx, evals):
             "note": "host arrays in and out (includes PCIe copies)"}
 
 
+def apply_traffic():
+    """HBM bytes per launch of k_apply_dl2 from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
+    or None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "pmc_apply_dl2_laplace_star_N65536.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(n, leaf, tol):
     """The oracle (numpy/scipy restatement, single-threaded BLAS) on the same workload."""
     from oracle import geometry as G
@@ -233,15 +248,31 @@ def main():
                          "t_tree", "t_lists", "t_id", "t_decouple", "t_root", "t_solve",
                          "t_refine_apply")}, "timings": tm}
 
-    # roofline of the dominant kernel (the LDS-resident ID kernel, FP64)
-    tm = res["fp64"]["timings"]
-    per_launch_flops = tm["id_flops"] / max(tm["id_launches"], 1)
-    per_launch_s = tm["id_kernel_s"] / max(tm["id_launches"], 1)
+    # roofline of the kernel that dominates the FP32+refinement step: the FP64
+    # matrix-free apply of the refinement residual (k_apply_dl2), timed with
+    # HIP events on the library's stream.  Algorithmic work: 12 flops per
+    # (target, source) pair (c_K = 10 for the Laplace entry, 2 for the
+    # multiply-add with the density).
+    tf = res["fp32_refine"]["timings"]
+    launches = max(tf["apply_launches"], 1)
+    per_launch_s = tf["apply_kernel_s"] / launches
+    per_launch_flops = 12.0 * tf["apply_pairs"] / launches
     achieved = per_launch_flops / per_launch_s / 1e12 if per_launch_s > 0 else 0.0
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                "kernel": "k_id<double,double,true> (ID matrix + pivoted QR + T, FP64)",
-                "avg_launch_us": per_launch_s * 1e6, "launches_per_factor": tm["id_launches"]}
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                "traffic": apply_traffic(),
+                "kernel": "k_apply_dl2 (FP64 Laplace double-layer apply, refinement residual; "
+                          "compute bound on the FP64 vector units)",
+                "avg_launch_us": per_launch_s * 1e6, "launches_per_step": tf["apply_launches"],
+                "flops_per_launch": per_launch_flops}
+    # the largest factorization kernel beside it (LDS-resident ID, FP64 path)
+    tm = res["fp64"]["timings"]
+    id_s = tm["id_kernel_s"] / max(tm["id_launches"], 1)
+    id_f = tm["id_flops"] / max(tm["id_launches"], 1)
+    id_kernel = {"kernel": "k_id<double,double,true> (ID matrix + pivoted QR + T)",
+                 "achieved": id_f / id_s / 1e12 if id_s > 0 else 0.0, "peak": PEAK_FP64_TFLOPS,
+                 "unit": "TFLOP/s", "avg_launch_us": id_s * 1e6,
+                 "launches_per_factor": tm["id_launches"]}
 
     fast = res["fp32_refine"]
     out = {
@@ -260,6 +291,7 @@ def main():
                                              "refine", "phases_s")},
         "speedup_fp32_refine_vs_fp64": res["fp64"]["s_per_step"] / fast["s_per_step"],
         "roofline": roofline,
+        "id_kernel": id_kernel,
     }
     if not args.no_extra:
         out["batch"] = bench_batch(torch, kigpu, W, ctx, 2, 1)

@@ -87,6 +87,10 @@ typedef struct {
   /* the same for the elimination kernel */
   int dec_launches;
   double dec_kernel_s, dec_flops;
+  /* FP64 refinement applies (Laplace, one right-hand side): launches, their
+   * summed kernel time and the number of (target, source) pairs */
+  int apply_launches;
+  double apply_kernel_s, apply_pairs;
 } kigpu_timings;
 
 /* ---- context ---------------------------------------------------------- */

@@ -458,6 +458,73 @@ __global__ __launch_bounds__(kAT) void k_apply_dl(Targets other_or_tg, Tiles tl,
   if (valid) part[(size_t)blockIdx.y * ldp + oi] = acc;
 }
 
+// K mu for the Laplace double layer when no two nodes coincide (the
+// refinement residual): two targets per thread, tiles of 2 kAT targets, so
+// each staged source serves both.  |r| = 0 then happens only for a node and
+// itself: the loop clamps |r|^2 away from zero (that pair contributes
+// 0 * 1e300 = 0) and the node's own diagonal 1/2 + kappa w / (4 pi) is added
+// once, by the split whose source range holds the node.
+constexpr double kTinyR2 = 1e-300;
+
+__global__ __launch_bounds__(kAT) void k_apply_dl2(Tiles tl, GeomView g, const double* mu,
+                                                   double* part, int ldp) {
+  __shared__ double2 s_p[kAT], s_a[kAT];
+  __shared__ double s_m[kAT];
+  const int t1 = tl.t1[blockIdx.x];
+  const int i0 = tl.t0[blockIdx.x] + threadIdx.x, i1 = i0 + kAT;
+  const bool v0 = i0 < t1, v1 = i1 < t1;
+  const double x0 = v0 ? g.px[i0] : 0, y0 = v0 ? g.py[i0] : 0;
+  const double x1 = v1 ? g.px[i1] : 0, y1 = v1 ? g.py[i1] : 0;
+  int j0, j1;
+  split_range(tl, &j0, &j1);
+  double a0 = 0, a1 = 0;
+  for (int b0 = j0; b0 < j1; b0 += kAT) {
+    const int nb = min(kAT, j1 - b0);
+    __syncthreads();
+    if (threadIdx.x < nb) {
+      const int j = b0 + threadIdx.x;
+      const double cw = -g.w[j] * kInv2Pi;
+      s_p[threadIdx.x] = make_double2(g.px[j], g.py[j]);
+      s_a[threadIdx.x] = make_double2(cw * g.nx[j], cw * g.ny[j]);
+      s_m[threadIdx.x] = mu[j];
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int jj = 0; jj < nb; ++jj) {
+      const double2 p = s_p[jj];
+      const double2 a = s_a[jj];
+      const double m = s_m[jj];
+      const double dx0 = x0 - p.x, dy0 = y0 - p.y;
+      const double dx1 = x1 - p.x, dy1 = y1 - p.y;
+      const double d0 = fmax(fma(dx0, dx0, dy0 * dy0), kTinyR2);
+      const double d1 = fmax(fma(dx1, dx1, dy1 * dy1), kTinyR2);
+      const double n0 = fma(dx0, a.x, dy0 * a.y);
+      const double n1 = fma(dx1, a.x, dy1 * a.y);
+      a0 = fma(n0 * recip(d0), m, a0);
+      a1 = fma(n1 * recip(d1), m, a1);
+    }
+  }
+  if (v0 && i0 >= j0 && i0 < j1) a0 += (0.5 + 0.5 * g.kappa[i0] * g.w[i0] * kInv2Pi) * mu[i0];
+  if (v1 && i1 >= j0 && i1 < j1) a1 += (0.5 + 0.5 * g.kappa[i1] * g.w[i1] * kInv2Pi) * mu[i1];
+  if (v0) part[(size_t)blockIdx.y * ldp + i0] = a0;
+  if (v1) part[(size_t)blockIdx.y * ldp + i1] = a1;
+}
+
+// flag = 1 if two points of one leaf have identical coordinates (coincident
+// points always share a leaf); one 64-thread block per node
+__global__ void k_dup_points(int nnodes, const int* is_leaf, const int64_t* pstart,
+                             const int* pcount, const int* perm, const double* px,
+                             const double* py, int* flag) {
+  const int x = blockIdx.x;
+  if (x >= nnodes || !is_leaf[x]) return;
+  const int c = pcount[x];
+  const int* pp = perm + pstart[x];
+  for (int e = threadIdx.x; e < c * c; e += blockDim.x) {
+    const int a = e / c, b = e - a * c;
+    if (a < b && px[pp[a]] == px[pp[b]] && py[pp[a]] == py[pp[b]]) atomicOr(flag, 1);
+  }
+}
+
 __global__ void k_sum_parts(const double* part, int nsplit, size_t len, double* y) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= len) return;
@@ -610,14 +677,15 @@ struct TileBuf {
   DBuf<int64_t> opt;
   int ntiles = 0;
   int maxlen = 0;  // longest other-segment
-  void build(const std::vector<int64_t>& own, const std::vector<int64_t>& other, hipStream_t s) {
+  void build(const std::vector<int64_t>& own, const std::vector<int64_t>& other, hipStream_t s,
+             int tile = kAT) {
     std::vector<int> hi, h0, h1;
     maxlen = 0;
     for (size_t b = 0; b + 1 < own.size(); ++b) {
-      for (int64_t p = own[b]; p < own[b + 1]; p += kAT) {
+      for (int64_t p = own[b]; p < own[b + 1]; p += tile) {
         hi.push_back((int)b);
         h0.push_back((int)p);
-        h1.push_back((int)std::min<int64_t>(own[b + 1], p + kAT));
+        h1.push_back((int)std::min<int64_t>(own[b + 1], p + tile));
       }
       maxlen = std::max(maxlen, (int)(other[b + 1] - other[b]));
     }
@@ -690,10 +758,54 @@ void run_apply(Context& c, const TreeData& t, int pde, bool fwd, const Targets& 
 
 }  // namespace
 
+// whether two nodes coincide (checked once per tree)
+static bool has_coincident_points(Context& c, const TreeData& t) {
+  if (t.dup_state < 0) {
+    DBuf<int> flag;
+    flag.alloc(1);
+    flag.zero(c.stream);
+    k_dup_points<<<t.nnodes, 64, 0, c.stream>>>(t.nnodes, t.is_leaf.p, t.pstart.p, t.pcount.p,
+                                                t.perm_all.p, t.px.p, t.py.p, flag.p);
+    int h = 0;
+    flag.download(&h, 1, c.stream);
+    KI_HIP(hipStreamSynchronize(c.stream));
+    t.dup_state = h;
+  }
+  return t.dup_state != 0;
+}
+
 void kernel_apply(Context& c, const TreeData& t, int pde, const double* d_mu, int nrhs,
                   double* d_y) {
   Targets tg{t.px.p, t.py.p, t.nx.p, t.ny.p, t.npts};
   TileBuf tb;
+  if (pde == KIGPU_LAPLACE && nrhs == 1 && !has_coincident_points(c, t)) {
+    hipStream_t s = c.stream;
+    const GeomView g{t.px.p, t.py.p, t.nx.p, t.ny.p, t.w.p, t.kappa.p, t.npts, t.sd};
+    tb.build(t.inst_pt, t.inst_pt, s, 2 * kAT);
+    const int split = tb.splits(64);
+    const size_t len = (size_t)t.ndof;
+    DBuf<double> part;
+    part.alloc(len * split);
+    KI_HIP(hipEventRecord(c.ev[12], s));
+    k_apply_dl2<<<dim3(tb.ntiles, split), kAT, 0, s>>>(tb.view(), g, d_mu, part.p, t.ndof);
+    KI_HIP(hipEventRecord(c.ev[13], s));
+    k_sum_parts<<<ceil_div((long)len, 256), 256, 0, s>>>(part.p, split, len, d_y);
+    KI_HIP(hipGetLastError());
+    KI_HIP(hipStreamSynchronize(s));
+    float ms = 0;
+    KI_HIP(hipEventElapsedTime(&ms, c.ev[12], c.ev[13]));
+    c.tim.apply_kernel_s += ms * 1e-3;
+    c.tim.apply_launches += 1;
+    c.tim.apply_pairs += [&] {
+      double p = 0;
+      for (int i = 0; i < t.ninst; ++i) {
+        const double ni = (double)(t.inst_pt[i + 1] - t.inst_pt[i]);
+        p += ni * ni;
+      }
+      return p;
+    }();
+    return;
+  }
   tb.build(t.inst_pt, t.inst_pt, c.stream);
   run_apply<false>(c, t, pde, false, tg, tb, d_mu, t.ndof, nrhs, d_y, t.ndof);
   KI_HIP(hipStreamSynchronize(c.stream));
@@ -761,6 +873,8 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
   hipStream_t s = c.stream;
   if (trans && t.nborder > 0)
     KI_FAIL(KIGPU_ERR_NOTSUPP, "transposed solve of a bordered (multiply connected) system");
+  c.tim.apply_launches = 0;
+  c.tim.apply_kernel_s = c.tim.apply_pairs = 0;
   const size_t len = (size_t)t.ndof * nrhs;
   const int h = t.nborder;
   const size_t alen = (size_t)h * nrhs;

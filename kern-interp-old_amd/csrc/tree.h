@@ -45,6 +45,10 @@ struct TreeData {
   std::vector<int64_t> h_pstart, h_cp_off;
   std::vector<double> h_corners, h_side;
 
+  // coincident nodes: -1 not checked yet, 0 none, 1 some (the fast apply
+  // assumes none)
+  mutable int dup_state = -1;
+
   // ---- factorization ----
   bool factored = false;
   kigpu_factor_opts opts{};

@@ -46,7 +46,8 @@ class Timings(C.Structure):
                                           "t_solve", "t_refine_apply")] + [
         ("syncs", C.c_int), ("id_launches", C.c_int), ("id_kernel_s", C.c_double),
         ("id_flops", C.c_double), ("dec_launches", C.c_int), ("dec_kernel_s", C.c_double),
-        ("dec_flops", C.c_double)]
+        ("dec_flops", C.c_double), ("apply_launches", C.c_int), ("apply_kernel_s", C.c_double),
+        ("apply_pairs", C.c_double)]
 
 
 # every symbol include/kigpu.h declares
