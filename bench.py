@@ -191,10 +191,10 @@ def bench_adjoint(kigpu, ctx, evals):
 
 
 def apply_traffic():
-    """HBM bytes per launch of k_apply_dl2 from the committed rocprofv3 --pmc
+    """HBM bytes per launch of k_apply_sym from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
     or None when the summary is absent."""
-    path = os.path.join(ROOT, "profiles", "pmc_apply_dl2_laplace_star_N65536.md")
+    path = os.path.join(ROOT, "profiles", "pmc_apply_sym_laplace_star_N65536.md")
     try:
         with open(path) as fh:
             for line in fh:
@@ -249,27 +249,29 @@ def main():
                          "t_refine_apply")}, "timings": tm}
 
     # roofline of the kernel that dominates the FP32+refinement step: the FP64
-    # matrix-free apply of the refinement residual (k_apply_dl2), timed with
-    # HIP events on the library's stream.  Algorithmic work: 12 flops per
-    # (target, source) pair (c_K = 10 for the Laplace entry, 2 for the
-    # multiply-add with the density).
+    # matrix-free apply of the refinement residual (k_apply_sym), timed with
+    # HIP events on the library's stream.  Algorithmic work: 16 flops per
+    # unordered pair of nodes {i, j}, i.e. 8 per (target, source) pair: r
+    # (2), |r|^2 (3), 1/|r|^2 (1), r . a_j and r . a_i (3 each), and the two
+    # multiply-adds into y_i and y_j (4).
     tf = res["fp32_refine"]["timings"]
     launches = max(tf["apply_launches"], 1)
     per_launch_s = tf["apply_kernel_s"] / launches
-    per_launch_flops = 12.0 * tf["apply_pairs"] / launches
+    per_launch_flops = 8.0 * tf["apply_pairs"] / launches
     achieved = per_launch_flops / per_launch_s / 1e12 if per_launch_s > 0 else 0.0
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
                 "traffic": apply_traffic(),
-                "kernel": "k_apply_dl2 (FP64 Laplace double-layer apply, refinement residual; "
-                          "compute bound on the FP64 vector units)",
+                "kernel": "k_apply_sym (FP64 Laplace double-layer apply by unordered node "
+                          "pairs, refinement residual; bound by the FP64 vector units)",
                 "avg_launch_us": per_launch_s * 1e6, "launches_per_step": tf["apply_launches"],
                 "flops_per_launch": per_launch_flops}
-    # the largest factorization kernel beside it (LDS-resident ID, FP64 path)
+    # the largest factorization kernel beside it (the ID, FP64 path)
     tm = res["fp64"]["timings"]
     id_s = tm["id_kernel_s"] / max(tm["id_launches"], 1)
     id_f = tm["id_flops"] / max(tm["id_launches"], 1)
-    id_kernel = {"kernel": "k_id<double,double,true> (ID matrix + pivoted QR + T)",
+    id_kernel = {"kernel": "k_id<double,double> (ID matrix + pivoted QR + T; LDS-resident and "
+                           "global-workspace boxes in one launch per level)",
                  "achieved": id_f / id_s / 1e12 if id_s > 0 else 0.0, "peak": PEAK_FP64_TFLOPS,
                  "unit": "TFLOP/s", "avg_launch_us": id_s * 1e6,
                  "launches_per_factor": tm["id_launches"]}

@@ -80,8 +80,9 @@ typedef struct {
   /* device time per phase of the last tree build / factor / solve (seconds) */
   double t_tree, t_lists, t_id, t_decouple, t_root, t_solve, t_refine_apply;
   int syncs;             /* host synchronizations in the last tree build + factor */
-  /* the ID kernel (ID matrix generation + pivoted QR + T), LDS-resident
-   * variant: launches, summed launch durations (s) and algorithmic flops */
+  /* the ID kernel (ID matrix generation + pivoted QR + T), one launch per
+   * level over every box compressed there: launches, summed launch
+   * durations (s) and algorithmic flops */
   int id_launches;
   double id_kernel_s, id_flops;
   /* the same for the elimination kernel */

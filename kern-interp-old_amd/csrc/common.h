@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "kigpu.h"
@@ -83,6 +85,64 @@ struct DBuf {
   }
   void zero(hipStream_t s) {
     if (n) KI_HIP(hipMemsetAsync(p, 0, n * sizeof(T), s));
+  }
+};
+
+// Page-locked host memory (hipHostMalloc) for the small per-level transfers:
+// copies from and to it are plain DMA, where pageable memory goes through a
+// staging copy on every call.  Grow-only; growing waits for the stream, so a
+// copy still in flight never loses its buffer.
+struct PinnedHost {
+  void* p = nullptr;
+  size_t cap = 0;
+  PinnedHost() = default;
+  PinnedHost(const PinnedHost&) = delete;
+  PinnedHost& operator=(const PinnedHost&) = delete;
+  ~PinnedHost() {
+    if (p) (void)hipHostFree(p);
+  }
+  template <typename T>
+  T* get(size_t count, hipStream_t s) {
+    const size_t bytes = count * sizeof(T) + 64;
+    if (bytes > cap) {
+      if (p) {
+        KI_HIP(hipStreamSynchronize(s));
+        KI_HIP(hipHostFree(p));
+        p = nullptr;
+        cap = 0;
+      }
+      size_t nb = 4096;
+      while (nb < bytes) nb *= 2;
+      KI_HIP(hipHostMalloc(&p, nb, hipHostMallocDefault));
+      cap = nb;
+    }
+    return static_cast<T*>(p);
+  }
+};
+
+// Several host arrays sent to the device with one copy through page-locked
+// memory: add() reserves a 256-byte aligned slot and returns its offset,
+// send() packs the block, copies it into dst and returns the device base
+// (slot o is at base + o).  The page-locked block is reused by the next
+// send(), so the caller synchronizes the stream between two sends.
+struct UploadPack {
+  std::vector<std::pair<const void*, size_t>> parts;
+  std::vector<size_t> offs;
+  size_t total = 0;
+  size_t add(const void* h, size_t bytes) {
+    const size_t o = total;
+    parts.push_back({h, bytes});
+    offs.push_back(o);
+    total = (o + bytes + 255) / 256 * 256;
+    return o;
+  }
+  unsigned char* send(PinnedHost& pin, DBuf<unsigned char>& dst, hipStream_t s) {
+    unsigned char* h = pin.get<unsigned char>(total, s);
+    for (size_t i = 0; i < parts.size(); ++i)
+      if (parts[i].second) std::memcpy(h + offs[i], parts[i].first, parts[i].second);
+    dst.alloc(total);
+    if (total) KI_HIP(hipMemcpyAsync(dst.p, h, total, hipMemcpyHostToDevice, s));
+    return dst.p;
   }
 };
 

@@ -22,14 +22,13 @@ struct LevelWork {
   int level = 0;
   int a = 0, b = 0;         // node id range
   int nmax = 0;             // largest |active_box| on the level
-  std::vector<IdWork> id_lds;  // boxes whose ID ran in the LDS-resident kernel
+  std::vector<IdWork> id_boxes;  // the level's ID work list (flop accounting)
   DBuf<double> slab64;      // factor slabs of this level's nodes
   DBuf<float> slab32;
   // per-level device temporaries, kept until the tree is freed so that no
   // hipFree stalls the level loop
-  DBuf<int64_t> soff, rows_off;
+  DBuf<unsigned char> pack;  // slab offsets, ID row offsets and the two ID work lists
   DBuf<int> rows;
-  DBuf<IdWork> wl_lds, wl_glb;
   DBuf<double> ws, dws64;
   DBuf<float> dws32;
 };
@@ -43,6 +42,8 @@ struct Context {
   DBuf<double> ring;        // proxy unit vectors cos | sin (host libm)
   DBuf<double> rsum;        // residual partial sums
   size_t lds_max = 65536;
+  PinnedHost pin_down;      // per-level downloads
+  PinnedHost pin_up;        // packed per-level uploads
   void ensure_events(size_t n) {
     while (evpool.size() < n) {
       hipEvent_t e;

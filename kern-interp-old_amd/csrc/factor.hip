@@ -286,13 +286,11 @@ constexpr int kIdNW = kIdTB / kWave;
 // T = R11^{-1} R12 inherits cond(R11) ~ 1/tol); TS: storage type of the
 // interpolation matrix in the factor slab.
 template <typename T, typename TS, bool LDS>
-__global__ __launch_bounds__(kIdTB) void k_id(const IdWork* work, GeomView g, int pde, double tol,
-                                              const int* rows_pool, const int* act_pool,
-                                              const int64_t* act_off, const double* ring, T* ws,
-                                              int* rank, int* sr_pool, int* pos_pool,
-                                              const uint64_t* fac_ptr, int* status) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const IdWork w = work[blockIdx.x];
+__device__ __forceinline__ void id_box(const IdWork& w, unsigned char* smem, GeomView g, int pde,
+                                       double tol, const int* rows_pool, const int* act_pool,
+                                       const int64_t* act_off, const double* ring, T* ws,
+                                       int* rank, int* sr_pool, int* pos_pool,
+                                       const uint64_t* fac_ptr, int* status) {
   const int n = w.n, q = w.q, m = w.m;
   const IdLds Ly = id_lds_layout(m, n, q, LDS, sizeof(T));
   T* A = LDS ? reinterpret_cast<T*>(smem + Ly.a) : ws + w.a_off;
@@ -484,6 +482,26 @@ __global__ __launch_bounds__(kIdTB) void k_id(const IdWork* work, GeomView g, in
     for (int row = 0; row < k; ++row) tc[row] = (TS)rc[row];
   }
   if (tid == 0) rank[w.node] = k;
+}
+
+// One launch per level over every box to compress: boxes whose ID matrix
+// fits in LDS run there, the others keep it in the global workspace ws (put
+// first in the work list, since they take longest), so both kinds share the
+// chip instead of running one launch after the other.
+template <typename T, typename TS>
+__global__ __launch_bounds__(kIdTB) void k_id(const IdWork* work, GeomView g, int pde, double tol,
+                                              const int* rows_pool, const int* act_pool,
+                                              const int64_t* act_off, const double* ring, T* ws,
+                                              int* rank, int* sr_pool, int* pos_pool,
+                                              const uint64_t* fac_ptr, int* status) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const IdWork w = work[blockIdx.x];
+  if (w.in_lds)
+    id_box<T, TS, true>(w, smem, g, pde, tol, rows_pool, act_pool, act_off, ring, ws, rank,
+                        sr_pool, pos_pool, fac_ptr, status);
+  else
+    id_box<T, TS, false>(w, smem, g, pde, tol, rows_pool, act_pool, act_off, ring, ws, rank,
+                         sr_pool, pos_pool, fac_ptr, status);
 }
 
 // ---------------------------------------------------------------------------
@@ -848,14 +866,16 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
                                         t.act_pool.p, t.sl_nbr.p, t.sl_cnt.p, t.cp_off.p,
                                         t.cp_nbr.p, t.corners.p, t.side.p, t.nrows.p, nullptr,
                                         nullptr, 0);
-    h_n.resize(cn);
-    h_q.assign(cn, 0);
-    KI_HIP(hipMemcpyAsync(h_n.data(), t.nact.p + a, sizeof(int) * cn, hipMemcpyDeviceToHost, s));
+    int* pin = c.pin_down.get<int>(2 * (size_t)cn, s);
+    KI_HIP(hipMemcpyAsync(pin, t.nact.p + a, sizeof(int) * cn, hipMemcpyDeviceToHost, s));
     if (l >= 3)
-      KI_HIP(hipMemcpyAsync(h_q.data(), t.nrows.p + a, sizeof(int) * cn, hipMemcpyDeviceToHost,
+      KI_HIP(hipMemcpyAsync(pin + cn, t.nrows.p + a, sizeof(int) * cn, hipMemcpyDeviceToHost,
                             s));
     KI_HIP(hipStreamSynchronize(s));
     ++c.tim.syncs;
+    h_n.assign(pin, pin + cn);
+    if (l >= 3) h_q.assign(pin + cn, pin + 2 * cn);
+    else h_q.assign(cn, 0);
     int64_t level_dofs = 0;
     for (int i = 0; i < cn; ++i) level_dofs += h_n[i];
     if (l <= 2) {
@@ -912,8 +932,20 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
       }
     }
     lw->nmax = nmax;
+    // one ID work list: the global-workspace boxes first, then the LDS ones
+    std::vector<IdWork> wl(wl_glb);
+    wl.insert(wl.end(), wl_lds.begin(), wl_lds.end());
+    const int nwl = (int)wl.size();
+    // ---- the level's lists in one copy
+    UploadPack pk;
+    const size_t o_soff = pk.add(soff.data(), sizeof(int64_t) * cn);
+    const size_t o_roff = pk.add(rows_off.data(), sizeof(int64_t) * cn);
+    const size_t o_wl = pk.add(wl.data(), sizeof(IdWork) * wl.size());
+    unsigned char* dpk = pk.send(c.pin_up, lw->pack, s);
+    const int64_t* d_soff = reinterpret_cast<const int64_t*>(dpk + o_soff);
+    const int64_t* d_roff = reinterpret_cast<const int64_t*>(dpk + o_roff);
+    const IdWork* d_wl = reinterpret_cast<const IdWork*>(dpk + o_wl);
     // ---- factor slabs for every node of the level
-    lw->soff.upload(soff, s);
     T* slab;
     if (sizeof(T) == 8) {
       lw->slab64.alloc(stot);
@@ -923,52 +955,36 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
       slab = reinterpret_cast<T*>(lw->slab32.p);
     }
     k_set_ptr<<<ceil_div(cn, kTB), kTB, 0, s>>>(t.fac_ptr.p, a, b, t.nact.p,
-                                               reinterpret_cast<uint64_t>(slab), lw->soff.p,
+                                               reinterpret_cast<uint64_t>(slab), d_soff,
                                                sizeof(T));
     // ---- ID rows
     lw->rows.alloc(std::max<int64_t>(rtot, 1));
     if (l >= 3) {
-      lw->rows_off.upload(rows_off, s);
       k_inner_rows<<<cn, kWave, 0, s>>>(a, b, o.min_dofs, g, t.is_leaf.p, t.obox_off.p,
                                         t.obox_pool.p, t.pcount.p, t.act_off.p, t.nact.p,
                                         t.act_pool.p, t.sl_nbr.p, t.sl_cnt.p, t.cp_off.p,
                                         t.cp_nbr.p, t.corners.p, t.side.p, t.nrows.p,
-                                        lw->rows_off.p - a, lw->rows.p, 1);
+                                        d_roff - a, lw->rows.p, 1);
     }
-    if (!wl_lds.empty()) lw->wl_lds.upload(wl_lds, s);
-    if (!wl_glb.empty()) lw->wl_glb.upload(wl_glb, s);
-    if (l <= 2) {
-      if (!wl_lds.empty())
-        k_outside_rows<<<(int)wl_lds.size(), kTB, 0, s>>>(
-            lw->wl_lds.p, a, b, l, (int)t.level_off[1], (int)t.level_off[2], t.is_leaf.p,
-            t.obox_off.p, t.obox_pool.p, t.pcount.p, t.sd, t.act_off.p, t.nact.p, t.act_pool.p,
-            t.inst.p, lw->rows.p);
-      if (!wl_glb.empty())
-        k_outside_rows<<<(int)wl_glb.size(), kTB, 0, s>>>(
-            lw->wl_glb.p, a, b, l, (int)t.level_off[1], (int)t.level_off[2], t.is_leaf.p,
-            t.obox_off.p, t.obox_pool.p, t.pcount.p, t.sd, t.act_off.p, t.nact.p, t.act_pool.p,
-            t.inst.p, lw->rows.p);
-    }
+    if (l <= 2 && nwl > 0)
+      k_outside_rows<<<nwl, kTB, 0, s>>>(
+          d_wl, a, b, l, (int)t.level_off[1], (int)t.level_off[2], t.is_leaf.p, t.obox_off.p,
+          t.obox_pool.p, t.pcount.p, t.sd, t.act_off.p, t.nact.p, t.act_pool.p, t.inst.p,
+          lw->rows.p);
     E.l1 = mark();
     // ---- ID
     E.id0 = E.l1;
-    if (!wl_lds.empty()) {
-      allow_lds(k_id<double, T, true>, lds_bytes);
+    if (nwl > 0) {
+      const size_t ids = std::max(lds_bytes, glds);
+      if (!wl_glb.empty()) lw->ws.alloc(atot);
+      allow_lds(k_id<double, T>, ids);
       E.idl0 = mark();
-      k_id<double, T, true><<<(int)wl_lds.size(), kIdTB, lds_bytes, s>>>(
-          lw->wl_lds.p, g, pde, o.id_tol, lw->rows.p, t.act_pool.p, t.act_off.p, c.ring.p,
-          (double*)nullptr, t.rank.p, t.sr_pool.p, t.pos_pool.p, t.fac_ptr.p, t.status.p);
+      k_id<double, T><<<nwl, kIdTB, ids, s>>>(d_wl, g, pde, o.id_tol, lw->rows.p, t.act_pool.p,
+                                              t.act_off.p, c.ring.p, lw->ws.p, t.rank.p,
+                                              t.sr_pool.p, t.pos_pool.p, t.fac_ptr.p, t.status.p);
       KI_HIP(hipGetLastError());
       E.idl1 = mark();
-      lw->id_lds = wl_lds;
-    }
-    if (!wl_glb.empty()) {
-      lw->ws.alloc(atot);
-      allow_lds(k_id<double, T, false>, glds);
-      k_id<double, T, false><<<(int)wl_glb.size(), kIdTB, glds, s>>>(
-          lw->wl_glb.p, g, pde, o.id_tol, lw->rows.p, t.act_pool.p, t.act_off.p, c.ring.p,
-          lw->ws.p, t.rank.p, t.sr_pool.p, t.pos_pool.p, t.fac_ptr.p, t.status.p);
-      KI_HIP(hipGetLastError());
+      lw->id_boxes = wl;
     }
     E.id1 = mark();
     // ---- elimination for every node of the level
@@ -1007,13 +1023,14 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
   // ---- roots (level 0), one per instance
   const int e_root0 = mark();
   const int ni = t.ninst;
-  t.h_root_n.assign(ni, 0);
-  std::vector<int64_t> r_off(ni);
-  KI_HIP(hipMemcpyAsync(t.h_root_n.data(), t.nact.p, sizeof(int) * ni, hipMemcpyDeviceToHost, s));
-  KI_HIP(hipMemcpyAsync(r_off.data(), t.act_off.p, sizeof(int64_t) * ni, hipMemcpyDeviceToHost,
-                        s));
+  int64_t* pin = c.pin_down.get<int64_t>(2 * (size_t)ni, s);
+  KI_HIP(hipMemcpyAsync(pin, t.act_off.p, sizeof(int64_t) * ni, hipMemcpyDeviceToHost, s));
+  KI_HIP(hipMemcpyAsync(pin + ni, t.nact.p, sizeof(int) * ni, hipMemcpyDeviceToHost, s));
   KI_HIP(hipStreamSynchronize(s));
   ++c.tim.syncs;
+  const int64_t r_off0 = pin[0];
+  const int* pn = reinterpret_cast<const int*>(pin + ni);
+  t.h_root_n.assign(pn, pn + ni);
   t.h_root_doff.assign(ni + 1, 0);
   t.h_root_moff.assign(ni + 1, 0);
   t.nroot_max = 0;
@@ -1028,11 +1045,16 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
   if (t.nroot > 0) {
     // the roots' active lists are contiguous in act_pool (level 0 is last)
     t.root_dofs.alloc(t.nroot);
-    KI_HIP(hipMemcpyAsync(t.root_dofs.p, t.act_pool.p + r_off[0], sizeof(int) * t.nroot,
+    KI_HIP(hipMemcpyAsync(t.root_dofs.p, t.act_pool.p + r_off0, sizeof(int) * t.nroot,
                           hipMemcpyDeviceToDevice, s));
-    t.root_n.upload(t.h_root_n, s);
-    t.root_doff.upload(t.h_root_doff, s);
-    t.root_moff.upload(t.h_root_moff, s);
+    UploadPack pk;
+    const size_t o_n = pk.add(t.h_root_n.data(), sizeof(int) * ni);
+    const size_t o_d = pk.add(t.h_root_doff.data(), sizeof(int64_t) * (ni + 1));
+    const size_t o_m = pk.add(t.h_root_moff.data(), sizeof(int64_t) * (ni + 1));
+    unsigned char* dpk = pk.send(c.pin_up, t.root_pack, s);
+    t.root_n_p = reinterpret_cast<const int*>(dpk + o_n);
+    t.root_doff_p = reinterpret_cast<const int64_t*>(dpk + o_d);
+    t.root_moff_p = reinterpret_cast<const int64_t*>(dpk + o_m);
     const size_t msz = std::max<int64_t>(t.h_root_moff[ni], 1);
     t.root_lu.alloc(msz);   // allskel_mat^{-1}
     t.root_mat.alloc(msz);  // allskel_mat
@@ -1041,13 +1063,13 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     if (lbytes + 40960 <= lds_cap) {  // static LDS of k_root: 2 int + 3 scratch vectors
       allow_lds(k_root<T, true>, lbytes);
       k_root<T, true><<<ni, 1024, lbytes, s>>>(g, pde, t.child.p, t.osize.p, t.acc_ptr.p,
-                                               t.acc_ld.p, t.root_n.p, t.root_doff.p,
-                                               t.root_moff.p, t.root_dofs.p, nullptr, t.root_lu.p,
+                                               t.acc_ld.p, t.root_n_p, t.root_doff_p,
+                                               t.root_moff_p, t.root_dofs.p, nullptr, t.root_lu.p,
                                                t.root_piv.p, t.root_mat.p, t.status.p);
     } else {
       t.ws64.alloc(msz);
       k_root<T, false><<<ni, 1024, 0, s>>>(g, pde, t.child.p, t.osize.p, t.acc_ptr.p,
-                                           t.acc_ld.p, t.root_n.p, t.root_doff.p, t.root_moff.p,
+                                           t.acc_ld.p, t.root_n_p, t.root_doff_p, t.root_moff_p,
                                            t.root_dofs.p, t.ws64.p, t.root_lu.p, t.root_piv.p,
                                            t.root_mat.p, t.status.p);
     }
@@ -1111,24 +1133,25 @@ void factor_tree(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     if (o.precision == KIGPU_FP32_REFINE) border_factor<float>(c, t);
     else border_factor<double>(c, t);
   }
-  int st = 0;
-  t.status.download(&st, 1, c.stream);
-  KI_HIP(hipStreamSynchronize(c.stream));
-  if (st & 2) KI_FAIL(KIGPU_ERR_RANK, "ID matrix with fewer rows than columns");
-  // host copies of the per-node results
+  // status word and host copies of the per-node results, one synchronization
   const int nn = t.nnodes;
-  t.h_rank.resize(nn);
-  t.h_nact.resize(nn);
-  t.h_act_off.resize(nn);
-  t.rank.download(t.h_rank.data(), nn, c.stream);
-  t.nact.download(t.h_nact.data(), nn, c.stream);
-  t.act_off.download(t.h_act_off.data(), nn, c.stream);
+  hipStream_t s = c.stream;
+  int64_t* pin = c.pin_down.get<int64_t>(2 * (size_t)nn + 8, s);
+  int* pi = reinterpret_cast<int*>(pin + nn);
+  KI_HIP(hipMemcpyAsync(pin, t.act_off.p, sizeof(int64_t) * nn, hipMemcpyDeviceToHost, s));
+  KI_HIP(hipMemcpyAsync(pi, t.rank.p, sizeof(int) * nn, hipMemcpyDeviceToHost, s));
+  KI_HIP(hipMemcpyAsync(pi + nn, t.nact.p, sizeof(int) * nn, hipMemcpyDeviceToHost, s));
+  KI_HIP(hipMemcpyAsync(pi + 2 * nn, t.status.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  KI_HIP(hipStreamSynchronize(s));
+  if (pi[2 * nn] & 2) KI_FAIL(KIGPU_ERR_RANK, "ID matrix with fewer rows than columns");
+  t.h_act_off.assign(pin, pin + nn);
+  t.h_rank.assign(pi, pi + nn);
+  t.h_nact.assign(pi + nn, pi + 2 * nn);
   t.fmirrors_ready = false;
-  KI_HIP(hipStreamSynchronize(c.stream));
   const double c_gen = o.pde == KIGPU_STOKES ? 20.0 : 11.0;
   for (auto& lw : t.work) {
     if (!lw) continue;
-    for (const IdWork& w : lw->id_lds) c.tim.id_flops += id_flops(w.m, w.n, t.h_rank[w.node], c_gen);
+    for (const IdWork& w : lw->id_boxes) c.tim.id_flops += id_flops(w.m, w.n, t.h_rank[w.node], c_gen);
     for (int x = lw->a; x < lw->b; ++x) c.tim.dec_flops += dec_flops(t.h_nact[x], t.h_rank[x], c_gen);
   }
   t.opts = o;

@@ -48,6 +48,13 @@ struct TreeData {
   // coincident nodes: -1 not checked yet, 0 none, 1 some (the fast apply
   // assumes none)
   mutable int dup_state = -1;
+  // block-pair table of the symmetric double-layer apply, built on first use:
+  // 6 ints per pair (target block start and length, source block start and
+  // length, target slot, source slot or -1 on a diagonal block) and 3 ints
+  // per block (start, length, slots of its instance)
+  mutable DBuf<int> sym_pairs, sym_blocks;
+  mutable DBuf<int64_t> d_dof_off;     // instance DOF offsets (residual norms)
+  mutable int sym_npairs = 0, sym_nblocks = 0, sym_slots = 0;
 
   // ---- factorization ----
   bool factored = false;
@@ -84,8 +91,10 @@ struct TreeData {
   int nroot_max = 0;
   std::vector<int> h_root_n;
   std::vector<int64_t> h_root_doff, h_root_moff;
-  DBuf<int> root_n;
-  DBuf<int64_t> root_doff, root_moff;
+  DBuf<unsigned char> root_pack;       // root_n | root_doff | root_moff on the device
+  const int* root_n_p = nullptr;
+  const int64_t* root_doff_p = nullptr;
+  const int64_t* root_moff_p = nullptr;
   DBuf<int> root_dofs;
   DBuf<double> root_lu;
   DBuf<int> root_piv;
@@ -111,7 +120,7 @@ struct TreeData {
 };
 
 // tree.hip
-void build_tree(TreeData& t, hipStream_t s, int* syncs);
+void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin);
 void ensure_tree_mirrors(TreeData& t);    // host copies used by the export calls
 void ensure_factor_mirrors(TreeData& t);
 

@@ -226,14 +226,22 @@ __global__ void k_rank_pairs(int c, const int* inst, const int64_t* seg, const i
 }
 
 // exclusive scan of the per-instance split counts (ninst + 1 entries)
-__global__ void k_seg_scan(int ninst, const int* seg_cnt, int* seg_split) {
+__global__ void k_seg_scan(int ninst, int* seg_cnt, int* seg_split, int64_t* next_seg,
+                           int* xfer) {
+  // also: the next level's node offsets (4 children per split), the offsets
+  // for the host (xfer), and seg_cnt cleared for the next level
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   int run = 0;
   for (int b = 0; b < ninst; ++b) {
     seg_split[b] = run;
+    xfer[b] = run;
+    next_seg[b] = 4 * (int64_t)run;
     run += seg_cnt[b];
+    seg_cnt[b] = 0;
   }
   seg_split[ninst] = run;
+  xfer[ninst] = run;
+  next_seg[ninst] = 4 * (int64_t)run;
 }
 
 // instance-local rank -> rank within the level
@@ -486,6 +494,9 @@ struct LevelBuf {
 struct FlatArgs {
   int c, base, parent_base, next_base, level, has_next;
   int64_t pbase;
+  int nperm;               // this level's point permutation, copied to perm_all + pbase
+  const int* perm;
+  int* perm_all;
   const int *parent_local, *slot, *split, *rank, *pcount, *inst;
   const int64_t* pstart;
   const double *corners, *side;
@@ -496,6 +507,7 @@ struct FlatArgs {
 
 __global__ void k_flatten(FlatArgs f) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < f.nperm) f.perm_all[f.pbase + i] = f.perm[i];
   if (i >= f.c) return;
   const int x = f.base + i;
   f.parent[x] = f.parent_local[i] < 0 ? -1 : f.parent_base + f.parent_local[i];
@@ -511,9 +523,49 @@ __global__ void k_flatten(FlatArgs f) {
   f.sidev[x] = f.side[i];
 }
 
+// level 0: one root box per instance around its points' min/max
+// (quadtree.cpp:80-88); also the level's instance offsets
+__global__ void k_root_level(int ni, const int64_t* inst_pt, const double* mm, int* parent_local,
+                             int* slot, int* pcount, int64_t* pstart, uint64_t* path,
+                             double* corners, double* side, int* inst, int64_t* seg) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > ni) return;
+  seg[i] = i;
+  if (i == ni) return;
+  const double lo = mm[2 * i] - 0.01, hi = mm[2 * i + 1] + 0.01;
+  double* c = corners + 8 * (size_t)i;
+  c[0] = lo;
+  c[1] = lo;
+  c[2] = lo;
+  c[3] = hi;
+  c[4] = hi;
+  c[5] = hi;
+  c[6] = hi;
+  c[7] = lo;
+  side[i] = hi - lo;
+  pcount[i] = (int)(inst_pt[i + 1] - inst_pt[i]);
+  pstart[i] = inst_pt[i];
+  parent_local[i] = -1;
+  slot[i] = 0;
+  path[i] = 0;
+  inst[i] = i;
+}
+
+// number of coarse-leaf entries (last offset + last count) into cp_off[nn],
+// and with the status word into xfer for the host
+__global__ void k_cp_total(int nn, int64_t* cp_off, const int* cnt, const int* status,
+                           int64_t* xfer) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int64_t tot = nn ? cp_off[nn - 1] + cnt[nn - 1] : 0;
+  cp_off[nn] = tot;
+  xfer[0] = tot;
+  xfer[1] = *status;
+}
+
 }  // namespace
 
-void build_tree(TreeData& t, hipStream_t s, int* syncs) {
+void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin) {
   const int n = t.npts;
   const int ni = t.ninst;
   if (t.inst_pt.empty()) t.inst_pt = {0, (int64_t)n};
@@ -521,7 +573,6 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs) {
     KI_FAIL(KIGPU_ERR_ARG, "instance point offsets do not cover the points");
   t.d_inst_pt.upload(t.inst_pt, s);
   // root boxes: min/max pooled over x and y (quadtree.cpp:80-88), per instance
-  std::vector<double> hmm(2 * (size_t)ni);
   DBuf<double> mm;
   mm.alloc(2 * (size_t)ni);
   if (ni == 1) {
@@ -532,46 +583,25 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs) {
   } else {
     k_minmax_seg<<<ni, kTB, 0, s>>>(t.px.p, t.py.p, t.d_inst_pt.p, mm.p);
   }
-  mm.download(hmm.data(), hmm.size(), s);
-  KI_HIP(hipStreamSynchronize(s));
-  ++*syncs;
-  t.inst_min.resize(ni);
-  t.inst_max.resize(ni);
-  for (int i = 0; i < ni; ++i) {
-    t.inst_min[i] = hmm[2 * i];
-    t.inst_max[i] = hmm[2 * i + 1];
-  }
-  t.min = t.inst_min[0];
-  t.max = t.inst_max[0];
 
   std::vector<std::unique_ptr<LevelBuf>> lv;
   {
     auto r = std::make_unique<LevelBuf>();
     r->count = ni;
-    std::vector<int> h_pl(ni, -1), h_slot(ni, 0), h_pc(ni), h_inst(ni);
-    std::vector<int64_t> h_ps(ni);
-    std::vector<uint64_t> h_path(ni, 0);
-    std::vector<double> h_c(8 * (size_t)ni), h_side(ni);
     r->seg.resize(ni + 1);
-    for (int i = 0; i < ni; ++i) {
-      const double lo = t.inst_min[i] - 0.01, hi = t.inst_max[i] + 0.01;
-      const double c8[8] = {lo, lo, lo, hi, hi, hi, hi, lo};
-      for (int j = 0; j < 8; ++j) h_c[8 * i + j] = c8[j];
-      h_side[i] = hi - lo;
-      h_pc[i] = (int)(t.inst_pt[i + 1] - t.inst_pt[i]);
-      h_ps[i] = t.inst_pt[i];
-      h_inst[i] = i;
-      r->seg[i] = i;
-    }
-    r->seg[ni] = ni;
-    r->parent_local.upload(h_pl, s);
-    r->slot.upload(h_slot, s);
-    r->pcount.upload(h_pc, s);
-    r->pstart.upload(h_ps, s);
-    r->path.upload(h_path, s);
-    r->corners.upload(h_c, s);
-    r->side.upload(h_side, s);
-    r->inst.upload(h_inst, s);
+    for (int i = 0; i <= ni; ++i) r->seg[i] = i;
+    r->parent_local.alloc(ni);
+    r->slot.alloc(ni);
+    r->pcount.alloc(ni);
+    r->pstart.alloc(ni);
+    r->path.alloc(ni);
+    r->corners.alloc(8 * (size_t)ni);
+    r->side.alloc(ni);
+    r->inst.alloc(ni);
+    r->dseg.alloc(ni + 1);
+    k_root_level<<<ceil_div(ni + 1, kTB), kTB, 0, s>>>(
+        ni, t.d_inst_pt.p, mm.p, r->parent_local.p, r->slot.p, r->pcount.p, r->pstart.p,
+        r->path.p, r->corners.p, r->side.p, r->inst.p, r->dseg.p);
     r->perm.alloc(n);
     r->pnode.alloc(n);
     r->nperm = n;
@@ -579,12 +609,14 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs) {
     k_fill_inst<<<ni, kTB, 0, s>>>(t.d_inst_pt.p, r->pnode.p);
     lv.push_back(std::move(r));
   }
-  DBuf<int> seg_cnt, seg_split, qcode, ccnt;
+  DBuf<int> seg_cnt, seg_split, qcode, ccnt, xfer;
   seg_cnt.alloc(ni);
+  seg_cnt.zero(s);
   seg_split.alloc(ni + 1);
+  xfer.alloc(ni + 1 + 4);  // seg_split | the level's quadrant totals (Q4)
+  Q4* qtot = reinterpret_cast<Q4*>(xfer.p + ni + 1);
   std::vector<int> h_split(ni + 1);
-  DBuf<Q4> pre, btot, qtot;
-  qtot.alloc(1);
+  DBuf<Q4> pre, btot;
   for (int l = 0;; ++l) {
     LevelBuf& L = *lv[l];
     const int c = L.count;
@@ -592,14 +624,14 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs) {
     L.split.alloc(c);
     L.st.alloc(c);
     L.rank.alloc(c);
-    L.dseg.upload(L.seg, s);
+    auto nx = std::make_unique<LevelBuf>();
+    nx->dseg.alloc(ni + 1);
     const int* pst = l ? lv[l - 1]->st.p : nullptr;
     k_split_info<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.pcount.p, L.pstart.p, L.perm.p,
                                                   L.parent_local.p, pst, t.sd, t.leaf_dofs,
                                                   L.split.p, L.st.p);
     int64_t segmax = 0;
     for (int i = 0; i < ni; ++i) segmax = std::max(segmax, L.seg[i + 1] - L.seg[i]);
-    seg_cnt.zero(s);
     if (segmax <= kSortMax && l <= 20 && (int64_t)n < (1ll << 24)) {
       k_rank_sort<<<ni, 1024, 0, s>>>(L.dseg.p, L.split.p, L.st.p, L.path.p, L.rank.p,
                                       seg_cnt.p);
@@ -607,7 +639,7 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs) {
       k_rank_pairs<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.inst.p, L.dseg.p, L.split.p, L.st.p,
                                                     L.path.p, L.rank.p, seg_cnt.p);
     }
-    k_seg_scan<<<1, 64, 0, s>>>(ni, seg_cnt.p, seg_split.p);
+    k_seg_scan<<<1, 64, 0, s>>>(ni, seg_cnt.p, seg_split.p, nx->dseg.p, xfer.p);
     if (ni > 1) k_rank_offset<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.inst.p, seg_split.p, L.rank.p);
     // classify and scan the points of splitting nodes
     const int nb = ceil_div(np, kScanTB);
@@ -617,17 +649,18 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs) {
     k_classify<<<ceil_div(np, kTB), kTB, 0, s>>>(np, L.perm.p, L.pnode.p, L.rank.p, L.corners.p,
                                                  t.px.p, t.py.p, qcode.p);
     k_q4_block<<<nb, kScanTB, 0, s>>>(np, qcode.p, pre.p, btot.p);
-    k_q4_top<<<1, 64, 0, s>>>(nb, btot.p, qtot.p);
-    k_q4_apply<<<ceil_div(np + 1, kTB), kTB, 0, s>>>(np, pre.p, btot.p, qtot.p);
-    Q4 tot;
-    seg_split.download(h_split.data(), ni + 1, s);
-    qtot.download(&tot, 1, s);
+    k_q4_top<<<1, 64, 0, s>>>(nb, btot.p, qtot);
+    k_q4_apply<<<ceil_div(np + 1, kTB), kTB, 0, s>>>(np, pre.p, btot.p, qtot);
+    int* hx = pin.get<int>(ni + 5, s);
+    KI_HIP(hipMemcpyAsync(hx, xfer.p, sizeof(int) * (ni + 5), hipMemcpyDeviceToHost, s));
     KI_HIP(hipStreamSynchronize(s));
     ++*syncs;
+    h_split.assign(hx, hx + ni + 1);
+    Q4 tot;
+    for (int q = 0; q < 4; ++q) tot.v[q] = hx[ni + 1 + q];
     const int S = h_split[ni];
     if (S == 0) break;
     if (l >= 30) KI_FAIL(KIGPU_ERR_NOTSUPP, "quadtree deeper than 31 levels");
-    auto nx = std::make_unique<LevelBuf>();
     LevelBuf& N = *nx;
     N.count = 4 * S;
     N.seg.resize(ni + 1);
@@ -704,10 +737,10 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs) {
     f.pstartv = t.pstart.p;
     f.cornersv = t.corners.p;
     f.sidev = t.side.p;
-    k_flatten<<<ceil_div(L.count, kTB), kTB, 0, s>>>(f);
-    if (L.nperm)
-      KI_HIP(hipMemcpyAsync(t.perm_all.p + pbase, L.perm.p, sizeof(int) * L.nperm,
-                            hipMemcpyDeviceToDevice, s));
+    f.nperm = L.nperm;
+    f.perm = L.perm.p;
+    f.perm_all = t.perm_all.p;
+    k_flatten<<<ceil_div(std::max(L.count, L.nperm), kTB), kTB, 0, s>>>(f);
     pbase += L.nperm;
   }
   // ---- neighbor lists ----
@@ -728,40 +761,53 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs) {
   k_coarse<<<ceil_div(nn, kTB), kTB, 0, s>>>(nn, t.parent.p, t.level.p, t.is_leaf.p,
                                               t.corners.p, t.sl_nbr.p, t.sl_cnt.p, nullptr,
                                               cpc.p, nullptr, 0);
-  exclusive_scan_i32_to_i64(cpc.p, t.cp_off.p, nn + 1 > nn ? nn : nn, s);
-  // total = last offset + last count
-  int64_t lo_ = 0;
-  int lc_ = 0;
-  KI_HIP(hipMemcpyAsync(&lo_, t.cp_off.p + nn - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  KI_HIP(hipMemcpyAsync(&lc_, cpc.p + nn - 1, sizeof(int), hipMemcpyDeviceToHost, s));
-  int hstat = 0;
-  t.status.download(&hstat, 1, s);
+  exclusive_scan_i32_to_i64(cpc.p, t.cp_off.p, nn, s);
+  DBuf<int64_t> cxf;
+  cxf.alloc(2);
+  k_cp_total<<<1, 64, 0, s>>>(nn, t.cp_off.p, cpc.p, t.status.p, cxf.p);
+  int64_t* hc = pin.get<int64_t>(2, s);
+  KI_HIP(hipMemcpyAsync(hc, cxf.p, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
   KI_HIP(hipStreamSynchronize(s));
   ++*syncs;
-  if (hstat) KI_FAIL(KIGPU_ERR_STATE, "more than 8 same-level neighbors (overlapping boxes)");
-  const int64_t ncp = lo_ + lc_;
-  KI_HIP(hipMemcpyAsync(t.cp_off.p + nn, &ncp, sizeof(int64_t), hipMemcpyHostToDevice, s));
+  if (hc[1]) KI_FAIL(KIGPU_ERR_STATE, "more than 8 same-level neighbors (overlapping boxes)");
+  const int64_t ncp = hc[0];
   t.cp_nbr.alloc(std::max<int64_t>(ncp, 1));
   k_coarse<<<ceil_div(nn, kTB), kTB, 0, s>>>(nn, t.parent.p, t.level.p, t.is_leaf.p,
                                               t.corners.p, t.sl_nbr.p, t.sl_cnt.p, t.cp_off.p,
                                               cpc.p, t.cp_nbr.p, 1);
 
-  // ---- host mirrors the factorization needs; the rest on export ----
+  // ---- host mirrors the factorization needs (the rest on export), one copy
+  // each into page-locked memory and one synchronization
   t.nperm_total = ptot;
   t.ncp = ncp;
   t.mirrors_ready = false;
-  t.h_is_leaf.resize(nn);
-  t.h_pcount.resize(nn);
-  t.h_inst.resize(nn);
-  t.inst.download(t.h_inst.data(), nn, s);
-  t.h_corners.resize(8 * (size_t)nn);
-  t.h_side.resize(nn);
-  t.is_leaf.download(t.h_is_leaf.data(), nn, s);
-  t.pcount.download(t.h_pcount.data(), nn, s);
-  t.corners.download(t.h_corners.data(), 8 * (size_t)nn, s);
-  t.side.download(t.h_side.data(), nn, s);
+  const size_t dbl = 9 * (size_t)nn + 2 * (size_t)ni;  // corners | side | min/max
+  double* hd = pin.get<double>(dbl + (3 * (size_t)nn + 1) / 2 + 1, s);
+  int* hi = reinterpret_cast<int*>(hd + dbl);
+  KI_HIP(hipMemcpyAsync(hd, t.corners.p, sizeof(double) * 8 * nn, hipMemcpyDeviceToHost, s));
+  KI_HIP(hipMemcpyAsync(hd + 8 * (size_t)nn, t.side.p, sizeof(double) * nn, hipMemcpyDeviceToHost,
+                        s));
+  KI_HIP(hipMemcpyAsync(hd + 9 * (size_t)nn, mm.p, sizeof(double) * 2 * ni, hipMemcpyDeviceToHost,
+                        s));
+  KI_HIP(hipMemcpyAsync(hi, t.inst.p, sizeof(int) * nn, hipMemcpyDeviceToHost, s));
+  KI_HIP(hipMemcpyAsync(hi + nn, t.is_leaf.p, sizeof(int) * nn, hipMemcpyDeviceToHost, s));
+  KI_HIP(hipMemcpyAsync(hi + 2 * (size_t)nn, t.pcount.p, sizeof(int) * nn, hipMemcpyDeviceToHost,
+                        s));
   KI_HIP(hipStreamSynchronize(s));
   ++*syncs;
+  t.h_corners.assign(hd, hd + 8 * (size_t)nn);
+  t.h_side.assign(hd + 8 * (size_t)nn, hd + 9 * (size_t)nn);
+  t.inst_min.resize(ni);
+  t.inst_max.resize(ni);
+  for (int i = 0; i < ni; ++i) {
+    t.inst_min[i] = hd[9 * (size_t)nn + 2 * i];
+    t.inst_max[i] = hd[9 * (size_t)nn + 2 * i + 1];
+  }
+  t.min = t.inst_min[0];
+  t.max = t.inst_max[0];
+  t.h_inst.assign(hi, hi + nn);
+  t.h_is_leaf.assign(hi + nn, hi + 2 * (size_t)nn);
+  t.h_pcount.assign(hi + 2 * (size_t)nn, hi + 3 * (size_t)nn);
 }
 
 void ensure_tree_mirrors(TreeData& t) {

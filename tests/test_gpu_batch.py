@@ -12,6 +12,8 @@ import pytest
 
 import kigpu
 from kigpu import workloads as W
+from oracle import geometry as G
+from oracle import kernel as K
 
 
 def instance_geoms(count, seed, lo=1024, hi=4096):
@@ -102,6 +104,26 @@ def test_forest_factor_solve_matches_single(ctx):
         assert d <= 1e-14
         s.close()
     print(f"  worst instance difference {worst:.3e}", flush=True)
+    f.close()
+
+
+@pytest.mark.gpu
+def test_forest_apply_one_density(ctx):
+    # block-diagonal apply over instances smaller than, equal to and larger
+    # than one 512-node block, against each instance's dense matrix
+    print("[batch] one-density apply over 4 instances", flush=True)
+    sizes = [300, 512, 1100, 2049]
+    geoms = [W.random_star(n, 70 + i) for i, n in enumerate(sizes)]
+    f = ctx.build_batch(geoms, 1, 64)
+    pt_off = f.instances()["point_offsets"]
+    mu = np.random.default_rng(8).standard_normal(int(pt_off[-1]))
+    y = f.apply(mu, kigpu.LAPLACE)
+    for i, n in enumerate(sizes):
+        b = G.random_star(n, 70 + i)
+        assert np.array_equal(b.points, geoms[i][0])
+        a, e = pt_off[i], pt_off[i + 1]
+        ref = K.Kernel(1, K.LAPLACE, b)(np.arange(n), np.arange(n)) @ mu[a:e]
+        assert np.allclose(y[a:e], ref, rtol=0, atol=1e-13 * np.abs(ref).max())
     f.close()
 
 

@@ -159,6 +159,23 @@ def test_fp32_refine(ctx):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("n", [300, 1000, 1536, 4099])
+def test_apply_one_density_blocks(ctx, n):
+    # one Laplace density goes through the block-pair apply (blocks of 512
+    # nodes, each unordered pair of distinct nodes evaluated once); sizes
+    # below one block, with a partial last block and with whole blocks
+    print(f"[apply] Laplace one density N={n}", flush=True)
+    b = G.star(n)
+    t = gpu_tree(ctx, b, 1, 64)
+    mu = np.random.default_rng(n).standard_normal(n)
+    ref = K.Kernel(1, K.LAPLACE, b)(np.arange(n), np.arange(n)) @ mu
+    y = t.apply(mu, K.LAPLACE)
+    assert np.allclose(y, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+    assert np.array_equal(y, t.apply(mu, K.LAPLACE))  # fixed summation order
+    t.close()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("pde", [K.LAPLACE, K.STOKES])
 def test_apply_matches_dense(ctx, pde):
     print(f"[apply] pde {pde}", flush=True)
