@@ -128,7 +128,7 @@ void run_sens(Context& c, const TreeData& t, int nrows, const double* rx, const 
                                                 d_mu, part.p, per);
   k_sum6<<<ceil_div((long)len, 256), 256, 0, s>>>(part.p, split, len, d_sens);
   KI_HIP(hipGetLastError());
-  KI_HIP(hipStreamSynchronize(s));
+  stream_wait(s);
 }
 
 }  // namespace

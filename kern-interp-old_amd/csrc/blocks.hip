@@ -54,7 +54,7 @@ void kernel_block(Context& c, const GeomView& g, const double* dpx, const double
     KI_HIP(hipGetLastError());
   }
   dout.download(h_out, (size_t)ntgt * nsrc, s);
-  KI_HIP(hipStreamSynchronize(s));
+  stream_wait(s);
 }
 
 }  // namespace kigpu

@@ -188,7 +188,7 @@ void border_factor(Context& c, TreeData& t) {
     k_dense_inverse<false><<<1, 1024, 0, s>>>(S.p, nS, t.Sinv.p, t.status.p);
   }
   KI_HIP(hipGetLastError());
-  KI_HIP(hipStreamSynchronize(s));
+  stream_wait(s);
   t.nS = nS;
 }
 template void border_factor<double>(Context&, TreeData&);

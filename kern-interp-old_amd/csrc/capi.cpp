@@ -147,7 +147,7 @@ kigpu_status kigpu_context_create(int device, kigpu_ctx** out) {
       ring[kProxyPoints + i] = std::sin(ang);
     }
     c.ring.upload(ring, c.stream);
-    KI_HIP(hipStreamSynchronize(c.stream));
+    stream_wait(c.stream);
   });
   if (st == KIGPU_OK) *out = ctx.release();
   return st;
@@ -199,7 +199,7 @@ static kigpu_status tree_build_common(kigpu_ctx* ctx, int npts, int sd, int leaf
     c.tim.syncs = 0;
     build_tree(t, c.stream, &c.tim.syncs, c.pin_down);
     KI_HIP(hipEventRecord(c.ev[11], c.stream));
-    KI_HIP(hipStreamSynchronize(c.stream));
+    stream_wait(c.stream);
     float ms = 0;
     KI_HIP(hipEventElapsedTime(&ms, c.ev[10], c.ev[11]));
     c.tim.t_tree = ms * 1e-3;
@@ -226,7 +226,7 @@ kigpu_status kigpu_tree_build(kigpu_ctx* ctx, const double* points, const double
     t.ny.upload(nb, s);
     t.w.upload(weights, npts, s);
     t.kappa.upload(curvatures, npts, s);
-    KI_HIP(hipStreamSynchronize(s));
+    stream_wait(s);
   });
 }
 
@@ -275,7 +275,7 @@ kigpu_status kigpu_tree_build_batch(kigpu_ctx* ctx, int ninst, const int* npts,
         t.ny.upload(nb, s);
         t.w.upload(weights, total, s);
         t.kappa.upload(curvatures, total, s);
-        KI_HIP(hipStreamSynchronize(s));
+        stream_wait(s);
       },
       &off);
 }
@@ -553,7 +553,7 @@ kigpu_status kigpu_solve(kigpu_ctx* ctx, kigpu_tree* tree, const double* b, int 
     KI_HIP(hipMemcpy2DAsync(x, sizeof(double) * ldx, dx.p, sizeof(double) * nd,
                             sizeof(double) * nd, nrhs, hipMemcpyDeviceToHost, c.stream));
     if (h && alpha) da.download(alpha, (size_t)h * nrhs, c.stream);
-    KI_HIP(hipStreamSynchronize(c.stream));
+    stream_wait(c.stream);
   });
 }
 
@@ -588,7 +588,7 @@ kigpu_status kigpu_geometry_create(kigpu_ctx* ctx, const double* points, const d
     g->kappa.upload(curvatures, npts, s);
     g->dx.upload(da, s);
     g->dy.upload(db, s);
-    KI_HIP(hipStreamSynchronize(s));
+    stream_wait(s);
   });
   if (st == KIGPU_OK) *out = g.release();
   return st;
@@ -636,7 +636,7 @@ kigpu_status kigpu_kernel_apply(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
     dy.alloc(len);
     kernel_apply(c, tree->t, pde, dm.p, nrhs, dy.p);
     dy.download(y, len, c.stream);
-    KI_HIP(hipStreamSynchronize(c.stream));
+    stream_wait(c.stream);
   });
 }
 
@@ -659,7 +659,7 @@ kigpu_status kigpu_forward_apply(kigpu_ctx* ctx, const kigpu_tree* tree, int pde
     du.alloc((size_t)sd * ntgt * nrhs);
     forward_apply(c, t, pde, dt.p, ntgt, dm.p, nrhs, du.p);
     du.download(u, (size_t)sd * ntgt * nrhs, c.stream);
-    KI_HIP(hipStreamSynchronize(c.stream));
+    stream_wait(c.stream);
   });
 }
 
@@ -697,7 +697,7 @@ kigpu_status kigpu_solve_transpose(kigpu_ctx* ctx, kigpu_tree* tree, const doubl
     solve_tree(c, tree->t, db.p, nrhs, dx.p, nullptr, refine, info, true);
     KI_HIP(hipMemcpy2DAsync(x, sizeof(double) * ldx, dx.p, sizeof(double) * nd,
                             sizeof(double) * nd, nrhs, hipMemcpyDeviceToHost, c.stream));
-    KI_HIP(hipStreamSynchronize(c.stream));
+    stream_wait(c.stream);
   });
 }
 
@@ -713,7 +713,7 @@ kigpu_status kigpu_kernel_apply_transpose(kigpu_ctx* ctx, const kigpu_tree* tree
     dy.alloc(len);
     kernel_apply_transpose(c, tree->t, pde, dl.p, nrhs, dy.p);
     dy.download(y, len, c.stream);
-    KI_HIP(hipStreamSynchronize(c.stream));
+    stream_wait(c.stream);
   });
 }
 
@@ -736,7 +736,7 @@ kigpu_status kigpu_forward_apply_transpose(kigpu_ctx* ctx, const kigpu_tree* tre
     dy.alloc((size_t)t.ndof * nrhs);
     forward_apply_transpose(c, t, pde, dt.p, ntgt, dv.p, nrhs, dy.p);
     dy.download(y, (size_t)t.ndof * nrhs, c.stream);
-    KI_HIP(hipStreamSynchronize(c.stream));
+    stream_wait(c.stream);
   });
 }
 
@@ -760,7 +760,7 @@ kigpu_status kigpu_kernel_sensitivity(kigpu_ctx* ctx, const kigpu_tree* tree, in
     ds.alloc(6 * (size_t)t.npts);
     kernel_sensitivity(c, t, dl.p, dm.p, ds.p);
     ds.download(sens, 6 * (size_t)t.npts, c.stream);
-    KI_HIP(hipStreamSynchronize(c.stream));
+    stream_wait(c.stream);
   });
 }
 
@@ -786,7 +786,7 @@ kigpu_status kigpu_forward_sensitivity(kigpu_ctx* ctx, const kigpu_tree* tree, i
     ds.alloc(6 * (size_t)t.npts);
     forward_sensitivity(c, t, dt.p, ntgt, dv.p, dm.p, ds.p);
     ds.download(sens, 6 * (size_t)t.npts, c.stream);
-    KI_HIP(hipStreamSynchronize(c.stream));
+    stream_wait(c.stream);
   });
 }
 

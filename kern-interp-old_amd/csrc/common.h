@@ -88,6 +88,16 @@ struct DBuf {
   }
 };
 
+// Wait for everything queued on stream s by polling hipStreamQuery.  The
+// per-level waits of the tree build, factorization and solve are short; a
+// blocking wait can put the host thread to sleep and wake it late.
+inline void stream_wait(hipStream_t s) {
+  hipError_t e;
+  while ((e = hipStreamQuery(s)) == hipErrorNotReady) {
+  }
+  KI_HIP(e);
+}
+
 // Page-locked host memory (hipHostMalloc) for the small per-level transfers:
 // copies from and to it are plain DMA, where pageable memory goes through a
 // staging copy on every call.  Grow-only; growing waits for the stream, so a
@@ -106,7 +116,7 @@ struct PinnedHost {
     const size_t bytes = count * sizeof(T) + 64;
     if (bytes > cap) {
       if (p) {
-        KI_HIP(hipStreamSynchronize(s));
+        stream_wait(s);
         KI_HIP(hipHostFree(p));
         p = nullptr;
         cap = 0;

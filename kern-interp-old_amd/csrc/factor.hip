@@ -275,6 +275,7 @@ __device__ __forceinline__ double staged_proxy_entry(int pde, int row, int bdof,
 }
 
 constexpr int kIdTB = 1024;
+constexpr int kIdCR = 8;  // rows per lane of the register-cached trailing update
 constexpr int kIdNW = kIdTB / kWave;
 
 // One workgroup per box: generate the ID matrix, Householder QR with column
@@ -424,6 +425,71 @@ __device__ __forceinline__ void id_box(const IdWork& w, unsigned char* smem, Geo
     if (tid == 0) ci[i] = (T)beta;
     // apply H(i)^T to A(i:m, i+1:n) and downdate the partial column norms
     const T ttau = (T)tau;
+    if (m - (i + 1) <= kIdCR * kWave) {
+      // Each wave keeps the Householder vector's rows in registers for the
+      // whole step and a column's rows while it works on it: one LDS read and
+      // one write per entry instead of five accesses (the update is bound by
+      // LDS bandwidth).  Same operations in the same order as below.
+      T cv[kIdCR];
+#pragma unroll
+      for (int u = 0; u < kIdCR; ++u) {
+        const int r = i + 1 + lane + u * kWave;
+        cv[u] = r < m ? ci[r] : T(0);
+      }
+      for (int j = i + 1 + wv; j < n; j += kIdNW) {
+        T* cj = A + (size_t)j * lda;
+        const T aij = cj[i];
+        T y[kIdCR];
+#pragma unroll
+        for (int u = 0; u < kIdCR; ++u) {
+          const int r = i + 1 + lane + u * kWave;
+          y[u] = r < m ? cj[r] : T(0);
+        }
+        T d = lane == 0 ? aij : T(0);
+#pragma unroll
+        for (int u = 0; u < kIdCR; ++u)
+          if (i + 1 + lane + u * kWave < m) d += cv[u] * y[u];
+        d = wave_sum(d);
+        const T f = ttau * d;
+#pragma unroll
+        for (int u = 0; u < kIdCR; ++u) {
+          const int r = i + 1 + lane + u * kWave;
+          if (r < m) {
+            y[u] -= f * cv[u];
+            cj[r] = y[u];
+          }
+        }
+        const T newi = aij - f;
+        if (lane == 0) cj[i] = newi;
+        const double v1 = vn1[j];
+        if (v1 != 0.0) {
+          double tt = fabs((double)newi) / v1;
+          double temp = fmax(1.0 - tt * tt, 0.0);
+          double rr = v1 / vn2[j];
+          double temp2 = temp * rr * rr;
+          if (temp2 <= tol3z) {
+            double s2 = 0;
+#pragma unroll
+            for (int u = 0; u < kIdCR; ++u) {
+              if (i + 1 + lane + u * kWave < m) {
+                double v = (double)y[u];
+                s2 += v * v;
+              }
+            }
+            s2 = wave_sum(s2);
+            if (lane == 0) {
+              double nv = i + 1 < m ? sqrt(s2) : 0.0;
+              vn1[j] = nv;
+              vn2[j] = nv;
+            }
+          } else if (lane == 0) {
+            vn1[j] = v1 * sqrt(temp);
+          }
+        }
+      }
+      __syncthreads();
+      continue;
+    }
     for (int j = i + 1 + wv; j < n; j += kIdNW) {
       T* cj = A + (size_t)j * lda;
       const T aij = cj[i];
@@ -871,7 +937,7 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     if (l >= 3)
       KI_HIP(hipMemcpyAsync(pin + cn, t.nrows.p + a, sizeof(int) * cn, hipMemcpyDeviceToHost,
                             s));
-    KI_HIP(hipStreamSynchronize(s));
+    stream_wait(s);
     ++c.tim.syncs;
     h_n.assign(pin, pin + cn);
     if (l >= 3) h_q.assign(pin + cn, pin + 2 * cn);
@@ -1026,7 +1092,7 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
   int64_t* pin = c.pin_down.get<int64_t>(2 * (size_t)ni, s);
   KI_HIP(hipMemcpyAsync(pin, t.act_off.p, sizeof(int64_t) * ni, hipMemcpyDeviceToHost, s));
   KI_HIP(hipMemcpyAsync(pin + ni, t.nact.p, sizeof(int) * ni, hipMemcpyDeviceToHost, s));
-  KI_HIP(hipStreamSynchronize(s));
+  stream_wait(s);
   ++c.tim.syncs;
   const int64_t r_off0 = pin[0];
   const int* pn = reinterpret_cast<const int*>(pin + ni);
@@ -1076,7 +1142,7 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     KI_HIP(hipGetLastError());
   }
   const int e_root1 = mark();
-  KI_HIP(hipStreamSynchronize(s));
+  stream_wait(s);
   ++c.tim.syncs;
   auto el = [&](int e0, int e1) {
     float ms = 0;
@@ -1142,7 +1208,7 @@ void factor_tree(Context& c, TreeData& t, const kigpu_factor_opts& o) {
   KI_HIP(hipMemcpyAsync(pi, t.rank.p, sizeof(int) * nn, hipMemcpyDeviceToHost, s));
   KI_HIP(hipMemcpyAsync(pi + nn, t.nact.p, sizeof(int) * nn, hipMemcpyDeviceToHost, s));
   KI_HIP(hipMemcpyAsync(pi + 2 * nn, t.status.p, sizeof(int), hipMemcpyDeviceToHost, s));
-  KI_HIP(hipStreamSynchronize(s));
+  stream_wait(s);
   if (pi[2 * nn] & 2) KI_FAIL(KIGPU_ERR_RANK, "ID matrix with fewer rows than columns");
   t.h_act_off.assign(pin, pin + nn);
   t.h_rank.assign(pi, pi + nn);

@@ -880,7 +880,7 @@ static bool has_coincident_points(Context& c, const TreeData& t) {
                                                 t.perm_all.p, t.px.p, t.py.p, flag.p);
     int h = 0;
     flag.download(&h, 1, c.stream);
-    KI_HIP(hipStreamSynchronize(c.stream));
+    stream_wait(c.stream);
     t.dup_state = h;
   }
   return t.dup_state != 0;
@@ -930,7 +930,7 @@ void kernel_apply(Context& c, const TreeData& t, int pde, const double* d_mu, in
     k_sum_sym<<<dim3(t.sym_nblocks, kSymT / kWave), 256, 0, s>>>(t.sym_blocks.p, part.p, t.ndof,
                                                                 d_y);
     KI_HIP(hipGetLastError());
-    KI_HIP(hipStreamSynchronize(s));
+    stream_wait(s);
     float ms = 0;
     KI_HIP(hipEventElapsedTime(&ms, c.ev[12], c.ev[13]));
     c.tim.apply_kernel_s += ms * 1e-3;
@@ -947,7 +947,7 @@ void kernel_apply(Context& c, const TreeData& t, int pde, const double* d_mu, in
   }
   tb.build(t.inst_pt, t.inst_pt, c.stream);
   run_apply<false>(c, t, pde, false, tg, tb, d_mu, t.ndof, nrhs, d_y, t.ndof);
-  KI_HIP(hipStreamSynchronize(c.stream));
+  stream_wait(c.stream);
 }
 
 void kernel_apply_transpose(Context& c, const TreeData& t, int pde, const double* d_lam, int nrhs,
@@ -956,7 +956,7 @@ void kernel_apply_transpose(Context& c, const TreeData& t, int pde, const double
   TileBuf tb;
   tb.build(t.inst_pt, t.inst_pt, c.stream);
   run_apply<true>(c, t, pde, false, rows, tb, d_lam, t.ndof, nrhs, d_y, t.ndof);
-  KI_HIP(hipStreamSynchronize(c.stream));
+  stream_wait(c.stream);
 }
 
 void forward_apply(Context& c, const TreeData& t, int pde, const double* d_tgt, int ntgt,
@@ -968,7 +968,7 @@ void forward_apply(Context& c, const TreeData& t, int pde, const double* d_tgt, 
   TileBuf tb;  // own = targets, other = all boundary points
   tb.build({0, (int64_t)ntgt}, {0, (int64_t)t.npts}, c.stream);
   run_apply<false>(c, t, pde, true, tg, tb, d_mu, t.ndof, nrhs, d_u, sd * ntgt);
-  KI_HIP(hipStreamSynchronize(c.stream));
+  stream_wait(c.stream);
 }
 
 void forward_apply_transpose(Context& c, const TreeData& t, int pde, const double* d_tgt, int ntgt,
@@ -980,7 +980,7 @@ void forward_apply_transpose(Context& c, const TreeData& t, int pde, const doubl
   TileBuf tb;  // own = boundary points, other = targets
   tb.build({0, (int64_t)t.npts}, {0, (int64_t)ntgt}, c.stream);
   run_apply<true>(c, t, pde, true, rows, tb, d_v, sd * ntgt, nrhs, d_y, t.ndof);
-  KI_HIP(hipStreamSynchronize(c.stream));
+  stream_wait(c.stream);
 }
 
 // r = f - y written out; per (instance, column) sums of r^2 and f^2
@@ -999,7 +999,7 @@ static void residual_sums(Context& c, const TreeData& t, const double* f, const 
   const size_t nh = 2 * (size_t)nb * nrhs * ni;
   double* h = c.pin_down.get<double>(nh, s);
   KI_HIP(hipMemcpyAsync(h, c.rsum.p, sizeof(double) * nh, hipMemcpyDeviceToHost, s));
-  KI_HIP(hipStreamSynchronize(s));
+  stream_wait(s);
   rr->assign((size_t)nrhs * ni, 0);
   ff->assign((size_t)nrhs * ni, 0);
   for (size_t q = 0; q < (size_t)nrhs * ni; ++q)
@@ -1032,7 +1032,7 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
       KI_HIP(hipMemcpyAsync(d_alpha, t.alpha_dev.p, sizeof(double) * alen,
                             hipMemcpyDeviceToDevice, s));
     KI_HIP(hipEventRecord(c.ev[7], s));
-    KI_HIP(hipStreamSynchronize(s));
+    stream_wait(s);
     float ms = 0;
     KI_HIP(hipEventElapsedTime(&ms, c.ev[6], c.ev[7]));
     c.tim.t_solve = ms * 1e-3;
@@ -1052,7 +1052,7 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
                           hipMemcpyDeviceToDevice, s));
   }
   KI_HIP(hipEventRecord(c.ev[7], s));
-  KI_HIP(hipStreamSynchronize(s));
+  stream_wait(s);
   float ms = 0;
   KI_HIP(hipEventElapsedTime(&ms, c.ev[6], c.ev[7]));
   c.tim.t_solve = ms * 1e-3;
@@ -1076,7 +1076,7 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
       residual_sums(c, t, d_b, t.f64.p, t.r64.p, nrhs, &rr, &ff);
       if (h) {
         t.rb64.download(hb.data(), alen, s);
-        KI_HIP(hipStreamSynchronize(s));
+        stream_wait(s);
         for (int col = 0; col < nrhs; ++col)
           for (int i = 0; i < h; ++i) rr[col] += hb[i + (size_t)col * h] * hb[i + (size_t)col * h];
       }
@@ -1084,7 +1084,7 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
       for (size_t q = 0; q < rr.size(); ++q)
         rel = std::max(rel, ff[q] > 0 ? std::sqrt(rr[q] / ff[q]) : std::sqrt(rr[q]));
       KI_HIP(hipEventRecord(c.ev[9], s));
-      KI_HIP(hipStreamSynchronize(s));
+      stream_wait(s);
       KI_HIP(hipEventElapsedTime(&ms, c.ev[8], c.ev[9]));
       c.tim.t_refine_apply += ms * 1e-3;
       if (info) {
@@ -1098,14 +1098,14 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
       k_axpy_f32<<<g, 256, 0, s>>>(d_x, t.x32.p, len);
       if (h) k_axpy_f64<<<ceil_div((long)alen, 256), 256, 0, s>>>(t.alpha64.p, t.alpha_dev.p, alen);
       KI_HIP(hipEventRecord(c.ev[9], s));
-      KI_HIP(hipStreamSynchronize(s));
+      stream_wait(s);
       KI_HIP(hipEventElapsedTime(&ms, c.ev[8], c.ev[9]));
       c.tim.t_solve += ms * 1e-3;
     }
   }
   if (h && d_alpha)
     KI_HIP(hipMemcpyAsync(d_alpha, t.alpha64.p, sizeof(double) * alen, hipMemcpyDeviceToDevice, s));
-  KI_HIP(hipStreamSynchronize(s));
+  stream_wait(s);
 }
 
 }  // namespace kigpu

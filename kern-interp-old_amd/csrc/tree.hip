@@ -653,7 +653,7 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin) {
     k_q4_apply<<<ceil_div(np + 1, kTB), kTB, 0, s>>>(np, pre.p, btot.p, qtot);
     int* hx = pin.get<int>(ni + 5, s);
     KI_HIP(hipMemcpyAsync(hx, xfer.p, sizeof(int) * (ni + 5), hipMemcpyDeviceToHost, s));
-    KI_HIP(hipStreamSynchronize(s));
+    stream_wait(s);
     ++*syncs;
     h_split.assign(hx, hx + ni + 1);
     Q4 tot;
@@ -767,7 +767,7 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin) {
   k_cp_total<<<1, 64, 0, s>>>(nn, t.cp_off.p, cpc.p, t.status.p, cxf.p);
   int64_t* hc = pin.get<int64_t>(2, s);
   KI_HIP(hipMemcpyAsync(hc, cxf.p, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  KI_HIP(hipStreamSynchronize(s));
+  stream_wait(s);
   ++*syncs;
   if (hc[1]) KI_FAIL(KIGPU_ERR_STATE, "more than 8 same-level neighbors (overlapping boxes)");
   const int64_t ncp = hc[0];
@@ -793,7 +793,7 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin) {
   KI_HIP(hipMemcpyAsync(hi + nn, t.is_leaf.p, sizeof(int) * nn, hipMemcpyDeviceToHost, s));
   KI_HIP(hipMemcpyAsync(hi + 2 * (size_t)nn, t.pcount.p, sizeof(int) * nn, hipMemcpyDeviceToHost,
                         s));
-  KI_HIP(hipStreamSynchronize(s));
+  stream_wait(s);
   ++*syncs;
   t.h_corners.assign(hd, hd + 8 * (size_t)nn);
   t.h_side.assign(hd + 8 * (size_t)nn, hd + 9 * (size_t)nn);
