@@ -131,6 +131,9 @@ kigpu_status kigpu_context_create(int device, kigpu_ctx** out) {
     c.device = device;
     KI_HIP(hipSetDevice(device));
     KI_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    KI_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
+    KI_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+    KI_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
     for (auto& e : c.ev) KI_HIP(hipEventCreate(&e));
     hipDeviceProp_t prop;
     KI_HIP(hipGetDeviceProperties(&prop, device));
@@ -158,6 +161,10 @@ kigpu_status kigpu_context_destroy(kigpu_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->c.stream);
   ctx->c.ring.release();
   for (auto& e : ctx->c.ev) (void)hipEventDestroy(e);
+  (void)hipStreamSynchronize(ctx->c.side);
+  (void)hipEventDestroy(ctx->c.ev_fork);
+  (void)hipEventDestroy(ctx->c.ev_join);
+  (void)hipStreamDestroy(ctx->c.side);
   (void)hipStreamDestroy(ctx->c.stream);
   delete ctx;
   return KIGPU_OK;

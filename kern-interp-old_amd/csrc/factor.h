@@ -36,6 +36,8 @@ struct LevelWork {
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // second stream for work that overlaps the main one
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipEvent_t ev[16];
   std::vector<hipEvent_t> evpool;  // per-level phase markers of a factorization
   kigpu_timings tim{};

@@ -604,9 +604,14 @@ __host__ __device__ inline size_t slab_elems(int n) { return ((size_t)n * n * 5 
 
 constexpr int kDecTB = 1024;
 constexpr int kDecNW = kDecTB / kWave;
+constexpr int kDecPair = 48;  // |active_box| up to which two eliminations share a CU (75 KiB LDS)
 
+// At most 64 VGPRs (8 waves per SIMD) so that two boxes share a CU when
+// their LDS fits: the elimination of one box is a chain of latency-bound
+// steps, and a second box fills the gaps.
 template <typename T, bool LDS>
-__global__ __launch_bounds__(kDecTB) void k_decouple(int a, GeomView g, int pde, const int* is_leaf,
+__global__ __launch_bounds__(kDecTB) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_decouple(const int* nodes, int a, GeomView g, int pde, const int* is_leaf,
                                                      const int* child, const int* osize_in,
                                                      const uint64_t* acc_ptr_in,
                                                      const int* acc_ld_in, const int* nact,
@@ -617,7 +622,7 @@ __global__ __launch_bounds__(kDecTB) void k_decouple(int a, GeomView g, int pde,
                                                      uint64_t* acc_ptr, int* acc_ld, T* ws,
                                                      size_t ws_stride, int* status) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int x = a + blockIdx.x;
+  const int x = nodes ? nodes[blockIdx.x] : a + blockIdx.x;
   const int n = nact[x];
   const int k = rank[x];
   const int tid = threadIdx.x;
@@ -1002,12 +1007,19 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     std::vector<IdWork> wl(wl_glb);
     wl.insert(wl.end(), wl_lds.begin(), wl_lds.end());
     const int nwl = (int)wl.size();
+    // elimination: boxes whose LDS lets two share a CU, and the larger ones
+    std::vector<int> dec_small, dec_large;
+    for (int i = 0; i < cn; ++i) (h_n[i] <= kDecPair ? dec_small : dec_large).push_back(a + i);
     // ---- the level's lists in one copy
     UploadPack pk;
     const size_t o_soff = pk.add(soff.data(), sizeof(int64_t) * cn);
     const size_t o_roff = pk.add(rows_off.data(), sizeof(int64_t) * cn);
     const size_t o_wl = pk.add(wl.data(), sizeof(IdWork) * wl.size());
+    const size_t o_ds = pk.add(dec_small.data(), sizeof(int) * dec_small.size());
+    const size_t o_dl = pk.add(dec_large.data(), sizeof(int) * dec_large.size());
     unsigned char* dpk = pk.send(c.pin_up, lw->pack, s);
+    const int* d_dec_small = reinterpret_cast<const int*>(dpk + o_ds);
+    const int* d_dec_large = reinterpret_cast<const int*>(dpk + o_dl);
     const int64_t* d_soff = reinterpret_cast<const int64_t*>(dpk + o_soff);
     const int64_t* d_roff = reinterpret_cast<const int64_t*>(dpk + o_roff);
     const IdWork* d_wl = reinterpret_cast<const IdWork*>(dpk + o_wl);
@@ -1057,12 +1069,24 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     E.dec0 = E.id1;
     {
       DecLds L = dec_layout(std::max(nmax, 1), sizeof(T));
-      if (L.total <= lds_cap) {
-        allow_lds(k_decouple<T, true>, L.total);
-        k_decouple<T, true><<<cn, kDecTB, L.total, s>>>(
-            a, g, pde, t.is_leaf.p, t.child.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, t.nact.p,
-            t.rank.p, t.act_off.p, t.act_pool.p, t.sr_pool.p, t.pos_pool.p, t.piv_pool.p,
-            t.fac_ptr.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, (T*)nullptr, 0, t.status.p);
+      auto dec_launch = [&](const int* nodes, int count, size_t lds, hipStream_t st) {
+        allow_lds(k_decouple<T, true>, lds);
+        k_decouple<T, true><<<count, kDecTB, lds, st>>>(
+            nodes, a, g, pde, t.is_leaf.p, t.child.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p,
+            t.nact.p, t.rank.p, t.act_off.p, t.act_pool.p, t.sr_pool.p, t.pos_pool.p,
+            t.piv_pool.p, t.fac_ptr.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, (T*)nullptr, 0,
+            t.status.p);
+      };
+      if (L.total <= lds_cap && !dec_large.empty() && !dec_small.empty()) {
+        // the large boxes on the side stream, the small ones two per CU
+        KI_HIP(hipEventRecord(c.ev_fork, s));
+        KI_HIP(hipStreamWaitEvent(c.side, c.ev_fork, 0));
+        dec_launch(d_dec_large, (int)dec_large.size(), L.total, c.side);
+        dec_launch(d_dec_small, (int)dec_small.size(), dec_layout(kDecPair, sizeof(T)).total, s);
+        KI_HIP(hipEventRecord(c.ev_join, c.side));
+        KI_HIP(hipStreamWaitEvent(s, c.ev_join, 0));
+      } else if (L.total <= lds_cap) {
+        dec_launch(nullptr, cn, L.total, s);
       } else {
         size_t stride = (L.total + sizeof(T) - 1) / sizeof(T);
         T* ws;
@@ -1074,7 +1098,7 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
           ws = reinterpret_cast<T*>(lw->dws32.p);
         }
         k_decouple<T, false><<<cn, kDecTB, 0, s>>>(
-            a, g, pde, t.is_leaf.p, t.child.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, t.nact.p,
+            nullptr, a, g, pde, t.is_leaf.p, t.child.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, t.nact.p,
             t.rank.p, t.act_off.p, t.act_pool.p, t.sr_pool.p, t.pos_pool.p, t.piv_pool.p,
             t.fac_ptr.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, ws, stride, t.status.p);
       }
