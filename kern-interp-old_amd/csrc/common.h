@@ -104,12 +104,18 @@ inline void stream_wait(hipStream_t s) {
 // copy still in flight never loses its buffer.
 struct PinnedHost {
   void* p = nullptr;
+  void* dev = nullptr;  // the same memory seen from the device (kernels may write it)
   size_t cap = 0;
   PinnedHost() = default;
   PinnedHost(const PinnedHost&) = delete;
   PinnedHost& operator=(const PinnedHost&) = delete;
   ~PinnedHost() {
     if (p) (void)hipHostFree(p);
+  }
+  // device view of the block returned by the last get()
+  template <typename T>
+  T* dget() const {
+    return static_cast<T*>(dev);
   }
   template <typename T>
   T* get(size_t count, hipStream_t s) {
@@ -123,7 +129,8 @@ struct PinnedHost {
       }
       size_t nb = 4096;
       while (nb < bytes) nb *= 2;
-      KI_HIP(hipHostMalloc(&p, nb, hipHostMallocDefault));
+      KI_HIP(hipHostMalloc(&p, nb, hipHostMallocMapped | hipHostMallocCoherent));
+      KI_HIP(hipHostGetDevicePointer(&dev, p, 0));
       cap = nb;
     }
     return static_cast<T*>(p);

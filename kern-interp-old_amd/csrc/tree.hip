@@ -21,6 +21,8 @@
 #include <algorithm>
 #include <cmath>
 
+#include <climits>
+
 #include "tree.h"
 
 namespace kigpu {
@@ -436,40 +438,53 @@ __device__ __forceinline__ bool wall_touch(const double* bc, const double* dc) {
 // ancestor level lb (ascending) the leaves among anc(d, lb)'s same-level
 // neighbors in level order, if the wall test passes.  pass 0 counts, pass 1
 // writes at cp_off.
-__global__ void k_coarse(int nnodes, const int* parent, const int* level, const int* is_leaf,
+// ancestors of every node, kMaxDepth per node (anc[d][k] = ancestor at level
+// k, anc[d][level] = d), filled level by level in k_flatten
+constexpr int kMaxDepth = 32;
+
+// the coarser leaves next to node d (get_descendent_neighbors,
+// quadtree.cpp:119-161): for each ancestor level lb = 1 .. l-1, the leaves
+// among the ancestor's same-level neighbors in ascending node id (the
+// reference's list order), kept if they touch d.  pass 0 counts, pass 1
+// writes.  Everything stays in registers: the up-to-8 candidates are
+// ordered by a sorting network.
+__global__ void k_coarse(int nnodes, const int* anc, const int* level, const int* is_leaf,
                          const double* corners, const int* sl_nbr, const int* sl_cnt,
                          const int64_t* cp_off, int* cp_cnt, int* cp_nbr, int pass) {
-  int d = blockIdx.x * blockDim.x + threadIdx.x;
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= nnodes) return;
-  int l = level[d];
-  int anc[64];
-  int cur = d;
-  for (int k = l; k >= 0; --k) {
-    anc[k] = cur;
-    cur = parent[cur];
-  }
-  const double* dc = corners + 8 * d;
+  const int l = level[d];
+  const double* dc = corners + 8 * (size_t)d;
   int cnt = 0;
-  int64_t o = pass ? cp_off[d] : 0;
+  const int64_t o = pass ? cp_off[d] : 0;
   for (int lb = 1; lb < l; ++lb) {
-    int n = anc[lb];
-    int cand[kMaxNbrs];
-    int nc = 0;
-    for (int j = 0; j < sl_cnt[n]; ++j) {
-      int bb = sl_nbr[kMaxNbrs * n + j];
-      if (is_leaf[bb]) cand[nc++] = bb;
+    const int nd = anc[(size_t)d * kMaxDepth + lb];
+    const int ns = sl_cnt[nd];
+    int c[kMaxNbrs];
+#pragma unroll
+    for (int j = 0; j < kMaxNbrs; ++j) {
+      const int bb = j < ns ? sl_nbr[kMaxNbrs * (size_t)nd + j] : -1;
+      c[j] = (bb >= 0 && is_leaf[bb]) ? bb : INT_MAX;
     }
-    for (int i = 1; i < nc; ++i) {  // level order = ascending node id
-      int v = cand[i], k = i - 1;
-      while (k >= 0 && cand[k] > v) {
-        cand[k + 1] = cand[k];
-        --k;
-      }
-      cand[k + 1] = v;
-    }
-    for (int i = 0; i < nc; ++i) {
-      if (wall_touch(corners + 8 * cand[i], dc)) {
-        if (pass) cp_nbr[o + cnt] = cand[i];
+    // Batcher's odd-even merge network for 8 keys
+#define KI_CE(a, b)                     \
+  {                                     \
+    const int lo_ = min(c[a], c[b]);    \
+    const int hi_ = max(c[a], c[b]);    \
+    c[a] = lo_;                         \
+    c[b] = hi_;                         \
+  }
+    KI_CE(0, 1) KI_CE(2, 3) KI_CE(4, 5) KI_CE(6, 7)
+    KI_CE(0, 2) KI_CE(1, 3) KI_CE(4, 6) KI_CE(5, 7)
+    KI_CE(1, 2) KI_CE(5, 6)
+    KI_CE(0, 4) KI_CE(1, 5) KI_CE(2, 6) KI_CE(3, 7)
+    KI_CE(2, 4) KI_CE(3, 5)
+    KI_CE(1, 2) KI_CE(3, 4) KI_CE(5, 6)
+#undef KI_CE
+#pragma unroll
+    for (int i = 0; i < kMaxNbrs; ++i) {
+      if (c[i] != INT_MAX && wall_touch(corners + 8 * (size_t)c[i], dc)) {
+        if (pass) cp_nbr[o + cnt] = c[i];
         ++cnt;
       }
     }
@@ -497,6 +512,7 @@ struct FlatArgs {
   int nperm;               // this level's point permutation, copied to perm_all + pbase
   const int* perm;
   int* perm_all;
+  int* anc;                // ancestor table, kMaxDepth per node
   const int *parent_local, *slot, *split, *rank, *pcount, *inst;
   const int64_t* pstart;
   const double *corners, *side;
@@ -510,6 +526,14 @@ __global__ void k_flatten(FlatArgs f) {
   if (i < f.nperm) f.perm_all[f.pbase + i] = f.perm[i];
   if (i >= f.c) return;
   const int x = f.base + i;
+  {
+    int* ax = f.anc + (size_t)x * kMaxDepth;
+    if (f.parent_local[i] >= 0) {
+      const int* ap = f.anc + (size_t)(f.parent_base + f.parent_local[i]) * kMaxDepth;
+      for (int k2 = 0; k2 < f.level; ++k2) ax[k2] = ap[k2];
+    }
+    ax[f.level] = x;
+  }
   f.parent[x] = f.parent_local[i] < 0 ? -1 : f.parent_base + f.parent_local[i];
   for (int s = 0; s < 4; ++s)
     f.child[4 * x + s] = (f.has_next && f.rank[i] >= 0) ? f.next_base + 4 * f.rank[i] + s : -1;
@@ -561,6 +585,22 @@ __global__ void k_cp_total(int nn, int64_t* cp_off, const int* cnt, const int* s
   cp_off[nn] = tot;
   xfer[0] = tot;
   xfer[1] = *status;
+}
+
+// the host mirrors of the factorization into mapped host memory: doubles
+// corners (8 nn) | side (nn) | min/max (2 ni), then ints inst | is_leaf | pcount
+__global__ void k_mirror_out(int nn, int ni, const double* corners, const double* side,
+                             const double* mm, const int* inst, const int* is_leaf,
+                             const int* pcount, double* hd, int* hi) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < 8 * (int64_t)nn) hd[e] = corners[e];
+  if (e < nn) {
+    hd[8 * (int64_t)nn + e] = side[e];
+    hi[e] = inst[e];
+    hi[nn + e] = is_leaf[e];
+    hi[2 * (int64_t)nn + e] = pcount[e];
+  }
+  if (e < 2 * ni) hd[9 * (int64_t)nn + e] = mm[e];
 }
 
 }  // namespace
@@ -707,6 +747,8 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin) {
   int64_t ptot = 0;
   for (auto& L : lv) ptot += L->nperm;
   t.perm_all.alloc(std::max<int64_t>(ptot, 1));
+  DBuf<int> anc;
+  anc.alloc((size_t)nn * kMaxDepth);
   int64_t pbase = 0;
   for (int l = 0; l < t.nlevels; ++l) {
     LevelBuf& L = *lv[l];
@@ -740,6 +782,7 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin) {
     f.nperm = L.nperm;
     f.perm = L.perm.p;
     f.perm_all = t.perm_all.p;
+    f.anc = anc.p;
     k_flatten<<<ceil_div(std::max(L.count, L.nperm), kTB), kTB, 0, s>>>(f);
     pbase += L.nperm;
   }
@@ -758,9 +801,8 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin) {
   DBuf<int> cpc;
   cpc.alloc(nn);
   t.cp_off.alloc(nn + 1);
-  k_coarse<<<ceil_div(nn, kTB), kTB, 0, s>>>(nn, t.parent.p, t.level.p, t.is_leaf.p,
-                                              t.corners.p, t.sl_nbr.p, t.sl_cnt.p, nullptr,
-                                              cpc.p, nullptr, 0);
+  k_coarse<<<ceil_div(nn, kTB), kTB, 0, s>>>(nn, anc.p, t.level.p, t.is_leaf.p, t.corners.p,
+                                              t.sl_nbr.p, t.sl_cnt.p, nullptr, cpc.p, nullptr, 0);
   exclusive_scan_i32_to_i64(cpc.p, t.cp_off.p, nn, s);
   DBuf<int64_t> cxf;
   cxf.alloc(2);
@@ -772,27 +814,28 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin) {
   if (hc[1]) KI_FAIL(KIGPU_ERR_STATE, "more than 8 same-level neighbors (overlapping boxes)");
   const int64_t ncp = hc[0];
   t.cp_nbr.alloc(std::max<int64_t>(ncp, 1));
-  k_coarse<<<ceil_div(nn, kTB), kTB, 0, s>>>(nn, t.parent.p, t.level.p, t.is_leaf.p,
-                                              t.corners.p, t.sl_nbr.p, t.sl_cnt.p, t.cp_off.p,
-                                              cpc.p, t.cp_nbr.p, 1);
+  k_coarse<<<ceil_div(nn, kTB), kTB, 0, s>>>(nn, anc.p, t.level.p, t.is_leaf.p, t.corners.p,
+                                              t.sl_nbr.p, t.sl_cnt.p, t.cp_off.p, cpc.p,
+                                              t.cp_nbr.p, 1);
 
   // ---- host mirrors the factorization needs (the rest on export), one copy
   // each into page-locked memory and one synchronization
   t.nperm_total = ptot;
   t.ncp = ncp;
   t.mirrors_ready = false;
+  // written by a kernel straight into the mapped page-locked block: a few
+  // hundred KB, and no copy-engine round trip (one of those, idle before,
+  // has been seen to take milliseconds)
   const size_t dbl = 9 * (size_t)nn + 2 * (size_t)ni;  // corners | side | min/max
   double* hd = pin.get<double>(dbl + (3 * (size_t)nn + 1) / 2 + 1, s);
   int* hi = reinterpret_cast<int*>(hd + dbl);
-  KI_HIP(hipMemcpyAsync(hd, t.corners.p, sizeof(double) * 8 * nn, hipMemcpyDeviceToHost, s));
-  KI_HIP(hipMemcpyAsync(hd + 8 * (size_t)nn, t.side.p, sizeof(double) * nn, hipMemcpyDeviceToHost,
-                        s));
-  KI_HIP(hipMemcpyAsync(hd + 9 * (size_t)nn, mm.p, sizeof(double) * 2 * ni, hipMemcpyDeviceToHost,
-                        s));
-  KI_HIP(hipMemcpyAsync(hi, t.inst.p, sizeof(int) * nn, hipMemcpyDeviceToHost, s));
-  KI_HIP(hipMemcpyAsync(hi + nn, t.is_leaf.p, sizeof(int) * nn, hipMemcpyDeviceToHost, s));
-  KI_HIP(hipMemcpyAsync(hi + 2 * (size_t)nn, t.pcount.p, sizeof(int) * nn, hipMemcpyDeviceToHost,
-                        s));
+  {
+    double* dd = pin.dget<double>();
+    k_mirror_out<<<ceil_div(9 * (int64_t)nn, kTB), kTB, 0, s>>>(
+        nn, ni, t.corners.p, t.side.p, mm.p, t.inst.p, t.is_leaf.p, t.pcount.p, dd,
+        reinterpret_cast<int*>(dd + dbl));
+    KI_HIP(hipGetLastError());
+  }
   stream_wait(s);
   ++*syncs;
   t.h_corners.assign(hd, hd + 8 * (size_t)nn);
