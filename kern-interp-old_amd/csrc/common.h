@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <cstring>
+#include <initializer_list>
 #include <string>
 #include <utility>
 #include <vector>
@@ -136,6 +137,19 @@ struct PinnedHost {
     return static_cast<T*>(p);
   }
 };
+
+// Device arrays copied into a mapped page-locked block by one kernel launch
+// (byte offsets into the block; sizes multiples of 4): the per-level results
+// the host needs, without the copy engine.  Read them after stream_wait.
+struct ToHost {
+  const void* src;
+  size_t off;
+  size_t bytes;
+};
+void copy_to_host(PinnedHost& pin, hipStream_t s, const ToHost* parts, int nparts);
+inline void copy_to_host(PinnedHost& pin, hipStream_t s, std::initializer_list<ToHost> parts) {
+  copy_to_host(pin, s, parts.begin(), (int)parts.size());
+}
 
 // Several host arrays sent to the device with one copy through page-locked
 // memory: add() reserves a 256-byte aligned slot and returns its offset,

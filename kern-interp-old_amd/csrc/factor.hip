@@ -938,10 +938,9 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
                                         t.cp_nbr.p, t.corners.p, t.side.p, t.nrows.p, nullptr,
                                         nullptr, 0);
     int* pin = c.pin_down.get<int>(2 * (size_t)cn, s);
-    KI_HIP(hipMemcpyAsync(pin, t.nact.p + a, sizeof(int) * cn, hipMemcpyDeviceToHost, s));
-    if (l >= 3)
-      KI_HIP(hipMemcpyAsync(pin + cn, t.nrows.p + a, sizeof(int) * cn, hipMemcpyDeviceToHost,
-                            s));
+    copy_to_host(c.pin_down, s,
+                 {{t.nact.p + a, 0, sizeof(int) * cn},
+                  {t.nrows.p + a, sizeof(int) * cn, l >= 3 ? sizeof(int) * cn : 0}});
     stream_wait(s);
     ++c.tim.syncs;
     h_n.assign(pin, pin + cn);
@@ -1114,8 +1113,9 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
   const int e_root0 = mark();
   const int ni = t.ninst;
   int64_t* pin = c.pin_down.get<int64_t>(2 * (size_t)ni, s);
-  KI_HIP(hipMemcpyAsync(pin, t.act_off.p, sizeof(int64_t) * ni, hipMemcpyDeviceToHost, s));
-  KI_HIP(hipMemcpyAsync(pin + ni, t.nact.p, sizeof(int) * ni, hipMemcpyDeviceToHost, s));
+  copy_to_host(c.pin_down, s,
+               {{t.act_off.p, 0, sizeof(int64_t) * ni}, {t.nact.p, sizeof(int64_t) * ni,
+                                                         sizeof(int) * ni}});
   stream_wait(s);
   ++c.tim.syncs;
   const int64_t r_off0 = pin[0];
@@ -1228,10 +1228,10 @@ void factor_tree(Context& c, TreeData& t, const kigpu_factor_opts& o) {
   hipStream_t s = c.stream;
   int64_t* pin = c.pin_down.get<int64_t>(2 * (size_t)nn + 8, s);
   int* pi = reinterpret_cast<int*>(pin + nn);
-  KI_HIP(hipMemcpyAsync(pin, t.act_off.p, sizeof(int64_t) * nn, hipMemcpyDeviceToHost, s));
-  KI_HIP(hipMemcpyAsync(pi, t.rank.p, sizeof(int) * nn, hipMemcpyDeviceToHost, s));
-  KI_HIP(hipMemcpyAsync(pi + nn, t.nact.p, sizeof(int) * nn, hipMemcpyDeviceToHost, s));
-  KI_HIP(hipMemcpyAsync(pi + 2 * nn, t.status.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  const size_t b8 = sizeof(int64_t) * nn, b4 = sizeof(int) * nn;
+  copy_to_host(c.pin_down, s,
+               {{t.act_off.p, 0, b8}, {t.rank.p, b8, b4}, {t.nact.p, b8 + b4, b4},
+                {t.status.p, b8 + 2 * b4, sizeof(int)}});
   stream_wait(s);
   if (pi[2 * nn] & 2) KI_FAIL(KIGPU_ERR_RANK, "ID matrix with fewer rows than columns");
   t.h_act_off.assign(pin, pin + nn);

@@ -998,7 +998,7 @@ static void residual_sums(Context& c, const TreeData& t, const double* f, const 
   k_residual<<<dim3(nb, nrhs, ni), kTB, 0, s>>>(f, y, r, n, nrhs, t.d_dof_off.p, c.rsum.p);
   const size_t nh = 2 * (size_t)nb * nrhs * ni;
   double* h = c.pin_down.get<double>(nh, s);
-  KI_HIP(hipMemcpyAsync(h, c.rsum.p, sizeof(double) * nh, hipMemcpyDeviceToHost, s));
+  copy_to_host(c.pin_down, s, {{c.rsum.p, 0, sizeof(double) * nh}});
   stream_wait(s);
   rr->assign((size_t)nrhs * ni, 0);
   ff->assign((size_t)nrhs * ni, 0);

@@ -692,7 +692,7 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin) {
     k_q4_top<<<1, 64, 0, s>>>(nb, btot.p, qtot);
     k_q4_apply<<<ceil_div(np + 1, kTB), kTB, 0, s>>>(np, pre.p, btot.p, qtot);
     int* hx = pin.get<int>(ni + 5, s);
-    KI_HIP(hipMemcpyAsync(hx, xfer.p, sizeof(int) * (ni + 5), hipMemcpyDeviceToHost, s));
+    copy_to_host(pin, s, {{xfer.p, 0, sizeof(int) * (ni + 5)}});
     stream_wait(s);
     ++*syncs;
     h_split.assign(hx, hx + ni + 1);
@@ -808,7 +808,7 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin) {
   cxf.alloc(2);
   k_cp_total<<<1, 64, 0, s>>>(nn, t.cp_off.p, cpc.p, t.status.p, cxf.p);
   int64_t* hc = pin.get<int64_t>(2, s);
-  KI_HIP(hipMemcpyAsync(hc, cxf.p, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  copy_to_host(pin, s, {{cxf.p, 0, 2 * sizeof(int64_t)}});
   stream_wait(s);
   ++*syncs;
   if (hc[1]) KI_FAIL(KIGPU_ERR_STATE, "more than 8 same-level neighbors (overlapping boxes)");
