@@ -486,13 +486,14 @@ __global__ __launch_bounds__(kAT) void k_apply_dl(Targets other_or_tg, Tiles tl,
 //          1/2 + kappa w / (4 pi) added once.
 // The contribution of pair (u, v) to block u goes to partial slot v, the one
 // to block v to slot u, so every node gets each of its instance's slots once
-// and k_sum_sym adds them.  Padding lanes and sources sit at x = +-1e150 with
+// and k_sum_sym adds them.  Padding lanes and sources sit at x = +-1e30 with
 // a = 0: their pairs with real nodes add exactly 0 and |r|^2 stays finite.
 constexpr int kSymT = 512;              // nodes per block
 constexpr int kSymP = 4;                // targets per lane
 constexpr int kSymTB = kSymT / kSymP;   // threads per workgroup
 constexpr int kSymNW = kSymTB / kWave;  // waves per workgroup
-constexpr double kFar = 1e150;
+static_assert(kSymP == 4, "the off-diagonal loop shares one reciprocal between four pairs");
+constexpr double kFar = 1e30;
 constexpr double kTinyR2 = 1e-300;
 
 struct SymPair {
@@ -575,15 +576,29 @@ __global__ __launch_bounds__(kSymTB) void k_apply_sym(const SymPair* pairs, Geom
     for (int s = 0; s < kWave; ++s) {
       const int j = (lane + s) & (kWave - 1);
       const double2 xy = cxy[j], a = ca[j];
+      double dx[kSymP], dy[kSymP], d[kSymP];
+#pragma unroll
+      for (int p = 0; p < kSymP; ++p) {
+        dx[p] = tx[p] - xy.x;
+        dy[p] = ty[p] - xy.y;
+        d[p] = fma(dx[p], dx[p], dy[p] * dy[p]);
+      }
+      // one reciprocal for the four pairs (v_rcp_f64 issues at about a
+      // quarter of the FMA rate): 1/(d0 d1 d2 d3) with one Newton step,
+      // then 1/d0 = d1 (d2 d3) / (d0 d1 d2 d3) and so on; all d lie in
+      // [1e-20, 4e60] (distinct nodes, padding at 1e30), so no product
+      // overflows or underflows
+      const double p01 = d[0] * d[1], p23 = d[2] * d[3];
+      const double rr = recip1(p01 * p23);
+      const double r01 = p23 * rr, r23 = p01 * rr;
+      const double inv[kSymP] = {d[1] * r01, d[0] * r01, d[3] * r23, d[2] * r23};
       double cs = 0;
 #pragma unroll
       for (int p = 0; p < kSymP; ++p) {
-        const double dx = tx[p] - xy.x, dy = ty[p] - xy.y;
-        const double r = recip1(fma(dx, dx, dy * dy));
-        const double ni = fma(dx, a.x, dy * a.y);
-        const double nj = fma(dx, tax[p], dy * tay[p]);
-        ta[p] = fma(ni, r, ta[p]);
-        cs = fma(nj, r, cs);
+        const double ni = fma(dx[p], a.x, dy[p] * a.y);
+        const double nj = fma(dx[p], tax[p], dy[p] * tay[p]);
+        ta[p] = fma(ni, inv[p], ta[p]);
+        cs = fma(nj, inv[p], cs);
       }
       js = __shfl(js, next) + cs;  // all lanes hold 0 before the first step
     }
