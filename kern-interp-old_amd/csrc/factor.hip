@@ -188,7 +188,7 @@ __global__ void k_outside_rows(const IdWork* work, int a, int b, int level, int 
 // ---------------------------------------------------------------------------
 
 struct IdLds {
-  size_t a, rgeo, cgeo, rows, act, vn1, vn2, perm, lp2, xn, vb, red, total;
+  size_t a, rgeo, cgeo, rows, act, vn1, vn2, perm, vb, red, total;
 };
 
 constexpr int kIdTB = 1024;
@@ -226,10 +226,6 @@ __host__ __device__ inline IdLds id_lds_layout(int m, int n, int q, bool a_in_ld
   o = align16(o + sizeof(double) * (size_t)n);
   L.perm = o;
   o = align16(o + sizeof(int) * (size_t)n);
-  L.lp2 = o;
-  o = align16(o + sizeof(int) * (size_t)n);
-  L.xn = o;
-  o = align16(o + sizeof(double) * (size_t)n);
   L.vb = o;  // Householder vector, beta and tau of the register-resident QR
   if (id_reg_box(m, n)) o = align16(o + sizeof(double) * ((size_t)m + 2));
   L.red = o;
@@ -329,159 +325,10 @@ __device__ __forceinline__ void id_finish(const IdWork& w, T* A, int lda, int n,
   if (tid == 0) rank[w.node] = k;
 }
 
-// The pivoted QR of id_box for boxes with m - 1 <= kIdCR * 64 rows below
-// the diagonal, with one block barrier per Householder step:
-//   * every wave finds the pivot itself (first maximum of the partial norms
-//     over logical positions i..n-1, as in dlaqp2);
-//   * dlaqp2's column interchange is a swap in the logical-to-physical map
-//     (double-buffered between perm and lp2), so no column moves;
-//   * dlarfg's |x|^2 below the diagonal of the pivot column comes from xn,
-//     which the previous step's update left for every column;
-//   * the Householder vector is scaled in registers, not in LDS;
-//   * R(i, i) = beta is written after the step's barrier (the pivot column
-//     is read during the step and retired after it).
-// Each wave keeps the vector's rows in registers for the step, and a column's
-// rows while it updates that column.  Outputs as in id_box.
-template <typename T, typename TS>
-__device__ __forceinline__ void id_qr_fast(const IdWork& w, T* A, int lda, int m, int n, double tol,
-                                           const int* s_act, double* vn1, double* vn2, int* lpa,
-                                           int* lpb, double* xn, const uint64_t* fac_ptr,
-                                           int* rank, int* sr_pool, int* pos_pool, int64_t aoff,
-                                           int* status) {
-  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
-  const double tol3z = sqrt(2.220446049250313e-16);
-  double thresh = 0;
-  int k = 0;
-  int fin = 0;  // 0: lpa holds the final map, 1: lpb
-  for (int i = 0; i < n; ++i) {
-    if (i >= m) {  // the reference reads R(i, i) past the last row here
-      if (tid == 0) flag(status, 2);
-      k = 0;
-      break;
-    }
-    const int* lpc = (i & 1) ? lpb : lpa;
-    int* lpn = (i & 1) ? lpa : lpb;
-    double best = -1.0;
-    int bi = n;
-    for (int j = i + lane; j < n; j += kWave) {
-      const double v = vn1[lpc[j]];
-      if (v > best) {
-        best = v;
-        bi = j;
-      }
-    }
-    wave_argmax(best, bi);
-    int pi = __builtin_amdgcn_readfirstlane(bi);
-    if (pi >= n) pi = i;  // NaN norms: keep the order
-    const int p = lpc[pi], pc = lpc[i];
-    if (wv == 0)
-      for (int j = lane; j < n; j += kWave) lpn[j] = j == i ? p : (j == pi ? pc : lpc[j]);
-    fin = (i + 1) & 1;
-    // dlarfg on the pivot column below row i
-    T* ci = A + (size_t)p * lda;
-    const double alpha = (double)ci[i];
-    const double xnorm = sqrt(fmax(xn[p], 0.0));
-    double beta, tau;
-    if (xnorm == 0.0) {
-      beta = alpha;
-      tau = 0.0;
-    } else {
-      beta = -copysign(hypot(alpha, xnorm), alpha);
-      tau = (beta - alpha) / beta;
-    }
-    if (i == 0) {
-      thresh = fabs(tol * beta);
-    } else if (fabs(beta) < thresh) {
-      k = i;
-      break;
-    }
-    if (i == n - 1) {  // no |R_ii| fell below the threshold: not compressed
-      k = 0;
-      break;
-    }
-    const T scal = xnorm == 0.0 ? T(0) : (T)(1.0 / (alpha - beta));
-    const T ttau = (T)tau;
-    T cv[kIdCR];
-#pragma unroll
-    for (int u = 0; u < kIdCR; ++u) {
-      const int r = i + 1 + lane + u * kWave;
-      cv[u] = r < m ? ci[r] * scal : T(0);
-    }
-    // H(i)^T on the trailing columns (logical i+1..n-1, after the swap)
-    for (int j = i + 1 + wv; j < n; j += kIdNW) {
-      const int qc = j == pi ? pc : lpc[j];
-      T* cj = A + (size_t)qc * lda;
-      const T aij = cj[i];
-      T y[kIdCR];
-#pragma unroll
-      for (int u = 0; u < kIdCR; ++u) {
-        const int r = i + 1 + lane + u * kWave;
-        y[u] = r < m ? cj[r] : T(0);
-      }
-      T d = lane == 0 ? aij : T(0);
-#pragma unroll
-      for (int u = 0; u < kIdCR; ++u)
-        if (i + 1 + lane + u * kWave < m) d += cv[u] * y[u];
-      d = wave_sum(d);
-      const T f = ttau * d;
-      double sn = 0;  // |column|^2 below row i + 1, for the next dlarfg
-#pragma unroll
-      for (int u = 0; u < kIdCR; ++u) {
-        const int r = i + 1 + lane + u * kWave;
-        if (r < m) {
-          y[u] -= f * cv[u];
-          cj[r] = y[u];
-          if (r > i + 1) sn += (double)y[u] * (double)y[u];
-        }
-      }
-      sn = wave_sum(sn);
-      const T newi = aij - f;
-      if (lane == 0) {
-        cj[i] = newi;
-        xn[qc] = sn;
-      }
-      const double v1 = vn1[qc];
-      if (v1 != 0.0) {
-        double tt = fabs((double)newi) / v1;
-        double temp = fmax(1.0 - tt * tt, 0.0);
-        double rr = v1 / vn2[qc];
-        double temp2 = temp * rr * rr;
-        if (temp2 <= tol3z) {
-          double s2 = 0;
-#pragma unroll
-          for (int u = 0; u < kIdCR; ++u) {
-            if (i + 1 + lane + u * kWave < m) {
-              double v = (double)y[u];
-              s2 += v * v;
-            }
-          }
-          s2 = wave_sum(s2);
-          if (lane == 0) {
-            double nv = i + 1 < m ? sqrt(s2) : 0.0;
-            vn1[qc] = nv;
-            vn2[qc] = nv;
-          }
-        } else if (lane == 0) {
-          vn1[qc] = v1 * sqrt(temp);
-        }
-      }
-    }
-    __syncthreads();
-    if (tid == 0) ci[i] = (T)beta;  // R(i, i); the column is retired
-  }
-  __syncthreads();
-  if (k == 0) {
-    if (tid == 0) rank[w.node] = -1;
-    return;
-  }
-  id_finish<T, TS>(w, A, lda, n, k, fin ? lpb : lpa, s_act, fac_ptr, rank, sr_pool, pos_pool,
-                   aoff);
-}
-
-// Householder QR with column pivoting, as in id_qr_fast, with the ID matrix
-// held in registers for the whole factorization: wave w keeps columns
-// w, w + 16, w + 32 (physical order), lane l rows l, l + 64, ..., and each
-// wave keeps its columns' partial norms.  Per step:
+// Householder QR with column pivoting, the same dlaqp2 steps as id_box below,
+// with the ID matrix held in registers for the whole factorization: wave w
+// keeps columns w, w + 16, w + 32 (physical order), lane l rows l, l + 64,
+// ..., and each wave keeps its columns' partial norms.  Per step:
 //   * every wave finds the pivot (first maximum of the partial norms over
 //     logical positions i..n-1) and swaps the logical-to-physical map, which
 //     lane j of every wave holds for position j;
@@ -688,24 +535,18 @@ __device__ __forceinline__ void id_box(const IdWork& w, unsigned char* smem, Geo
   });
   __syncthreads();
 
-  // ---- column norms (dgeqp3: DNRM2 of each column), FP64; xn: the squared
-  // norm below row 0, which the first dlarfg needs
-  int* lp2 = reinterpret_cast<int*>(smem + Ly.lp2);
-  double* xn = reinterpret_cast<double*>(smem + Ly.xn);
+  // ---- column norms (dgeqp3: DNRM2 of each column), FP64
   for (int j = wv; j < n; j += kIdNW) {
     const T* col = A + (size_t)j * lda;
-    double s = 0, s1 = 0;
+    double s = 0;
     for (int i = lane; i < m; i += kWave) {
       double v = (double)col[i];
       s += v * v;
-      if (i > 0) s1 += v * v;
     }
     s = wave_sum(s);
-    s1 = wave_sum(s1);
     if (lane == 0) {
       vn1[j] = sqrt(s);
       vn2[j] = vn1[j];
-      xn[j] = s1;
     }
   }
   __syncthreads();
@@ -713,11 +554,6 @@ __device__ __forceinline__ void id_box(const IdWork& w, unsigned char* smem, Geo
     id_qr_reg<T, TS>(w, A, lda, m, n, tol, s_act, vn1, vn2, perm,
                      reinterpret_cast<double*>(smem + Ly.vb), fac_ptr, rank, sr_pool, pos_pool,
                      aoff, status);
-    return;
-  }
-  if (m - 1 <= kIdCR * kWave) {
-    id_qr_fast<T, TS>(w, A, lda, m, n, tol, s_act, vn1, vn2, perm, lp2, xn, fac_ptr, rank,
-                      sr_pool, pos_pool, aoff, status);
     return;
   }
 

@@ -619,6 +619,170 @@ __global__ __launch_bounds__(kSymTB) void k_apply_sym(const SymPair* pairs, Geom
   }
 }
 
+// Apply to many right-hand sides as a product on the FP64 matrix cores:
+// y_part[split] = K(tile rows, split's sources) * mu(split's sources, :).
+// A block of 4 waves owns 128 target DOFs (32 per wave, two 16-row blocks)
+// and 16 * NCB right-hand sides per pass.  Sources go through LDS in chunks
+// of 32 DOFs (per-source values and the density block), the next chunk
+// loaded into registers while the current one is multiplied.  Each lane
+// computes the kernel entries of its own A-operand slots of
+// v_mfma_f64_16x16x4f64 (row lane % 16, source lane / 16 of each 4-wide
+// step) and reads its B-operand slot (source lane / 16, column lane % 16)
+// from LDS; accumulator element v of lane l is row l / 16 + 4 v, column
+// l % 16 of its 16 x 16 block.  Rows past the tile are computed and not
+// stored (a row of the product depends only on its own row of K); sources
+// past the split are placed far away with zero weight and zero density, so
+// their entries are 0.
+//   Laplace: per source (x, y), a = -w n / (2 pi) and the diagonal value;
+//   the entry is (r . a) / |r|^2 (one Newton step on v_rcp_f64, as in
+//   k_apply_sym), or the diagonal value where r is exactly zero.
+//   Laplace-Neumann and Stokes: the entries of laplace_entry / stokes_entry.
+typedef double mm_d4 __attribute__((ext_vector_type(4)));
+constexpr int kMmTB = 256;
+constexpr int kMmRows = 128;  // target DOFs per block
+constexpr int kMmS = 32;      // source DOFs per LDS chunk
+constexpr int kMmLd = 72;     // LDS row stride of the density chunk (64 columns + 8)
+
+template <int PDE, int NCB>
+__global__ __launch_bounds__(kMmTB) void k_apply_mm(Targets tg, Tiles tl, GeomView g,
+                                                    const double* mu, int ldm, int nrhs,
+                                                    double* part, int ldp) {
+  constexpr int SD = PDE == KIGPU_STOKES ? 2 : 1;
+  constexpr int SP = kMmS / SD;                    // source points per chunk
+  constexpr int NM = kMmS * 16 * NCB / kMmTB;      // density values per thread and chunk
+  static_assert(16 * NCB <= kMmLd && NM >= 1, "density chunk");
+  __shared__ double s_src[6][SP];
+  __shared__ double s_mu[kMmS * kMmLd];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  const int tp0 = tl.t0[blockIdx.x], tp1 = tl.t1[blockIdx.x];
+  double tx[2], ty[2], tnx[2], tny[2];
+  int tc[2];
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb) {
+    const int row = 32 * wv + 16 * rb + (lane & 15);
+    const int pt = min(tp0 + row / SD, tp1 - 1);  // rows past the tile repeat the last point
+    tc[rb] = row % SD;
+    tx[rb] = tg.x[pt];
+    ty[rb] = tg.y[pt];
+    tnx[rb] = tg.nx[pt];
+    tny[rb] = tg.ny[pt];
+  }
+  // this split's source points, in whole chunks
+  const int b = tl.inst[blockIdx.x];
+  const int oa = (int)tl.opt[b], oe = (int)tl.opt[b + 1];
+  int per = (oe - oa + (int)gridDim.y - 1) / (int)gridDim.y;
+  per = (per + SP - 1) / SP * SP;
+  const int s0 = min(oe, oa + (int)blockIdx.y * per), s1 = min(oe, s0 + per);
+  const size_t pbase = (size_t)blockIdx.y * ldp * nrhs;
+  for (int c0 = 0; c0 < nrhs; c0 += 16 * NCB) {
+    mm_d4 acc[2][NCB];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb) acc[rb][cb] = mm_d4{0.0, 0.0, 0.0, 0.0};
+    // registers holding the next chunk
+    double rs[6];
+    double rm[NM];
+    auto fetch = [&](int b0) {
+      const int nb = min(SP, s1 - b0);
+      if (tid < SP) {
+        const int sp = b0 + tid;
+        if (tid < nb) {
+          const double w = g.w[sp], kap = g.kappa[sp];
+          rs[0] = g.px[sp];
+          rs[1] = g.py[sp];
+          if (PDE == KIGPU_LAPLACE) {
+            rs[2] = -w * kInv2Pi * g.nx[sp];
+            rs[3] = -w * kInv2Pi * g.ny[sp];
+            rs[4] = 0.5 + 0.5 * kap * w * kInv2Pi;
+            rs[5] = 0.0;
+          } else {
+            rs[2] = g.nx[sp];
+            rs[3] = g.ny[sp];
+            rs[4] = w;
+            rs[5] = kap;
+          }
+        } else {
+          rs[0] = kFar;
+          rs[1] = kFar;
+          rs[2] = rs[3] = rs[4] = rs[5] = 0.0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < NM; ++u) {
+        const int e = tid + kMmTB * u;
+        const int cc = e / kMmS, d = e - cc * kMmS;
+        const int col = c0 + cc;
+        rm[u] = (d < SD * nb && col < nrhs) ? mu[(size_t)SD * b0 + d + (size_t)col * ldm] : 0.0;
+      }
+    };
+    if (s0 < s1) fetch(s0);
+    for (int b0 = s0; b0 < s1; b0 += SP) {
+      __syncthreads();
+      if (tid < SP)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) s_src[q][tid] = rs[q];
+#pragma unroll
+      for (int u = 0; u < NM; ++u) {
+        const int e = tid + kMmTB * u;
+        const int cc = e / kMmS, d = e - cc * kMmS;
+        s_mu[d * kMmLd + cc] = rm[u];
+      }
+      __syncthreads();
+      if (b0 + SP < s1) fetch(b0 + SP);
+#pragma unroll 2
+      for (int ks = 0; ks < kMmS / 4; ++ks) {
+        const int d = 4 * ks + (lane >> 4);  // source DOF in the chunk
+        const int sp = d / SD, sc = d % SD;
+        const double sx = s_src[0][sp], sy = s_src[1][sp];
+        double a[2];
+        if (PDE == KIGPU_LAPLACE) {
+          const double ax = s_src[2][sp], ay = s_src[3][sp], dg = s_src[4][sp];
+#pragma unroll
+          for (int rb = 0; rb < 2; ++rb) {
+            const double r1 = tx[rb] - sx, r2 = ty[rb] - sy;
+            const double rr = fma(r1, r1, r2 * r2);
+            const double v = fma(r1, ax, r2 * ay) * recip1(rr);
+            a[rb] = rr == 0.0 ? dg : v;
+          }
+        } else {
+          const double snx = s_src[2][sp], sny = s_src[3][sp], sw = s_src[4][sp],
+                       sk = s_src[5][sp];
+#pragma unroll
+          for (int rb = 0; rb < 2; ++rb) {
+            double v;
+            if (PDE == KIGPU_STOKES)
+              v = stokes_entry(tc[rb], sc, tx[rb], ty[rb], tnx[rb], tny[rb], sx, sy, snx, sny,
+                               sw, sk);
+            else
+              v = laplace_entry(PDE, tx[rb] - sx, ty[rb] - sy, sw, sk, snx, sny, tnx[rb],
+                                tny[rb], false);
+            a[rb] = sx == kFar ? 0.0 : v;  // padding sources
+          }
+        }
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) {
+          const double bv = s_mu[d * kMmLd + 16 * cb + (lane & 15)];
+#pragma unroll
+          for (int rb = 0; rb < 2; ++rb)
+            acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], bv, acc[rb][cb], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = 32 * wv + 16 * rb + (lane >> 4) + 4 * v;
+          const int col = c0 + 16 * cb + (lane & 15);
+          if (tp0 + row / SD < tp1 && col < nrhs)
+            part[pbase + (size_t)SD * tp0 + row + (size_t)col * ldp] = acc[rb][cb][v];
+        }
+  }
+}
+
 // y over the blocks (3 ints per block: start, length, slots of its
 // instance): node i of block blockIdx.x adds its slots; 64 nodes x 4 slot
 // phases per workgroup, the four phase sums combined in a fixed order
@@ -802,10 +966,13 @@ __global__ void k_split_xy(const double* xy, int n, double* x, double* y) {
 struct TileBuf {
   DBuf<int> inst, t0, t1;
   DBuf<int64_t> opt;
+  std::vector<int64_t> own, other;  // the segments it was built from
   int ntiles = 0;
   int maxlen = 0;  // longest other-segment
-  void build(const std::vector<int64_t>& own, const std::vector<int64_t>& other, hipStream_t s,
+  void build(const std::vector<int64_t>& own_, const std::vector<int64_t>& other_, hipStream_t s,
              int tile = kAT) {
+    own = own_;
+    other = other_;
     std::vector<int> hi, h0, h1;
     maxlen = 0;
     for (size_t b = 0; b + 1 < own.size(); ++b) {
@@ -849,9 +1016,37 @@ void run_apply(Context& c, const TreeData& t, int pde, bool fwd, const Targets& 
                const TileBuf& tb, const double* d_in, int ldi, int nrhs, double* d_out, int ldo) {
   hipStream_t s = c.stream;
   const GeomView g{t.px.p, t.py.p, t.nx.p, t.ny.p, t.w.p, t.kappa.p, t.npts, t.sd};
-  const int split = tb.splits(TRANS ? 256 : (fwd ? 256 : 64));
   const size_t len = (size_t)ldo * nrhs;
   DBuf<double> part;
+  if (!TRANS && !fwd && nrhs >= 2) {
+    // many right-hand sides: the product on the matrix cores, tiles of
+    // kMmRows target DOFs
+    const int sd = pde == KIGPU_STOKES ? 2 : 1;
+    TileBuf mt;
+    mt.build(tb.own, tb.other, s, kMmRows / sd);
+    const int msplit = std::max(1, std::min(ceil_div(2048, std::max(mt.ntiles, 1)),
+                                            ceil_div(mt.maxlen, kMmS / sd * 8)));
+    part.alloc(len * msplit);
+    const dim3 mgrid(mt.ntiles, msplit);
+    const Tiles mtl = mt.view();
+#define KI_MM(P)                                                                              \
+  do {                                                                                        \
+    if (nrhs <= 16)                                                                           \
+      k_apply_mm<P, 1><<<mgrid, kMmTB, 0, s>>>(other, mtl, g, d_in, ldi, nrhs, part.p, ldo);  \
+    else if (nrhs <= 32)                                                                      \
+      k_apply_mm<P, 2><<<mgrid, kMmTB, 0, s>>>(other, mtl, g, d_in, ldi, nrhs, part.p, ldo);  \
+    else                                                                                      \
+      k_apply_mm<P, 4><<<mgrid, kMmTB, 0, s>>>(other, mtl, g, d_in, ldi, nrhs, part.p, ldo);  \
+  } while (0)
+    if (pde == KIGPU_LAPLACE) KI_MM(KIGPU_LAPLACE);
+    else if (pde == KIGPU_LAPLACE_NEUMANN) KI_MM(KIGPU_LAPLACE_NEUMANN);
+    else KI_MM(KIGPU_STOKES);
+#undef KI_MM
+    k_sum_parts<<<ceil_div((long)len, 256), 256, 0, s>>>(part.p, msplit, len, d_out);
+    KI_HIP(hipGetLastError());
+    return;
+  }
+  const int split = tb.splits(TRANS ? 256 : (fwd ? 256 : 64));
   part.alloc(len * split);
   dim3 grid(tb.ntiles, split);
   const Tiles tl = tb.view();

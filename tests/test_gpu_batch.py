@@ -128,6 +128,24 @@ def test_forest_apply_one_density(ctx):
 
 
 @pytest.mark.gpu
+def test_forest_apply_many_rhs(ctx):
+    # the matrix-core apply over instances of uneven sizes, 24 right-hand sides
+    print("[batch] 24-column apply over 4 instances", flush=True)
+    sizes = [300, 512, 1100, 2049]
+    geoms = [W.random_star(n, 70 + i) for i, n in enumerate(sizes)]
+    f = ctx.build_batch(geoms, 1, 64)
+    pt_off = f.instances()["point_offsets"]
+    mu = np.random.default_rng(9).standard_normal((int(pt_off[-1]), 24))
+    y = f.apply(mu, kigpu.LAPLACE)
+    for i, n in enumerate(sizes):
+        b = G.random_star(n, 70 + i)
+        a, e = pt_off[i], pt_off[i + 1]
+        ref = K.Kernel(1, K.LAPLACE, b)(np.arange(n), np.arange(n)) @ mu[a:e]
+        assert np.allclose(y[a:e], ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+    f.close()
+
+
+@pytest.mark.gpu
 def test_forest_fp32_refined(ctx):
     print("[batch] FP32 + refinement, 8 instances", flush=True)
     geoms = instance_geoms(8, 13)

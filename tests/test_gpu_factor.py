@@ -176,6 +176,30 @@ def test_apply_one_density_blocks(ctx, n):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("pde", [K.LAPLACE, K.LAPLACE_NEUMANN, K.STOKES])
+@pytest.mark.parametrize("nrhs", [2, 16, 21, 70])
+def test_apply_many_rhs(ctx, pde, nrhs):
+    # K mu for several right-hand sides (the matrix-core product): 1, 2 and
+    # 4 blocks of 16 columns per pass, two passes at 70, a ragged last tile
+    # of targets and of sources (1000 points is not a multiple of 128 or 32)
+    print(f"[apply] pde {pde} nrhs {nrhs}", flush=True)
+    b = G.star(1000) if pde != K.STOKES else G.circle(600)
+    sd = 2 if pde == K.STOKES else 1
+    t = gpu_tree(ctx, b, sd, 32)
+    nd = sd * b.num_points
+    mu = np.random.default_rng(11 + nrhs).standard_normal((nd, nrhs))
+    Kd = K.Kernel(sd, pde, b)(np.arange(nd), np.arange(nd))
+    y = t.apply(mu, pde)
+    ref = Kd @ mu
+    assert y.shape == ref.shape
+    assert np.allclose(y, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+    # the one-density kernels give the same columns
+    y1 = t.apply(np.ascontiguousarray(mu[:, :1]), pde)
+    assert np.allclose(y[:, :1], y1, rtol=0, atol=1e-13 * np.abs(y1).max())
+    t.close()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("pde", [K.LAPLACE, K.STOKES])
 def test_apply_matches_dense(ctx, pde):
     print(f"[apply] pde {pde}", flush=True)
