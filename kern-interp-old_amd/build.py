@@ -26,9 +26,15 @@ FLAGS = [
 ]
 
 
+# per-file additions: solve.hip's matrix-core apply keeps its loop-carried
+# accumulators in the VGPR form of the MFMA instructions (the default
+# heuristic put them in AGPRs and copied them back and forth every step)
+FILE_FLAGS = {"solve.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *FLAGS]
+    cmd = [HIPCC, *FLAGS, *FILE_FLAGS.get(src, [])]
     if src.endswith(".cpp"):
         cmd += ["-x", "hip"]
     cmd += ["-c", os.path.join(CSRC, src), "-o", obj]

@@ -506,7 +506,9 @@ struct SymPair {
 __global__ __launch_bounds__(kSymTB) void k_apply_sym(const SymPair* pairs, GeomView g,
                                                       const double* mu, double* part, int ldp) {
   __shared__ double2 s_xy[kSymT], s_a[kSymT];
-  __shared__ double s_acc[kSymNW][kSymT];
+  // after the pair loop s_xy holds the waves' source sums, kSymNW rows of kSymT
+  static_assert(sizeof(double2) >= sizeof(double) * kSymNW, "source sums fit in s_xy");
+  double* s_acc = reinterpret_cast<double*>(s_xy);
   const SymPair pr = pairs[blockIdx.x];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
   for (int j = tid; j < kSymT; j += kSymTB) {
@@ -565,7 +567,13 @@ __global__ __launch_bounds__(kSymTB) void k_apply_sym(const SymPair* pairs, Geom
   }
   const int ns = (pr.sn + kWave - 1) / kWave * kWave;
   const int next = (lane + 1) & (kWave - 1);
-  for (int base = 0; base < ns; base += kWave) {
+  // source sums stay in registers until every wave is done with s_xy
+  double jsv[kSymT / kWave];
+#pragma unroll
+  for (int bb = 0; bb < kSymT / kWave; ++bb) {
+    const int base = bb * kWave;
+    jsv[bb] = 0;
+    if (base >= ns) continue;
     const double2* cxy = s_xy + base;
     const double2* ca = s_a + base;
     // js: running sum of the source this lane meets in the current step; the
@@ -603,7 +611,7 @@ __global__ __launch_bounds__(kSymTB) void k_apply_sym(const SymPair* pairs, Geom
       js = __shfl(js, next) + cs;  // all lanes hold 0 before the first step
     }
     // after the last step this lane holds the whole wave's sum of source lane - 1
-    s_acc[wv][base + ((lane + kWave - 1) & (kWave - 1))] = js;
+    jsv[bb] = js;
   }
 #pragma unroll
   for (int p = 0; p < kSymP; ++p) {
@@ -611,10 +619,14 @@ __global__ __launch_bounds__(kSymTB) void k_apply_sym(const SymPair* pairs, Geom
     if (i < pr.tn) part[(size_t)pr.slot_t * ldp + pr.t0 + i] = ta[p];
   }
   __syncthreads();
+#pragma unroll
+  for (int bb = 0; bb < kSymT / kWave; ++bb)
+    s_acc[wv * kSymT + bb * kWave + ((lane + kWave - 1) & (kWave - 1))] = jsv[bb];
+  __syncthreads();
   for (int j = tid; j < pr.sn; j += kSymTB) {
     double s = 0;
 #pragma unroll
-    for (int w = 0; w < kSymNW; ++w) s += s_acc[w][j];
+    for (int w = 0; w < kSymNW; ++w) s += s_acc[w * kSymT + j];
     part[(size_t)pr.slot_s * ldp + pr.s0 + j] = -s;
   }
 }
