@@ -170,6 +170,32 @@ def bench_batch(torch, kigpu, W, ctx, steps, warmup):
             "refine": {"iterations": info.iterations, "rel_residual": info.rel_residual}}
 
 
+def bench_multi_rhs(torch, kigpu, W, ctx, args, counts=(16, 128)):
+    """The same star workload with several right-hand sides (seeded point
+    sources at radius 2-4 besides the one at (3, 3)): both paths, 1 warm-up
+    and 2 timed steps each.  Interior error is that of the first column."""
+    import argparse
+    pts, nrm, w, kap = W.star(args.n)
+    out = {}
+    for nrhs in counts:
+        srcs = [(3.0, 3.0)] + [tuple(s) for s in W.point_sources(nrhs - 1, 7)]
+        F = np.stack([W.point_source_bc(pts, s) for s in srcs[:nrhs]], 1)
+        prob = DeviceProblem(torch, (pts, nrm, w, kap), F)
+        sub = argparse.Namespace(**{**vars(args), "steps": 2, "warmup": 1})
+        leg = {}
+        for name, prec in (("fp64", kigpu.FP64), ("fp32_refine", kigpu.FP32_REFINE)):
+            dt, tims, info, h, mu = time_path(torch, kigpu, ctx, prob, prec, sub)
+            err = interior_error(kigpu, ctx, h, mu)
+            ctx.lib.kigpu_tree_destroy(h)
+            tm = {k: float(np.median([t[k] for t in tims])) for k in tims[0]}
+            leg[name] = {"s_per_step": dt, "rhs_dof_per_s": args.n * nrhs / dt,
+                         "interior_rel_err_col0": err, "refine": info,
+                         "t_solve": tm["t_solve"], "t_refine_apply": tm["t_refine_apply"]}
+        out[str(nrhs)] = leg
+        del prob
+    return out
+
+
 def bench_adjoint(kigpu, ctx, evals):
     """Shape gradient of L = sum u(x_j)^2 (1024 targets) in 16 Fourier
     coefficients of a star with N = 32768: factor, solve, transposed solve,
@@ -296,6 +322,7 @@ def main():
         "id_kernel": id_kernel,
     }
     if not args.no_extra:
+        out["multi_rhs"] = bench_multi_rhs(torch, kigpu, W, ctx, args)
         out["batch"] = bench_batch(torch, kigpu, W, ctx, 2, 1)
         out["adjoint"] = bench_adjoint(kigpu, ctx, 2)
     if not args.no_cpu:

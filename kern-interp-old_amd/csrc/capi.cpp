@@ -631,6 +631,29 @@ kigpu_status kigpu_kernel_apply_device(kigpu_ctx* ctx, const kigpu_tree* tree, i
   return guard([&] { kernel_apply(ctx->c, tree->t, pde, d_mu, nrhs, d_y); });
 }
 
+kigpu_status kigpu_kernel_apply_treecode(kigpu_ctx* ctx, const kigpu_tree* tree,
+                                         const double* mu, double* y) {
+  KI_REQUIRE(ctx && tree && mu && y, "invalid argument");
+  KI_REQUIRE(tree->t.sd == 1 && treecode_usable(tree->t),
+             "the treecode apply takes the Laplace double layer (one DOF per node)");
+  return guard([&] {
+    Context& c = ctx->c;
+    const size_t len = (size_t)tree->t.ndof;
+    DBuf<double> dm, dy;
+    dm.upload(mu, len, c.stream);
+    dy.alloc(len);
+    treecode_apply(c, tree->t, dm.p, dy.p);
+    dy.download(y, len, c.stream);
+    stream_wait(c.stream);
+  });
+}
+
+kigpu_status kigpu_set_refine_apply(kigpu_ctx* ctx, int mode) {
+  KI_REQUIRE(ctx && (mode == 0 || mode == 1), "invalid argument");
+  ctx->c.refine_apply = mode;
+  return KIGPU_OK;
+}
+
 kigpu_status kigpu_kernel_apply(kigpu_ctx* ctx, const kigpu_tree* tree, int pde, const double* mu,
                                 int nrhs, double* y) {
   KI_REQUIRE(ctx && tree && mu && y && nrhs > 0, "invalid argument");

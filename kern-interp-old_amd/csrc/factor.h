@@ -41,6 +41,7 @@ struct Context {
   hipEvent_t ev[16];
   std::vector<hipEvent_t> evpool;  // per-level phase markers of a factorization
   kigpu_timings tim{};
+  int refine_apply = 0;     // refinement residual: 0 treecode where it applies, 1 direct sum
   DBuf<double> ring;        // proxy unit vectors cos | sin (host libm)
   DBuf<double> rsum;        // residual partial sums
   size_t lds_max = 65536;
@@ -67,6 +68,10 @@ void border_step(Context& c, TreeData& t, T* x, int ldx, int nrhs, const double*
 void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_x,
                 double* d_alpha, const kigpu_refine_opts* ro, kigpu_solve_info* info,
                 bool trans = false);
+// treecode.hip: K mu for the Laplace double layer, one density, by a
+// treecode (series truncation below 1e-14 relative); the refinement's apply
+bool treecode_usable(const TreeData& t);
+void treecode_apply(Context& c, const TreeData& t, const double* d_mu, double* d_y);
 void kernel_apply(Context& c, const TreeData& t, int pde, const double* d_mu, int nrhs,
                   double* d_y);
 void kernel_apply_transpose(Context& c, const TreeData& t, int pde, const double* d_lam, int nrhs,

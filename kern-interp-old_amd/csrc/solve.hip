@@ -380,14 +380,6 @@ __device__ __forceinline__ double recip(double d) {
   return fma(r, e, r);
 }
 
-// 1/d for d > 0 with one Newton step: relative error below 2.2e-15 over
-// d in [1e-20, 1e20] (measured on gfx950 against 1/d in long double on 4M
-// log-uniform samples: v_rcp_f64 alone 4.6e-8, one step 2.2e-15), which is
-// below the rounding of an N-term sum of the refinement residual.
-__device__ __forceinline__ double recip1(double d) {
-  const double r = __builtin_amdgcn_rcp(d);
-  return fma(r, fma(-d, r, 1.0), r);
-}
 
 // Laplace double layer, one right-hand side (the refinement residual and the
 // interior evaluation).  Per source s the LDS tile holds (x, y), the scaled
@@ -1238,6 +1230,8 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
     KI_FAIL(KIGPU_ERR_NOTSUPP, "transposed solve of a bordered (multiply connected) system");
   c.tim.apply_launches = 0;
   c.tim.apply_kernel_s = c.tim.apply_pairs = 0;
+  c.tim.tc_launches = 0;
+  c.tim.tc_kernel_s = c.tim.tc_eval_s = c.tim.tc_series_pairs = c.tim.tc_direct_pairs = 0;
   const size_t len = (size_t)t.ndof * nrhs;
   const int h = t.nborder;
   const size_t alen = (size_t)h * nrhs;
@@ -1280,6 +1274,11 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
   c.tim.t_solve = ms * 1e-3;
   c.tim.t_refine_apply = 0;
   const int iters = ro ? ro->max_iters : 0;
+  // the residual apply: the treecode for the Laplace double layer with one
+  // density and no coincident nodes (unless the context asks for the direct
+  // sum), else the direct kernels
+  const bool tc = iters > 0 && !trans && c.refine_apply == 0 && t.opts.pde == KIGPU_LAPLACE &&
+                  nrhs == 1 && treecode_usable(t) && !has_coincident_points(c, t);
   if (iters > 0) {
     t.r64.alloc(len);
     t.f64.alloc(len);
@@ -1288,6 +1287,7 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
     for (int it = 0; it <= iters; ++it) {
       KI_HIP(hipEventRecord(c.ev[8], s));
       if (trans) kernel_apply_transpose(c, t, t.opts.pde, d_x, nrhs, t.f64.p);
+      else if (tc) treecode_apply(c, t, d_x, t.f64.p);
       else kernel_apply(c, t, t.opts.pde, d_x, nrhs, t.f64.p);
       if (h) {
         k_add_ualpha<<<dim3(ceil_div(t.ndof, 256), nrhs), 256, 0, s>>>(t.ndof, h, t.bU64in.p,
@@ -1327,6 +1327,13 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
   }
   if (h && d_alpha)
     KI_HIP(hipMemcpyAsync(d_alpha, t.alpha64.p, sizeof(double) * alen, hipMemcpyDeviceToDevice, s));
+  if (tc) {
+    const unsigned long long* hw = c.pin_down.get<unsigned long long>(2, s);
+    copy_to_host(c.pin_down, s, {{t.tc_work.p, 0, 2 * sizeof(unsigned long long)}});
+    stream_wait(s);
+    c.tim.tc_series_pairs = (double)hw[0];
+    c.tim.tc_direct_pairs = (double)hw[1];
+  }
   stream_wait(s);
 }
 

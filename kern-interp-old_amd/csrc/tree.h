@@ -55,6 +55,14 @@ struct TreeData {
   mutable DBuf<int> sym_pairs, sym_blocks;
   mutable DBuf<int64_t> d_dof_off;     // instance DOF offsets (residual norms)
   mutable int sym_npairs = 0, sym_nblocks = 0, sym_slots = 0;
+  // treecode apply of the refinement residual (treecode.hip): box centres and
+  // series (per apply), interaction lists of every leaf (once per tree),
+  // pair counts of one apply
+  mutable bool tc_ready = false;
+  mutable DBuf<double4> tc_ctr;
+  mutable DBuf<double2> tc_coef;
+  mutable DBuf<int> tc_far, tc_near, tc_nfar, tc_nnear;
+  mutable DBuf<unsigned long long> tc_work;
 
   // ---- factorization ----
   bool factored = false;

@@ -47,7 +47,9 @@ class Timings(C.Structure):
         ("syncs", C.c_int), ("id_launches", C.c_int), ("id_kernel_s", C.c_double),
         ("id_flops", C.c_double), ("dec_launches", C.c_int), ("dec_kernel_s", C.c_double),
         ("dec_flops", C.c_double), ("apply_launches", C.c_int), ("apply_kernel_s", C.c_double),
-        ("apply_pairs", C.c_double)]
+        ("apply_pairs", C.c_double), ("tc_launches", C.c_int), ("tc_kernel_s", C.c_double),
+        ("tc_eval_s", C.c_double), ("tc_series_pairs", C.c_double),
+        ("tc_direct_pairs", C.c_double)]
 
 
 # every symbol include/kigpu.h declares
@@ -62,6 +64,7 @@ EXPORTS = [
     "kigpu_solve_transpose", "kigpu_solve_transpose_device", "kigpu_kernel_apply_transpose",
     "kigpu_forward_apply_transpose", "kigpu_kernel_sensitivity", "kigpu_forward_sensitivity",
     "kigpu_tree_build_batch", "kigpu_tree_build_batch_device", "kigpu_tree_instances",
+    "kigpu_kernel_apply_treecode", "kigpu_set_refine_apply",
 ]
 
 _lib = None
@@ -107,6 +110,8 @@ def load() -> C.CDLL:
         "kigpu_solve_device": (C.c_int, [P, P, P, C.c_int, P, P, C.POINTER(RefineOpts),
                                          C.POINTER(SolveInfo)]),
         "kigpu_kernel_apply": (C.c_int, [P, P, C.c_int, _f64p, C.c_int, _f64p]),
+        "kigpu_kernel_apply_treecode": (C.c_int, [P, P, _f64p, _f64p]),
+        "kigpu_set_refine_apply": (C.c_int, [P, C.c_int]),
         "kigpu_kernel_apply_device": (C.c_int, [P, P, C.c_int, P, C.c_int, P]),
         "kigpu_forward_apply": (C.c_int, [P, P, C.c_int, _f64p, C.c_int, _f64p, C.c_int, _f64p]),
         "kigpu_solve_transpose": (C.c_int, [P, P, _f64p, C.c_int, C.c_int, _f64p, C.c_int,
@@ -159,6 +164,10 @@ class Context:
         if self.h:
             self.lib.kigpu_context_destroy(self.h)
             self.h = None
+
+    def set_refine_apply(self, mode: int):
+        """Residual apply of the FP64 refinement: 0 treecode (default), 1 direct sum."""
+        _check(self.lib.kigpu_set_refine_apply(self.h, mode), "kigpu_set_refine_apply")
 
     def __del__(self):
         try:
@@ -393,6 +402,15 @@ class Tree:
                                                   _f64(v), _f64(mu), out),
                "kigpu_forward_sensitivity")
         return out.reshape(-1, 6)
+
+    def apply_treecode(self, mu):
+        """K mu by the treecode (Laplace double layer, one density): the
+        refinement's residual apply."""
+        mu = _f64(mu)
+        y = np.zeros(mu.size)
+        _check(self.lib.kigpu_kernel_apply_treecode(self.ctx.h, self.h, mu, y),
+               "kigpu_kernel_apply_treecode")
+        return y
 
     def apply(self, mu, pde=None):
         pde = self.pde if pde is None else pde
