@@ -216,11 +216,11 @@ def bench_adjoint(kigpu, ctx, evals):
             "note": "host arrays in and out (includes PCIe copies)"}
 
 
-def apply_traffic():
-    """HBM bytes per launch of k_apply_sym from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
-    or None when the summary is absent."""
-    path = os.path.join(ROOT, "profiles", "pmc_apply_sym_laplace_star_N65536.md")
+def pmc_traffic(name):
+    """HBM bytes per launch of a kernel from its committed rocprofv3 --pmc
+    summary under profiles/ (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration), or None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as fh:
             for line in fh:
@@ -274,33 +274,46 @@ def main():
                          "t_tree", "t_lists", "t_id", "t_decouple", "t_root", "t_solve",
                          "t_refine_apply")}, "timings": tm}
 
-    # roofline of the kernel that dominates the FP32+refinement step: the FP64
-    # matrix-free apply of the refinement residual (k_apply_sym), timed with
-    # HIP events on the library's stream.  Algorithmic work: 16 flops per
-    # unordered pair of nodes {i, j}, i.e. 8 per (target, source) pair: r
-    # (2), |r|^2 (3), 1/|r|^2 (1), r . a_j and r . a_i (3 each), and the two
-    # multiply-adds into y_i and y_j (4).
+    # roofline of the kernel that dominates the FP32+refinement step: the ID
+    # kernel k_id<double, float> (ID matrix generation, pivoted QR to the rank
+    # step and T = R11^-1 R12, in FP64), one launch per tree level, timed with
+    # HIP events on the library's stream.  Algorithmic flops per box (m rows,
+    # n columns, rank k, generation cost c_K per entry): c_K m n + 2 m n +
+    # sum_{i<=k} [2 (m - i) + 4 (m - i)(n - i - 1)] + k^2 (n - k), summed over
+    # the level's boxes.
     tf = res["fp32_refine"]["timings"]
-    launches = max(tf["apply_launches"], 1)
-    per_launch_s = tf["apply_kernel_s"] / launches
-    per_launch_flops = 8.0 * tf["apply_pairs"] / launches
-    achieved = per_launch_flops / per_launch_s / 1e12 if per_launch_s > 0 else 0.0
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
-                "traffic": apply_traffic(),
-                "kernel": "k_apply_sym (FP64 Laplace double-layer apply by unordered node "
-                          "pairs, refinement residual; bound by the FP64 vector units)",
-                "avg_launch_us": per_launch_s * 1e6, "launches_per_step": tf["apply_launches"],
-                "flops_per_launch": per_launch_flops}
-    # the largest factorization kernel beside it (the ID, FP64 path)
-    tm = res["fp64"]["timings"]
-    id_s = tm["id_kernel_s"] / max(tm["id_launches"], 1)
-    id_f = tm["id_flops"] / max(tm["id_launches"], 1)
-    id_kernel = {"kernel": "k_id<double,double> (ID matrix + pivoted QR + T; LDS-resident and "
-                           "global-workspace boxes in one launch per level)",
-                 "achieved": id_f / id_s / 1e12 if id_s > 0 else 0.0, "peak": PEAK_FP64_TFLOPS,
-                 "unit": "TFLOP/s", "avg_launch_us": id_s * 1e6,
-                 "launches_per_factor": tm["id_launches"]}
+    id_s = tf["id_kernel_s"] / max(tf["id_launches"], 1)
+    id_f = tf["id_flops"] / max(tf["id_launches"], 1)
+    id_achieved = id_f / id_s / 1e12 if id_s > 0 else 0.0
+    roofline = {"bound": "mfma", "achieved": id_achieved, "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": id_achieved / PEAK_FP64_TFLOPS,
+                "traffic": pmc_traffic("pmc_id_laplace_star_N65536.md"),
+                "kernel": "k_id<double,float> (ID matrix + pivoted QR + T in FP64, one launch "
+                          "per level; each box is a chain of dependent Householder steps, "
+                          "bound by latency, measured against the FP64 peak)",
+                "avg_launch_us": id_s * 1e6, "launches_per_step": tf["id_launches"],
+                "flops_per_launch": id_f}
+    # the refinement's residual apply: the treecode (box series + direct near
+    # field).  Algorithmic flops: 8 (P - 1) + 15 per (target, series box) for
+    # the Horner evaluation of P = 48 terms, 10 per (target, source) pair
+    # summed directly.
+    tc_launches = max(tf["tc_launches"], 1)
+    tc_eval_s = tf["tc_eval_s"] / tc_launches
+    tc_flops = 391.0 * tf["tc_series_pairs"] + 10.0 * tf["tc_direct_pairs"]
+    refine_apply = {"kernel": "k_tc_coef + k_tc_eval (treecode, Laplace double layer)",
+                    "applies_per_step": tf["tc_launches"],
+                    "avg_apply_us": tf["tc_kernel_s"] / tc_launches * 1e6,
+                    "avg_eval_us": tc_eval_s * 1e6,
+                    "series_pairs": tf["tc_series_pairs"], "direct_pairs": tf["tc_direct_pairs"],
+                    "eval_achieved_tflops": tc_flops / tc_eval_s / 1e12 if tc_eval_s > 0 else 0.0,
+                    "direct_sum_equivalent_pairs": float(args.n) ** 2}
+    # the elimination kernel beside it (FP32 path)
+    dec_s = tf["dec_kernel_s"] / max(tf["dec_launches"], 1)
+    dec_f = tf["dec_flops"] / max(tf["dec_launches"], 1)
+    dec_kernel = {"kernel": "k_decouple<float> (elimination and Schur update per box)",
+                  "achieved": dec_f / dec_s / 1e12 if dec_s > 0 else 0.0,
+                  "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "avg_launch_us": dec_s * 1e6,
+                  "launches_per_step": tf["dec_launches"]}
 
     fast = res["fp32_refine"]
     out = {
@@ -319,7 +332,8 @@ def main():
                                              "refine", "phases_s")},
         "speedup_fp32_refine_vs_fp64": res["fp64"]["s_per_step"] / fast["s_per_step"],
         "roofline": roofline,
-        "id_kernel": id_kernel,
+        "refine_apply": refine_apply,
+        "dec_kernel": dec_kernel,
     }
     if not args.no_extra:
         out["multi_rhs"] = bench_multi_rhs(torch, kigpu, W, ctx, args)

@@ -63,6 +63,8 @@ struct TreeData {
   mutable DBuf<double2> tc_coef;
   mutable DBuf<int> tc_far, tc_near, tc_nfar, tc_nnear;
   mutable DBuf<unsigned long long> tc_work;
+  mutable DBuf<int4> tc_wlist;           // series work list (4 boxes or 1 large box per block)
+  mutable int tc_nwork = 0;
 
   // ---- factorization ----
   bool factored = false;

@@ -13,11 +13,12 @@
 // from c to the leaf's box); the first kTcP terms then leave a relative
 // truncation below theta^P / (1 - theta) = 7e-15.  Everything else is summed
 // directly, with the diagonal value 1/2 + kappa w / (4 pi) where r is exactly
-// zero, as in k_apply_sym.  Boxes of levels 0 and 1 are always opened.
+// zero, as in k_apply_sym.  Boxes of levels 0 to 2 are always opened.
 //
-// Three kernels: k_tc_coef (centres for the acceptance test, series of every
-// box of level >= 2 with at least kTcMinPts points; per apply), k_tc_plan
-// (interaction lists of every leaf; once per tree), k_tc_eval (per apply).
+// Kernels: k_tc_ctr (box centres and radii; once per tree), k_tc_coef
+// (series of every box of level >= 3 with at least kTcMinPts points; per
+// apply), k_tc_plan (interaction lists of every leaf; once per tree),
+// k_tc_eval (per apply).
 #include <algorithm>
 #include <cmath>
 
@@ -31,7 +32,7 @@ namespace {
 
 constexpr int kTcP = 48;              // series terms
 constexpr double kTcTheta2 = 0.25;    // acceptance: rho^2 <= theta^2 dist^2
-constexpr int kTcMinLevel = 2;        // boxes of coarser levels are always opened
+constexpr int kTcMinLevel = 3;        // boxes of coarser levels are always opened
 constexpr int kTcMinPts = 16;         // smaller accepted boxes are summed directly
 constexpr int kTcFar = 768;           // list capacities per leaf
 constexpr int kTcNear = 256;
@@ -40,72 +41,105 @@ constexpr int kTcTB = 256;            // evaluation: 4 waves per leaf
 constexpr int kTcNW = kTcTB / kWave;
 constexpr double kTcFar2 = 1e30;      // padding sources, as in k_apply_sym
 
-// Centre (x, y), radius and its inverse of every box of level >= kTcMinLevel,
-// and the series of those with at least kTcMinPts points.  One block per box;
-// each thread takes every 256th point, 16 powers per pass, wave sums then a
-// sum over the four waves in a fixed order.
-__global__ __launch_bounds__(256) void k_tc_coef(int base, const int64_t* pstart,
-                                                 const int* pcount, const int* perm,
-                                                 const double* corners, GeomView g,
-                                                 const double* mu, double4* ctr, double2* coef) {
-  __shared__ double red[4][32];
-  const int node = base + blockIdx.x;
-  const int cnt = pcount[node];
+// per-wave LDS handshake: the wave's lanes see each other's LDS writes
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Centre (x, y), radius and its inverse of every box (once per tree).
+__global__ void k_tc_ctr(int nn, const double* corners, double4* ctr) {
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= nn) return;
   const double* cc = corners + 8 * (size_t)node;
   const double hx = 0.5 * (cc[6] - cc[0]), hy = 0.5 * (cc[3] - cc[1]);
-  const double cx = cc[0] + hx, cy = cc[1] + hy;
   const double rho = sqrt(hx * hx + hy * hy);
-  const double irho = 1.0 / rho;
-  if (threadIdx.x == 0) ctr[node] = make_double4(cx, cy, rho, irho);
-  if (cnt < kTcMinPts) return;
+  ctr[node] = make_double4(cc[0] + hx, cc[1] + hy, rho, 1.0 / rho);
+}
+
+// Series of the boxes in the work list (boxes of level >= kTcMinLevel with at
+// least kTcMinPts points).  A block takes four boxes of at most kTcSmall
+// points, one per wave, or one larger box over its four waves (entry .y ==
+// -2).  Each lane sums its points' a_j zeta_j^k for all k in registers (one
+// pass over the points, loads kept in flight two points at a time); the 64
+// partial sums of each power go through LDS, lanes 4k .. 4k + 3 add a
+// quarter each and a quad exchange completes the sum; a large box adds its
+// four waves in wave order.  Fixed summation order throughout.
+constexpr int kTcSmall = 256;
+
+__global__ __launch_bounds__(kTcTB) void k_tc_coef(const int4* work, const int64_t* pstart,
+                                                   const int* pcount, const int* perm,
+                                                   GeomView g, const double* mu,
+                                                   const double4* ctr, double2* coef) {
+  __shared__ double2 part[kTcNW][16][kWave];
+  __shared__ double2 wsum[kTcNW][kTcP];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
-  const int64_t p0 = pstart[node];
-  for (int q = 0; q < kTcP / 16; ++q) {
-    double sr[16], si[16];
+  const int4 wk = work[blockIdx.x];
+  const bool big = wk.y == -2;
+  const int node = big ? wk.x : (wv == 0 ? wk.x : wv == 1 ? wk.y : wv == 2 ? wk.z : wk.w);
+  const int first = big ? tid : lane, stride = big ? kTcTB : kWave;
+  double sr[kTcP], si[kTcP];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) sr[k] = si[k] = 0.0;
-    for (int j = tid; j < cnt; j += 256) {
-      const int p = perm[p0 + j];
-      const double s = -g.w[p] * mu[p] * kInv2Pi;
-      double ar = s * g.nx[p], ai = s * g.ny[p];
-      const double zr = (g.px[p] - cx) * irho, zi = (g.py[p] - cy) * irho;
-      if (q > 0) {  // a zeta^(16 q)
-        double er = zr, ei = zi;
+  for (int k = 0; k < kTcP; ++k) sr[k] = si[k] = 0.0;
+  if (node >= 0) {
+    const int cnt = pcount[node];
+    const int64_t p0 = pstart[node];
+    const double4 q = ctr[node];
+    for (int j = first; j < cnt; j += 2 * stride) {
+      const int j2 = j + stride;
+      const int pa = perm[p0 + j], pb = perm[p0 + (j2 < cnt ? j2 : j)];
+      const double wa = g.w[pa], ma = mu[pa], xa = g.px[pa], ya = g.py[pa], na = g.nx[pa],
+                   nb_ = g.ny[pa];
+      const double wb = g.w[pb], mb = mu[pb], xb = g.px[pb], yb = g.py[pb], nxb = g.nx[pb],
+                   nyb = g.ny[pb];
+      for (int h = 0; h < 2; ++h) {
+        if (h == 1 && j2 >= cnt) break;
+        const double s = -(h ? wb * mb : wa * ma) * kInv2Pi;
+        double ar = s * (h ? nxb : na), ai = s * (h ? nyb : nb_);
+        const double zr = ((h ? xb : xa) - q.x) * q.w, zi = ((h ? yb : ya) - q.y) * q.w;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const double nr = er * er - ei * ei, ni = 2.0 * er * ei;
-          er = nr;
-          ei = ni;
-        }
-        for (int t = 0; t < q; ++t) {
-          const double nr = ar * er - ai * ei, ni = ar * ei + ai * er;
+        for (int k = 0; k < kTcP; ++k) {
+          sr[k] += ar;
+          si[k] += ai;
+          const double nr = ar * zr - ai * zi, ni = ar * zi + ai * zr;
           ar = nr;
           ai = ni;
         }
       }
+    }
+  }
+  // per wave: 64 partials of each power -> one sum, 16 powers at a time
 #pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        sr[k] += ar;
-        si[k] += ai;
-        const double nr = ar * zr - ai * zi, ni = ar * zi + ai * zr;
-        ar = nr;
-        ai = ni;
-      }
-    }
+  for (int c = 0; c < kTcP / 16; ++c) {
+    wave_sync();
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const double a = wave_sum(sr[k]), b = wave_sum(si[k]);
-      if (lane == 0) {
-        red[wv][2 * k] = a;
-        red[wv][2 * k + 1] = b;
-      }
+    for (int k = 0; k < 16; ++k) part[wv][k][lane] = make_double2(sr[16 * c + k], si[16 * c + k]);
+    wave_sync();
+    const int k = lane >> 2, r = lane & 3;
+    double tr = 0, ti = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const double2 v = part[wv][k][16 * r + i];
+      tr += v.x;
+      ti += v.y;
     }
-    __syncthreads();
-    if (tid < 32) {
-      const double s = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
-      reinterpret_cast<double*>(coef + (size_t)node * kTcP + 16 * q)[tid] = s;
+    tr += dpp::mov<dpp::kXor1>(tr);
+    ti += dpp::mov<dpp::kXor1>(ti);
+    tr += dpp::mov<dpp::kXor2>(tr);
+    ti += dpp::mov<dpp::kXor2>(ti);
+    if (r == 0) {
+      if (big) wsum[wv][16 * c + k] = make_double2(tr, ti);
+      else if (node >= 0) coef[(size_t)node * kTcP + 16 * c + k] = make_double2(tr, ti);
     }
+  }
+  if (big) {
     __syncthreads();
+    if (tid < kTcP) {
+      const double2 a = wsum[0][tid], b = wsum[1][tid], c = wsum[2][tid], d = wsum[3][tid];
+      coef[(size_t)node * kTcP + tid] = make_double2((a.x + b.x) + (c.x + d.x),
+                                                      (a.y + b.y) + (c.y + d.y));
+    }
   }
 }
 
@@ -191,57 +225,55 @@ __global__ __launch_bounds__(64) void k_tc_plan(const int* is_leaf, const int* p
   }
 }
 
-// per-wave LDS handshake: the wave's lanes see each other's LDS writes
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
+// near-field sources in LDS: all of a leaf's near boxes at once when their
+// points fit, else 64 per wave at a time
+constexpr int kTcNearPts = 512;
 struct TcSrc {
-  double x[kTcNW][kWave], y[kTcNW][kWave], ax[kTcNW][kWave], ay[kTcNW][kWave],
-      dg[kTcNW][kWave];
+  double x[kTcNearPts], y[kTcNearPts], ax[kTcNearPts], ay[kTcNearPts], dg[kTcNearPts];
 };
 
+// source point p into slot i of the LDS store
+__device__ __forceinline__ void tc_stage(TcSrc& S, int i, const GeomView& g, const double* mu,
+                                         int p) {
+  const double w = g.w[p], m = mu[p];
+  const double s = -w * m * kInv2Pi;
+  S.x[i] = g.px[p];
+  S.y[i] = g.py[p];
+  S.ax[i] = s * g.nx[p];
+  S.ay[i] = s * g.ny[p];
+  S.dg[i] = (0.5 + 0.5 * g.kappa[p] * w * kInv2Pi) * m;
+}
+
+// u += sum over the LDS sources [lo, hi) of K(target, j) mu_j
+__device__ __forceinline__ void tc_sum(const TcSrc& S, int lo, int hi, double zx, double zy,
+                                       double& u) {
+  for (int k = lo; k < hi; ++k) {
+    const double rx = zx - S.x[k], ry = zy - S.y[k];
+    const double rr = fma(rx, rx, ry * ry);
+    const double v = fma(rx, S.ax[k], ry * S.ay[k]) * recip1(rr);
+    u += rr == 0.0 ? S.dg[k] : v;
+  }
+}
+
 // u += sum over n sources (point index src(j)) of K(target, j) mu_j, the
-// sources staged 64 at a time in the wave's LDS rows
+// sources staged 64 at a time in the wave's LDS slots [64 wv, 64 wv + 64)
 template <typename F>
 __device__ __forceinline__ void tc_direct(TcSrc& S, int wv, int lane, const GeomView& g,
                                           const double* mu, double zx, double zy, int n, F src,
                                           double& u) {
+  const int o = kWave * wv;
   for (int j0 = 0; j0 < n; j0 += kWave) {
     const int j = j0 + lane;
     wave_sync();
-    if (j < n) {
-      const int p = src(j);
-      const double w = g.w[p], m = mu[p];
-      const double s = -w * m * kInv2Pi;
-      S.x[wv][lane] = g.px[p];
-      S.y[wv][lane] = g.py[p];
-      S.ax[wv][lane] = s * g.nx[p];
-      S.ay[wv][lane] = s * g.ny[p];
-      S.dg[wv][lane] = (0.5 + 0.5 * g.kappa[p] * w * kInv2Pi) * m;
-    } else {
-      S.x[wv][lane] = kTcFar2;
-      S.y[wv][lane] = kTcFar2;
-      S.ax[wv][lane] = 0.0;
-      S.ay[wv][lane] = 0.0;
-      S.dg[wv][lane] = 0.0;
-    }
+    if (j < n) tc_stage(S, o + lane, g, mu, src(j));
     wave_sync();
-    const int m = min(kWave, n - j0);
-    for (int k = 0; k < m; ++k) {
-      const double rx = zx - S.x[wv][k], ry = zy - S.y[wv][k];
-      const double rr = fma(rx, rx, ry * ry);
-      const double v = fma(rx, S.ax[wv][k], ry * S.ay[wv][k]) * recip1(rr);
-      u += rr == 0.0 ? S.dg[wv][k] : v;
-    }
+    tc_sum(S, o, o + min(kWave, n - j0), zx, zy, u);
   }
 }
 
 // y over the points of leaf B (one block of 4 waves, a target per lane):
-// wave w takes every 4th series box and every 4th direct box of B's lists;
-// the four partial sums are added in wave order.
+// wave w takes every 4th series box of B's list and a quarter of the points
+// of its near boxes; the four partial sums are added in wave order.
 __global__ __launch_bounds__(kTcTB) void k_tc_eval(const int* is_leaf, const int* pcount,
                                                    const int64_t* pstart, const int* perm,
                                                    const int* inst, const int64_t* inst_pt,
@@ -252,14 +284,45 @@ __global__ __launch_bounds__(kTcTB) void k_tc_eval(const int* is_leaf, const int
   __shared__ double2 s_cf[kTcNW][kTcP];
   __shared__ TcSrc S;
   __shared__ double s_u[kTcNW][kWave];
+  __shared__ int fl[kTcFar], nl[kTcNear];
   const int B = blockIdx.x;
   const int cntB = pcount[B];
   if (!is_leaf[B] || cntB == 0) return;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
   const int64_t p0B = pstart[B];
   const int nf = nfar[B], nn = nnear[B];
-  const int* fl = far_list + (size_t)B * kTcFar;
-  const int* nl = near_list + (size_t)B * kTcNear;
+  __shared__ int s_off[kWave + 1];
+  for (int e = tid; e < nf; e += kTcTB) fl[e] = far_list[(size_t)B * kTcFar + e];
+  for (int e = tid; e < nn; e += kTcTB) nl[e] = near_list[(size_t)B * kTcNear + e];
+  __syncthreads();
+  // the near boxes' points, staged once for every target of B when at most
+  // 64 boxes with at most kTcNearPts points in all
+  int nt = -1;
+  if (nf >= 0 && nn <= kWave) {
+    if (wv == 0) {
+      int inc = lane < nn ? pcount[nl[lane]] : 0;
+      for (int o = 1; o < kWave; o <<= 1) {
+        const int t = __shfl_up(inc, o);
+        if (lane >= o) inc += t;
+      }
+      s_off[lane + 1] = inc;
+      if (lane == 0) s_off[0] = 0;
+    }
+    __syncthreads();
+    if (s_off[nn] <= kTcNearPts) {
+      nt = s_off[nn];
+      for (int idx = tid; idx < nt; idx += kTcTB) {
+        int lo = 0, hi = nn;  // s_off[lo] <= idx < s_off[hi]
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (s_off[mid] <= idx) lo = mid;
+          else hi = mid;
+        }
+        tc_stage(S, idx, g, mu, perm[pstart[nl[lo]] + (idx - s_off[lo])]);
+      }
+      __syncthreads();
+    }
+  }
   for (int t0 = 0; t0 < cntB; t0 += kWave) {
     const int ti = t0 + lane;
     const bool tv = ti < cntB;
@@ -267,29 +330,57 @@ __global__ __launch_bounds__(kTcTB) void k_tc_eval(const int* is_leaf, const int
     const double zx = g.px[tp], zy = g.py[tp];
     double u = 0;
     if (nf >= 0) {
+      // the next box's centre and series are loaded while this one is summed
+      double4 qn = make_double4(0.0, 0.0, 0.0, 0.0);
+      double2 cn = make_double2(0.0, 0.0);
+      if (wv < nf) {
+        qn = ctr[fl[wv]];
+        if (lane < kTcP) cn = coef[(size_t)fl[wv] * kTcP + lane];
+      }
       for (int e = wv; e < nf; e += kTcNW) {
-        const int C = fl[e];
-        const double4 q = ctr[C];
+        const double4 q = qn;
         wave_sync();
-        if (lane < kTcP) s_cf[wv][lane] = coef[(size_t)C * kTcP + lane];
+        if (lane < kTcP) s_cf[wv][lane] = cn;
         wave_sync();
+        if (e + kTcNW < nf) {
+          const int C2 = fl[e + kTcNW];
+          qn = ctr[C2];
+          if (lane < kTcP) cn = coef[(size_t)C2 * kTcP + lane];
+        }
         const double dx = zx - q.x, dy = zy - q.y;
         const double inv = recip1(fma(dx, dx, dy * dy));
         const double wr = q.z * dx * inv, wi = -q.z * dy * inv;
-        double sr = s_cf[wv][kTcP - 1].x, si = s_cf[wv][kTcP - 1].y;
-        for (int k = kTcP - 2; k >= 0; --k) {
-          const double2 m = s_cf[wv][k];
-          const double nr = fma(sr, wr, fma(-si, wi, m.x));
-          const double ni = fma(sr, wi, fma(si, wr, m.y));
-          sr = nr;
-          si = ni;
+        // sum_k M_k w^k = E(w^2) + w O(w^2): two Horner chains of half the
+        // length over the even and the odd terms
+        const double vr = wr * wr - wi * wi, vi = 2.0 * wr * wi;
+        double er = s_cf[wv][kTcP - 2].x, ei = s_cf[wv][kTcP - 2].y;
+        double orr = s_cf[wv][kTcP - 1].x, oi = s_cf[wv][kTcP - 1].y;
+#pragma unroll 4
+        for (int k = kTcP - 4; k >= 0; k -= 2) {
+          const double2 me = s_cf[wv][k], mo = s_cf[wv][k + 1];
+          const double ner = fma(er, vr, fma(-ei, vi, me.x));
+          const double nei = fma(er, vi, fma(ei, vr, me.y));
+          const double nor = fma(orr, vr, fma(-oi, vi, mo.x));
+          const double noi = fma(orr, vi, fma(oi, vr, mo.y));
+          er = ner;
+          ei = nei;
+          orr = nor;
+          oi = noi;
         }
+        const double sr = er + (orr * wr - oi * wi), si = ei + (orr * wi + oi * wr);
         u = fma(sr * wr - si * wi, q.w, u);
       }
-      for (int e = wv; e < nn; e += kTcNW) {
-        const int D = nl[e];
-        const int64_t pd = pstart[D];
-        tc_direct(S, wv, lane, g, mu, zx, zy, pcount[D], [&](int j) { return perm[pd + j]; }, u);
+      if (nt >= 0) {  // staged: wave w sums a quarter of the points
+        const int per = (nt + kTcNW - 1) / kTcNW;
+        const int lo = min(nt, wv * per), hi = min(nt, lo + per);
+        tc_sum(S, lo, hi, zx, zy, u);
+      } else {
+        for (int e = wv; e < nn; e += kTcNW) {
+          const int D = nl[e];
+          const int64_t pd = pstart[D];
+          tc_direct(S, wv, lane, g, mu, zx, zy, pcount[D], [&](int j) { return perm[pd + j]; },
+                    u);
+        }
       }
     } else {  // lists overflowed: the whole instance, directly
       const int b = inst[B];
@@ -324,11 +415,36 @@ void treecode_apply(Context& c, const TreeData& t, const double* d_mu, double* d
     t.tc_nnear.alloc(nn);
     t.tc_work.alloc(2);
     t.tc_work.zero(s);
+    k_tc_ctr<<<ceil_div(nn, 256), 256, 0, s>>>(nn, t.corners.p, t.tc_ctr.p);
+    // series work list: boxes of at most kTcSmall points four to a block,
+    // larger ones a block each (the node's point counts are mirrored on the
+    // host by the tree build)
+    std::vector<int4> wl;
+    int4 cur = make_int4(-1, -1, -1, -1);
+    int fill = 0;
+    for (int x = base; x < nn; ++x) {
+      const int cnt = t.h_pcount[x];
+      if (cnt < kTcMinPts) continue;
+      if (cnt > kTcSmall) {
+        wl.push_back(make_int4(x, -2, -1, -1));
+        continue;
+      }
+      int* slot = fill == 0 ? &cur.x : fill == 1 ? &cur.y : fill == 2 ? &cur.z : &cur.w;
+      *slot = x;
+      if (++fill == 4) {
+        wl.push_back(cur);
+        cur = make_int4(-1, -1, -1, -1);
+        fill = 0;
+      }
+    }
+    if (fill > 0) wl.push_back(cur);
+    t.tc_nwork = (int)wl.size();
+    t.tc_wlist.upload(wl, s);
   }
   KI_HIP(hipEventRecord(c.ev[12], s));
-  if (nn > base)
-    k_tc_coef<<<nn - base, 256, 0, s>>>(base, t.pstart.p, t.pcount.p, t.perm_all.p,
-                                        t.corners.p, g, d_mu, t.tc_ctr.p, t.tc_coef.p);
+  if (t.tc_nwork > 0)
+    k_tc_coef<<<t.tc_nwork, kTcTB, 0, s>>>(t.tc_wlist.p, t.pstart.p, t.pcount.p, t.perm_all.p, g,
+                                          d_mu, t.tc_ctr.p, t.tc_coef.p);
   if (plan) {
     k_tc_plan<<<nn, kWave, 0, s>>>(t.is_leaf.p, t.pcount.p, t.level.p, t.child.p, t.inst.p,
                                    t.corners.p, t.tc_ctr.p, t.tc_far.p, t.tc_near.p,
