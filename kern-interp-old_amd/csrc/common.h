@@ -191,7 +191,9 @@ struct GeomView {
 
 // device-side scan helpers (scan.hip)
 void exclusive_scan_i64(const int64_t* in, int64_t* out, int n, hipStream_t s);
-void exclusive_scan_i32_to_i64(const int* in, int64_t* out, int n, hipStream_t s);
+// out[i] = add + in[0] + ... + in[i - 1]
+void exclusive_scan_i32_to_i64(const int* in, int64_t* out, int n, hipStream_t s,
+                               int64_t add = 0);
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 

@@ -72,6 +72,8 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
 // treecode (series truncation below 1e-14 relative); the refinement's apply
 bool treecode_usable(const TreeData& t);
 void treecode_apply(Context& c, const TreeData& t, const double* d_mu, double* d_y);
+// the tree's treecode lists and box centres, on stream st (once per tree)
+void treecode_prepare(Context& c, const TreeData& t, hipStream_t st);
 void kernel_apply(Context& c, const TreeData& t, int pde, const double* d_mu, int nrhs,
                   double* d_y);
 void kernel_apply_transpose(Context& c, const TreeData& t, int pde, const double* d_lam, int nrhs,

@@ -1041,11 +1041,6 @@ __global__ __launch_bounds__(1024) void k_root(GeomView g, int pde, const int* c
   for (int i = tid; i < n; i += nt) piv_out[i] = piv[i];
 }
 
-__global__ void k_add_i64(int64_t* p, int n, int64_t add) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] += add;
-}
-
 __global__ void k_set_ptr(uint64_t* dst, int a, int b, const int* nact, uint64_t base,
                           const int64_t* offs, size_t tsz) {
   int x = a + blockIdx.x * blockDim.x + threadIdx.x;
@@ -1126,8 +1121,7 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     // ---- active lists (remove_inactive_dofs_at_level, quadtree.cpp:664-691)
     k_active_size<<<ceil_div(cn, kTB), kTB, 0, s>>>(a, b, t.is_leaf.p, t.child.p, t.pcount.p,
                                                    t.osize.p, t.sd, t.nact.p, t.rank.p);
-    exclusive_scan_i32_to_i64(t.nact.p + a, t.act_off.p + a, cn, s);
-    k_add_i64<<<ceil_div(cn, kTB), kTB, 0, s>>>(t.act_off.p + a, cn, pool_base);
+    exclusive_scan_i32_to_i64(t.nact.p + a, t.act_off.p + a, cn, s, pool_base);
     k_active_fill<<<cn, 64, 0, s>>>(a, t.is_leaf.p, t.child.p, t.obox_off.p, t.obox_pool.p,
                                     t.osize.p, t.rank.p, t.act_off.p, t.nact.p, t.act_pool.p,
                                     t.sr_pool.p);

@@ -10,10 +10,11 @@ namespace {
 constexpr int kScanTB = 1024;
 
 template <typename TIn>
-__global__ __launch_bounds__(kScanTB) void k_scan(const TIn* in, int64_t* out, int n) {
+__global__ __launch_bounds__(kScanTB) void k_scan(const TIn* in, int64_t* out, int n,
+                                                  int64_t add) {
   __shared__ int64_t buf[kScanTB];
   __shared__ int64_t carry;
-  if (threadIdx.x == 0) carry = 0;
+  if (threadIdx.x == 0) carry = add;
   __syncthreads();
   for (int base = 0; base < n; base += kScanTB) {
     int i = base + threadIdx.x;
@@ -37,12 +38,12 @@ __global__ __launch_bounds__(kScanTB) void k_scan(const TIn* in, int64_t* out, i
 
 void exclusive_scan_i64(const int64_t* in, int64_t* out, int n, hipStream_t s) {
   if (n <= 0) return;
-  k_scan<int64_t><<<1, kScanTB, 0, s>>>(in, out, n);
+  k_scan<int64_t><<<1, kScanTB, 0, s>>>(in, out, n, 0);
 }
 
-void exclusive_scan_i32_to_i64(const int* in, int64_t* out, int n, hipStream_t s) {
+void exclusive_scan_i32_to_i64(const int* in, int64_t* out, int n, hipStream_t s, int64_t add) {
   if (n <= 0) return;
-  k_scan<int><<<1, kScanTB, 0, s>>>(in, out, n);
+  k_scan<int><<<1, kScanTB, 0, s>>>(in, out, n, add);
 }
 
 namespace {

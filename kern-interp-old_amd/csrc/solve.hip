@@ -1257,6 +1257,15 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
   }
   // FP32 factor: solve in FP32, then FP64 refinement
   //   r = [f - K mu - U alpha; alpha - Psi mu],  [mu; alpha] += F^{-1} r
+  const int iters = ro ? ro->max_iters : 0;
+  // the residual apply: the treecode for the Laplace double layer with one
+  // density and no coincident nodes (unless the context asks for the direct
+  // sum), else the direct kernels
+  const bool tc = iters > 0 && !trans && c.refine_apply == 0 && t.opts.pde == KIGPU_LAPLACE &&
+                  nrhs == 1 && treecode_usable(t) && !has_coincident_points(c, t);
+  // the treecode's lists are built on the side stream while the first FP32
+  // solve runs (the first apply waits for them)
+  if (tc) treecode_prepare(c, t, c.side);
   t.x32.alloc(len);
   const int g = ceil_div((long)len, 256);
   k_convert<double, float><<<g, 256, 0, s>>>(d_b, t.x32.p, len);
@@ -1273,12 +1282,6 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
   KI_HIP(hipEventElapsedTime(&ms, c.ev[6], c.ev[7]));
   c.tim.t_solve = ms * 1e-3;
   c.tim.t_refine_apply = 0;
-  const int iters = ro ? ro->max_iters : 0;
-  // the residual apply: the treecode for the Laplace double layer with one
-  // density and no coincident nodes (unless the context asks for the direct
-  // sum), else the direct kernels
-  const bool tc = iters > 0 && !trans && c.refine_apply == 0 && t.opts.pde == KIGPU_LAPLACE &&
-                  nrhs == 1 && treecode_usable(t) && !has_coincident_points(c, t);
   if (iters > 0) {
     t.r64.alloc(len);
     t.f64.alloc(len);

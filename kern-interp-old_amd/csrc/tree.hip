@@ -23,6 +23,7 @@
 
 #include <climits>
 
+#include "device_util.h"
 #include "tree.h"
 
 namespace kigpu {
@@ -151,14 +152,33 @@ __global__ void k_split_info(int c, const int* pcount, const int64_t* pstart, co
 // number of splitting nodes.
 constexpr int kSortMax = 4096;
 
-__global__ __launch_bounds__(1024) void k_rank_sort(const int64_t* seg, const int* split,
-                                                    const int* st, const uint64_t* path,
-                                                    int* rank, int* seg_cnt) {
+// The block first computes its segment's split flags and split times (the
+// work of k_split_info); with a single instance (one block) it also does the
+// segment scan of k_seg_scan at the end.
+__global__ __launch_bounds__(1024) void k_rank_sort(const int64_t* seg, const int* pcount,
+                                                    const int64_t* pstart, const int* perm,
+                                                    const int* parent_local,
+                                                    const int* parent_st, int sd, int leaf,
+                                                    int* split, int* st, const uint64_t* path,
+                                                    int* rank, int* seg_cnt, int* seg_split,
+                                                    int64_t* next_seg, int* xfer) {
   __shared__ uint64_t key[kSortMax];
   __shared__ int val[kSortMax];
   __shared__ int s_cnt;
   const int base = (int)seg[blockIdx.x];
   const int c = (int)seg[blockIdx.x + 1] - base;
+  for (int i = threadIdx.x; i < c; i += 1024) {
+    const int x = base + i;
+    const int sp = (long)pcount[x] * sd > leaf;
+    int t = -1;
+    if (sp) {
+      t = perm[pstart[x] + leaf / sd];
+      if (parent_st) t = max(t, parent_st[parent_local[x]]);
+    }
+    split[x] = sp;
+    st[x] = t;
+  }
+  __syncthreads();
   int P = 1;
   while (P < c) P <<= 1;
   if (threadIdx.x == 0) s_cnt = 0;
@@ -201,7 +221,18 @@ __global__ __launch_bounds__(1024) void k_rank_sort(const int64_t* seg, const in
   }
   if (cnt) atomicAdd(&s_cnt, cnt);
   __syncthreads();
-  if (threadIdx.x == 0) seg_cnt[blockIdx.x] = s_cnt;
+  if (threadIdx.x != 0) return;
+  if (gridDim.x > 1) {
+    seg_cnt[blockIdx.x] = s_cnt;
+    return;
+  }
+  const int run = s_cnt;  // k_seg_scan for one instance
+  seg_split[0] = 0;
+  xfer[0] = 0;
+  next_seg[0] = 0;
+  seg_split[1] = run;
+  xfer[1] = run;
+  next_seg[1] = 4 * (int64_t)run;
 }
 
 // the same by counting, one thread per node, for segments the sort cannot take
@@ -252,13 +283,13 @@ __global__ void k_rank_offset(int c, const int* inst, const int* seg_split, int*
   if (i < c && rank[i] >= 0) rank[i] += seg_split[inst[i]];
 }
 
-// records of the 4 children of every splitting node (node_subdivide, quadtree.cpp:203-253)
-__global__ void k_children(int c, const int* rank, const double* corners, const double* side,
-                           const uint64_t* path, const int* inst, int* n_parent, int* n_slot,
-                           uint64_t* n_path, double* n_corners, double* n_side, int* n_inst) {
+// records of the 4 children of splitting node i (node_subdivide, quadtree.cpp:203-253)
+__device__ __forceinline__ void child_records(int i, const int* rank, const double* corners,
+                                              const double* side, const uint64_t* path,
+                                              const int* inst, int* n_parent, int* n_slot,
+                                              uint64_t* n_path, double* n_corners,
+                                              double* n_side, int* n_inst) {
 #pragma clang fp contract(off)
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= c || rank[i] < 0) return;
   const double* cc = corners + 8 * i;
   double midx, midy;
   node_mid(cc, &midx, &midy);
@@ -296,31 +327,30 @@ __device__ __forceinline__ Q4 q4_add(const Q4& a, const Q4& b) {
   return r;
 }
 
-// quadrant of each point position of the level (-1: its node does not split)
-__global__ void k_classify(int np, const int* perm, const int* pnode, const int* rank,
-                           const double* corners, const double* px, const double* py, int* qcode) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np) return;
-  int nd = pnode[p];
-  int q = -1;
-  if (rank[nd] >= 0) {
-    double midx, midy;
-    node_mid(corners + 8 * nd, &midx, &midy);
-    int pt = perm[p];
-    q = quadrant(px[pt], py[pt], midx, midy);
-  }
-  qcode[p] = q;
-}
-
 constexpr int kScanTB = 1024;
 
-// block-level inclusive scan of the one-hot quadrant flags; block totals out
-__global__ __launch_bounds__(kScanTB) void k_q4_block(int np, const int* qcode, Q4* pre,
-                                                      Q4* btot) {
+// quadrant of each point position (-1: its node does not split; to qcode),
+// then the block-level inclusive scan of the one-hot quadrant flags; block
+// totals out
+__global__ __launch_bounds__(kScanTB) void k_q4_block(int np, const int* perm, const int* pnode,
+                                                      const int* rank, const double* corners,
+                                                      const double* px, const double* py,
+                                                      int* qcode, Q4* pre, Q4* btot) {
   __shared__ Q4 buf[kScanTB];
   int p = blockIdx.x * kScanTB + threadIdx.x;
   Q4 v = {{0, 0, 0, 0}};
-  if (p < np && qcode[p] >= 0) v.v[qcode[p]] = 1;
+  if (p < np) {
+    const int nd = pnode[p];
+    int q = -1;
+    if (rank[nd] >= 0) {
+      double midx, midy;
+      node_mid(corners + 8 * nd, &midx, &midy);
+      const int pt = perm[p];
+      q = quadrant(px[pt], py[pt], midx, midy);
+    }
+    qcode[p] = q;
+    if (q >= 0) v.v[q] = 1;
+  }
   buf[threadIdx.x] = v;
   __syncthreads();
   for (int off = 1; off < kScanTB; off <<= 1) {
@@ -338,29 +368,59 @@ __global__ __launch_bounds__(kScanTB) void k_q4_block(int np, const int* qcode, 
   if (threadIdx.x == kScanTB - 1) btot[blockIdx.x] = buf[kScanTB - 1];
 }
 
-// exclusive scan of the block totals (one thread; few blocks); total at [nb]
-__global__ void k_q4_top(int nb, Q4* btot, Q4* total) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  Q4 run = {{0, 0, 0, 0}};
-  for (int b = 0; b < nb; ++b) {
-    Q4 t = btot[b];
-    btot[b] = run;
-    run = q4_add(run, t);
+// pre += the exclusive prefix of the block totals (each block of kTB = 256
+// positions lies in one scan block b and sums btot[0 .. b-1] itself, in wave
+// order); the block holding position np also writes the level total
+__global__ void k_q4_apply(int np, int nb, Q4* pre, const Q4* btot, Q4* total) {
+  __shared__ int s_add[8];
+  const int p0 = blockIdx.x * blockDim.x;
+  const int p = p0 + threadIdx.x;
+  const int b = p0 / kScanTB;
+  const bool last = np >= p0 && np < p0 + (int)blockDim.x;
+  if (threadIdx.x < kWave) {
+    int a[4] = {0, 0, 0, 0}, t[4] = {0, 0, 0, 0};
+    for (int i = threadIdx.x; i < nb; i += kWave) {
+      const Q4 v = btot[i];
+      for (int q = 0; q < 4; ++q) {
+        if (i < b) a[q] += v.v[q];
+        t[q] += v.v[q];
+      }
+    }
+    for (int q = 0; q < 4; ++q) {
+      a[q] = wave_sum(a[q]);
+      t[q] = wave_sum(t[q]);
+    }
+    if (threadIdx.x == 0)
+      for (int q = 0; q < 4; ++q) {
+        s_add[q] = a[q];
+        s_add[4 + q] = t[q];
+      }
   }
-  *total = run;
+  __syncthreads();
+  if (p < np) {
+    Q4 v = pre[p];
+    for (int q = 0; q < 4; ++q) v.v[q] += s_add[q];
+    pre[p] = v;
+  }
+  if (last && p == np) {
+    Q4 v;
+    for (int q = 0; q < 4; ++q) v.v[q] = s_add[4 + q];
+    pre[np] = v;
+    *total = v;
+  }
 }
 
-__global__ void k_q4_apply(int np, Q4* pre, const Q4* btot, const Q4* total) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < np) pre[p] = q4_add(pre[p], btot[p / kScanTB]);
-  if (p == np) pre[np] = *total;
-}
-
-// points per child: counts between the node's first and one-past-last position
-__global__ void k_child_counts(int c, const int* rank, const int64_t* pstart, const int* pcount,
-                               const Q4* pre, int* ccnt) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+// records of the 4 children of every splitting node and its points per child
+// (counts between the node's first and one-past-last position)
+__global__ void k_children_counts(int c, const int* rank, const double* corners,
+                                  const double* side, const uint64_t* path, const int* inst,
+                                  const int64_t* pstart, const int* pcount, const Q4* pre,
+                                  int* n_parent, int* n_slot, uint64_t* n_path,
+                                  double* n_corners, double* n_side, int* n_inst, int* ccnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c || rank[i] < 0) return;
+  child_records(i, rank, corners, side, path, inst, n_parent, n_slot, n_path, n_corners, n_side,
+                n_inst);
   const Q4 a = pre[pstart[i]], b = pre[pstart[i] + pcount[i]];
   for (int q = 0; q < 4; ++q) ccnt[4 * rank[i] + q] = b.v[q] - a.v[q];
 }
@@ -667,30 +727,31 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin) {
     auto nx = std::make_unique<LevelBuf>();
     nx->dseg.alloc(ni + 1);
     const int* pst = l ? lv[l - 1]->st.p : nullptr;
-    k_split_info<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.pcount.p, L.pstart.p, L.perm.p,
-                                                  L.parent_local.p, pst, t.sd, t.leaf_dofs,
-                                                  L.split.p, L.st.p);
     int64_t segmax = 0;
     for (int i = 0; i < ni; ++i) segmax = std::max(segmax, L.seg[i + 1] - L.seg[i]);
-    if (segmax <= kSortMax && l <= 20 && (int64_t)n < (1ll << 24)) {
-      k_rank_sort<<<ni, 1024, 0, s>>>(L.dseg.p, L.split.p, L.st.p, L.path.p, L.rank.p,
-                                      seg_cnt.p);
+    const bool sorted = segmax <= kSortMax && l <= 20 && (int64_t)n < (1ll << 24);
+    if (sorted) {  // split flags and times inside; the segment scan too for one instance
+      k_rank_sort<<<ni, 1024, 0, s>>>(L.dseg.p, L.pcount.p, L.pstart.p, L.perm.p,
+                                      L.parent_local.p, pst, t.sd, t.leaf_dofs, L.split.p,
+                                      L.st.p, L.path.p, L.rank.p, seg_cnt.p, seg_split.p,
+                                      nx->dseg.p, xfer.p);
     } else {
+      k_split_info<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.pcount.p, L.pstart.p, L.perm.p,
+                                                    L.parent_local.p, pst, t.sd, t.leaf_dofs,
+                                                    L.split.p, L.st.p);
       k_rank_pairs<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.inst.p, L.dseg.p, L.split.p, L.st.p,
                                                     L.path.p, L.rank.p, seg_cnt.p);
     }
-    k_seg_scan<<<1, 64, 0, s>>>(ni, seg_cnt.p, seg_split.p, nx->dseg.p, xfer.p);
+    if (!sorted || ni > 1) k_seg_scan<<<1, 64, 0, s>>>(ni, seg_cnt.p, seg_split.p, nx->dseg.p, xfer.p);
     if (ni > 1) k_rank_offset<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.inst.p, seg_split.p, L.rank.p);
     // classify and scan the points of splitting nodes
     const int nb = ceil_div(np, kScanTB);
     qcode.alloc(np);
     pre.alloc((size_t)np + 1);
     btot.alloc(nb);
-    k_classify<<<ceil_div(np, kTB), kTB, 0, s>>>(np, L.perm.p, L.pnode.p, L.rank.p, L.corners.p,
-                                                 t.px.p, t.py.p, qcode.p);
-    k_q4_block<<<nb, kScanTB, 0, s>>>(np, qcode.p, pre.p, btot.p);
-    k_q4_top<<<1, 64, 0, s>>>(nb, btot.p, qtot);
-    k_q4_apply<<<ceil_div(np + 1, kTB), kTB, 0, s>>>(np, pre.p, btot.p, qtot);
+    k_q4_block<<<nb, kScanTB, 0, s>>>(np, L.perm.p, L.pnode.p, L.rank.p, L.corners.p, t.px.p,
+                                      t.py.p, qcode.p, pre.p, btot.p);
+    k_q4_apply<<<ceil_div(np + 1, kTB), kTB, 0, s>>>(np, nb, pre.p, btot.p, qtot);
     int* hx = pin.get<int>(ni + 5, s);
     copy_to_host(pin, s, {{xfer.p, 0, sizeof(int) * (ni + 5)}});
     stream_wait(s);
@@ -713,11 +774,9 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin) {
     N.pcount.alloc(N.count);
     N.pstart.alloc(N.count);
     N.inst.alloc(N.count);
-    k_children<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.rank.p, L.corners.p, L.side.p, L.path.p,
-                                                L.inst.p, N.parent_local.p, N.slot.p, N.path.p,
-                                                N.corners.p, N.side.p, N.inst.p);
-    k_child_counts<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.rank.p, L.pstart.p, L.pcount.p, pre.p,
-                                                    N.pcount.p);
+    k_children_counts<<<ceil_div(c, kTB), kTB, 0, s>>>(
+        c, L.rank.p, L.corners.p, L.side.p, L.path.p, L.inst.p, L.pstart.p, L.pcount.p, pre.p,
+        N.parent_local.p, N.slot.p, N.path.p, N.corners.p, N.side.p, N.inst.p, N.pcount.p);
     exclusive_scan_i32_to_i64(N.pcount.p, N.pstart.p, N.count, s);
     N.nperm = tot.v[0] + tot.v[1] + tot.v[2] + tot.v[3];
     N.perm.alloc(std::max(N.nperm, 1));

@@ -400,57 +400,64 @@ __global__ __launch_bounds__(kTcTB) void k_tc_eval(const int* is_leaf, const int
 
 bool treecode_usable(const TreeData& t) { return t.nnodes > 0 && t.sd == 1; }
 
+// Everything the applies reuse for the tree's lifetime: box centres, the
+// series work list and the interaction lists, on stream st; ev[15] marks
+// their completion.
+void treecode_prepare(Context& c, const TreeData& t, hipStream_t st) {
+  if (t.tc_ready) return;
+  const int nn = t.nnodes;
+  const int base = t.nlevels > kTcMinLevel ? (int)t.level_off[kTcMinLevel] : nn;
+  t.tc_ctr.alloc((size_t)nn);
+  t.tc_coef.alloc((size_t)nn * kTcP);
+  t.tc_far.alloc((size_t)nn * kTcFar);
+  t.tc_near.alloc((size_t)nn * kTcNear);
+  t.tc_nfar.alloc(nn);
+  t.tc_nnear.alloc(nn);
+  t.tc_work.alloc(2);
+  t.tc_work.zero(st);
+  k_tc_ctr<<<ceil_div(nn, 256), 256, 0, st>>>(nn, t.corners.p, t.tc_ctr.p);
+  // series work list: boxes of at most kTcSmall points four to a block,
+  // larger ones a block each (the node's point counts are mirrored on the
+  // host by the tree build)
+  std::vector<int4> wl;
+  int4 cur = make_int4(-1, -1, -1, -1);
+  int fill = 0;
+  for (int x = base; x < nn; ++x) {
+    const int cnt = t.h_pcount[x];
+    if (cnt < kTcMinPts) continue;
+    if (cnt > kTcSmall) {
+      wl.push_back(make_int4(x, -2, -1, -1));
+      continue;
+    }
+    int* slot = fill == 0 ? &cur.x : fill == 1 ? &cur.y : fill == 2 ? &cur.z : &cur.w;
+    *slot = x;
+    if (++fill == 4) {
+      wl.push_back(cur);
+      cur = make_int4(-1, -1, -1, -1);
+      fill = 0;
+    }
+  }
+  if (fill > 0) wl.push_back(cur);
+  t.tc_nwork = (int)wl.size();
+  t.tc_wlist.upload(wl, st);
+  k_tc_plan<<<nn, kWave, 0, st>>>(t.is_leaf.p, t.pcount.p, t.level.p, t.child.p, t.inst.p,
+                                  t.corners.p, t.tc_ctr.p, t.tc_far.p, t.tc_near.p, t.tc_nfar.p,
+                                  t.tc_nnear.p, t.tc_work.p);
+  KI_HIP(hipEventRecord(c.ev[15], st));
+  KI_HIP(hipGetLastError());
+  t.tc_ready = true;
+}
+
 void treecode_apply(Context& c, const TreeData& t, const double* d_mu, double* d_y) {
   hipStream_t s = c.stream;
   const int nn = t.nnodes;
-  const int base = t.nlevels > kTcMinLevel ? (int)t.level_off[kTcMinLevel] : nn;
   const GeomView g{t.px.p, t.py.p, t.nx.p, t.ny.p, t.w.p, t.kappa.p, t.npts, t.sd};
-  const bool plan = !t.tc_ready;
-  if (plan) {
-    t.tc_ctr.alloc((size_t)nn);
-    t.tc_coef.alloc((size_t)nn * kTcP);
-    t.tc_far.alloc((size_t)nn * kTcFar);
-    t.tc_near.alloc((size_t)nn * kTcNear);
-    t.tc_nfar.alloc(nn);
-    t.tc_nnear.alloc(nn);
-    t.tc_work.alloc(2);
-    t.tc_work.zero(s);
-    k_tc_ctr<<<ceil_div(nn, 256), 256, 0, s>>>(nn, t.corners.p, t.tc_ctr.p);
-    // series work list: boxes of at most kTcSmall points four to a block,
-    // larger ones a block each (the node's point counts are mirrored on the
-    // host by the tree build)
-    std::vector<int4> wl;
-    int4 cur = make_int4(-1, -1, -1, -1);
-    int fill = 0;
-    for (int x = base; x < nn; ++x) {
-      const int cnt = t.h_pcount[x];
-      if (cnt < kTcMinPts) continue;
-      if (cnt > kTcSmall) {
-        wl.push_back(make_int4(x, -2, -1, -1));
-        continue;
-      }
-      int* slot = fill == 0 ? &cur.x : fill == 1 ? &cur.y : fill == 2 ? &cur.z : &cur.w;
-      *slot = x;
-      if (++fill == 4) {
-        wl.push_back(cur);
-        cur = make_int4(-1, -1, -1, -1);
-        fill = 0;
-      }
-    }
-    if (fill > 0) wl.push_back(cur);
-    t.tc_nwork = (int)wl.size();
-    t.tc_wlist.upload(wl, s);
-  }
+  if (!t.tc_ready) treecode_prepare(c, t, s);
+  KI_HIP(hipStreamWaitEvent(s, c.ev[15], 0));  // the lists may come from the side stream
   KI_HIP(hipEventRecord(c.ev[12], s));
   if (t.tc_nwork > 0)
     k_tc_coef<<<t.tc_nwork, kTcTB, 0, s>>>(t.tc_wlist.p, t.pstart.p, t.pcount.p, t.perm_all.p, g,
                                           d_mu, t.tc_ctr.p, t.tc_coef.p);
-  if (plan) {
-    k_tc_plan<<<nn, kWave, 0, s>>>(t.is_leaf.p, t.pcount.p, t.level.p, t.child.p, t.inst.p,
-                                   t.corners.p, t.tc_ctr.p, t.tc_far.p, t.tc_near.p,
-                                   t.tc_nfar.p, t.tc_nnear.p, t.tc_work.p);
-    t.tc_ready = true;
-  }
   KI_HIP(hipEventRecord(c.ev[14], s));
   k_tc_eval<<<nn, kTcTB, 0, s>>>(t.is_leaf.p, t.pcount.p, t.pstart.p, t.perm_all.p, t.inst.p,
                                  t.d_inst_pt.p, g, d_mu, t.tc_ctr.p, t.tc_coef.p, t.tc_far.p,
