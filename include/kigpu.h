@@ -92,8 +92,8 @@ typedef struct {
    * summed kernel time and the number of (target, source) pairs */
   int apply_launches;
   double apply_kernel_s, apply_pairs;
-  /* the treecode apply the FP64 refinement uses by default (Laplace, one
-   * right-hand side): applies, their summed kernel time (series +
+  /* the treecode apply the FP64 refinement uses by default (Laplace double
+   * layer): applies, their summed kernel time (series +
    * evaluation), the evaluation kernel's share (s), and the (target, series
    * box) and (target, source) pairs of one apply */
   int tc_launches;
@@ -231,18 +231,18 @@ kigpu_status kigpu_kernel_apply(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
                                 const double* mu, int nrhs, double* y);
 kigpu_status kigpu_kernel_apply_device(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
                                        const double* d_mu, int nrhs, double* d_y);
-/* The same product for the Laplace double layer with one density by a
- * treecode: box series of 48 terms used where the box radius is at most half
- * the distance from its centre to the target leaf, the rest summed directly;
- * the truncation stays below 1e-14 relative to the direct sum.  mu, y:
- * num_points doubles.  This is the FP64 refinement's residual apply (below);
- * it has no counterpart in the reference, whose path has no refinement. */
+/* The same product for the Laplace double layer by a treecode: box series of
+ * 48 terms used where the box radius is at most half the distance from its
+ * centre to the target leaf, the rest summed directly; the truncation stays
+ * below 1e-14 relative to the direct sum.  mu, y: num_points x nrhs,
+ * column-major.  This is the FP64 refinement's residual apply (below); it has
+ * no counterpart in the reference, whose path has no refinement. */
 kigpu_status kigpu_kernel_apply_treecode(kigpu_ctx* ctx, const kigpu_tree* tree,
-                                         const double* mu, double* y);
+                                         const double* mu, int nrhs, double* y);
 /* Residual apply of the FP64 refinement on this context: 0 (default) the
- * treecode where it applies (Laplace, one right-hand side, no coincident
- * nodes), 1 always the direct sum of kigpu_kernel_apply.  No counterpart in
- * the reference. */
+ * treecode where it applies (Laplace double layer, no coincident nodes, not
+ * the adjoint), 1 always the direct sum of kigpu_kernel_apply.  No
+ * counterpart in the reference. */
 kigpu_status kigpu_set_refine_apply(kigpu_ctx* ctx, int mode);
 /* u = Kernel::forward() mu (kernel.cpp:337-357): boundary -> domain targets,
  * targets: 2*ntgt interleaved.  u: (sd*ntgt) x nrhs. */

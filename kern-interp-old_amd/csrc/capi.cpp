@@ -632,17 +632,17 @@ kigpu_status kigpu_kernel_apply_device(kigpu_ctx* ctx, const kigpu_tree* tree, i
 }
 
 kigpu_status kigpu_kernel_apply_treecode(kigpu_ctx* ctx, const kigpu_tree* tree,
-                                         const double* mu, double* y) {
-  KI_REQUIRE(ctx && tree && mu && y, "invalid argument");
+                                         const double* mu, int nrhs, double* y) {
+  KI_REQUIRE(ctx && tree && mu && y && nrhs > 0, "invalid argument");
   KI_REQUIRE(tree->t.sd == 1 && treecode_usable(tree->t),
              "the treecode apply takes the Laplace double layer (one DOF per node)");
   return guard([&] {
     Context& c = ctx->c;
-    const size_t len = (size_t)tree->t.ndof;
+    const size_t len = (size_t)tree->t.ndof * nrhs;
     DBuf<double> dm, dy;
     dm.upload(mu, len, c.stream);
     dy.alloc(len);
-    treecode_apply(c, tree->t, dm.p, dy.p);
+    treecode_apply(c, tree->t, dm.p, nrhs, dy.p);
     dy.download(y, len, c.stream);
     stream_wait(c.stream);
   });

@@ -68,10 +68,11 @@ void border_step(Context& c, TreeData& t, T* x, int ldx, int nrhs, const double*
 void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_x,
                 double* d_alpha, const kigpu_refine_opts* ro, kigpu_solve_info* info,
                 bool trans = false);
-// treecode.hip: K mu for the Laplace double layer, one density, by a
-// treecode (series truncation below 1e-14 relative); the refinement's apply
+// treecode.hip: K mu for the Laplace double layer by a treecode (series
+// truncation below 1e-14 relative); the refinement's apply.  mu, y: ndof x
+// nrhs, column-major with leading dimension ndof
 bool treecode_usable(const TreeData& t);
-void treecode_apply(Context& c, const TreeData& t, const double* d_mu, double* d_y);
+void treecode_apply(Context& c, const TreeData& t, const double* d_mu, int nrhs, double* d_y);
 // the tree's treecode lists and box centres, on stream st (once per tree)
 void treecode_prepare(Context& c, const TreeData& t, hipStream_t st);
 void kernel_apply(Context& c, const TreeData& t, int pde, const double* d_mu, int nrhs,

@@ -1258,11 +1258,11 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
   // FP32 factor: solve in FP32, then FP64 refinement
   //   r = [f - K mu - U alpha; alpha - Psi mu],  [mu; alpha] += F^{-1} r
   const int iters = ro ? ro->max_iters : 0;
-  // the residual apply: the treecode for the Laplace double layer with one
-  // density and no coincident nodes (unless the context asks for the direct
-  // sum), else the direct kernels
+  // the residual apply: the treecode for the Laplace double layer with no
+  // coincident nodes (unless the context asks for the direct sum), else the
+  // direct kernels
   const bool tc = iters > 0 && !trans && c.refine_apply == 0 && t.opts.pde == KIGPU_LAPLACE &&
-                  nrhs == 1 && treecode_usable(t) && !has_coincident_points(c, t);
+                  treecode_usable(t) && !has_coincident_points(c, t);
   // the treecode's lists are built on the side stream while the first FP32
   // solve runs (the first apply waits for them)
   if (tc) treecode_prepare(c, t, c.side);
@@ -1290,7 +1290,7 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
     for (int it = 0; it <= iters; ++it) {
       KI_HIP(hipEventRecord(c.ev[8], s));
       if (trans) kernel_apply_transpose(c, t, t.opts.pde, d_x, nrhs, t.f64.p);
-      else if (tc) treecode_apply(c, t, d_x, t.f64.p);
+      else if (tc) treecode_apply(c, t, d_x, nrhs, t.f64.p);
       else kernel_apply(c, t, t.opts.pde, d_x, nrhs, t.f64.p);
       if (h) {
         k_add_ualpha<<<dim3(ceil_div(t.ndof, 256), nrhs), 256, 0, s>>>(t.ndof, h, t.bU64in.p,

@@ -70,10 +70,13 @@ constexpr int kTcSmall = 256;
 
 __global__ __launch_bounds__(kTcTB) void k_tc_coef(const int4* work, const int64_t* pstart,
                                                    const int* pcount, const int* perm,
-                                                   GeomView g, const double* mu,
-                                                   const double4* ctr, double2* coef) {
+                                                   GeomView g, const double* mu, int ldm,
+                                                   const double4* ctr, double2* coef,
+                                                   size_t cstride) {
   __shared__ double2 part[kTcNW][16][kWave];
   __shared__ double2 wsum[kTcNW][kTcP];
+  mu += (size_t)blockIdx.y * ldm;  // one density per grid row
+  coef += (size_t)blockIdx.y * cstride;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
   const int4 wk = work[blockIdx.x];
   const bool big = wk.y == -2;
@@ -277,10 +280,11 @@ __device__ __forceinline__ void tc_direct(TcSrc& S, int wv, int lane, const Geom
 __global__ __launch_bounds__(kTcTB) void k_tc_eval(const int* is_leaf, const int* pcount,
                                                    const int64_t* pstart, const int* perm,
                                                    const int* inst, const int64_t* inst_pt,
-                                                   GeomView g, const double* mu,
+                                                   GeomView g, const double* mu, int ldm,
                                                    const double4* ctr, const double2* coef,
-                                                   const int* far_list, const int* near_list,
-                                                   const int* nfar, const int* nnear, double* y) {
+                                                   size_t cstride, const int* far_list,
+                                                   const int* near_list, const int* nfar,
+                                                   const int* nnear, double* y, int ldy) {
   __shared__ double2 s_cf[kTcNW][kTcP];
   __shared__ TcSrc S;
   __shared__ double s_u[kTcNW][kWave];
@@ -288,6 +292,9 @@ __global__ __launch_bounds__(kTcTB) void k_tc_eval(const int* is_leaf, const int
   const int B = blockIdx.x;
   const int cntB = pcount[B];
   if (!is_leaf[B] || cntB == 0) return;
+  mu += (size_t)blockIdx.y * ldm;  // one density per grid row
+  coef += (size_t)blockIdx.y * cstride;
+  y += (size_t)blockIdx.y * ldy;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
   const int64_t p0B = pstart[B];
   const int nf = nfar[B], nn = nnear[B];
@@ -408,7 +415,6 @@ void treecode_prepare(Context& c, const TreeData& t, hipStream_t st) {
   const int nn = t.nnodes;
   const int base = t.nlevels > kTcMinLevel ? (int)t.level_off[kTcMinLevel] : nn;
   t.tc_ctr.alloc((size_t)nn);
-  t.tc_coef.alloc((size_t)nn * kTcP);
   t.tc_far.alloc((size_t)nn * kTcFar);
   t.tc_near.alloc((size_t)nn * kTcNear);
   t.tc_nfar.alloc(nn);
@@ -448,20 +454,24 @@ void treecode_prepare(Context& c, const TreeData& t, hipStream_t st) {
   t.tc_ready = true;
 }
 
-void treecode_apply(Context& c, const TreeData& t, const double* d_mu, double* d_y) {
+void treecode_apply(Context& c, const TreeData& t, const double* d_mu, int nrhs, double* d_y) {
   hipStream_t s = c.stream;
   const int nn = t.nnodes;
   const GeomView g{t.px.p, t.py.p, t.nx.p, t.ny.p, t.w.p, t.kappa.p, t.npts, t.sd};
   if (!t.tc_ready) treecode_prepare(c, t, s);
+  const size_t cstride = (size_t)nn * kTcP;
+  t.tc_coef.alloc(cstride * nrhs);
   KI_HIP(hipStreamWaitEvent(s, c.ev[15], 0));  // the lists may come from the side stream
   KI_HIP(hipEventRecord(c.ev[12], s));
   if (t.tc_nwork > 0)
-    k_tc_coef<<<t.tc_nwork, kTcTB, 0, s>>>(t.tc_wlist.p, t.pstart.p, t.pcount.p, t.perm_all.p, g,
-                                          d_mu, t.tc_ctr.p, t.tc_coef.p);
+    k_tc_coef<<<dim3(t.tc_nwork, nrhs), kTcTB, 0, s>>>(t.tc_wlist.p, t.pstart.p, t.pcount.p,
+                                                      t.perm_all.p, g, d_mu, t.ndof, t.tc_ctr.p,
+                                                      t.tc_coef.p, cstride);
   KI_HIP(hipEventRecord(c.ev[14], s));
-  k_tc_eval<<<nn, kTcTB, 0, s>>>(t.is_leaf.p, t.pcount.p, t.pstart.p, t.perm_all.p, t.inst.p,
-                                 t.d_inst_pt.p, g, d_mu, t.tc_ctr.p, t.tc_coef.p, t.tc_far.p,
-                                 t.tc_near.p, t.tc_nfar.p, t.tc_nnear.p, d_y);
+  k_tc_eval<<<dim3(nn, nrhs), kTcTB, 0, s>>>(t.is_leaf.p, t.pcount.p, t.pstart.p, t.perm_all.p,
+                                             t.inst.p, t.d_inst_pt.p, g, d_mu, t.ndof, t.tc_ctr.p,
+                                             t.tc_coef.p, cstride, t.tc_far.p, t.tc_near.p,
+                                             t.tc_nfar.p, t.tc_nnear.p, d_y, t.ndof);
   KI_HIP(hipEventRecord(c.ev[13], s));
   KI_HIP(hipGetLastError());
   stream_wait(s);

@@ -110,7 +110,7 @@ def load() -> C.CDLL:
         "kigpu_solve_device": (C.c_int, [P, P, P, C.c_int, P, P, C.POINTER(RefineOpts),
                                          C.POINTER(SolveInfo)]),
         "kigpu_kernel_apply": (C.c_int, [P, P, C.c_int, _f64p, C.c_int, _f64p]),
-        "kigpu_kernel_apply_treecode": (C.c_int, [P, P, _f64p, _f64p]),
+        "kigpu_kernel_apply_treecode": (C.c_int, [P, P, _f64p, C.c_int, _f64p]),
         "kigpu_set_refine_apply": (C.c_int, [P, C.c_int]),
         "kigpu_kernel_apply_device": (C.c_int, [P, P, C.c_int, P, C.c_int, P]),
         "kigpu_forward_apply": (C.c_int, [P, P, C.c_int, _f64p, C.c_int, _f64p, C.c_int, _f64p]),
@@ -404,13 +404,18 @@ class Tree:
         return out.reshape(-1, 6)
 
     def apply_treecode(self, mu):
-        """K mu by the treecode (Laplace double layer, one density): the
-        refinement's residual apply."""
-        mu = _f64(mu)
-        y = np.zeros(mu.size)
-        _check(self.lib.kigpu_kernel_apply_treecode(self.ctx.h, self.h, mu, y),
+        """K mu by the treecode (Laplace double layer; mu: (ndof,) or
+        (ndof, nrhs)): the refinement's residual apply."""
+        mu = np.asarray(mu, np.float64)
+        one = mu.ndim == 1
+        M = mu.reshape(self.ndof, -1)
+        nrhs = M.shape[1]
+        flat = np.ascontiguousarray(M.T).reshape(-1)
+        y = np.zeros_like(flat)
+        _check(self.lib.kigpu_kernel_apply_treecode(self.ctx.h, self.h, flat, nrhs, y),
                "kigpu_kernel_apply_treecode")
-        return y
+        Y = y.reshape(nrhs, self.ndof).T
+        return Y[:, 0].copy() if one else np.array(Y)
 
     def apply(self, mu, pde=None):
         pde = self.pde if pde is None else pde

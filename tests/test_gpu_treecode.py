@@ -1,7 +1,7 @@
 """The treecode apply of the refinement residual (Laplace double layer, one
-density) against the dense kernel of the oracle and the GPU's direct sum, and
-the FP32 + refinement solve with it against the same solve with the direct
-residual."""
+or several densities) against the dense kernel of the oracle and the GPU's
+direct sum, and the FP32 + refinement solve with it against the same solve
+with the direct residual."""
 import numpy as np
 import pytest
 
@@ -59,6 +59,26 @@ def test_treecode_matches_direct_large(ctx, geom):
     t.close()
 
 
+@pytest.mark.parametrize("nrhs", [3, 17])
+def test_treecode_many_rhs(ctx, nrhs):
+    # several densities in one apply (one grid row per column, shared lists):
+    # each column against the direct sum and against the one-density apply
+    n = 20000
+    print(f"[treecode] random_star {n}, {nrhs} densities", flush=True)
+    pts, nrm, w, kap = W.random_star(n, 9)
+    t = ctx.build_tree(pts, nrm, w, kap, 1, 64)
+    mu = np.random.default_rng(nrhs).standard_normal((n, nrhs))
+    ref = t.apply(mu, kigpu.LAPLACE)
+    y = t.apply_treecode(mu)
+    assert y.shape == ref.shape
+    err = (np.abs(y - ref).max(axis=0) / np.abs(ref).max(axis=0)).max()
+    print(f"  worst column: max error / max |y| = {err:.2e}", flush=True)
+    assert err <= RTOL
+    for j in (0, nrhs - 1):
+        assert np.array_equal(y[:, j], t.apply_treecode(np.ascontiguousarray(mu[:, j])))
+    t.close()
+
+
 def test_treecode_forest(ctx):
     # block-diagonal over instances of uneven sizes
     print("[treecode] forest of 4 instances", flush=True)
@@ -100,6 +120,34 @@ def test_refinement_treecode_vs_direct(ctx):
     d = np.abs(mus[0] - mus[1]).max() / np.abs(mus[1]).max()
     print(f"  direct residual of the treecode-refined density {res:.2e}; "
           f"densities differ by {d:.2e} (relative)", flush=True)
+    assert res <= 1e-10
+    assert d <= 1e-9
+    t.close()
+
+
+def test_refinement_treecode_many_rhs(ctx):
+    # eight right-hand sides refined with the treecode residual and with the
+    # direct one; every column converges and the two agree
+    print("[treecode] FP32 + refinement, 8 right-hand sides", flush=True)
+    b = star_problem(8192)
+    t = gpu_tree(ctx, b, 1, 64)
+    t.factor(kigpu.LAPLACE, 1e-6, kigpu.FP32_REFINE)
+    rng = np.random.default_rng(21)
+    F = np.column_stack([b.boundary_values] + [rng.standard_normal(b.num_points)
+                                                for _ in range(7)])
+    mus = []
+    for mode in (0, 1):
+        ctx.set_refine_apply(mode)
+        mus.append(t.solve(F, refine_iters=10, rel_residual=1e-10))
+        print(f"  mode {mode}: {t.last_info}", flush=True)
+        assert t.last_info["rel_residual"] <= 1e-10
+        tm = ctx.timings()
+        assert (tm["tc_launches"] > 0) == (mode == 0)
+    ctx.set_refine_apply(0)
+    R = F - t.apply(mus[0], kigpu.LAPLACE)
+    res = (np.linalg.norm(R, axis=0) / np.linalg.norm(F, axis=0)).max()
+    d = (np.abs(mus[0] - mus[1]).max(axis=0) / np.abs(mus[1]).max(axis=0)).max()
+    print(f"  worst direct residual {res:.2e}; densities differ by {d:.2e}", flush=True)
     assert res <= 1e-10
     assert d <= 1e-9
     t.close()
