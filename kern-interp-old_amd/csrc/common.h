@@ -99,6 +99,14 @@ inline void stream_wait(hipStream_t s) {
   KI_HIP(e);
 }
 
+// Wait, the same way, for the work queued before event ev only.
+inline void event_wait(hipEvent_t ev) {
+  hipError_t e;
+  while ((e = hipEventQuery(ev)) == hipErrorNotReady) {
+  }
+  KI_HIP(e);
+}
+
 // Page-locked host memory (hipHostMalloc) for the small per-level transfers:
 // copies from and to it are plain DMA, where pageable memory goes through a
 // staging copy on every call.  Grow-only; growing waits for the stream, so a

@@ -53,16 +53,23 @@ __global__ void k_leaf_dofs(int nnodes, const int* is_leaf, const int* pcount, i
 }
 
 // |active_box| (remove_inactive_dofs_at_box, quadtree.cpp:694-720)
+// size of the list node y hands to its parent once its ID has run: its rank
+// if compressed, else its active count (the elimination stores the same
+// value in osize; the lists of the parent level are built from the ID's
+// output before that elimination runs)
+__device__ __forceinline__ int out_size(int y, const int* rank, const int* nact) {
+  return rank[y] >= 0 ? rank[y] : nact[y];
+}
+
 __global__ void k_active_size(int a, int b, const int* is_leaf, const int* child,
-                              const int* pcount, const int* osize, int sd, int* nact,
-                              int* rank) {
+                              const int* pcount, int sd, int* nact, int* rank) {
   int x = a + blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= b) return;
   int n = 0;
   if (is_leaf[x]) {
     n = pcount[x] * sd;
   } else {
-    for (int s = 0; s < 4; ++s) n += osize[child[4 * x + s]];
+    for (int s = 0; s < 4; ++s) n += out_size(child[4 * x + s], rank, nact);
   }
   nact[x] = n;
   rank[x] = -1;
@@ -75,9 +82,8 @@ __device__ __forceinline__ const int* out_list(int y, const int* rank, const int
 }
 
 __global__ void k_active_fill(int a, const int* is_leaf, const int* child, const int64_t* obox_off,
-                              const int* obox, const int* osize, const int* rank,
-                              const int64_t* act_off, const int* nact, int* act_pool,
-                              const int* sr_pool) {
+                              const int* obox, const int* rank, const int64_t* act_off,
+                              const int* nact, int* act_pool, const int* sr_pool) {
   int x = a + blockIdx.x;
   int* out = act_pool + act_off[x];
   if (is_leaf[x]) {
@@ -89,7 +95,7 @@ __global__ void k_active_fill(int a, const int* is_leaf, const int* child, const
   for (int s = 0; s < 4; ++s) {
     int y = child[4 * x + s];
     const int* src = out_list(y, rank, act_off, act_pool, sr_pool);
-    int cnt = osize[y];
+    int cnt = out_size(y, rank, nact);
     for (int j = threadIdx.x; j < cnt; j += blockDim.x) out[o + j] = src[j];
     o += cnt;
   }
@@ -1061,9 +1067,9 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
   const int pde = o.pde;
   const int prow = pde == KIGPU_STOKES ? 4 * kProxyPoints : 2 * kProxyPoints;
   const size_t lds_cap = c.lds_max;
-  // at most 6 markers per level (lists, ID LDS start/end, ID end, elimination
-  // end) plus 2 for the root
-  c.ensure_events(8 * (size_t)t.nlevels + 8);
+  // at most 9 markers per level (lists start/end, rows, ID start, ID kernel
+  // start/end, ID end, elimination start/end) plus 2 for the root
+  c.ensure_events(10 * (size_t)t.nlevels + 8);
   int ev = 0;
   auto mark = [&]() {
     if ((size_t)ev >= c.evpool.size()) KI_FAIL(KIGPU_ERR_STATE, "event pool exhausted");
@@ -1105,46 +1111,61 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
   std::vector<int> h_n, h_q;
   int64_t pool_base = 0;
   struct LevelEv {
-    int l0, l1, id0, id1, dec0, dec1, idl0, idl1;
+    int l0, l1, pre, r0, id0, id1, dec0, dec1, idl0, idl1;
   };
   std::vector<LevelEv> lev;
 
+  // ---- active lists of level l (remove_inactive_dofs_at_level,
+  // quadtree.cpp:664-691) and, for l >= 3, the inner-row counts, whose sizes
+  // are copied to page-locked memory (read on the host once marker l1 is
+  // done).  They need only the ID of level l + 1, so they are queued ahead of
+  // that level's elimination, which then runs while the host builds level
+  // l's work lists.
+  auto launch_lists = [&](int l, LevelEv& E) -> int* {
+    const int a = (int)t.level_off[l], b = (int)t.level_off[l + 1], cn = b - a;
+    E = LevelEv{};
+    E.idl0 = E.idl1 = -1;
+    E.l0 = mark();
+    k_active_size<<<ceil_div(cn, kTB), kTB, 0, s>>>(a, b, t.is_leaf.p, t.child.p, t.pcount.p,
+                                                   t.sd, t.nact.p, t.rank.p);
+    exclusive_scan_i32_to_i64(t.nact.p + a, t.act_off.p + a, cn, s, pool_base);
+    k_active_fill<<<cn, 64, 0, s>>>(a, t.is_leaf.p, t.child.p, t.obox_off.p, t.obox_pool.p,
+                                    t.rank.p, t.act_off.p, t.nact.p, t.act_pool.p, t.sr_pool.p);
+    int* hp = nullptr;
+    if (l > 0) {
+      if (l >= 3)
+        k_inner_rows<<<cn, kWave, 0, s>>>(a, b, o.min_dofs, g, t.is_leaf.p, t.obox_off.p,
+                                          t.obox_pool.p, t.pcount.p, t.act_off.p, t.nact.p,
+                                          t.act_pool.p, t.sl_nbr.p, t.sl_cnt.p, t.cp_off.p,
+                                          t.cp_nbr.p, t.corners.p, t.side.p, t.nrows.p, nullptr,
+                                          nullptr, 0);
+      hp = c.pin_down.get<int>(2 * (size_t)cn, s);
+      copy_to_host(c.pin_down, s,
+                   {{t.nact.p + a, 0, sizeof(int) * cn},
+                    {t.nrows.p + a, sizeof(int) * cn, l >= 3 ? sizeof(int) * cn : 0}});
+    }
+    E.l1 = mark();
+    E.pre = E.l1;
+    return hp;
+  };
+
+  LevelEv E;
+  int* hlists = launch_lists(t.nlevels - 1, E);
   for (int l = t.nlevels - 1; l >= 0; --l) {
     const int a = (int)t.level_off[l], b = (int)t.level_off[l + 1], cn = b - a;
     auto lw = std::make_unique<LevelWork>();
     lw->level = l;
     lw->a = a;
     lw->b = b;
-    LevelEv E{};
-    E.idl0 = E.idl1 = -1;
-    E.l0 = mark();
-    // ---- active lists (remove_inactive_dofs_at_level, quadtree.cpp:664-691)
-    k_active_size<<<ceil_div(cn, kTB), kTB, 0, s>>>(a, b, t.is_leaf.p, t.child.p, t.pcount.p,
-                                                   t.osize.p, t.sd, t.nact.p, t.rank.p);
-    exclusive_scan_i32_to_i64(t.nact.p + a, t.act_off.p + a, cn, s, pool_base);
-    k_active_fill<<<cn, 64, 0, s>>>(a, t.is_leaf.p, t.child.p, t.obox_off.p, t.obox_pool.p,
-                                    t.osize.p, t.rank.p, t.act_off.p, t.nact.p, t.act_pool.p,
-                                    t.sr_pool.p);
-    if (l == 0) {
-      E.l1 = mark();
+    if (l == 0) {  // the roots' lists (queued ahead of level 1's elimination)
       lev.push_back(E);
       t.work[l] = std::move(lw);
       break;
     }
-    if (l >= 3)
-      k_inner_rows<<<cn, kWave, 0, s>>>(a, b, o.min_dofs, g, t.is_leaf.p, t.obox_off.p,
-                                        t.obox_pool.p, t.pcount.p, t.act_off.p, t.nact.p,
-                                        t.act_pool.p, t.sl_nbr.p, t.sl_cnt.p, t.cp_off.p,
-                                        t.cp_nbr.p, t.corners.p, t.side.p, t.nrows.p, nullptr,
-                                        nullptr, 0);
-    int* pin = c.pin_down.get<int>(2 * (size_t)cn, s);
-    copy_to_host(c.pin_down, s,
-                 {{t.nact.p + a, 0, sizeof(int) * cn},
-                  {t.nrows.p + a, sizeof(int) * cn, l >= 3 ? sizeof(int) * cn : 0}});
-    stream_wait(s);
+    event_wait(c.evpool[E.l1]);
     ++c.tim.syncs;
-    h_n.assign(pin, pin + cn);
-    if (l >= 3) h_q.assign(pin + cn, pin + 2 * cn);
+    h_n.assign(hlists, hlists + cn);
+    if (l >= 3) h_q.assign(hlists + cn, hlists + 2 * cn);
     else h_q.assign(cn, 0);
     int64_t level_dofs = 0;
     for (int i = 0; i < cn; ++i) level_dofs += h_n[i];
@@ -1223,6 +1244,7 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     const int64_t* d_roff = reinterpret_cast<const int64_t*>(dpk + o_roff);
     const IdWork* d_wl = reinterpret_cast<const IdWork*>(dpk + o_wl);
     // ---- factor slabs for every node of the level
+    E.r0 = mark();
     T* slab;
     if (sizeof(T) == 8) {
       lw->slab64.alloc(stot);
@@ -1248,9 +1270,8 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
           d_wl, a, b, l, (int)t.level_off[1], (int)t.level_off[2], t.is_leaf.p, t.obox_off.p,
           t.obox_pool.p, t.pcount.p, t.sd, t.act_off.p, t.nact.p, t.act_pool.p, t.inst.p,
           lw->rows.p);
-    E.l1 = mark();
     // ---- ID
-    E.id0 = E.l1;
+    E.id0 = mark();
     if (nwl > 0) {
       const size_t ids = std::max(lds_bytes, glds);
       if (!wl_glb.empty()) lw->ws.alloc(atot);
@@ -1264,8 +1285,12 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
       lw->id_boxes = wl;
     }
     E.id1 = mark();
+    // the next level's lists, then this level's elimination
+    pool_base += level_dofs;  // the next level's lists start after this level's
+    LevelEv En;
+    hlists = launch_lists(l - 1, En);
     // ---- elimination for every node of the level
-    E.dec0 = E.id1;
+    E.dec0 = mark();
     {
       DecLds L = dec_layout(std::max(nmax, 1), sizeof(T));
       auto dec_launch = [&](const int* nodes, int count, size_t lds, hipStream_t st) {
@@ -1306,7 +1331,8 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     E.dec1 = mark();
     lev.push_back(E);
     t.work[l] = std::move(lw);
-    pool_base += level_dofs;  // the next level's lists start after this level's
+    En.pre = E.dec1;  // the next level's upload follows this elimination
+    E = En;
   }
 
   // ---- roots (level 0), one per instance
@@ -1374,7 +1400,10 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     return ms * 1e-3;
   };
   for (const LevelEv& E : lev) {
+    // the lists' kernels, and the wait for the host's work list plus the
+    // ID rows (from the marker before the level's upload to the ID)
     c.tim.t_lists += el(E.l0, E.l1);
+    if (E.id0 > E.r0 && E.r0 > 0) c.tim.t_lists += el(E.pre, E.id0);
     if (E.id1 > E.id0) {
       c.tim.t_id += el(E.id0, E.id1);
       c.tim.t_decouple += el(E.dec0, E.dec1);

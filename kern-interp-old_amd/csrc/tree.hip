@@ -440,40 +440,59 @@ __global__ void k_scatter(int np, const int* perm, const int* pnode, const int* 
 }
 
 // same-level neighbor lists of level [a, b): siblings, then cousins
-// (compute_neighbor_lists, quadtree.cpp:33-50)
+// (compute_neighbor_lists, quadtree.cpp:33-50).  kMaxNbrs lanes per node:
+// lane j tests the children of the parent's j-th neighbor, and a prefix sum
+// over the lanes puts the cousins in the serial loop's (j, s) order.
+constexpr int kSlLanes = kMaxNbrs;
+static_assert(kSlLanes == 8 && kWave % kSlLanes == 0, "8 lanes per node");
 __global__ void k_same_level(int a, int b, const int* parent, const int* slot, const int* child,
                              const int* is_leaf, const int* level, const double* corners,
                              const double* side, int* sl_nbr, int* sl_cnt, int* status) {
 #pragma clang fp contract(off)
-  int x = a + blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= b) return;
-  int p = parent[x];
-  int cnt = 0;
-  int* out = sl_nbr + kMaxNbrs * x;
-  if (p >= 0) {
-    for (int s = 0; s < 4; ++s)
-      if (s != slot[x]) out[cnt++] = child[4 * p + s];
-    double thr = side[x] * sqrt(2.0) + 1e-5;
-    double x0 = corners[8 * x], y0 = corners[8 * x + 1];
-    for (int j = 0; j < sl_cnt[p]; ++j) {
-      int y = sl_nbr[kMaxNbrs * p + j];
-      if (is_leaf[y]) continue;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int x = a + gid / kSlLanes;
+  const int j = gid % kSlLanes;
+  const int p = x < b ? parent[x] : -1;
+  int found[4];
+  int nf = 0;
+  if (p >= 0 && j < sl_cnt[p]) {
+    const int y = sl_nbr[kMaxNbrs * p + j];
+    if (!is_leaf[y]) {
+      const double thr = side[x] * sqrt(2.0) + 1e-5;
+      const double x0 = corners[8 * x], y0 = corners[8 * x + 1];
+      const int lx = level[x];
       for (int s = 0; s < 4; ++s) {
-        int z = child[4 * y + s];
-        if (z < 0 || level[z] != level[x]) continue;
-        double dx = x0 - corners[8 * z];
-        double dy = y0 - corners[8 * z + 1];
-        if (sqrt(dx * dx + dy * dy) < thr) {
-          if (cnt >= kMaxNbrs) {
-            atomicOr(status, 1);
-            continue;
-          }
-          out[cnt++] = z;
-        }
+        const int z = child[4 * y + s];
+        if (z < 0 || level[z] != lx) continue;
+        const double dx = x0 - corners[8 * z];
+        const double dy = y0 - corners[8 * z + 1];
+        if (sqrt(dx * dx + dy * dy) < thr) found[nf++] = z;
       }
     }
   }
-  sl_cnt[x] = cnt;
+  // exclusive prefix of nf over the node's lanes
+  int incl = nf;
+  for (int d = 1; d < kSlLanes; d <<= 1) {
+    const int t = __shfl_up(incl, d, kSlLanes);
+    if (j >= d) incl += t;
+  }
+  const int total = __shfl(incl, kSlLanes - 1, kSlLanes);
+  if (p < 0) {
+    if (x < b && j == 0) sl_cnt[x] = 0;
+    return;
+  }
+  int* out = sl_nbr + kMaxNbrs * x;
+  if (j < 3) {  // the siblings first
+    const int sx = slot[x];
+    const int s = j < sx ? j : j + 1;
+    out[j] = child[4 * p + s];
+  }
+  const int o = 3 + incl - nf;
+  for (int i = 0; i < nf; ++i) {
+    if (o + i < kMaxNbrs) out[o + i] = found[i];
+    else atomicOr(status, 1);
+  }
+  if (j == 0) sl_cnt[x] = min(3 + total, kMaxNbrs);
 }
 
 // get_descendent_neighbors' wall test (quadtree.cpp:121-155) for big box B and
@@ -508,19 +527,23 @@ constexpr int kMaxDepth = 32;
 // reference's list order), kept if they touch d.  pass 0 counts, pass 1
 // writes.  Everything stays in registers: the up-to-8 candidates are
 // ordered by a sorting network.
+// kMaxDepth lanes per node d: lane lb handles ancestor level lb, and a
+// prefix sum over the lanes keeps the ascending-lb order.
+static_assert(kMaxDepth == 32 && kWave % kMaxDepth == 0, "32 lanes per node");
 __global__ void k_coarse(int nnodes, const int* anc, const int* level, const int* is_leaf,
                          const double* corners, const int* sl_nbr, const int* sl_cnt,
                          const int64_t* cp_off, int* cp_cnt, int* cp_nbr, int pass) {
-  const int d = blockIdx.x * blockDim.x + threadIdx.x;
-  if (d >= nnodes) return;
-  const int l = level[d];
-  const double* dc = corners + 8 * (size_t)d;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int d = gid / kMaxDepth;
+  const int lb = gid % kMaxDepth;
+  const int l = d < nnodes ? level[d] : 0;
+  const double* dc = corners + 8 * (size_t)(d < nnodes ? d : 0);
+  int c[kMaxNbrs];
+  bool touch[kMaxNbrs] = {};
   int cnt = 0;
-  const int64_t o = pass ? cp_off[d] : 0;
-  for (int lb = 1; lb < l; ++lb) {
+  if (lb >= 1 && lb < l) {
     const int nd = anc[(size_t)d * kMaxDepth + lb];
     const int ns = sl_cnt[nd];
-    int c[kMaxNbrs];
 #pragma unroll
     for (int j = 0; j < kMaxNbrs; ++j) {
       const int bb = j < ns ? sl_nbr[kMaxNbrs * (size_t)nd + j] : -1;
@@ -543,13 +566,26 @@ __global__ void k_coarse(int nnodes, const int* anc, const int* level, const int
 #undef KI_CE
 #pragma unroll
     for (int i = 0; i < kMaxNbrs; ++i) {
-      if (c[i] != INT_MAX && wall_touch(corners + 8 * (size_t)c[i], dc)) {
-        if (pass) cp_nbr[o + cnt] = c[i];
-        ++cnt;
-      }
+      touch[i] = c[i] != INT_MAX && wall_touch(corners + 8 * (size_t)c[i], dc);
+      cnt += touch[i];
     }
   }
-  if (!pass) cp_cnt[d] = cnt;
+  // exclusive prefix of cnt over the node's lanes
+  int incl = cnt;
+  for (int k = 1; k < kMaxDepth; k <<= 1) {
+    const int t = __shfl_up(incl, k, kMaxDepth);
+    if (lb >= k) incl += t;
+  }
+  if (d >= nnodes) return;
+  if (!pass) {
+    if (lb == kMaxDepth - 1) cp_cnt[d] = incl;
+    return;
+  }
+  if (cnt == 0) return;
+  int64_t o = cp_off[d] + (incl - cnt);
+#pragma unroll
+  for (int i = 0; i < kMaxNbrs; ++i)
+    if (touch[i]) cp_nbr[o++] = c[i];
 }
 
 struct LevelBuf {
@@ -852,16 +888,16 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin) {
   t.status.zero(s);
   for (int l = 0; l < t.nlevels; ++l) {
     int a = (int)t.level_off[l], b = (int)t.level_off[l + 1];
-    k_same_level<<<ceil_div(b - a, kTB), kTB, 0, s>>>(a, b, t.parent.p, t.slot.p, t.child.p,
-                                                       t.is_leaf.p, t.level.p, t.corners.p,
-                                                       t.side.p, t.sl_nbr.p, t.sl_cnt.p,
-                                                       t.status.p);
+    k_same_level<<<ceil_div((b - a) * kSlLanes, kTB), kTB, 0, s>>>(
+        a, b, t.parent.p, t.slot.p, t.child.p, t.is_leaf.p, t.level.p, t.corners.p, t.side.p,
+        t.sl_nbr.p, t.sl_cnt.p, t.status.p);
   }
   DBuf<int> cpc;
   cpc.alloc(nn);
   t.cp_off.alloc(nn + 1);
-  k_coarse<<<ceil_div(nn, kTB), kTB, 0, s>>>(nn, anc.p, t.level.p, t.is_leaf.p, t.corners.p,
-                                              t.sl_nbr.p, t.sl_cnt.p, nullptr, cpc.p, nullptr, 0);
+  const int ncoarse = (int)ceil_div((int64_t)nn * kMaxDepth, kTB);
+  k_coarse<<<ncoarse, kTB, 0, s>>>(nn, anc.p, t.level.p, t.is_leaf.p, t.corners.p, t.sl_nbr.p,
+                                   t.sl_cnt.p, nullptr, cpc.p, nullptr, 0);
   exclusive_scan_i32_to_i64(cpc.p, t.cp_off.p, nn, s);
   DBuf<int64_t> cxf;
   cxf.alloc(2);
@@ -873,9 +909,8 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin) {
   if (hc[1]) KI_FAIL(KIGPU_ERR_STATE, "more than 8 same-level neighbors (overlapping boxes)");
   const int64_t ncp = hc[0];
   t.cp_nbr.alloc(std::max<int64_t>(ncp, 1));
-  k_coarse<<<ceil_div(nn, kTB), kTB, 0, s>>>(nn, anc.p, t.level.p, t.is_leaf.p, t.corners.p,
-                                              t.sl_nbr.p, t.sl_cnt.p, t.cp_off.p, cpc.p,
-                                              t.cp_nbr.p, 1);
+  k_coarse<<<ncoarse, kTB, 0, s>>>(nn, anc.p, t.level.p, t.is_leaf.p, t.corners.p, t.sl_nbr.p,
+                                   t.sl_cnt.p, t.cp_off.p, cpc.p, t.cp_nbr.p, 1);
 
   // ---- host mirrors the factorization needs (the rest on export), one copy
   // each into page-locked memory and one synchronization
