@@ -61,18 +61,47 @@ __device__ __forceinline__ int out_size(int y, const int* rank, const int* nact)
   return rank[y] >= 0 ? rank[y] : nact[y];
 }
 
-__global__ void k_active_size(int a, int b, const int* is_leaf, const int* child,
-                              const int* pcount, int sd, int* nact, int* rank) {
-  int x = a + blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= b) return;
-  int n = 0;
-  if (is_leaf[x]) {
-    n = pcount[x] * sd;
-  } else {
-    for (int s = 0; s < 4; ++s) n += out_size(child[4 * x + s], rank, nact);
+// active sizes of level [a, b) and their exclusive prefix sums from `add`
+// (the level's offsets in act_pool); one workgroup, chunks of kLsTB nodes.
+// The sizes also go to host_nact (page-locked, device-visible).
+constexpr int kLsTB = 1024;
+__global__ __launch_bounds__(kLsTB) void k_active_size(int a, int b, const int* is_leaf,
+                                                       const int* child, const int* pcount,
+                                                       int sd, int* nact, int* rank,
+                                                       int64_t* act_off, int64_t add,
+                                                       int* host_nact) {
+  __shared__ int64_t wtot[kLsTB / kWave];
+  __shared__ int64_t carry;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  if (tid == 0) carry = add;
+  __syncthreads();
+  for (int base = a; base < b; base += kLsTB) {
+    const int x = base + tid;
+    int n = 0;
+    if (x < b) {
+      if (is_leaf[x]) {
+        n = pcount[x] * sd;
+      } else {
+        for (int s = 0; s < 4; ++s) n += out_size(child[4 * x + s], rank, nact);
+      }
+      nact[x] = n;
+      rank[x] = -1;
+      if (host_nact) host_nact[x - a] = n;
+    }
+    int64_t incl = n;
+    for (int d = 1; d < kWave; d <<= 1) {
+      const int64_t t = __shfl_up(incl, d, kWave);
+      if (lane >= d) incl += t;
+    }
+    if (lane == kWave - 1) wtot[wv] = incl;
+    __syncthreads();
+    int64_t before = carry;
+    for (int w = 0; w < wv; ++w) before += wtot[w];
+    if (x < b) act_off[x] = before + incl - n;
+    __syncthreads();
+    if (tid == kLsTB - 1) carry = before + incl;
+    __syncthreads();
   }
-  nact[x] = n;
-  rank[x] = -1;
 }
 
 // the list a node hands to its parent: skel (compressed) or active_box
@@ -123,13 +152,17 @@ __global__ void k_inner_rows(int a, int b, int min_dofs, GeomView g, const int* 
                              const int64_t* act_off, const int* nact, const int* act_pool,
                              const int* sl_nbr, const int* sl_cnt, const int64_t* cp_off,
                              const int* cp_nbr, const double* corners, const double* side,
-                             int* nrows, const int64_t* rows_off, int* rows_pool, int pass) {
+                             int* nrows, const int64_t* rows_off, int* rows_pool, int pass,
+                             int* host_rows) {
 #pragma clang fp contract(off)
   int x = a + blockIdx.x;
   if (x >= b) return;
   const int lane = threadIdx.x;
   if (nact[x] < min_dofs) {
-    if (!pass && lane == 0) nrows[x] = 0;
+    if (!pass && lane == 0) {
+      nrows[x] = 0;
+      if (host_rows) host_rows[x - a] = 0;
+    }
     return;
   }
   const double cx = corners[8 * x] + side[x] / 2.0;
@@ -158,7 +191,10 @@ __global__ void k_inner_rows(int a, int b, int min_dofs, GeomView g, const int* 
       cnt += __popcll(m);
     }
   }
-  if (!pass && lane == 0) nrows[x] = cnt;
+  if (!pass && lane == 0) {
+    nrows[x] = cnt;
+    if (host_rows) host_rows[x - a] = cnt;
+  }
 }
 
 // outside rows for levels 1 and 2 (kernel.cpp:164-207): every other node's
@@ -1126,24 +1162,20 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     E = LevelEv{};
     E.idl0 = E.idl1 = -1;
     E.l0 = mark();
-    k_active_size<<<ceil_div(cn, kTB), kTB, 0, s>>>(a, b, t.is_leaf.p, t.child.p, t.pcount.p,
-                                                   t.sd, t.nact.p, t.rank.p);
-    exclusive_scan_i32_to_i64(t.nact.p + a, t.act_off.p + a, cn, s, pool_base);
+    // sizes and row counts go straight to the page-locked block (the level
+    // above's were read before this call)
+    int* hp = l > 0 ? c.pin_down.get<int>(2 * (size_t)cn, s) : nullptr;
+    int* dp = l > 0 ? c.pin_down.dget<int>() : nullptr;
+    k_active_size<<<1, kLsTB, 0, s>>>(a, b, t.is_leaf.p, t.child.p, t.pcount.p, t.sd, t.nact.p,
+                                      t.rank.p, t.act_off.p, pool_base, dp);
     k_active_fill<<<cn, 64, 0, s>>>(a, t.is_leaf.p, t.child.p, t.obox_off.p, t.obox_pool.p,
                                     t.rank.p, t.act_off.p, t.nact.p, t.act_pool.p, t.sr_pool.p);
-    int* hp = nullptr;
-    if (l > 0) {
-      if (l >= 3)
-        k_inner_rows<<<cn, kWave, 0, s>>>(a, b, o.min_dofs, g, t.is_leaf.p, t.obox_off.p,
-                                          t.obox_pool.p, t.pcount.p, t.act_off.p, t.nact.p,
-                                          t.act_pool.p, t.sl_nbr.p, t.sl_cnt.p, t.cp_off.p,
-                                          t.cp_nbr.p, t.corners.p, t.side.p, t.nrows.p, nullptr,
-                                          nullptr, 0);
-      hp = c.pin_down.get<int>(2 * (size_t)cn, s);
-      copy_to_host(c.pin_down, s,
-                   {{t.nact.p + a, 0, sizeof(int) * cn},
-                    {t.nrows.p + a, sizeof(int) * cn, l >= 3 ? sizeof(int) * cn : 0}});
-    }
+    if (l >= 3)
+      k_inner_rows<<<cn, kWave, 0, s>>>(a, b, o.min_dofs, g, t.is_leaf.p, t.obox_off.p,
+                                        t.obox_pool.p, t.pcount.p, t.act_off.p, t.nact.p,
+                                        t.act_pool.p, t.sl_nbr.p, t.sl_cnt.p, t.cp_off.p,
+                                        t.cp_nbr.p, t.corners.p, t.side.p, t.nrows.p, nullptr,
+                                        nullptr, 0, dp + cn);
     E.l1 = mark();
     E.pre = E.l1;
     return hp;
@@ -1263,7 +1295,7 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
                                         t.obox_pool.p, t.pcount.p, t.act_off.p, t.nact.p,
                                         t.act_pool.p, t.sl_nbr.p, t.sl_cnt.p, t.cp_off.p,
                                         t.cp_nbr.p, t.corners.p, t.side.p, t.nrows.p,
-                                        d_roff - a, lw->rows.p, 1);
+                                        d_roff - a, lw->rows.p, 1, nullptr);
     }
     if (l <= 2 && nwl > 0)
       k_outside_rows<<<nwl, kTB, 0, s>>>(
