@@ -47,9 +47,16 @@ __global__ void k_obox(int nnodes, const int* is_leaf, const int* pcount, const 
     for (int c = 0; c < sd; ++c) out[sd * j + c] = sd * pp[j] + c;
 }
 
-__global__ void k_leaf_dofs(int nnodes, const int* is_leaf, const int* pcount, int sd, int* cnt) {
+// leaf DOF counts, and the per-node factor records and status word cleared
+__global__ void k_factor_init(int nnodes, const int* is_leaf, const int* pcount, int sd, int* cnt,
+                              uint64_t* fac_ptr, uint64_t* acc_ptr, int* osize, int* status) {
   int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x < nnodes) cnt[x] = is_leaf[x] ? pcount[x] * sd : 0;
+  if (x >= nnodes) return;
+  cnt[x] = is_leaf[x] ? pcount[x] * sd : 0;
+  fac_ptr[x] = 0;
+  acc_ptr[x] = 0;
+  osize[x] = 0;
+  if (x == 0) *status = 0;
 }
 
 // |active_box| (remove_inactive_dofs_at_box, quadtree.cpp:694-720)
@@ -1117,12 +1124,6 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
   DBuf<int> cnt;
   cnt.alloc(nn);
   t.obox_off.alloc(nn);
-  k_leaf_dofs<<<ceil_div(nn, kTB), kTB, 0, s>>>(nn, t.is_leaf.p, t.pcount.p, t.sd, cnt.p);
-  exclusive_scan_i32_to_i64(cnt.p, t.obox_off.p, nn, s);
-  t.obox_pool.alloc(t.ndof);
-  k_obox<<<nn, 64, 0, s>>>(nn, t.is_leaf.p, t.pcount.p, t.pstart.p, t.perm_all.p, t.obox_off.p,
-                           t.sd, t.obox_pool.p);
-
   t.act_off.alloc(nn);
   t.nact.alloc(nn);
   t.nrows.alloc(nn);
@@ -1131,11 +1132,13 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
   t.fac_ptr.alloc(nn);
   t.acc_ptr.alloc(nn);
   t.acc_ld.alloc(nn);
-  t.fac_ptr.zero(s);
-  t.acc_ptr.zero(s);
-  t.osize.zero(s);
   t.status.alloc(1);
-  t.status.zero(s);
+  k_factor_init<<<ceil_div(nn, kTB), kTB, 0, s>>>(nn, t.is_leaf.p, t.pcount.p, t.sd, cnt.p,
+                                                 t.fac_ptr.p, t.acc_ptr.p, t.osize.p, t.status.p);
+  exclusive_scan_i32_to_i64(cnt.p, t.obox_off.p, nn, s);
+  t.obox_pool.alloc(t.ndof);
+  k_obox<<<nn, 64, 0, s>>>(nn, t.is_leaf.p, t.pcount.p, t.pstart.p, t.perm_all.p, t.obox_off.p,
+                           t.sd, t.obox_pool.p);
   const int64_t pool = (int64_t)t.ndof * t.nlevels + 64;
   t.act_pool.alloc(pool);
   t.sr_pool.alloc(pool);

@@ -161,34 +161,52 @@ __global__ __launch_bounds__(1024) void k_rank_sort(const int64_t* seg, const in
                                                     const int* parent_st, int sd, int leaf,
                                                     int* split, int* st, const uint64_t* path,
                                                     int* rank, int* seg_cnt, int* seg_split,
-                                                    int64_t* next_seg, int* xfer) {
+                                                    int64_t* next_seg, int* xfer,
+                                                    int64_t* rank_pstart) {
   __shared__ uint64_t key[kSortMax];
   __shared__ int val[kSortMax];
+  __shared__ int s_wtot[1024 / kWave];
   __shared__ int s_cnt;
   const int base = (int)seg[blockIdx.x];
   const int c = (int)seg[blockIdx.x + 1] - base;
-  for (int i = threadIdx.x; i < c; i += 1024) {
-    const int x = base + i;
-    const int sp = (long)pcount[x] * sd > leaf;
-    int t = -1;
-    if (sp) {
-      t = perm[pstart[x] + leaf / sd];
-      if (parent_st) t = max(t, parent_st[parent_local[x]]);
-    }
-    split[x] = sp;
-    st[x] = t;
-  }
-  __syncthreads();
-  int P = 1;
-  while (P < c) P <<= 1;
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
   if (threadIdx.x == 0) s_cnt = 0;
-  for (int i = threadIdx.x; i < P; i += 1024) {
-    if (i < c && split[base + i]) {
-      key[i] = ((uint64_t)st[base + i] << 40) | path[base + i];
-    } else {
-      key[i] = ~0ull;
+  __syncthreads();
+  // split flags and times; the splitting nodes' keys compacted in node order
+  for (int i0 = 0; i0 < c; i0 += 1024) {
+    const int i = i0 + threadIdx.x;
+    const int x = base + i;
+    int sp = 0, t = -1;
+    if (i < c) {
+      sp = (long)pcount[x] * sd > leaf;
+      if (sp) {
+        t = perm[pstart[x] + leaf / sd];
+        if (parent_st) t = max(t, parent_st[parent_local[x]]);
+      }
+      split[x] = sp;
+      st[x] = t;
+      if (!sp) rank[x] = -1;
     }
-    val[i] = i;
+    const unsigned long long m = __ballot(sp);
+    if (lane == 0) s_wtot[wv] = __popcll(m);
+    __syncthreads();
+    int o = s_cnt;
+    for (int w = 0; w < wv; ++w) o += s_wtot[w];
+    if (sp) {
+      const int q = o + __popcll(m & ((1ull << lane) - 1));
+      key[q] = ((uint64_t)t << 40) | path[x];
+      val[q] = i;
+    }
+    __syncthreads();
+    if (threadIdx.x == 1023) s_cnt = o + __popcll(m);
+    __syncthreads();
+  }
+  const int ns = s_cnt;
+  int P = 1;
+  while (P < ns) P <<= 1;
+  for (int i = ns + threadIdx.x; i < P; i += 1024) {
+    key[i] = ~0ull;
+    val[i] = -1;
   }
   __syncthreads();
   for (int size = 2; size <= P; size <<= 1) {
@@ -210,17 +228,31 @@ __global__ __launch_bounds__(1024) void k_rank_sort(const int64_t* seg, const in
       __syncthreads();
     }
   }
-  int cnt = 0;
-  for (int i = threadIdx.x; i < P; i += 1024) {
-    if (key[i] != ~0ull) {
-      rank[base + val[i]] = i;
-      ++cnt;
-    } else if (val[i] < c) {
-      rank[base + val[i]] = -1;
+  for (int i = threadIdx.x; i < ns; i += 1024) rank[base + val[i]] = i;
+  if (gridDim.x == 1 && rank_pstart) {
+    // one instance: the offset of each splitting node's points among the
+    // next level's, i.e. the prefix of the point counts in rank order
+    int64_t carry = 0;
+    for (int i0 = 0; i0 < ns; i0 += 1024) {
+      const int i = i0 + threadIdx.x;
+      const int v = i < ns ? pcount[base + val[i]] : 0;
+      int incl = v;
+      for (int d = 1; d < kWave; d <<= 1) {
+        const int t = __shfl_up(incl, d, kWave);
+        if (lane >= d) incl += t;
+      }
+      if (lane == kWave - 1) s_wtot[wv] = incl;
+      __syncthreads();
+      int64_t o = carry, tot = 0;
+      for (int w = 0; w < 1024 / kWave; ++w) {
+        if (w < wv) o += s_wtot[w];
+        tot += s_wtot[w];
+      }
+      if (i < ns) rank_pstart[base + val[i]] = o + incl - v;
+      __syncthreads();
+      carry += tot;
     }
   }
-  if (cnt) atomicAdd(&s_cnt, cnt);
-  __syncthreads();
   if (threadIdx.x != 0) return;
   if (gridDim.x > 1) {
     seg_cnt[blockIdx.x] = s_cnt;
@@ -321,27 +353,21 @@ struct Q4 {
   int v[4];
 };
 
-__device__ __forceinline__ Q4 q4_add(const Q4& a, const Q4& b) {
-  Q4 r;
-  for (int i = 0; i < 4; ++i) r.v[i] = a.v[i] + b.v[i];
-  return r;
-}
-
 constexpr int kScanTB = 1024;
 
 // quadrant of each point position (-1: its node does not split; to qcode),
-// then the block-level inclusive scan of the one-hot quadrant flags; block
-// totals out
+// then per quadrant the number of earlier positions of the block with that
+// quadrant (wave ballots, then the waves' totals); block totals out
 __global__ __launch_bounds__(kScanTB) void k_q4_block(int np, const int* perm, const int* pnode,
                                                       const int* rank, const double* corners,
                                                       const double* px, const double* py,
                                                       int* qcode, Q4* pre, Q4* btot) {
-  __shared__ Q4 buf[kScanTB];
-  int p = blockIdx.x * kScanTB + threadIdx.x;
-  Q4 v = {{0, 0, 0, 0}};
+  __shared__ int wtot[kScanTB / kWave][4];
+  const int p = blockIdx.x * kScanTB + threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+  int q = -1;
   if (p < np) {
     const int nd = pnode[p];
-    int q = -1;
     if (rank[nd] >= 0) {
       double midx, midy;
       node_mid(corners + 8 * nd, &midx, &midy);
@@ -349,23 +375,25 @@ __global__ __launch_bounds__(kScanTB) void k_q4_block(int np, const int* perm, c
       q = quadrant(px[pt], py[pt], midx, midy);
     }
     qcode[p] = q;
-    if (q >= 0) v.v[q] = 1;
   }
-  buf[threadIdx.x] = v;
+  // per-quadrant counts: lanes below in the wave (ballots), then the waves
+  // below in the block
+  const unsigned long long below = (1ull << lane) - 1;
+  Q4 e;
+  for (int k = 0; k < 4; ++k) {
+    const unsigned long long m = __ballot(q == k);
+    e.v[k] = __popcll(m & below);
+    if (lane == 0) wtot[wv][k] = __popcll(m);
+  }
   __syncthreads();
-  for (int off = 1; off < kScanTB; off <<= 1) {
-    Q4 t = {{0, 0, 0, 0}};
-    if (threadIdx.x >= off) t = buf[threadIdx.x - off];
-    __syncthreads();
-    buf[threadIdx.x] = q4_add(buf[threadIdx.x], t);
-    __syncthreads();
+  for (int w = 0; w < wv; ++w)
+    for (int k = 0; k < 4; ++k) e.v[k] += wtot[w][k];
+  if (p < np) pre[p] = e;  // exclusive within the block
+  if (threadIdx.x == kScanTB - 1) {
+    Q4 tot = e;
+    if (q >= 0) tot.v[q] += 1;
+    btot[blockIdx.x] = tot;
   }
-  if (p < np) {  // exclusive within the block
-    Q4 e = buf[threadIdx.x];
-    for (int i = 0; i < 4; ++i) e.v[i] -= v.v[i];
-    pre[p] = e;
-  }
-  if (threadIdx.x == kScanTB - 1) btot[blockIdx.x] = buf[kScanTB - 1];
 }
 
 // pre += the exclusive prefix of the block totals (each block of kTB = 256
@@ -412,17 +440,29 @@ __global__ void k_q4_apply(int np, int nb, Q4* pre, const Q4* btot, Q4* total) {
 
 // records of the 4 children of every splitting node and its points per child
 // (counts between the node's first and one-past-last position)
+// (counts between the node's first and one-past-last position).  With
+// rank_pstart (the node's first point among the next level's) the children's
+// first points go to n_pstart as well.
 __global__ void k_children_counts(int c, const int* rank, const double* corners,
                                   const double* side, const uint64_t* path, const int* inst,
                                   const int64_t* pstart, const int* pcount, const Q4* pre,
                                   int* n_parent, int* n_slot, uint64_t* n_path,
-                                  double* n_corners, double* n_side, int* n_inst, int* ccnt) {
+                                  double* n_corners, double* n_side, int* n_inst, int* ccnt,
+                                  const int64_t* rank_pstart, int64_t* n_pstart) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c || rank[i] < 0) return;
   child_records(i, rank, corners, side, path, inst, n_parent, n_slot, n_path, n_corners, n_side,
                 n_inst);
   const Q4 a = pre[pstart[i]], b = pre[pstart[i] + pcount[i]];
-  for (int q = 0; q < 4; ++q) ccnt[4 * rank[i] + q] = b.v[q] - a.v[q];
+  int64_t o = rank_pstart ? rank_pstart[i] : 0;
+  for (int q = 0; q < 4; ++q) {
+    const int cq = b.v[q] - a.v[q];
+    ccnt[4 * rank[i] + q] = cq;
+    if (rank_pstart) {
+      n_pstart[4 * rank[i] + q] = o;
+      o += cq;
+    }
+  }
 }
 
 __global__ void k_scatter(int np, const int* perm, const int* pnode, const int* qcode,
@@ -745,12 +785,16 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin) {
     k_fill_inst<<<ni, kTB, 0, s>>>(t.d_inst_pt.p, r->pnode.p);
     lv.push_back(std::move(r));
   }
-  DBuf<int> seg_cnt, seg_split, qcode, ccnt, xfer;
+  DBuf<int> seg_cnt, seg_split, qcode, ccnt;
+  DBuf<int64_t> rank_pstart;
   seg_cnt.alloc(ni);
   seg_cnt.zero(s);
   seg_split.alloc(ni + 1);
-  xfer.alloc(ni + 1 + 4);  // seg_split | the level's quadrant totals (Q4)
-  Q4* qtot = reinterpret_cast<Q4*>(xfer.p + ni + 1);
+  // seg_split | the level's quadrant totals (Q4), written by the kernels
+  // straight into page-locked memory and read after each level's wait
+  int* hx = pin.get<int>(ni + 5, s);
+  int* xfer = pin.dget<int>();
+  Q4* qtot = reinterpret_cast<Q4*>(xfer + ni + 1);
   std::vector<int> h_split(ni + 1);
   DBuf<Q4> pre, btot;
   for (int l = 0;; ++l) {
@@ -766,11 +810,15 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin) {
     int64_t segmax = 0;
     for (int i = 0; i < ni; ++i) segmax = std::max(segmax, L.seg[i + 1] - L.seg[i]);
     const bool sorted = segmax <= kSortMax && l <= 20 && (int64_t)n < (1ll << 24);
-    if (sorted) {  // split flags and times inside; the segment scan too for one instance
+    // one sorted instance: the segment scan and the children's point
+    // offsets come out of the sort
+    const bool one = sorted && ni == 1;
+    if (one) rank_pstart.alloc(c);
+    if (sorted) {  // split flags and times inside
       k_rank_sort<<<ni, 1024, 0, s>>>(L.dseg.p, L.pcount.p, L.pstart.p, L.perm.p,
                                       L.parent_local.p, pst, t.sd, t.leaf_dofs, L.split.p,
                                       L.st.p, L.path.p, L.rank.p, seg_cnt.p, seg_split.p,
-                                      nx->dseg.p, xfer.p);
+                                      nx->dseg.p, xfer, one ? rank_pstart.p : nullptr);
     } else {
       k_split_info<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.pcount.p, L.pstart.p, L.perm.p,
                                                     L.parent_local.p, pst, t.sd, t.leaf_dofs,
@@ -778,7 +826,7 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin) {
       k_rank_pairs<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.inst.p, L.dseg.p, L.split.p, L.st.p,
                                                     L.path.p, L.rank.p, seg_cnt.p);
     }
-    if (!sorted || ni > 1) k_seg_scan<<<1, 64, 0, s>>>(ni, seg_cnt.p, seg_split.p, nx->dseg.p, xfer.p);
+    if (!one) k_seg_scan<<<1, 64, 0, s>>>(ni, seg_cnt.p, seg_split.p, nx->dseg.p, xfer);
     if (ni > 1) k_rank_offset<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.inst.p, seg_split.p, L.rank.p);
     // classify and scan the points of splitting nodes
     const int nb = ceil_div(np, kScanTB);
@@ -788,8 +836,6 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin) {
     k_q4_block<<<nb, kScanTB, 0, s>>>(np, L.perm.p, L.pnode.p, L.rank.p, L.corners.p, t.px.p,
                                       t.py.p, qcode.p, pre.p, btot.p);
     k_q4_apply<<<ceil_div(np + 1, kTB), kTB, 0, s>>>(np, nb, pre.p, btot.p, qtot);
-    int* hx = pin.get<int>(ni + 5, s);
-    copy_to_host(pin, s, {{xfer.p, 0, sizeof(int) * (ni + 5)}});
     stream_wait(s);
     ++*syncs;
     h_split.assign(hx, hx + ni + 1);
@@ -812,8 +858,9 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin) {
     N.inst.alloc(N.count);
     k_children_counts<<<ceil_div(c, kTB), kTB, 0, s>>>(
         c, L.rank.p, L.corners.p, L.side.p, L.path.p, L.inst.p, L.pstart.p, L.pcount.p, pre.p,
-        N.parent_local.p, N.slot.p, N.path.p, N.corners.p, N.side.p, N.inst.p, N.pcount.p);
-    exclusive_scan_i32_to_i64(N.pcount.p, N.pstart.p, N.count, s);
+        N.parent_local.p, N.slot.p, N.path.p, N.corners.p, N.side.p, N.inst.p, N.pcount.p,
+        one ? rank_pstart.p : nullptr, N.pstart.p);
+    if (!one) exclusive_scan_i32_to_i64(N.pcount.p, N.pstart.p, N.count, s);
     N.nperm = tot.v[0] + tot.v[1] + tot.v[2] + tot.v[3];
     N.perm.alloc(std::max(N.nperm, 1));
     N.pnode.alloc(std::max(N.nperm, 1));
