@@ -204,7 +204,7 @@ static kigpu_status tree_build_common(kigpu_ctx* ctx, int npts, int sd, int leaf
     KI_HIP(hipEventRecord(c.ev[10], c.stream));
     load(t, c.stream);
     c.tim.syncs = 0;
-    build_tree(t, c.stream, &c.tim.syncs, c.pin_down);
+    build_tree(t, c.stream, &c.tim.syncs, c.pin_down, c.evpool);
     KI_HIP(hipEventRecord(c.ev[11], c.stream));
     stream_wait(c.stream);
     float ms = 0;

@@ -99,6 +99,15 @@ inline void stream_wait(hipStream_t s) {
   KI_HIP(e);
 }
 
+// Grow an event pool to n events.
+inline void ensure_events(std::vector<hipEvent_t>& pool, size_t n) {
+  while (pool.size() < n) {
+    hipEvent_t e;
+    KI_HIP(hipEventCreate(&e));
+    pool.push_back(e);
+  }
+}
+
 // Wait, the same way, for the work queued before event ev only.
 inline void event_wait(hipEvent_t ev) {
   hipError_t e;
@@ -202,6 +211,8 @@ void exclusive_scan_i64(const int64_t* in, int64_t* out, int n, hipStream_t s);
 // out[i] = add + in[0] + ... + in[i - 1]
 void exclusive_scan_i32_to_i64(const int* in, int64_t* out, int n, hipStream_t s,
                                int64_t add = 0);
+// the same with the length *dn read on the device
+void exclusive_scan_i32_to_i64_dev(const int* in, int64_t* out, const int64_t* dn, hipStream_t s);
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 

@@ -47,13 +47,7 @@ struct Context {
   size_t lds_max = 65536;
   PinnedHost pin_down;      // per-level downloads
   PinnedHost pin_up;        // packed per-level uploads
-  void ensure_events(size_t n) {
-    while (evpool.size() < n) {
-      hipEvent_t e;
-      KI_HIP(hipEventCreate(&e));
-      evpool.push_back(e);
-    }
-  }
+  void ensure_events(size_t n) { kigpu::ensure_events(evpool, n); }
   ~Context() {
     for (auto e : evpool) (void)hipEventDestroy(e);
   }

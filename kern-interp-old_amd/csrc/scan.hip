@@ -11,9 +11,10 @@ constexpr int kScanTB = 1024;
 
 template <typename TIn>
 __global__ __launch_bounds__(kScanTB) void k_scan(const TIn* in, int64_t* out, int n,
-                                                  int64_t add) {
+                                                  int64_t add, const int64_t* dn) {
   __shared__ int64_t buf[kScanTB];
   __shared__ int64_t carry;
+  if (dn) n = (int)*dn;  // the length, when only the device has it
   if (threadIdx.x == 0) carry = add;
   __syncthreads();
   for (int base = 0; base < n; base += kScanTB) {
@@ -38,12 +39,16 @@ __global__ __launch_bounds__(kScanTB) void k_scan(const TIn* in, int64_t* out, i
 
 void exclusive_scan_i64(const int64_t* in, int64_t* out, int n, hipStream_t s) {
   if (n <= 0) return;
-  k_scan<int64_t><<<1, kScanTB, 0, s>>>(in, out, n, 0);
+  k_scan<int64_t><<<1, kScanTB, 0, s>>>(in, out, n, 0, nullptr);
 }
 
 void exclusive_scan_i32_to_i64(const int* in, int64_t* out, int n, hipStream_t s, int64_t add) {
   if (n <= 0) return;
-  k_scan<int><<<1, kScanTB, 0, s>>>(in, out, n, add);
+  k_scan<int><<<1, kScanTB, 0, s>>>(in, out, n, add, nullptr);
+}
+
+void exclusive_scan_i32_to_i64_dev(const int* in, int64_t* out, const int64_t* dn, hipStream_t s) {
+  k_scan<int><<<1, kScanTB, 0, s>>>(in, out, 0, 0, dn);
 }
 
 namespace {

@@ -130,7 +130,9 @@ struct TreeData {
 };
 
 // tree.hip
-void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin);
+// evs: an event pool (grown to the tree's depth bound) for the pipelined levels
+void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin,
+                std::vector<hipEvent_t>& evs);
 void ensure_tree_mirrors(TreeData& t);    // host copies used by the export calls
 void ensure_factor_mirrors(TreeData& t);
 

@@ -310,7 +310,9 @@ __global__ void k_seg_scan(int ninst, int* seg_cnt, int* seg_split, int64_t* nex
 }
 
 // instance-local rank -> rank within the level
-__global__ void k_rank_offset(int c, const int* inst, const int* seg_split, int* rank) {
+__global__ void k_rank_offset(int c, const int64_t* dc, const int* inst, const int* seg_split,
+                              int* rank) {
+  if (dc) c = (int)*dc;  // the level's node count, when only the device has it
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < c && rank[i] >= 0) rank[i] += seg_split[inst[i]];
 }
@@ -358,11 +360,13 @@ constexpr int kScanTB = 1024;
 // quadrant of each point position (-1: its node does not split; to qcode),
 // then per quadrant the number of earlier positions of the block with that
 // quadrant (wave ballots, then the waves' totals); block totals out
-__global__ __launch_bounds__(kScanTB) void k_q4_block(int np, const int* perm, const int* pnode,
+__global__ __launch_bounds__(kScanTB) void k_q4_block(int np, const int* dnp, const int* perm,
+                                                      const int* pnode,
                                                       const int* rank, const double* corners,
                                                       const double* px, const double* py,
                                                       int* qcode, Q4* pre, Q4* btot) {
   __shared__ int wtot[kScanTB / kWave][4];
+  if (dnp) np = *dnp;  // the level's point count, when only the device has it
   const int p = blockIdx.x * kScanTB + threadIdx.x;
   const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
   int q = -1;
@@ -399,8 +403,10 @@ __global__ __launch_bounds__(kScanTB) void k_q4_block(int np, const int* perm, c
 // pre += the exclusive prefix of the block totals (each block of kTB = 256
 // positions lies in one scan block b and sums btot[0 .. b-1] itself, in wave
 // order); the block holding position np also writes the level total
-__global__ void k_q4_apply(int np, int nb, Q4* pre, const Q4* btot, Q4* total) {
+__global__ void k_q4_apply(int np, const int* dnp, int nb, Q4* pre, const Q4* btot, Q4* total,
+                           int* np_next) {
   __shared__ int s_add[8];
+  if (dnp) np = *dnp;
   const int p0 = blockIdx.x * blockDim.x;
   const int p = p0 + threadIdx.x;
   const int b = p0 / kScanTB;
@@ -435,6 +441,7 @@ __global__ void k_q4_apply(int np, int nb, Q4* pre, const Q4* btot, Q4* total) {
     for (int q = 0; q < 4; ++q) v.v[q] = s_add[4 + q];
     pre[np] = v;
     *total = v;
+    if (np_next) *np_next = v.v[0] + v.v[1] + v.v[2] + v.v[3];
   }
 }
 
@@ -443,12 +450,13 @@ __global__ void k_q4_apply(int np, int nb, Q4* pre, const Q4* btot, Q4* total) {
 // (counts between the node's first and one-past-last position).  With
 // rank_pstart (the node's first point among the next level's) the children's
 // first points go to n_pstart as well.
-__global__ void k_children_counts(int c, const int* rank, const double* corners,
+__global__ void k_children_counts(int c, const int64_t* dc, const int* rank, const double* corners,
                                   const double* side, const uint64_t* path, const int* inst,
                                   const int64_t* pstart, const int* pcount, const Q4* pre,
                                   int* n_parent, int* n_slot, uint64_t* n_path,
                                   double* n_corners, double* n_side, int* n_inst, int* ccnt,
                                   const int64_t* rank_pstart, int64_t* n_pstart) {
+  if (dc) c = (int)*dc;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c || rank[i] < 0) return;
   child_records(i, rank, corners, side, path, inst, n_parent, n_slot, n_path, n_corners, n_side,
@@ -465,9 +473,10 @@ __global__ void k_children_counts(int c, const int* rank, const double* corners,
   }
 }
 
-__global__ void k_scatter(int np, const int* perm, const int* pnode, const int* qcode,
-                          const int* rank, const int64_t* pstart, const Q4* pre,
+__global__ void k_scatter(int np, const int* dnp, const int* perm, const int* pnode,
+                          const int* qcode, const int* rank, const int64_t* pstart, const Q4* pre,
                           const int64_t* cstart, int* nperm_out, int* npnode_out) {
+  if (dnp) np = *dnp;
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   const int q = qcode[p];
@@ -741,7 +750,8 @@ __global__ void k_mirror_out(int nn, int ni, const double* corners, const double
 
 }  // namespace
 
-void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin) {
+void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin,
+                std::vector<hipEvent_t>& evs) {
   const int n = t.npts;
   const int ni = t.ninst;
   if (t.inst_pt.empty()) t.inst_pt = {0, (int64_t)n};
@@ -760,114 +770,170 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin) {
     k_minmax_seg<<<ni, kTB, 0, s>>>(t.px.p, t.py.p, t.d_inst_pt.p, mm.p);
   }
 
-  std::vector<std::unique_ptr<LevelBuf>> lv;
-  {
-    auto r = std::make_unique<LevelBuf>();
-    r->count = ni;
-    r->seg.resize(ni + 1);
-    for (int i = 0; i <= ni; ++i) r->seg[i] = i;
-    r->parent_local.alloc(ni);
-    r->slot.alloc(ni);
-    r->pcount.alloc(ni);
-    r->pstart.alloc(ni);
-    r->path.alloc(ni);
-    r->corners.alloc(8 * (size_t)ni);
-    r->side.alloc(ni);
-    r->inst.alloc(ni);
-    r->dseg.alloc(ni + 1);
-    k_root_level<<<ceil_div(ni + 1, kTB), kTB, 0, s>>>(
-        ni, t.d_inst_pt.p, mm.p, r->parent_local.p, r->slot.p, r->pcount.p, r->pstart.p,
-        r->path.p, r->corners.p, r->side.p, r->inst.p, r->dseg.p);
-    r->perm.alloc(n);
-    r->pnode.alloc(n);
-    r->nperm = n;
-    k_iota<<<ceil_div(n, kTB), kTB, 0, s>>>(r->perm.p, n);
-    k_fill_inst<<<ni, kTB, 0, s>>>(t.d_inst_pt.p, r->pnode.p);
-    lv.push_back(std::move(r));
-  }
-  DBuf<int> seg_cnt, seg_split, qcode, ccnt;
+  std::vector<std::unique_ptr<LevelBuf>> lv, spare;
+  // Pipelined build: a splitting node holds at least q1 points, so below the
+  // root an instance has at most 4 floor(n_i / q1) nodes per level and a
+  // level at most `cap` nodes and n points.  When every instance's bound fits
+  // the sort, level l + 1 is queued with those bounds and the counts the
+  // kernels leave on the device, and the host reads level l's counts while it
+  // runs; a level queued past the last one finds no nodes and does nothing.
+  // Levels deeper than the sort takes (paths of 20 levels) restart the build
+  // with a wait per level.
+  const int q1 = t.leaf_dofs / t.sd + 1;
+  int64_t inst_bound = 1;
+  for (int i = 0; i < ni; ++i)
+    inst_bound = std::max<int64_t>(inst_bound, 4 * ((t.inst_pt[i + 1] - t.inst_pt[i]) / q1));
+  const int cap = (int)std::max<int64_t>(4 * ((int64_t)n / q1), 1);
+  bool pipe = inst_bound <= kSortMax && (int64_t)n < (1ll << 24);
+  const int slot = ni + 5;  // per level: seg_split (ni + 1) | quadrant totals (Q4)
+  int* hx = pin.get<int>((size_t)slot * kMaxDepth, s);
+  int* dx = pin.dget<int>();
+  if (pipe) ensure_events(evs, kMaxDepth);
+  DBuf<int> seg_cnt, seg_split, qcode, d_np;
   DBuf<int64_t> rank_pstart;
-  seg_cnt.alloc(ni);
-  seg_cnt.zero(s);
-  seg_split.alloc(ni + 1);
-  // seg_split | the level's quadrant totals (Q4), written by the kernels
-  // straight into page-locked memory and read after each level's wait
-  int* hx = pin.get<int>(ni + 5, s);
-  int* xfer = pin.dget<int>();
-  Q4* qtot = reinterpret_cast<Q4*>(xfer + ni + 1);
-  std::vector<int> h_split(ni + 1);
   DBuf<Q4> pre, btot;
-  for (int l = 0;; ++l) {
-    LevelBuf& L = *lv[l];
-    const int c = L.count;
-    const int np = L.nperm;
-    L.split.alloc(c);
-    L.st.alloc(c);
-    L.rank.alloc(c);
-    auto nx = std::make_unique<LevelBuf>();
-    nx->dseg.alloc(ni + 1);
-    const int* pst = l ? lv[l - 1]->st.p : nullptr;
-    int64_t segmax = 0;
-    for (int i = 0; i < ni; ++i) segmax = std::max(segmax, L.seg[i + 1] - L.seg[i]);
-    const bool sorted = segmax <= kSortMax && l <= 20 && (int64_t)n < (1ll << 24);
-    // one sorted instance: the segment scan and the children's point
-    // offsets come out of the sort
-    const bool one = sorted && ni == 1;
-    if (one) rank_pstart.alloc(c);
-    if (sorted) {  // split flags and times inside
-      k_rank_sort<<<ni, 1024, 0, s>>>(L.dseg.p, L.pcount.p, L.pstart.p, L.perm.p,
-                                      L.parent_local.p, pst, t.sd, t.leaf_dofs, L.split.p,
-                                      L.st.p, L.path.p, L.rank.p, seg_cnt.p, seg_split.p,
-                                      nx->dseg.p, xfer, one ? rank_pstart.p : nullptr);
-    } else {
-      k_split_info<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.pcount.p, L.pstart.p, L.perm.p,
-                                                    L.parent_local.p, pst, t.sd, t.leaf_dofs,
-                                                    L.split.p, L.st.p);
-      k_rank_pairs<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.inst.p, L.dseg.p, L.split.p, L.st.p,
-                                                    L.path.p, L.rank.p, seg_cnt.p);
+  seg_split.alloc(ni + 1);
+  d_np.alloc(kMaxDepth + 1);  // points per level (pipelined)
+  for (int attempt = 0;; ++attempt) {
+    lv.clear();
+    seg_cnt.alloc(ni);
+    seg_cnt.zero(s);
+    {
+      auto r = std::make_unique<LevelBuf>();
+      r->count = ni;
+      r->seg.resize(ni + 1);
+      for (int i = 0; i <= ni; ++i) r->seg[i] = i;
+      r->parent_local.alloc(ni);
+      r->slot.alloc(ni);
+      r->pcount.alloc(ni);
+      r->pstart.alloc(ni);
+      r->path.alloc(ni);
+      r->corners.alloc(8 * (size_t)ni);
+      r->side.alloc(ni);
+      r->inst.alloc(ni);
+      r->dseg.alloc(ni + 1);
+      k_root_level<<<ceil_div(ni + 1, kTB), kTB, 0, s>>>(
+          ni, t.d_inst_pt.p, mm.p, r->parent_local.p, r->slot.p, r->pcount.p, r->pstart.p,
+          r->path.p, r->corners.p, r->side.p, r->inst.p, r->dseg.p);
+      r->perm.alloc(n);
+      r->pnode.alloc(n);
+      r->nperm = n;
+      k_iota<<<ceil_div(n, kTB), kTB, 0, s>>>(r->perm.p, n);
+      k_fill_inst<<<ni, kTB, 0, s>>>(t.d_inst_pt.p, r->pnode.p);
+      lv.push_back(std::move(r));
     }
-    if (!one) k_seg_scan<<<1, 64, 0, s>>>(ni, seg_cnt.p, seg_split.p, nx->dseg.p, xfer);
-    if (ni > 1) k_rank_offset<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.inst.p, seg_split.p, L.rank.p);
-    // classify and scan the points of splitting nodes
-    const int nb = ceil_div(np, kScanTB);
-    qcode.alloc(np);
-    pre.alloc((size_t)np + 1);
-    btot.alloc(nb);
-    k_q4_block<<<nb, kScanTB, 0, s>>>(np, L.perm.p, L.pnode.p, L.rank.p, L.corners.p, t.px.p,
-                                      t.py.p, qcode.p, pre.p, btot.p);
-    k_q4_apply<<<ceil_div(np + 1, kTB), kTB, 0, s>>>(np, nb, pre.p, btot.p, qtot);
-    stream_wait(s);
-    ++*syncs;
-    h_split.assign(hx, hx + ni + 1);
-    Q4 tot;
-    for (int q = 0; q < 4; ++q) tot.v[q] = hx[ni + 1 + q];
-    const int S = h_split[ni];
-    if (S == 0) break;
-    if (l >= 30) KI_FAIL(KIGPU_ERR_NOTSUPP, "quadtree deeper than 31 levels");
-    LevelBuf& N = *nx;
-    N.count = 4 * S;
-    N.seg.resize(ni + 1);
-    for (int i = 0; i <= ni; ++i) N.seg[i] = 4 * (int64_t)h_split[i];
-    N.parent_local.alloc(N.count);
-    N.slot.alloc(N.count);
-    N.path.alloc(N.count);
-    N.corners.alloc(8 * (size_t)N.count);
-    N.side.alloc(N.count);
-    N.pcount.alloc(N.count);
-    N.pstart.alloc(N.count);
-    N.inst.alloc(N.count);
-    k_children_counts<<<ceil_div(c, kTB), kTB, 0, s>>>(
-        c, L.rank.p, L.corners.p, L.side.p, L.path.p, L.inst.p, L.pstart.p, L.pcount.p, pre.p,
-        N.parent_local.p, N.slot.p, N.path.p, N.corners.p, N.side.p, N.inst.p, N.pcount.p,
-        one ? rank_pstart.p : nullptr, N.pstart.p);
-    if (!one) exclusive_scan_i32_to_i64(N.pcount.p, N.pstart.p, N.count, s);
-    N.nperm = tot.v[0] + tot.v[1] + tot.v[2] + tot.v[3];
-    N.perm.alloc(std::max(N.nperm, 1));
-    N.pnode.alloc(std::max(N.nperm, 1));
-    k_scatter<<<ceil_div(np, kTB), kTB, 0, s>>>(np, L.perm.p, L.pnode.p, qcode.p, L.rank.p,
-                                                L.pstart.p, pre.p, N.pstart.p, N.perm.p,
-                                                N.pnode.p);
-    lv.push_back(std::move(nx));
+    bool restart = false;
+    // host copy of level l's counts (its slot): next level's segments and points
+    auto read_slot = [&](int l, LevelBuf& N) {
+      const int* h = hx + (size_t)slot * l;
+      N.count = 4 * h[ni];
+      N.seg.resize(ni + 1);
+      for (int i = 0; i <= ni; ++i) N.seg[i] = 4 * (int64_t)h[i];
+      N.nperm = h[ni + 1] + h[ni + 2] + h[ni + 3] + h[ni + 4];
+      return h[ni];
+    };
+    for (int l = 0;; ++l) {
+      LevelBuf& L = *lv[l];
+      if (pipe && l > 20) {  // beyond the sort's path length
+        stream_wait(s);
+        restart = true;
+        break;
+      }
+      if (l >= kMaxDepth - 1) KI_FAIL(KIGPU_ERR_NOTSUPP, "quadtree deeper than 31 levels");
+      const bool dev = pipe && l > 0;  // this level's counts are on the device only
+      const int c = dev ? cap : L.count;
+      const int np = dev ? n : L.nperm;
+      const int* dnp = dev ? d_np.p + l : nullptr;
+      const int64_t* dc = dev ? L.dseg.p + ni : nullptr;
+      int* xf = dx + (size_t)slot * l;
+      Q4* qtot = reinterpret_cast<Q4*>(xf + ni + 1);
+      L.split.alloc(c);
+      L.st.alloc(c);
+      L.rank.alloc(c);
+      auto nx = std::make_unique<LevelBuf>();
+      nx->dseg.alloc(ni + 1);
+      const int* pst = l ? lv[l - 1]->st.p : nullptr;
+      bool sorted = pipe;
+      if (!pipe) {
+        int64_t segmax = 0;
+        for (int i = 0; i < ni; ++i) segmax = std::max(segmax, L.seg[i + 1] - L.seg[i]);
+        sorted = segmax <= kSortMax && l <= 20 && (int64_t)n < (1ll << 24);
+      }
+      // one sorted instance: the segment scan and the children's point
+      // offsets come out of the sort
+      const bool one = sorted && ni == 1;
+      if (one) rank_pstart.alloc(c);
+      if (sorted) {  // split flags and times inside
+        k_rank_sort<<<ni, 1024, 0, s>>>(L.dseg.p, L.pcount.p, L.pstart.p, L.perm.p,
+                                        L.parent_local.p, pst, t.sd, t.leaf_dofs, L.split.p,
+                                        L.st.p, L.path.p, L.rank.p, seg_cnt.p, seg_split.p,
+                                        nx->dseg.p, xf, one ? rank_pstart.p : nullptr);
+      } else {
+        k_split_info<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.pcount.p, L.pstart.p, L.perm.p,
+                                                      L.parent_local.p, pst, t.sd, t.leaf_dofs,
+                                                      L.split.p, L.st.p);
+        k_rank_pairs<<<ceil_div(c, kTB), kTB, 0, s>>>(c, L.inst.p, L.dseg.p, L.split.p, L.st.p,
+                                                      L.path.p, L.rank.p, seg_cnt.p);
+      }
+      if (!one) k_seg_scan<<<1, 64, 0, s>>>(ni, seg_cnt.p, seg_split.p, nx->dseg.p, xf);
+      if (ni > 1)
+        k_rank_offset<<<ceil_div(c, kTB), kTB, 0, s>>>(c, dc, L.inst.p, seg_split.p, L.rank.p);
+      // classify and scan the points of splitting nodes
+      const int nb = ceil_div(std::max(np, 1), kScanTB);
+      qcode.alloc(np);
+      pre.alloc((size_t)np + 1);
+      btot.alloc(nb);
+      k_q4_block<<<nb, kScanTB, 0, s>>>(np, dnp, L.perm.p, L.pnode.p, L.rank.p, L.corners.p,
+                                        t.px.p, t.py.p, qcode.p, pre.p, btot.p);
+      k_q4_apply<<<ceil_div(np + 1, kTB), kTB, 0, s>>>(np, dnp, nb, pre.p, btot.p, qtot,
+                                                       pipe ? d_np.p + l + 1 : nullptr);
+      LevelBuf& N = *nx;
+      if (!pipe) {
+        stream_wait(s);
+        ++*syncs;
+        if (read_slot(l, N) == 0) break;
+      } else {
+        N.count = cap;  // bounds until the host reads the level's slot
+        N.nperm = n;
+      }
+      N.parent_local.alloc(N.count);
+      N.slot.alloc(N.count);
+      N.path.alloc(N.count);
+      N.corners.alloc(8 * (size_t)N.count);
+      N.side.alloc(N.count);
+      N.pcount.alloc(N.count);
+      N.pstart.alloc(N.count);
+      N.inst.alloc(N.count);
+      k_children_counts<<<ceil_div(c, kTB), kTB, 0, s>>>(
+          c, dc, L.rank.p, L.corners.p, L.side.p, L.path.p, L.inst.p, L.pstart.p, L.pcount.p,
+          pre.p, N.parent_local.p, N.slot.p, N.path.p, N.corners.p, N.side.p, N.inst.p,
+          N.pcount.p, one ? rank_pstart.p : nullptr, N.pstart.p);
+      if (!one) {
+        if (pipe) exclusive_scan_i32_to_i64_dev(N.pcount.p, N.pstart.p, N.dseg.p + ni, s);
+        else exclusive_scan_i32_to_i64(N.pcount.p, N.pstart.p, N.count, s);
+      }
+      N.perm.alloc(std::max(N.nperm, 1));
+      N.pnode.alloc(std::max(N.nperm, 1));
+      k_scatter<<<ceil_div(np, kTB), kTB, 0, s>>>(np, dnp, L.perm.p, L.pnode.p, qcode.p,
+                                                  L.rank.p, L.pstart.p, pre.p, N.pstart.p,
+                                                  N.perm.p, N.pnode.p);
+      lv.push_back(std::move(nx));
+      if (pipe) {
+        KI_HIP(hipEventRecord(evs[l], s));
+        if (l > 0) {  // level l - 1's counts, while level l runs
+          event_wait(evs[l - 1]);
+          ++*syncs;
+          if (read_slot(l - 1, *lv[l]) == 0) {  // level l has no nodes
+            // its buffers stay alive until the queued no-op kernels are done
+            for (size_t k = l; k < lv.size(); ++k) spare.push_back(std::move(lv[k]));
+            lv.resize(l);
+            break;
+          }
+        }
+      }
+    }
+    if (!restart) break;
+    pipe = false;
   }
 
   // ---- flatten to global node ids ----

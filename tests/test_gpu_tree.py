@@ -1,5 +1,7 @@
 """GPU quadtree vs the sequential-insertion oracle: bit-exact node order,
 corners, DOF lists and neighbor lists (quadtree.cpp:27-285)."""
+import types
+
 import numpy as np
 import pytest
 
@@ -8,8 +10,22 @@ from oracle.tree import QuadTree
 
 from helpers import gpu_tree
 
+def clustered():
+    # 48 points within 1e-7 of (0.1, 0.2) inside a unit circle of 64: the
+    # tree runs deeper than 20 levels (the pipelined build's sort limit),
+    # so the build restarts with a wait per level
+    b = G.circle(64)
+    i = np.arange(48)
+    extra = np.stack([0.1 + 2e-9 * i, 0.2 + 1.5e-9 * (i % 7)], 1)
+    pts = np.concatenate([b.points.reshape(-1, 2), extra]).reshape(-1)
+    n = pts.size // 2
+    return types.SimpleNamespace(points=pts, normals=np.tile([1.0, 0.0], n),
+                                 weights=np.full(n, 1.0 / n), curvatures=np.zeros(n))
+
+
 CASES = [
     ("star4096_leaf64", lambda: G.star(4096), 1, 64),
+    ("clustered_deep", clustered, 1, 32),
     ("circle1024_leaf32", lambda: G.circle(1024), 1, 32),
     ("annulus1024_leaf32", lambda: G.annulus(1024), 1, 32),
     ("circle1024_stokes_leaf32", lambda: G.circle(1024), 2, 32),
