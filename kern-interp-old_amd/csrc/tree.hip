@@ -488,62 +488,6 @@ __global__ void k_scatter(int np, const int* dnp, const int* perm, const int* pn
   npnode_out[dst] = ch;
 }
 
-// same-level neighbor lists of level [a, b): siblings, then cousins
-// (compute_neighbor_lists, quadtree.cpp:33-50).  kMaxNbrs lanes per node:
-// lane j tests the children of the parent's j-th neighbor, and a prefix sum
-// over the lanes puts the cousins in the serial loop's (j, s) order.
-constexpr int kSlLanes = kMaxNbrs;
-static_assert(kSlLanes == 8 && kWave % kSlLanes == 0, "8 lanes per node");
-__global__ void k_same_level(int a, int b, const int* parent, const int* slot, const int* child,
-                             const int* is_leaf, const int* level, const double* corners,
-                             const double* side, int* sl_nbr, int* sl_cnt, int* status) {
-#pragma clang fp contract(off)
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int x = a + gid / kSlLanes;
-  const int j = gid % kSlLanes;
-  const int p = x < b ? parent[x] : -1;
-  int found[4];
-  int nf = 0;
-  if (p >= 0 && j < sl_cnt[p]) {
-    const int y = sl_nbr[kMaxNbrs * p + j];
-    if (!is_leaf[y]) {
-      const double thr = side[x] * sqrt(2.0) + 1e-5;
-      const double x0 = corners[8 * x], y0 = corners[8 * x + 1];
-      const int lx = level[x];
-      for (int s = 0; s < 4; ++s) {
-        const int z = child[4 * y + s];
-        if (z < 0 || level[z] != lx) continue;
-        const double dx = x0 - corners[8 * z];
-        const double dy = y0 - corners[8 * z + 1];
-        if (sqrt(dx * dx + dy * dy) < thr) found[nf++] = z;
-      }
-    }
-  }
-  // exclusive prefix of nf over the node's lanes
-  int incl = nf;
-  for (int d = 1; d < kSlLanes; d <<= 1) {
-    const int t = __shfl_up(incl, d, kSlLanes);
-    if (j >= d) incl += t;
-  }
-  const int total = __shfl(incl, kSlLanes - 1, kSlLanes);
-  if (p < 0) {
-    if (x < b && j == 0) sl_cnt[x] = 0;
-    return;
-  }
-  int* out = sl_nbr + kMaxNbrs * x;
-  if (j < 3) {  // the siblings first
-    const int sx = slot[x];
-    const int s = j < sx ? j : j + 1;
-    out[j] = child[4 * p + s];
-  }
-  const int o = 3 + incl - nf;
-  for (int i = 0; i < nf; ++i) {
-    if (o + i < kMaxNbrs) out[o + i] = found[i];
-    else atomicOr(status, 1);
-  }
-  if (j == 0) sl_cnt[x] = min(3 + total, kMaxNbrs);
-}
-
 // get_descendent_neighbors' wall test (quadtree.cpp:121-155) for big box B and
 // smaller box d, including the left/right test `y < top && y < bottom` (:143).
 __device__ __forceinline__ bool wall_touch(const double* bc, const double* dc) {
@@ -666,8 +610,69 @@ struct FlatArgs {
   double *cornersv, *sidev;
 };
 
-__global__ void k_flatten(FlatArgs f) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Same-level neighbor lists (compute_neighbor_lists, quadtree.cpp:33-50):
+// siblings, then cousins.  kSlLanes lanes per node: lane j tests the
+// children of the parent's j-th neighbor, and a prefix sum over the lanes
+// puts the cousins in the serial loop's (j, s) order.  The parent level's
+// flat records (child, is_leaf, lists) come from the previous launch; this
+// level's own records are read from the level's arrays, and a child of a
+// parent-level node is always on this level.
+constexpr int kSlLanes = kMaxNbrs;
+static_assert(kSlLanes == 8 && kWave % kSlLanes == 0, "8 lanes per node");
+__device__ __forceinline__ void same_level_lane(const FlatArgs& f, int i, int j, int* sl_nbr,
+                                                int* sl_cnt, int* status) {
+#pragma clang fp contract(off)
+  const int p = i < f.c && f.parent_local[i] >= 0 ? f.parent_base + f.parent_local[i] : -1;
+  int found[4];
+  int nf = 0;
+  if (p >= 0 && j < sl_cnt[p]) {
+    const int y = sl_nbr[kMaxNbrs * p + j];
+    if (!f.is_leaf[y]) {
+      const double thr = f.side[i] * sqrt(2.0) + 1e-5;
+      const double x0 = f.corners[8 * i], y0 = f.corners[8 * i + 1];
+      for (int s = 0; s < 4; ++s) {
+        const int z = f.child[4 * y + s];
+        if (z < 0) continue;
+        const int zl = z - f.base;
+        const double dx = x0 - f.corners[8 * zl];
+        const double dy = y0 - f.corners[8 * zl + 1];
+        if (sqrt(dx * dx + dy * dy) < thr) found[nf++] = z;
+      }
+    }
+  }
+  // exclusive prefix of nf over the node's lanes
+  int incl = nf;
+  for (int d = 1; d < kSlLanes; d <<= 1) {
+    const int t = __shfl_up(incl, d, kSlLanes);
+    if (j >= d) incl += t;
+  }
+  const int total = __shfl(incl, kSlLanes - 1, kSlLanes);
+  if (i >= f.c) return;
+  const int x = f.base + i;
+  if (p < 0) {
+    if (j == 0) sl_cnt[x] = 0;
+    return;
+  }
+  int* out = sl_nbr + kMaxNbrs * x;
+  if (j < 3) {  // the siblings first
+    const int sx = f.slot[i];
+    const int s = j < sx ? j : j + 1;
+    out[j] = f.child[4 * p + s];
+  }
+  const int o = 3 + incl - nf;
+  for (int k = 0; k < nf; ++k) {
+    if (o + k < kMaxNbrs) out[o + k] = found[k];
+    else atomicOr(status, 1);
+  }
+  if (j == 0) sl_cnt[x] = min(3 + total, kMaxNbrs);
+}
+
+// one launch per level: the level's point permutation and records into the
+// flat arrays, its ancestor rows, and its same-level neighbor lists
+__global__ void k_flatten(FlatArgs f, int* sl_nbr, int* sl_cnt, int* status) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  same_level_lane(f, g / kSlLanes, g % kSlLanes, sl_nbr, sl_cnt, status);
+  const int i = g;
   if (i < f.nperm) f.perm_all[f.pbase + i] = f.perm[i];
   if (i >= f.c) return;
   const int x = f.base + i;
@@ -957,6 +962,11 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin,
   t.perm_all.alloc(std::max<int64_t>(ptot, 1));
   DBuf<int> anc;
   anc.alloc((size_t)nn * kMaxDepth);
+  // neighbor lists, filled level by level with the flat records
+  t.sl_nbr.alloc(kMaxNbrs * (size_t)nn);
+  t.sl_cnt.alloc(nn);
+  t.status.alloc(1);
+  t.status.zero(s);
   int64_t pbase = 0;
   for (int l = 0; l < t.nlevels; ++l) {
     LevelBuf& L = *lv[l];
@@ -991,19 +1001,9 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin,
     f.perm = L.perm.p;
     f.perm_all = t.perm_all.p;
     f.anc = anc.p;
-    k_flatten<<<ceil_div(std::max(L.count, L.nperm), kTB), kTB, 0, s>>>(f);
+    const int64_t nthr = std::max<int64_t>((int64_t)kSlLanes * L.count, L.nperm);
+    k_flatten<<<ceil_div(nthr, kTB), kTB, 0, s>>>(f, t.sl_nbr.p, t.sl_cnt.p, t.status.p);
     pbase += L.nperm;
-  }
-  // ---- neighbor lists ----
-  t.sl_nbr.alloc(kMaxNbrs * (size_t)nn);
-  t.sl_cnt.alloc(nn);
-  t.status.alloc(1);
-  t.status.zero(s);
-  for (int l = 0; l < t.nlevels; ++l) {
-    int a = (int)t.level_off[l], b = (int)t.level_off[l + 1];
-    k_same_level<<<ceil_div((b - a) * kSlLanes, kTB), kTB, 0, s>>>(
-        a, b, t.parent.p, t.slot.p, t.child.p, t.is_leaf.p, t.level.p, t.corners.p, t.side.p,
-        t.sl_nbr.p, t.sl_cnt.p, t.status.p);
   }
   DBuf<int> cpc;
   cpc.alloc(nn);
