@@ -216,12 +216,30 @@ void exclusive_scan_i32_to_i64_dev(const int* in, int64_t* out, const int64_t* d
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 
-// allow a kernel to use `bytes` of dynamic LDS (above the default limit)
+// allow a kernel to use `bytes` of dynamic LDS (above the default limit).
+// The attribute is set only when it grows for that kernel on the current
+// device: the per-level launches ask again and again, and each set costs a
+// runtime call on the host between two launches.
 template <typename K>
 inline void allow_lds(K kern, size_t bytes) {
-  if (bytes > 0)
-    KI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  if (bytes == 0) return;
+  struct Key {
+    const void* f;
+    int dev;
+  };
+  thread_local std::vector<std::pair<Key, size_t>> granted;
+  int dev = 0;
+  KI_HIP(hipGetDevice(&dev));
+  const void* f = reinterpret_cast<const void*>(kern);
+  for (auto& g : granted)
+    if (g.first.f == f && g.first.dev == dev) {
+      if (bytes <= g.second) return;
+      KI_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+      g.second = bytes;
+      return;
+    }
+  KI_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  granted.push_back({Key{f, dev}, bytes});
 }
 
 }  // namespace kigpu

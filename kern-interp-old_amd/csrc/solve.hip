@@ -29,44 +29,62 @@ constexpr int kTB = 256;
 constexpr int kNW = kTB / kWave;
 constexpr int kRC = 16;  // right-hand sides per block
 
+// The threads that work on one node: the whole block, or one wavefront
+// (four children of a node side by side)
+struct BlockGroup {
+  int tid;
+  static constexpr int nt = kTB;
+  __device__ void sync() const { __syncthreads(); }
+};
+struct WaveGroup {
+  int tid;
+  static constexpr int nt = kWave;
+  __device__ void sync() const { wave_sync(); }
+};
+
+// Factor records the sweeps read, per node
+struct SweepArgs {
+  const int* rank;
+  const int* nact;
+  const int64_t* act_off;
+  const int* sr_pool;
+  const uint64_t* fac_ptr;
+};
+
+// One node of the upward pass on right-hand sides [c0, c0 + nc); v: LDS for
+// 2 n nc values.
 // mode 0: x_r -= T^T x_s; x_s -= L x_r        (sweeps of U in skeletonize, :169-185)
 // mode 1: the same, then x_r <- X_rr^{-1} x_r   (solve's upward pass with D^{-1})
 // mode 2: x_r -= T^T x_s; x_s -= U^T x_r       (sweeps of Psi^T, :187-204)
 // mode 3: mode 2, then x_r <- X_rr^{-T} x_r     (transposed solve's upward pass)
-template <typename T>
-__global__ __launch_bounds__(kTB) void k_up(int a, const int* rank, const int* nact,
-                                            const int64_t* act_off, const int* sr_pool,
-                                            const uint64_t* fac_ptr, T* x, int ldx, int nrhs,
-                                            int mode) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int node = a + blockIdx.x;
-  const int k = rank[node];
+template <typename T, typename G>
+__device__ __forceinline__ void up_node(const G& g, const SweepArgs& f, int node, T* x, int ldx,
+                                        int c0, int nc, int mode, T* v) {
+  const int k = f.rank[node];
   if (k < 0) return;
-  const int n = nact[node], r = n - k;
-  const int c0 = blockIdx.y * kRC, nc = min(kRC, nrhs - c0);
-  const int* dofs = sr_pool + act_off[node];
-  const T* Tm = reinterpret_cast<const T*>(fac_ptr[node]);
+  const int n = f.nact[node], r = n - k;
+  const int* dofs = f.sr_pool + f.act_off[node];
+  const T* Tm = reinterpret_cast<const T*>(f.fac_ptr[node]);
   const T* Lm = Tm + (size_t)k * r;
   const T* Xinv = Tm + 3 * (size_t)k * r;
-  T* v = reinterpret_cast<T*>(smem);
   T* tmp = v + (size_t)n * nc;
-  const int tid = threadIdx.x;
-  for (int e = tid; e < n * nc; e += kTB) {
+  const int tid = g.tid;
+  for (int e = tid; e < n * nc; e += G::nt) {
     int i = e % n, c = e / n;
     v[e] = x[dofs[i] + (size_t)(c0 + c) * ldx];
   }
-  __syncthreads();
-  for (int e = tid; e < r * nc; e += kTB) {  // x_r -= T^T x_s
+  g.sync();
+  for (int e = tid; e < r * nc; e += G::nt) {  // x_r -= T^T x_s
     int i = e % r, c = e / r;
     const T* vs = v + (size_t)c * n;
     T s = vs[k + i];
     for (int t = 0; t < k; ++t) s -= Tm[t + (size_t)i * k] * vs[t];
     v[k + i + (size_t)c * n] = s;
   }
-  __syncthreads();
+  g.sync();
   if (mode >= 2) {
     const T* Um = Tm + 2 * (size_t)k * r;  // r x k
-    for (int e = tid; e < k * nc; e += kTB) {  // x_s -= U^T x_r
+    for (int e = tid; e < k * nc; e += G::nt) {  // x_s -= U^T x_r
       int t = e % k, c = e / k;
       const T* vs = v + (size_t)c * n;
       T s = vs[t];
@@ -74,7 +92,7 @@ __global__ __launch_bounds__(kTB) void k_up(int a, const int* rank, const int* n
       v[t + (size_t)c * n] = s;
     }
   } else {
-    for (int e = tid; e < k * nc; e += kTB) {  // x_s -= L x_r
+    for (int e = tid; e < k * nc; e += G::nt) {  // x_s -= L x_r
       int t = e % k, c = e / k;
       const T* vs = v + (size_t)c * n;
       T s = vs[t];
@@ -82,7 +100,7 @@ __global__ __launch_bounds__(kTB) void k_up(int a, const int* rank, const int* n
       v[t + (size_t)c * n] = s;
     }
   }
-  for (int e = tid; e < r * nc; e += kTB) {  // x_r <- X_rr^{-1} x_r (mode 1, ^{-T} mode 3)
+  for (int e = tid; e < r * nc; e += G::nt) {  // x_r <- X_rr^{-1} x_r (mode 1, ^{-T} mode 3)
     int i = e % r, c = e / r;
     const T* vs = v + (size_t)c * n + k;
     T s;
@@ -97,41 +115,36 @@ __global__ __launch_bounds__(kTB) void k_up(int a, const int* rank, const int* n
     }
     tmp[e] = s;
   }
-  __syncthreads();
-  for (int e = tid; e < k * nc; e += kTB) {
+  g.sync();
+  for (int e = tid; e < k * nc; e += G::nt) {
     int t = e % k, c = e / k;
     x[dofs[t] + (size_t)(c0 + c) * ldx] = v[t + (size_t)c * n];
   }
-  for (int e = tid; e < r * nc; e += kTB) {
+  for (int e = tid; e < r * nc; e += G::nt) {
     int i = e % r, c = e / r;
     x[dofs[k + i] + (size_t)(c0 + c) * ldx] = tmp[e];
   }
 }
 
+// One node of the downward pass; v: LDS for n nc values.
 // trans = 0: x_r -= U x_s; x_s -= T x_r.  trans = 1: x_r -= L^T x_s; x_s -= T x_r.
-template <typename T>
-__global__ __launch_bounds__(kTB) void k_down(int a, const int* rank, const int* nact,
-                                              const int64_t* act_off, const int* sr_pool,
-                                              const uint64_t* fac_ptr, T* x, int ldx, int nrhs,
-                                              int trans) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int node = a + blockIdx.x;
-  const int k = rank[node];
+template <typename T, typename G>
+__device__ __forceinline__ void down_node(const G& g, const SweepArgs& f, int node, T* x, int ldx,
+                                          int c0, int nc, int trans, T* v) {
+  const int k = f.rank[node];
   if (k < 0) return;
-  const int n = nact[node], r = n - k;
-  const int c0 = blockIdx.y * kRC, nc = min(kRC, nrhs - c0);
-  const int* dofs = sr_pool + act_off[node];
-  const T* Tm = reinterpret_cast<const T*>(fac_ptr[node]);
+  const int n = f.nact[node], r = n - k;
+  const int* dofs = f.sr_pool + f.act_off[node];
+  const T* Tm = reinterpret_cast<const T*>(f.fac_ptr[node]);
   const T* Lm = Tm + (size_t)k * r;
   const T* Um = Tm + 2 * (size_t)k * r;
-  T* v = reinterpret_cast<T*>(smem);
-  const int tid = threadIdx.x;
-  for (int e = tid; e < n * nc; e += kTB) {
+  const int tid = g.tid;
+  for (int e = tid; e < n * nc; e += G::nt) {
     int i = e % n, c = e / n;
     v[e] = x[dofs[i] + (size_t)(c0 + c) * ldx];
   }
-  __syncthreads();
-  for (int e = tid; e < r * nc; e += kTB) {  // x_r -= U x_s  (or L^T x_s)
+  g.sync();
+  for (int e = tid; e < r * nc; e += G::nt) {  // x_r -= U x_s  (or L^T x_s)
     int i = e % r, c = e / r;
     const T* vs = v + (size_t)c * n;
     T s = vs[k + i];
@@ -141,23 +154,81 @@ __global__ __launch_bounds__(kTB) void k_down(int a, const int* rank, const int*
       for (int t = 0; t < k; ++t) s -= Um[i + (size_t)t * r] * vs[t];
     v[k + i + (size_t)c * n] = s;
   }
-  __syncthreads();
-  for (int e = tid; e < k * nc; e += kTB) {  // x_s -= T x_r
+  g.sync();
+  for (int e = tid; e < k * nc; e += G::nt) {  // x_s -= T x_r
     int t = e % k, c = e / k;
     const T* vs = v + (size_t)c * n;
     T s = vs[t];
     for (int i = 0; i < r; ++i) s -= Tm[t + (size_t)i * k] * vs[k + i];
     v[t + (size_t)c * n] = s;
   }
-  __syncthreads();
-  for (int e = tid; e < r * nc; e += kTB) {  // only the redundant part changed last
+  g.sync();
+  for (int e = tid; e < r * nc; e += G::nt) {  // only the redundant part changed last
     int i = e % r, c = e / r;
     x[dofs[k + i] + (size_t)(c0 + c) * ldx] = v[k + i + (size_t)c * n];
   }
-  for (int e = tid; e < k * nc; e += kTB) {
+  for (int e = tid; e < k * nc; e += G::nt) {
     int t = e % k, c = e / k;
     x[dofs[t] + (size_t)(c0 + c) * ldx] = v[t + (size_t)c * n];
   }
+}
+
+// one level: a block per node of [a, a + gridDim.x), kRC right-hand sides
+// per blockIdx.y
+template <typename T>
+__global__ __launch_bounds__(kTB) void k_up(int a, SweepArgs f, T* x, int ldx, int nrhs,
+                                            int mode) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int c0 = blockIdx.y * kRC;
+  up_node<T>(BlockGroup{(int)threadIdx.x}, f, a + blockIdx.x, x, ldx, c0, min(kRC, nrhs - c0),
+             mode, reinterpret_cast<T*>(smem));
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTB) void k_down(int a, SweepArgs f, T* x, int ldx, int nrhs,
+                                              int trans) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int c0 = blockIdx.y * kRC;
+  down_node<T>(BlockGroup{(int)threadIdx.x}, f, a + blockIdx.x, x, ldx, c0, min(kRC, nrhs - c0),
+               trans, reinterpret_cast<T*>(smem));
+}
+
+// Two levels in one launch: a block per node of the coarser level [a, ...),
+// its four children one wavefront each (LDS regions of `wstride` values),
+// then the node itself with the whole block.  A node's sweep touches only
+// its own active DOFs, which lie among its children's, so a block needs no
+// other block's results.  Upward: children first; downward: node first.
+static_assert(kTB == 4 * kWave, "one wavefront per child");
+template <typename T>
+__global__ __launch_bounds__(kTB) void k_up_pair(int a, const int* child, SweepArgs f, T* x,
+                                                 int ldx, int nrhs, int mode, int wstride) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* lds = reinterpret_cast<T*>(smem);
+  const int node = a + blockIdx.x;
+  const int c0 = blockIdx.y * kRC, nc = min(kRC, nrhs - c0);
+  const int w = threadIdx.x / kWave;
+  const int ch = child[4 * node + w];
+  if (ch >= 0)
+    up_node<T>(WaveGroup{(int)threadIdx.x % kWave}, f, ch, x, ldx, c0, nc, mode,
+               lds + (size_t)w * wstride);
+  __syncthreads();
+  up_node<T>(BlockGroup{(int)threadIdx.x}, f, node, x, ldx, c0, nc, mode, lds);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTB) void k_down_pair(int a, const int* child, SweepArgs f, T* x,
+                                                   int ldx, int nrhs, int trans, int wstride) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* lds = reinterpret_cast<T*>(smem);
+  const int node = a + blockIdx.x;
+  const int c0 = blockIdx.y * kRC, nc = min(kRC, nrhs - c0);
+  down_node<T>(BlockGroup{(int)threadIdx.x}, f, node, x, ldx, c0, nc, trans, lds);
+  __syncthreads();
+  const int w = threadIdx.x / kWave;
+  const int ch = child[4 * node + w];
+  if (ch >= 0)
+    down_node<T>(WaveGroup{(int)threadIdx.x % kWave}, f, ch, x, ldx, c0, nc, trans,
+                 lds + (size_t)w * wstride);
 }
 
 // x[allskel] <- allskel_mat^{-1} x[allskel] (trans: ^{-T}) in FP64 with the
@@ -876,19 +947,46 @@ __global__ void k_axpy_f32(double* x, const float* d, size_t n) {
 
 }  // namespace
 
-// upward pass over all levels, finest first (see k_up for the modes)
+// Levels l and l + 1 of the sweeps in one launch (k_up_pair / k_down_pair)
+// when both exist and four children's LDS regions plus the node's fit; the
+// LDS bytes of that launch and each child's region (values) come back.
+template <typename T>
+static bool pair_fits(const Context& c, const TreeData& t, int l, int rcol, int per_n,
+                      size_t* lds, int* wstride) {
+  if (l < 1 || l + 1 >= t.nlevels || !t.work[l] || !t.work[l + 1]) return false;
+  const LevelWork& wp = *t.work[l];
+  const LevelWork& wc = *t.work[l + 1];
+  *wstride = per_n * wc.nmax * rcol;
+  const size_t vals = std::max<size_t>(4 * (size_t)*wstride, (size_t)per_n * wp.nmax * rcol);
+  *lds = sizeof(T) * vals;
+  return *lds > 0 && *lds <= c.lds_max;
+}
+
+// upward pass over all levels, finest first (see up_node for the modes)
 template <typename T>
 void up_sweeps(Context& c, TreeData& t, T* x, int ldx, int nrhs, int mode) {
   hipStream_t s = c.stream;
   const int ncb = ceil_div(nrhs, kRC);
   const int rcol = std::min(nrhs, kRC);
-  for (int l = t.nlevels - 1; l >= 1; --l) {
+  const SweepArgs f{t.rank.p, t.nact.p, t.act_off.p, t.sr_pool.p, t.fac_ptr.p};
+  for (int l = t.nlevels - 1; l >= 1;) {
+    size_t lds = 0;
+    int ws = 0;
+    if (pair_fits<T>(c, t, l - 1, rcol, 2, &lds, &ws)) {  // levels l - 1 and l
+      const LevelWork& w = *t.work[l - 1];
+      allow_lds(k_up_pair<T>, lds);
+      k_up_pair<T><<<dim3(w.b - w.a, ncb), kTB, lds, s>>>(w.a, t.child.p, f, x, ldx, nrhs, mode,
+                                                          ws);
+      l -= 2;
+      continue;
+    }
     LevelWork& w = *t.work[l];
-    if (w.nmax == 0) continue;
-    size_t lds = 2 * sizeof(T) * (size_t)w.nmax * rcol;
-    allow_lds(k_up<T>, lds);
-    k_up<T><<<dim3(w.b - w.a, ncb), kTB, lds, s>>>(w.a, t.rank.p, t.nact.p, t.act_off.p,
-                                                   t.sr_pool.p, t.fac_ptr.p, x, ldx, nrhs, mode);
+    if (w.nmax > 0) {
+      lds = 2 * sizeof(T) * (size_t)w.nmax * rcol;
+      allow_lds(k_up<T>, lds);
+      k_up<T><<<dim3(w.b - w.a, ncb), kTB, lds, s>>>(w.a, f, x, ldx, nrhs, mode);
+    }
+    l -= 1;
   }
 }
 template void up_sweeps<double>(Context&, TreeData&, double*, int, int, int);
@@ -900,14 +998,25 @@ void down_sweeps(Context& c, TreeData& t, T* x, int ldx, int nrhs, int trans = 0
   hipStream_t s = c.stream;
   const int ncb = ceil_div(nrhs, kRC);
   const int rcol = std::min(nrhs, kRC);
-  for (int l = 1; l < t.nlevels; ++l) {
+  const SweepArgs f{t.rank.p, t.nact.p, t.act_off.p, t.sr_pool.p, t.fac_ptr.p};
+  for (int l = 1; l < t.nlevels;) {
+    size_t lds = 0;
+    int ws = 0;
+    if (pair_fits<T>(c, t, l, rcol, 1, &lds, &ws)) {  // levels l and l + 1
+      const LevelWork& w = *t.work[l];
+      allow_lds(k_down_pair<T>, lds);
+      k_down_pair<T><<<dim3(w.b - w.a, ncb), kTB, lds, s>>>(w.a, t.child.p, f, x, ldx, nrhs,
+                                                            trans, ws);
+      l += 2;
+      continue;
+    }
     LevelWork& w = *t.work[l];
-    if (w.nmax == 0) continue;
-    size_t lds = sizeof(T) * (size_t)w.nmax * rcol;
-    allow_lds(k_down<T>, lds);
-    k_down<T><<<dim3(w.b - w.a, ncb), kTB, lds, s>>>(w.a, t.rank.p, t.nact.p, t.act_off.p,
-                                                     t.sr_pool.p, t.fac_ptr.p, x, ldx, nrhs,
-                                                     trans);
+    if (w.nmax > 0) {
+      lds = sizeof(T) * (size_t)w.nmax * rcol;
+      allow_lds(k_down<T>, lds);
+      k_down<T><<<dim3(w.b - w.a, ncb), kTB, lds, s>>>(w.a, f, x, ldx, nrhs, trans);
+    }
+    l += 1;
   }
   KI_HIP(hipGetLastError());
 }
