@@ -642,9 +642,11 @@ kigpu_status kigpu_kernel_apply_treecode(kigpu_ctx* ctx, const kigpu_tree* tree,
     DBuf<double> dm, dy;
     dm.upload(mu, len, c.stream);
     dy.alloc(len);
-    treecode_apply(c, tree->t, dm.p, nrhs, dy.p);
+    const hipEvent_t tev[3] = {c.ev[12], c.ev[14], c.ev[13]};
+    treecode_apply(c, tree->t, dm.p, nrhs, dy.p, tev);
     dy.download(y, len, c.stream);
     stream_wait(c.stream);
+    treecode_timing(c, tev);
   });
 }
 

@@ -66,7 +66,12 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
 // truncation below 1e-14 relative); the refinement's apply.  mu, y: ndof x
 // nrhs, column-major with leading dimension ndof
 bool treecode_usable(const TreeData& t);
-void treecode_apply(Context& c, const TreeData& t, const double* d_mu, int nrhs, double* d_y);
+// tev: three events recorded around the series and evaluation kernels (or
+// nullptr); treecode_timing adds their times to the context's timings once
+// they have completed.  No synchronization inside.
+void treecode_apply(Context& c, const TreeData& t, const double* d_mu, int nrhs, double* d_y,
+                    const hipEvent_t* tev);
+void treecode_timing(Context& c, const hipEvent_t* tev);
 // the tree's treecode lists and box centres, on stream st (once per tree)
 void treecode_prepare(Context& c, const TreeData& t, hipStream_t st);
 void kernel_apply(Context& c, const TreeData& t, int pde, const double* d_mu, int nrhs,

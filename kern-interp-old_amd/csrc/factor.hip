@@ -1461,11 +1461,13 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
 // T = R11^{-1} R12 solve.
 static double id_flops(int m, int n, int k, double c_gen) {
   double f = c_gen * m * n + 2.0 * m * n;
-  const int steps = k > 0 ? k + 1 : n;
-  for (int i = 0; i < steps; ++i) {
-    f += 2.0 * (m - i);
-    if (i + 1 < steps) f += 4.0 * (m - i) * (n - i - 1);
-  }
+  const double st = k > 0 ? k + 1 : n;  // QR steps
+  // sum_{i<st} 2(m - i)  +  sum_{i<st-1} 4(m - i)(n - 1 - i), in closed form
+  const double s1 = st - 1;                       // terms of the second sum
+  const double j1 = s1 * (s1 - 1) / 2;            // sum of i
+  const double j2 = (s1 - 1) * s1 * (2 * s1 - 1) / 6;  // sum of i^2
+  f += 2.0 * (m * st - st * (st - 1) / 2);
+  if (s1 > 0) f += 4.0 * ((double)m * (n - 1) * s1 - (double)(m + n - 1) * j1 + j2);
   if (k > 0) f += (double)k * k * (n - k);
   return f;
 }

@@ -887,7 +887,7 @@ __global__ void k_dup_points(int nnodes, const int* is_leaf, const int64_t* psta
   const int* pp = perm + pstart[x];
   for (int e = threadIdx.x; e < c * c; e += blockDim.x) {
     const int a = e / c, b = e - a * c;
-    if (a < b && px[pp[a]] == px[pp[b]] && py[pp[a]] == py[pp[b]]) atomicOr(flag, 1);
+    if (a < b && px[pp[a]] == px[pp[b]] && py[pp[a]] == py[pp[b]]) *flag = 1;  // all write 1
   }
 }
 
@@ -1366,15 +1366,27 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
   }
   // FP32 factor: solve in FP32, then FP64 refinement
   //   r = [f - K mu - U alpha; alpha - Psi mu],  [mu; alpha] += F^{-1} r
+  // The host waits only for each residual's norms; every phase is timed by
+  // events read after the last wait.
   const int iters = ro ? ro->max_iters : 0;
   // the residual apply: the treecode for the Laplace double layer with no
   // coincident nodes (unless the context asks for the direct sum), else the
   // direct kernels
-  const bool tc = iters > 0 && !trans && c.refine_apply == 0 && t.opts.pde == KIGPU_LAPLACE &&
-                  treecode_usable(t) && !has_coincident_points(c, t);
-  // the treecode's lists are built on the side stream while the first FP32
-  // solve runs (the first apply waits for them)
-  if (tc) treecode_prepare(c, t, c.side);
+  const bool tc_try = iters > 0 && !trans && c.refine_apply == 0 &&
+                      t.opts.pde == KIGPU_LAPLACE && treecode_usable(t);
+  int* dup_h = nullptr;
+  if (tc_try && t.dup_state < 0) {
+    // coincident-node test on the side stream, its flag written straight to
+    // page-locked memory and read once the first FP32 solve is queued
+    dup_h = c.pin_down.get<int>(1, s);
+    *dup_h = 0;
+    KI_HIP(hipEventRecord(c.ev_fork, s));
+    KI_HIP(hipStreamWaitEvent(c.side, c.ev_fork, 0));
+    k_dup_points<<<t.nnodes, 64, 0, c.side>>>(t.nnodes, t.is_leaf.p, t.pstart.p, t.pcount.p,
+                                               t.perm_all.p, t.px.p, t.py.p,
+                                               c.pin_down.dget<int>());
+    KI_HIP(hipEventRecord(c.ev[5], c.side));
+  }
   t.x32.alloc(len);
   const int g = ceil_div((long)len, 256);
   k_convert<double, float><<<g, 256, 0, s>>>(d_b, t.x32.p, len);
@@ -1386,21 +1398,39 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
                           hipMemcpyDeviceToDevice, s));
   }
   KI_HIP(hipEventRecord(c.ev[7], s));
-  stream_wait(s);
-  float ms = 0;
-  KI_HIP(hipEventElapsedTime(&ms, c.ev[6], c.ev[7]));
-  c.tim.t_solve = ms * 1e-3;
-  c.tim.t_refine_apply = 0;
+  if (dup_h) {
+    event_wait(c.ev[5]);
+    t.dup_state = *dup_h;
+  }
+  const bool tc = tc_try && t.dup_state == 0;
+  // the treecode's lists are built on the side stream while the first FP32
+  // solve runs (the first apply waits for them)
+  if (tc) treecode_prepare(c, t, c.side);
+  // event pairs of the refinement's applies and corrections, and the
+  // treecode's three events per apply
+  c.ensure_events(7 * (size_t)(iters + 1) + 1);
+  int evn = 0;
+  auto mark = [&]() {
+    KI_HIP(hipEventRecord(c.evpool[evn], s));
+    return evn++;
+  };
+  std::vector<int> seg_apply, seg_solve, seg_tc;
   if (iters > 0) {
     t.r64.alloc(len);
     t.f64.alloc(len);
     if (h) t.rb64.alloc(alen);
     std::vector<double> rr, ff, hb(alen);
     for (int it = 0; it <= iters; ++it) {
-      KI_HIP(hipEventRecord(c.ev[8], s));
-      if (trans) kernel_apply_transpose(c, t, t.opts.pde, d_x, nrhs, t.f64.p);
-      else if (tc) treecode_apply(c, t, d_x, nrhs, t.f64.p);
-      else kernel_apply(c, t, t.opts.pde, d_x, nrhs, t.f64.p);
+      seg_apply.push_back(mark());
+      if (trans) {
+        kernel_apply_transpose(c, t, t.opts.pde, d_x, nrhs, t.f64.p);
+      } else if (tc) {
+        seg_tc.push_back(evn);
+        evn += 3;
+        treecode_apply(c, t, d_x, nrhs, t.f64.p, &c.evpool[seg_tc.back()]);
+      } else {
+        kernel_apply(c, t, t.opts.pde, d_x, nrhs, t.f64.p);
+      }
       if (h) {
         k_add_ualpha<<<dim3(ceil_div(t.ndof, 256), nrhs), 256, 0, s>>>(t.ndof, h, t.bU64in.p,
                                                                       t.alpha64.p, t.f64.p);
@@ -1417,36 +1447,44 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
       double rel = 0;  // worst column of the worst instance
       for (size_t q = 0; q < rr.size(); ++q)
         rel = std::max(rel, ff[q] > 0 ? std::sqrt(rr[q] / ff[q]) : std::sqrt(rr[q]));
-      KI_HIP(hipEventRecord(c.ev[9], s));
-      stream_wait(s);
-      KI_HIP(hipEventElapsedTime(&ms, c.ev[8], c.ev[9]));
-      c.tim.t_refine_apply += ms * 1e-3;
+      seg_apply.push_back(mark());
       if (info) {
         info->rel_residual = rel;
         info->iterations = it;
       }
       if (rel <= ro->rel_residual || it == iters) break;
-      KI_HIP(hipEventRecord(c.ev[8], s));
+      seg_solve.push_back(mark());
       k_convert<double, float><<<g, 256, 0, s>>>(t.r64.p, t.x32.p, len);
       sweeps<float>(c, t, t.x32.p, t.ndof, nrhs, h ? t.rb64.p : nullptr, trans);
       k_axpy_f32<<<g, 256, 0, s>>>(d_x, t.x32.p, len);
       if (h) k_axpy_f64<<<ceil_div((long)alen, 256), 256, 0, s>>>(t.alpha64.p, t.alpha_dev.p, alen);
-      KI_HIP(hipEventRecord(c.ev[9], s));
-      stream_wait(s);
-      KI_HIP(hipEventElapsedTime(&ms, c.ev[8], c.ev[9]));
-      c.tim.t_solve += ms * 1e-3;
+      seg_solve.push_back(mark());
     }
   }
   if (h && d_alpha)
     KI_HIP(hipMemcpyAsync(d_alpha, t.alpha64.p, sizeof(double) * alen, hipMemcpyDeviceToDevice, s));
+  const unsigned long long* hw = nullptr;
   if (tc) {
-    const unsigned long long* hw = c.pin_down.get<unsigned long long>(2, s);
+    hw = c.pin_down.get<unsigned long long>(2, s);
     copy_to_host(c.pin_down, s, {{t.tc_work.p, 0, 2 * sizeof(unsigned long long)}});
-    stream_wait(s);
+  }
+  stream_wait(s);
+  if (tc) {
     c.tim.tc_series_pairs = (double)hw[0];
     c.tim.tc_direct_pairs = (double)hw[1];
   }
-  stream_wait(s);
+  auto el = [&](hipEvent_t e0, hipEvent_t e1) {
+    float ms = 0;
+    KI_HIP(hipEventElapsedTime(&ms, e0, e1));
+    return ms * 1e-3;
+  };
+  c.tim.t_solve = el(c.ev[6], c.ev[7]);
+  for (size_t i = 0; i + 1 < seg_solve.size(); i += 2)
+    c.tim.t_solve += el(c.evpool[seg_solve[i]], c.evpool[seg_solve[i + 1]]);
+  c.tim.t_refine_apply = 0;
+  for (size_t i = 0; i + 1 < seg_apply.size(); i += 2)
+    c.tim.t_refine_apply += el(c.evpool[seg_apply[i]], c.evpool[seg_apply[i + 1]]);
+  for (int e : seg_tc) treecode_timing(c, &c.evpool[e]);
 }
 
 }  // namespace kigpu

@@ -454,7 +454,8 @@ void treecode_prepare(Context& c, const TreeData& t, hipStream_t st) {
   t.tc_ready = true;
 }
 
-void treecode_apply(Context& c, const TreeData& t, const double* d_mu, int nrhs, double* d_y) {
+void treecode_apply(Context& c, const TreeData& t, const double* d_mu, int nrhs, double* d_y,
+                    const hipEvent_t* tev) {
   hipStream_t s = c.stream;
   const int nn = t.nnodes;
   const GeomView g{t.px.p, t.py.p, t.nx.p, t.ny.p, t.w.p, t.kappa.p, t.npts, t.sd};
@@ -462,23 +463,25 @@ void treecode_apply(Context& c, const TreeData& t, const double* d_mu, int nrhs,
   const size_t cstride = (size_t)nn * kTcP;
   t.tc_coef.alloc(cstride * nrhs);
   KI_HIP(hipStreamWaitEvent(s, c.ev[15], 0));  // the lists may come from the side stream
-  KI_HIP(hipEventRecord(c.ev[12], s));
+  if (tev) KI_HIP(hipEventRecord(tev[0], s));
   if (t.tc_nwork > 0)
     k_tc_coef<<<dim3(t.tc_nwork, nrhs), kTcTB, 0, s>>>(t.tc_wlist.p, t.pstart.p, t.pcount.p,
                                                       t.perm_all.p, g, d_mu, t.ndof, t.tc_ctr.p,
                                                       t.tc_coef.p, cstride);
-  KI_HIP(hipEventRecord(c.ev[14], s));
+  if (tev) KI_HIP(hipEventRecord(tev[1], s));
   k_tc_eval<<<dim3(nn, nrhs), kTcTB, 0, s>>>(t.is_leaf.p, t.pcount.p, t.pstart.p, t.perm_all.p,
                                              t.inst.p, t.d_inst_pt.p, g, d_mu, t.ndof, t.tc_ctr.p,
                                              t.tc_coef.p, cstride, t.tc_far.p, t.tc_near.p,
                                              t.tc_nfar.p, t.tc_nnear.p, d_y, t.ndof);
-  KI_HIP(hipEventRecord(c.ev[13], s));
+  if (tev) KI_HIP(hipEventRecord(tev[2], s));
   KI_HIP(hipGetLastError());
-  stream_wait(s);
-  float ms = 0, ms_eval = 0;
-  KI_HIP(hipEventElapsedTime(&ms, c.ev[12], c.ev[13]));
-  KI_HIP(hipEventElapsedTime(&ms_eval, c.ev[14], c.ev[13]));
   c.tim.tc_launches += 1;
+}
+
+void treecode_timing(Context& c, const hipEvent_t* tev) {
+  float ms = 0, ms_eval = 0;
+  KI_HIP(hipEventElapsedTime(&ms, tev[0], tev[2]));
+  KI_HIP(hipEventElapsedTime(&ms_eval, tev[1], tev[2]));
   c.tim.tc_kernel_s += ms * 1e-3;
   c.tim.tc_eval_s += ms_eval * 1e-3;
 }
