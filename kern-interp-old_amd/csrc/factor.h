@@ -43,7 +43,6 @@ struct Context {
   kigpu_timings tim{};
   int refine_apply = 0;     // refinement residual: 0 treecode where it applies, 1 direct sum
   DBuf<double> ring;        // proxy unit vectors cos | sin (host libm)
-  DBuf<double> rsum;        // residual partial sums
   size_t lds_max = 65536;
   PinnedHost pin_down;      // per-level downloads
   PinnedHost pin_up;        // packed per-level uploads

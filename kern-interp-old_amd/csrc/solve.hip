@@ -1317,11 +1317,11 @@ static void residual_sums(Context& c, const TreeData& t, const double* f, const 
     for (int i = 0; i <= ni; ++i) doff[i] = t.inst_pt[i] * t.sd;
     t.d_dof_off.upload(doff, s);
   }
-  c.rsum.alloc(2 * (size_t)nb * nrhs * ni);
-  k_residual<<<dim3(nb, nrhs, ni), kTB, 0, s>>>(f, y, r, n, nrhs, t.d_dof_off.p, c.rsum.p);
+  // the partial sums go straight to page-locked memory
   const size_t nh = 2 * (size_t)nb * nrhs * ni;
   double* h = c.pin_down.get<double>(nh, s);
-  copy_to_host(c.pin_down, s, {{c.rsum.p, 0, sizeof(double) * nh}});
+  k_residual<<<dim3(nb, nrhs, ni), kTB, 0, s>>>(f, y, r, n, nrhs, t.d_dof_off.p,
+                                                c.pin_down.dget<double>());
   stream_wait(s);
   rr->assign((size_t)nrhs * ni, 0);
   ff->assign((size_t)nrhs * ni, 0);
