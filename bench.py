@@ -293,16 +293,19 @@ def main():
                 "flops_per_launch": id_f}
     # the refinement's residual apply: the treecode (box series + direct near
     # field).  Algorithmic flops: 8 (P - 1) + 15 per (target, series box) for
-    # the Horner evaluation of P = 48 terms, 10 per (target, source) pair
-    # summed directly.
+    # the Horner evaluation of the P <= 48 terms the box needs for the
+    # target's leaf (8 per term summed + 7 per pair), 10 per (target, source)
+    # pair summed directly.
     tc_launches = max(tf["tc_launches"], 1)
     tc_eval_s = tf["tc_eval_s"] / tc_launches
-    tc_flops = 391.0 * tf["tc_series_pairs"] + 10.0 * tf["tc_direct_pairs"]
+    tc_flops = (8.0 * tf["tc_series_terms"] + 7.0 * tf["tc_series_pairs"]
+                + 10.0 * tf["tc_direct_pairs"])
     refine_apply = {"kernel": "k_tc_coef + k_tc_eval (treecode, Laplace double layer)",
                     "applies_per_step": tf["tc_launches"],
                     "avg_apply_us": tf["tc_kernel_s"] / tc_launches * 1e6,
                     "avg_eval_us": tc_eval_s * 1e6,
                     "series_pairs": tf["tc_series_pairs"], "direct_pairs": tf["tc_direct_pairs"],
+                    "series_terms": tf["tc_series_terms"],
                     "eval_achieved_tflops": tc_flops / tc_eval_s / 1e12 if tc_eval_s > 0 else 0.0,
                     "direct_sum_equivalent_pairs": float(args.n) ** 2}
     # the elimination kernel beside it (FP32 path)

@@ -94,10 +94,11 @@ typedef struct {
   double apply_kernel_s, apply_pairs;
   /* the treecode apply the FP64 refinement uses by default (Laplace double
    * layer): applies, their summed kernel time (series +
-   * evaluation), the evaluation kernel's share (s), and the (target, series
-   * box) and (target, source) pairs of one apply */
+   * evaluation), the evaluation kernel's share (s), the (target, series
+   * box) and (target, source) pairs of one apply, and the series terms
+   * summed (each pair sums the terms its box needs for its leaf, <= 48) */
   int tc_launches;
-  double tc_kernel_s, tc_eval_s, tc_series_pairs, tc_direct_pairs;
+  double tc_kernel_s, tc_eval_s, tc_series_pairs, tc_direct_pairs, tc_series_terms;
 } kigpu_timings;
 
 /* ---- context ---------------------------------------------------------- */

@@ -1341,6 +1341,7 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
   c.tim.apply_kernel_s = c.tim.apply_pairs = 0;
   c.tim.tc_launches = 0;
   c.tim.tc_kernel_s = c.tim.tc_eval_s = c.tim.tc_series_pairs = c.tim.tc_direct_pairs = 0;
+  c.tim.tc_series_terms = 0;
   const size_t len = (size_t)t.ndof * nrhs;
   const int h = t.nborder;
   const size_t alen = (size_t)h * nrhs;
@@ -1465,13 +1466,14 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
     KI_HIP(hipMemcpyAsync(d_alpha, t.alpha64.p, sizeof(double) * alen, hipMemcpyDeviceToDevice, s));
   const unsigned long long* hw = nullptr;
   if (tc) {
-    hw = c.pin_down.get<unsigned long long>(2, s);
-    copy_to_host(c.pin_down, s, {{t.tc_work.p, 0, 2 * sizeof(unsigned long long)}});
+    hw = c.pin_down.get<unsigned long long>(3, s);
+    copy_to_host(c.pin_down, s, {{t.tc_work.p, 0, 3 * sizeof(unsigned long long)}});
   }
   stream_wait(s);
   if (tc) {
     c.tim.tc_series_pairs = (double)hw[0];
     c.tim.tc_direct_pairs = (double)hw[1];
+    c.tim.tc_series_terms = (double)hw[2];
   }
   auto el = [&](hipEvent_t e0, hipEvent_t e1) {
     float ms = 0;

@@ -32,6 +32,19 @@ namespace {
 
 constexpr int kTcP = 48;              // series terms
 constexpr double kTcTheta2 = 0.25;    // acceptance: rho^2 <= theta^2 dist^2
+
+// Terms a box's series needs for a leaf: with r = rho / dist(centre, leaf
+// box) <= 1/2 the tail after P terms is below r^P / (1 - r) <= 2 r^P of
+// sum|a_j| / |x - c|.  P = 48 at the acceptance limit gives 2^-47; for
+// smaller r, P = 96 / log2(1 / r^2) (rounded up to even, with a margin for
+// the single-precision logarithm) keeps 2 r^P <= 2^-47 as well.
+__device__ __forceinline__ int tc_nterms(double rho2, double g2) {
+  if (!(g2 > 0.0)) return kTcP;
+  const float x = -__log2f((float)(rho2 / g2));  // log2(1 / r^2) >= 2
+  if (!(x > 2.0f)) return kTcP;
+  int P = (int)ceilf(96.0f * 1.001f / x);
+  return min(kTcP, max(2, P + (P & 1)));
+}
 constexpr int kTcMinLevel = 3;        // boxes of coarser levels are always opened
 constexpr int kTcMinPts = 16;         // smaller accepted boxes are summed directly
 constexpr int kTcFar = 768;           // list capacities per leaf
@@ -150,7 +163,8 @@ __global__ __launch_bounds__(kTcTB) void k_tc_coef(const int4* work, const int64
 // root; a box is accepted (series, or direct when small), summed directly
 // when it is a leaf, or else opened.  Counts -1: a list overflowed, and the
 // evaluation sums the whole instance directly for B.  work[0], work[1]:
-// totals of (target, series) and (target, source) pairs over all leaves.
+// totals of (target, series) and (target, source) pairs over all leaves;
+// work[2]: the series terms those pairs sum.
 __global__ __launch_bounds__(64) void k_tc_plan(const int* is_leaf, const int* pcount,
                                                 const int* level, const int* child,
                                                 const int* inst, const double* corners,
@@ -169,14 +183,14 @@ __global__ __launch_bounds__(64) void k_tc_plan(const int* is_leaf, const int* p
   if (lane == 0) fr[0][0] = inst[B];  // the instance's root is node inst[B]
   __syncthreads();
   int nf = 0, nn = 0, nfr = 1, cur = 0;
-  long long npts_near = 0;
+  long long npts_near = 0, nterms = 0;
   bool ovf = false;
   const uint64_t below = (1ull << lane) - 1;
   while (nfr > 0) {
     int nnext = 0;
     for (int i0 = 0; i0 < nfr; i0 += kWave) {
       const int i = i0 + lane;
-      int kind = 0, C = -1, cnt = 0;
+      int kind = 0, C = -1, cnt = 0, pterms = 0;
       if (i < nfr) {
         C = fr[cur][i];
         cnt = C >= 0 ? pcount[C] : 0;
@@ -187,6 +201,7 @@ __global__ __launch_bounds__(64) void k_tc_plan(const int* is_leaf, const int* p
             const double dx = fmax(fabs(q.x - bx) - hbx, 0.0);
             const double dy = fmax(fabs(q.y - by) - hby, 0.0);
             acc = q.z * q.z <= kTcTheta2 * (dx * dx + dy * dy);
+            if (acc) pterms = tc_nterms(q.z * q.z, dx * dx + dy * dy);
           }
           if (acc) kind = cnt >= kTcMinPts ? 1 : 2;
           else kind = is_leaf[C] ? 2 : 3;
@@ -196,6 +211,7 @@ __global__ __launch_bounds__(64) void k_tc_plan(const int* is_leaf, const int* p
       if (kind == 1) {
         const int pos = nf + __popcll(mf & below);
         if (pos < kTcFar) fl[pos] = C;
+        nterms += pterms;
       } else if (kind == 2) {
         const int pos = nn + __popcll(mn & below);
         if (pos < kTcNear) nl[pos] = C;
@@ -218,12 +234,14 @@ __global__ __launch_bounds__(64) void k_tc_plan(const int* is_leaf, const int* p
     nfr = nnext;
   }
   npts_near = wave_sum((double)npts_near);
+  nterms = wave_sum((double)nterms);
   if (lane == 0) {
     nfar[B] = ovf ? -1 : nf;
     nnear[B] = ovf ? -1 : nn;
     if (!ovf) {
       atomicAdd(work, (unsigned long long)cntB * (unsigned long long)nf);
       atomicAdd(work + 1, (unsigned long long)cntB * (unsigned long long)npts_near);
+      atomicAdd(work + 2, (unsigned long long)cntB * (unsigned long long)nterms);
     }
   }
 }
@@ -284,7 +302,8 @@ __global__ __launch_bounds__(kTcTB) void k_tc_eval(const int* is_leaf, const int
                                                    const double4* ctr, const double2* coef,
                                                    size_t cstride, const int* far_list,
                                                    const int* near_list, const int* nfar,
-                                                   const int* nnear, double* y, int ldy) {
+                                                   const int* nnear, double* y, int ldy,
+                                                   const double* corners) {
   __shared__ double2 s_cf[kTcNW][kTcP];
   __shared__ TcSrc S;
   __shared__ double s_u[kTcNW][kWave];
@@ -298,6 +317,10 @@ __global__ __launch_bounds__(kTcTB) void k_tc_eval(const int* is_leaf, const int
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
   const int64_t p0B = pstart[B];
   const int nf = nfar[B], nn = nnear[B];
+  // the leaf's box, for each series box's distance to it
+  const double* cb = corners + 8 * (size_t)B;
+  const double hbx = 0.5 * (cb[6] - cb[0]), hby = 0.5 * (cb[3] - cb[1]);
+  const double bx = cb[0] + hbx, by = cb[1] + hby;
   __shared__ int s_off[kWave + 1];
   for (int e = tid; e < nf; e += kTcTB) fl[e] = far_list[(size_t)B * kTcFar + e];
   for (int e = tid; e < nn; e += kTcTB) nl[e] = near_list[(size_t)B * kTcNear + e];
@@ -354,16 +377,21 @@ __global__ __launch_bounds__(kTcTB) void k_tc_eval(const int* is_leaf, const int
           qn = ctr[C2];
           if (lane < kTcP) cn = coef[(size_t)C2 * kTcP + lane];
         }
+        // terms this box needs for this leaf (the same for the whole wave)
+        const double gx = fmax(fabs(q.x - bx) - hbx, 0.0), gy = fmax(fabs(q.y - by) - hby, 0.0);
+        // the same for every lane (box and leaf are): made scalar, so that
+        // the Horner loop below runs as a uniform loop
+        const int P = __builtin_amdgcn_readfirstlane(tc_nterms(q.z * q.z, gx * gx + gy * gy));
         const double dx = zx - q.x, dy = zy - q.y;
         const double inv = recip1(fma(dx, dx, dy * dy));
         const double wr = q.z * dx * inv, wi = -q.z * dy * inv;
-        // sum_k M_k w^k = E(w^2) + w O(w^2): two Horner chains of half the
-        // length over the even and the odd terms
+        // sum_{k<P} M_k w^k = E(w^2) + w O(w^2): two Horner chains of half
+        // the length over the even and the odd terms
         const double vr = wr * wr - wi * wi, vi = 2.0 * wr * wi;
-        double er = s_cf[wv][kTcP - 2].x, ei = s_cf[wv][kTcP - 2].y;
-        double orr = s_cf[wv][kTcP - 1].x, oi = s_cf[wv][kTcP - 1].y;
+        double er = s_cf[wv][P - 2].x, ei = s_cf[wv][P - 2].y;
+        double orr = s_cf[wv][P - 1].x, oi = s_cf[wv][P - 1].y;
 #pragma unroll 4
-        for (int k = kTcP - 4; k >= 0; k -= 2) {
+        for (int k = P - 4; k >= 0; k -= 2) {
           const double2 me = s_cf[wv][k], mo = s_cf[wv][k + 1];
           const double ner = fma(er, vr, fma(-ei, vi, me.x));
           const double nei = fma(er, vi, fma(ei, vr, me.y));
@@ -419,7 +447,7 @@ void treecode_prepare(Context& c, const TreeData& t, hipStream_t st) {
   t.tc_near.alloc((size_t)nn * kTcNear);
   t.tc_nfar.alloc(nn);
   t.tc_nnear.alloc(nn);
-  t.tc_work.alloc(2);
+  t.tc_work.alloc(3);
   t.tc_work.zero(st);
   k_tc_ctr<<<ceil_div(nn, 256), 256, 0, st>>>(nn, t.corners.p, t.tc_ctr.p);
   // series work list: boxes of at most kTcSmall points four to a block,
@@ -472,7 +500,7 @@ void treecode_apply(Context& c, const TreeData& t, const double* d_mu, int nrhs,
   k_tc_eval<<<dim3(nn, nrhs), kTcTB, 0, s>>>(t.is_leaf.p, t.pcount.p, t.pstart.p, t.perm_all.p,
                                              t.inst.p, t.d_inst_pt.p, g, d_mu, t.ndof, t.tc_ctr.p,
                                              t.tc_coef.p, cstride, t.tc_far.p, t.tc_near.p,
-                                             t.tc_nfar.p, t.tc_nnear.p, d_y, t.ndof);
+                                             t.tc_nfar.p, t.tc_nnear.p, d_y, t.ndof, t.corners.p);
   if (tev) KI_HIP(hipEventRecord(tev[2], s));
   KI_HIP(hipGetLastError());
   c.tim.tc_launches += 1;

@@ -49,7 +49,7 @@ class Timings(C.Structure):
         ("dec_flops", C.c_double), ("apply_launches", C.c_int), ("apply_kernel_s", C.c_double),
         ("apply_pairs", C.c_double), ("tc_launches", C.c_int), ("tc_kernel_s", C.c_double),
         ("tc_eval_s", C.c_double), ("tc_series_pairs", C.c_double),
-        ("tc_direct_pairs", C.c_double)]
+        ("tc_direct_pairs", C.c_double), ("tc_series_terms", C.c_double)]
 
 
 # every symbol include/kigpu.h declares
