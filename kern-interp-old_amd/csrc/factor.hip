@@ -153,18 +153,21 @@ __device__ __forceinline__ NearSrc near_segment(int y, const int* is_leaf, const
 }
 
 // inner-circle rows of the ID matrix (kernel.cpp:209-218) for levels >= 3;
-// one wave per node.  pass 0 counts into nrows, pass 1 writes rows_pool.
+// one wave per node.  pass 0 counts into nrows, pass 1 writes rows_pool and,
+// with fac_ptr, every node's factor slab address (slab + soff * tsz).
 __global__ void k_inner_rows(int a, int b, int min_dofs, GeomView g, const int* is_leaf,
                              const int64_t* obox_off, const int* obox, const int* pcount,
                              const int64_t* act_off, const int* nact, const int* act_pool,
                              const int* sl_nbr, const int* sl_cnt, const int64_t* cp_off,
                              const int* cp_nbr, const double* corners, const double* side,
                              int* nrows, const int64_t* rows_off, int* rows_pool, int pass,
-                             int* host_rows) {
+                             int* host_rows, uint64_t* fac_ptr = nullptr, uint64_t slab = 0,
+                             const int64_t* soff = nullptr, size_t tsz = 0) {
 #pragma clang fp contract(off)
   int x = a + blockIdx.x;
   if (x >= b) return;
   const int lane = threadIdx.x;
+  if (fac_ptr && lane == 0) fac_ptr[x] = slab + (uint64_t)soff[x - a] * tsz;
   if (nact[x] < min_dofs) {
     if (!pass && lane == 0) {
       nrows[x] = 0;
@@ -1288,9 +1291,11 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
       lw->slab32.alloc(stot);
       slab = reinterpret_cast<T*>(lw->slab32.p);
     }
-    k_set_ptr<<<ceil_div(cn, kTB), kTB, 0, s>>>(t.fac_ptr.p, a, b, t.nact.p,
-                                               reinterpret_cast<uint64_t>(slab), d_soff,
-                                               sizeof(T));
+    // (levels >= 3: set by the row kernel below)
+    if (l <= 2)
+      k_set_ptr<<<ceil_div(cn, kTB), kTB, 0, s>>>(t.fac_ptr.p, a, b, t.nact.p,
+                                                 reinterpret_cast<uint64_t>(slab), d_soff,
+                                                 sizeof(T));
     // ---- ID rows
     lw->rows.alloc(std::max<int64_t>(rtot, 1));
     if (l >= 3) {
@@ -1298,7 +1303,8 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
                                         t.obox_pool.p, t.pcount.p, t.act_off.p, t.nact.p,
                                         t.act_pool.p, t.sl_nbr.p, t.sl_cnt.p, t.cp_off.p,
                                         t.cp_nbr.p, t.corners.p, t.side.p, t.nrows.p,
-                                        d_roff - a, lw->rows.p, 1, nullptr);
+                                        d_roff - a, lw->rows.p, 1, nullptr, t.fac_ptr.p,
+                                        reinterpret_cast<uint64_t>(slab), d_soff, sizeof(T));
     }
     if (l <= 2 && nwl > 0)
       k_outside_rows<<<nwl, kTB, 0, s>>>(
