@@ -1015,36 +1015,37 @@ void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin,
   DBuf<int64_t> cxf;
   cxf.alloc(2);
   k_cp_total<<<1, 64, 0, s>>>(nn, t.cp_off.p, cpc.p, t.status.p, cxf.p);
-  int64_t* hc = pin.get<int64_t>(2, s);
-  copy_to_host(pin, s, {{cxf.p, 0, 2 * sizeof(int64_t)}});
-  stream_wait(s);
-  ++*syncs;
-  if (hc[1]) KI_FAIL(KIGPU_ERR_STATE, "more than 8 same-level neighbors (overlapping boxes)");
-  const int64_t ncp = hc[0];
-  t.cp_nbr.alloc(std::max<int64_t>(ncp, 1));
+  // a node's coarse list holds at most kMaxNbrs leaves per coarser level, so
+  // the lists are written without waiting for their total
+  t.cp_nbr.alloc((size_t)nn * kMaxNbrs * std::max(t.nlevels - 1, 1));
   k_coarse<<<ncoarse, kTB, 0, s>>>(nn, anc.p, t.level.p, t.is_leaf.p, t.corners.p, t.sl_nbr.p,
                                    t.sl_cnt.p, t.cp_off.p, cpc.p, t.cp_nbr.p, 1);
 
   // ---- host mirrors the factorization needs (the rest on export), one copy
-  // each into page-locked memory and one synchronization
+  // each into page-locked memory and one synchronization; the coarse lists'
+  // total and the status word come with them
   t.nperm_total = ptot;
-  t.ncp = ncp;
   t.mirrors_ready = false;
   // written by a kernel straight into the mapped page-locked block: a few
   // hundred KB, and no copy-engine round trip (one of those, idle before,
   // has been seen to take milliseconds)
   const size_t dbl = 9 * (size_t)nn + 2 * (size_t)ni;  // corners | side | min/max
-  double* hd = pin.get<double>(dbl + (3 * (size_t)nn + 1) / 2 + 1, s);
+  const size_t ints = (3 * (size_t)nn + 1) / 2 + 1;      // inst | is_leaf | pcount
+  double* hd = pin.get<double>(dbl + ints + 2, s);
   int* hi = reinterpret_cast<int*>(hd + dbl);
+  const int64_t* hc = reinterpret_cast<const int64_t*>(hd + dbl + ints);
   {
     double* dd = pin.dget<double>();
     k_mirror_out<<<ceil_div(9 * (int64_t)nn, kTB), kTB, 0, s>>>(
         nn, ni, t.corners.p, t.side.p, mm.p, t.inst.p, t.is_leaf.p, t.pcount.p, dd,
         reinterpret_cast<int*>(dd + dbl));
     KI_HIP(hipGetLastError());
+    copy_to_host(pin, s, {{cxf.p, sizeof(double) * (dbl + ints), 2 * sizeof(int64_t)}});
   }
   stream_wait(s);
   ++*syncs;
+  if (hc[1]) KI_FAIL(KIGPU_ERR_STATE, "more than 8 same-level neighbors (overlapping boxes)");
+  t.ncp = hc[0];
   t.h_corners.assign(hd, hd + 8 * (size_t)nn);
   t.h_side.assign(hd + 8 * (size_t)nn, hd + 9 * (size_t)nn);
   t.inst_min.resize(ni);
