@@ -59,15 +59,22 @@ __global__ void k_minmax_part(const double* px, const double* py, int n, double*
   }
 }
 
+// one wave: min and max are exact, so the order of the reduction is free
 __global__ void k_minmax_final(const double* part, int np, double* out) {
-  if (threadIdx.x != 0) return;
+  const int l = threadIdx.x;
   double mn = part[0], mx = part[1];
-  for (int i = 1; i < np; ++i) {
+  for (int i = l; i < np; i += kWave) {
     mn = fmin(mn, part[2 * i]);
     mx = fmax(mx, part[2 * i + 1]);
   }
-  out[0] = mn;
-  out[1] = mx;
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    mn = fmin(mn, __shfl_xor(mn, o));
+    mx = fmax(mx, __shfl_xor(mx, o));
+  }
+  if (l == 0) {
+    out[0] = mn;
+    out[1] = mx;
+  }
 }
 
 __global__ void k_iota(int* p, int n) {
