@@ -33,7 +33,11 @@ enum {
   KIGPU_ERR_RANK = 5,         /* ID matrix with fewer rows than columns */
   KIGPU_ERR_NOCOMPRESS = 6,   /* bordered system with nothing compressed */
   KIGPU_ERR_NOTSUPP = 7,      /* combination not supported */
-  KIGPU_ERR_NOCONV = 8        /* iterative refinement did not converge */
+  KIGPU_ERR_NOCONV = 8,       /* iterative refinement did not converge */
+  KIGPU_ERR_SINGULAR = 9      /* a zero or non-finite pivot in an X_rr, root or
+                                 bordered inverse: the factorization is unusable
+                                 (the reference leaves dgetrf's info unchecked,
+                                 ki_mat.cpp:431-432, and goes on with inf/NaN) */
 };
 
 /* Kernel::Pde (kern_interp/kernel/kernel.h:16-20) */

@@ -1,6 +1,8 @@
 // Caching device allocator behind DBuf (see common.h).
 #include <map>
 #include <mutex>
+#include <utility>
+#include <vector>
 
 #include "common.h"
 
@@ -44,10 +46,31 @@ void* dev_alloc(size_t bytes) {
   return p;
 }
 
+// Blocks freed on this thread wait here until dev_release_pending(): a
+// block can still be read or written by kernels queued before its free, on
+// either stream of a context, so it is handed out again only once the
+// thread has waited for those streams.
+namespace {
+struct Pending {
+  std::vector<std::pair<size_t, void*>> blocks;
+  ~Pending() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (auto& b : blocks) g_free.emplace(b.first, b.second);
+  }
+};
+thread_local Pending t_pending;
+}  // namespace
+
 void dev_free(void* p, size_t bytes) {
   if (!p) return;
+  t_pending.blocks.emplace_back(bytes, p);
+}
+
+void dev_release_pending() {
+  if (t_pending.blocks.empty()) return;
   std::lock_guard<std::mutex> lk(g_mu);
-  g_free.emplace(bytes, p);
+  for (auto& b : t_pending.blocks) g_free.emplace(b.first, b.second);
+  t_pending.blocks.clear();
 }
 
 }  // namespace kigpu

@@ -30,20 +30,30 @@ void set_error(const std::string& msg) { g_err = msg; }
 
 namespace {
 
+// Runs an entry point's body and maps exceptions to status codes.  With a
+// context (entry points that wait for their work before returning), both of
+// its streams are waited for once the body's temporaries are gone, and the
+// device blocks freed on this thread become reusable.
 template <typename F>
-kigpu_status guard(F&& f) {
+kigpu_status guard(F&& f, Context* settle = nullptr) {
+  kigpu_status st = KIGPU_OK;
   try {
     f();
-    return KIGPU_OK;
   } catch (const HipError& e) {
-    return e.code;
+    st = e.code;
   } catch (const std::bad_alloc&) {
     set_error("host allocation failed");
-    return KIGPU_ERR_NOMEM;
+    st = KIGPU_ERR_NOMEM;
   } catch (...) {
     set_error("unexpected exception");
-    return KIGPU_ERR_STATE;
+    st = KIGPU_ERR_STATE;
   }
+  if (settle) {
+    if (hipStreamSynchronize(settle->stream) == hipSuccess &&
+        hipStreamSynchronize(settle->side) == hipSuccess)
+      dev_release_pending();
+  }
+  return st;
 }
 
 #define KI_REQUIRE(cond, msg)              \
@@ -116,6 +126,7 @@ const char* kigpu_status_string(kigpu_status s) {
     case KIGPU_ERR_NOCOMPRESS: return "no compression possible";
     case KIGPU_ERR_NOTSUPP: return "not supported";
     case KIGPU_ERR_NOCONV: return "refinement did not converge";
+    case KIGPU_ERR_SINGULAR: return "singular factorization (zero or non-finite pivot)";
     default: return "unknown status";
   }
 }
@@ -151,7 +162,7 @@ kigpu_status kigpu_context_create(int device, kigpu_ctx** out) {
     }
     c.ring.upload(ring, c.stream);
     stream_wait(c.stream);
-  });
+  }, &ctx->c);
   if (st == KIGPU_OK) *out = ctx.release();
   return st;
 }
@@ -210,7 +221,7 @@ static kigpu_status tree_build_common(kigpu_ctx* ctx, int npts, int sd, int leaf
     float ms = 0;
     KI_HIP(hipEventElapsedTime(&ms, c.ev[10], c.ev[11]));
     c.tim.t_tree = ms * 1e-3;
-  });
+  }, &ctx->c);
   if (st == KIGPU_OK) *out = tr.release();
   return st;
 }
@@ -440,7 +451,7 @@ kigpu_status kigpu_factor(kigpu_ctx* ctx, kigpu_tree* tree, const kigpu_factor_o
     if (o.min_dofs <= 0) o.min_dofs = 16;
     tree->t.factored = false;
     factor_tree(ctx->c, tree->t, o);
-  });
+  }, &ctx->c);
 }
 
 kigpu_status kigpu_factor_ranks(const kigpu_tree* tree, int* rank, int* active) {
@@ -533,7 +544,8 @@ kigpu_status kigpu_solve_device(kigpu_ctx* ctx, kigpu_tree* tree, const double* 
     set_error("solve before factor");
     return KIGPU_ERR_STATE;
   }
-  return guard([&] { solve_tree(ctx->c, tree->t, d_b, nrhs, d_x, d_alpha, refine, info); });
+  return guard([&] { solve_tree(ctx->c, tree->t, d_b, nrhs, d_x, d_alpha, refine, info); },
+               &ctx->c);
 }
 
 kigpu_status kigpu_solve(kigpu_ctx* ctx, kigpu_tree* tree, const double* b, int ldb, int nrhs,
@@ -561,7 +573,7 @@ kigpu_status kigpu_solve(kigpu_ctx* ctx, kigpu_tree* tree, const double* b, int 
                             sizeof(double) * nd, nrhs, hipMemcpyDeviceToHost, c.stream));
     if (h && alpha) da.download(alpha, (size_t)h * nrhs, c.stream);
     stream_wait(c.stream);
-  });
+  }, &ctx->c);
 }
 
 kigpu_status kigpu_geometry_create(kigpu_ctx* ctx, const double* points, const double* normals,
@@ -596,7 +608,7 @@ kigpu_status kigpu_geometry_create(kigpu_ctx* ctx, const double* points, const d
     g->dx.upload(da, s);
     g->dy.upload(db, s);
     stream_wait(s);
-  });
+  }, &ctx->c);
   if (st == KIGPU_OK) *out = g.release();
   return st;
 }
@@ -647,7 +659,7 @@ kigpu_status kigpu_kernel_apply_treecode(kigpu_ctx* ctx, const kigpu_tree* tree,
     dy.download(y, len, c.stream);
     stream_wait(c.stream);
     treecode_timing(c, tev);
-  });
+  }, &ctx->c);
 }
 
 kigpu_status kigpu_set_refine_apply(kigpu_ctx* ctx, int mode) {
@@ -669,7 +681,7 @@ kigpu_status kigpu_kernel_apply(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
     kernel_apply(c, tree->t, pde, dm.p, nrhs, dy.p);
     dy.download(y, len, c.stream);
     stream_wait(c.stream);
-  });
+  }, &ctx->c);
 }
 
 kigpu_status kigpu_forward_apply(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
@@ -692,7 +704,7 @@ kigpu_status kigpu_forward_apply(kigpu_ctx* ctx, const kigpu_tree* tree, int pde
     forward_apply(c, t, pde, dt.p, ntgt, dm.p, nrhs, du.p);
     du.download(u, (size_t)sd * ntgt * nrhs, c.stream);
     stream_wait(c.stream);
-  });
+  }, &ctx->c);
 }
 
 kigpu_status kigpu_solve_transpose_device(kigpu_ctx* ctx, kigpu_tree* tree, const double* d_b,
@@ -705,7 +717,8 @@ kigpu_status kigpu_solve_transpose_device(kigpu_ctx* ctx, kigpu_tree* tree, cons
     return KIGPU_ERR_STATE;
   }
   return guard(
-      [&] { solve_tree(ctx->c, tree->t, d_b, nrhs, d_x, nullptr, refine, info, true); });
+      [&] { solve_tree(ctx->c, tree->t, d_b, nrhs, d_x, nullptr, refine, info, true); },
+      &ctx->c);
 }
 
 kigpu_status kigpu_solve_transpose(kigpu_ctx* ctx, kigpu_tree* tree, const double* b, int ldb,
@@ -730,7 +743,7 @@ kigpu_status kigpu_solve_transpose(kigpu_ctx* ctx, kigpu_tree* tree, const doubl
     KI_HIP(hipMemcpy2DAsync(x, sizeof(double) * ldx, dx.p, sizeof(double) * nd,
                             sizeof(double) * nd, nrhs, hipMemcpyDeviceToHost, c.stream));
     stream_wait(c.stream);
-  });
+  }, &ctx->c);
 }
 
 kigpu_status kigpu_kernel_apply_transpose(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
@@ -746,7 +759,7 @@ kigpu_status kigpu_kernel_apply_transpose(kigpu_ctx* ctx, const kigpu_tree* tree
     kernel_apply_transpose(c, tree->t, pde, dl.p, nrhs, dy.p);
     dy.download(y, len, c.stream);
     stream_wait(c.stream);
-  });
+  }, &ctx->c);
 }
 
 kigpu_status kigpu_forward_apply_transpose(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
@@ -769,7 +782,7 @@ kigpu_status kigpu_forward_apply_transpose(kigpu_ctx* ctx, const kigpu_tree* tre
     forward_apply_transpose(c, t, pde, dt.p, ntgt, dv.p, nrhs, dy.p);
     dy.download(y, (size_t)t.ndof * nrhs, c.stream);
     stream_wait(c.stream);
-  });
+  }, &ctx->c);
 }
 
 kigpu_status kigpu_kernel_sensitivity(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
@@ -793,7 +806,7 @@ kigpu_status kigpu_kernel_sensitivity(kigpu_ctx* ctx, const kigpu_tree* tree, in
     kernel_sensitivity(c, t, dl.p, dm.p, ds.p);
     ds.download(sens, 6 * (size_t)t.npts, c.stream);
     stream_wait(c.stream);
-  });
+  }, &ctx->c);
 }
 
 kigpu_status kigpu_forward_sensitivity(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
@@ -819,7 +832,7 @@ kigpu_status kigpu_forward_sensitivity(kigpu_ctx* ctx, const kigpu_tree* tree, i
     forward_sensitivity(c, t, dt.p, ntgt, dv.p, dm.p, ds.p);
     ds.download(sens, 6 * (size_t)t.npts, c.stream);
     stream_wait(c.stream);
-  });
+  }, &ctx->c);
 }
 
 }  // extern "C"

@@ -45,8 +45,12 @@ struct HipError {
 // Caching device allocator: freed blocks go to a free list keyed by their
 // rounded size and are reused, so the per-level buffers of tree builds,
 // factorizations and solves never call hipMalloc/hipFree in steady state.
+// A freed block is reused only after the freeing thread calls
+// dev_release_pending(), which it does once it has waited for every stream
+// that may still touch the block (the end of each synchronous entry point).
 void* dev_alloc(size_t bytes);
 void dev_free(void* p, size_t bytes);
+void dev_release_pending();
 size_t dev_round(size_t bytes);
 
 // owning device buffer

@@ -226,18 +226,29 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// A pivot the elimination can divide by: nonzero and finite (false for NaN).
+// The pivot searches below start from the first candidate row, so a column
+// without any finite candidate keeps the diagonal as its pivot: the row
+// index stays inside the matrix, and the zero, infinite or NaN pivot is
+// reported through the return value instead.
+template <typename T>
+__host__ __device__ __forceinline__ bool pivot_usable(T d) {
+  const double a = fabs((double)d);
+  return a > 0.0 && a <= 1.7976931348623157e308;
+}
+
 // In-place Gauss-Jordan inverse of the n x n matrix A (LDS, column-major,
 // leading dimension n) with partial row pivoting: the pivot of step c is the
 // first largest |A(i, c)|, i >= c, of the current matrix -- the same choice
 // as dgetrf's -- and piv[c] records it.  Executed by ONE wavefront; prow and
-// fcol are n-element LDS scratch vectors.  Returns false on a zero pivot.
+// fcol are n-element LDS scratch vectors.  Returns false on a zero or non-finite pivot.
 template <typename T>
 __device__ bool wave_gj_inverse(T* A, int n, int* piv, T* prow, T* fcol) {
   const int lane = threadIdx.x & 63;
   bool ok = true;
   for (int c = 0; c < n; ++c) {
     double best = -1.0;
-    int bi = n;
+    int bi = c;  // stays c when no candidate is finite (NaN compares false)
     for (int i = c + lane; i < n; i += 64) {
       double v = fabs((double)A[i + (size_t)c * n]);
       if (v > best) {
@@ -256,7 +267,7 @@ __device__ bool wave_gj_inverse(T* A, int n, int* piv, T* prow, T* fcol) {
       }
     wave_sync();
     const T d = A[c + (size_t)c * n];
-    if (d == T(0)) ok = false;
+    if (!pivot_usable(d)) ok = false;
     const T inv = T(1) / d;
     for (int j = lane; j < n; j += 64) prow[j] = j == c ? inv : A[c + (size_t)j * n] * inv;
     for (int i = lane; i < n; i += 64) fcol[i] = A[i + (size_t)c * n];
@@ -293,7 +304,7 @@ __device__ bool block_gj_inverse(T* A, int n, int* piv, T* prow, T* fcol, int* s
   for (int c = 0; c < n; ++c) {
     if (wv == 0) {
       double best = -1.0;
-      int bi = n;
+      int bi = c;  // stays c when no candidate is finite (NaN compares false)
       for (int i = c + lane; i < n; i += 64) {
         double v = fabs((double)A[i + (size_t)c * n]);
         if (v > best) {
@@ -317,7 +328,7 @@ __device__ bool block_gj_inverse(T* A, int n, int* piv, T* prow, T* fcol, int* s
       }
     __syncthreads();
     const T d = A[c + (size_t)c * n];
-    if (d == T(0)) ok = false;
+    if (!pivot_usable(d)) ok = false;
     const T inv = T(1) / d;
     for (int j = tid; j < n; j += nt) prow[j] = j == c ? inv : A[c + (size_t)j * n] * inv;
     for (int i = tid; i < n; i += nt) fcol[i] = A[i + (size_t)c * n];
@@ -356,7 +367,7 @@ __device__ bool block_gj_inverse(T* A, int n, int* piv, T* prow, T* fcol, int* s
 // records them.  The column interchanges that finish the inverse are not
 // applied: on return inv(A)(i, j) = A[i + sigma[j] * n].  prow, orow, fcol
 // are n-element LDS scratch vectors, piv and sigma n-element LDS int arrays.
-// Returns false on a zero pivot.
+// Returns false on a zero or non-finite pivot.
 template <typename T>
 __device__ bool block_gj_inverse_perm(T* __restrict__ A, int n, int* __restrict__ piv,
                                       int* __restrict__ sigma, T* __restrict__ prow,
@@ -366,7 +377,7 @@ __device__ bool block_gj_inverse_perm(T* __restrict__ A, int n, int* __restrict_
   bool ok = true;
   if (wv == 0) {  // pivot of column 0
     double best = -1.0;
-    int bi = n;
+    int bi = 0;  // stays 0 when no candidate is finite (NaN compares false)
     for (int i = lane; i < n; i += 64) {
       double v = fabs((double)A[i]);
       if (v > best) {
@@ -381,7 +392,7 @@ __device__ bool block_gj_inverse_perm(T* __restrict__ A, int n, int* __restrict_
   for (int c = 0; c < n; ++c) {
     const int p = *s_p;
     const T d = A[p + (size_t)c * n];
-    if (d == T(0)) ok = false;
+    if (!pivot_usable(d)) ok = false;
     const T inv = T(1) / d;
     for (int j = tid; j < n; j += nt) {
       prow[j] = j == c ? inv : A[p + (size_t)j * n] * inv;
@@ -396,7 +407,7 @@ __device__ bool block_gj_inverse_perm(T* __restrict__ A, int n, int* __restrict_
     // columns are issued before any store so that they overlap.  The wave
     // with jj = 0 (column c + 1) also finds the next pivot.
     double best = -1.0;
-    int bi = n;
+    int bi = c + 1;  // the first candidate of column c + 1; kept when none is finite
     for (int i = lane; i < n; i += 64) {
       const bool rp = i == p, rc = i == c;
       const T f = rp ? fp : fcol[i];

@@ -5,6 +5,13 @@
 
 namespace kigpu {
 
+// Bits of the device status word (TreeData::status) that the factorization
+// kernels set: 2 an ID matrix with fewer rows than columns, 4 an X_rr inverse
+// with a zero or non-finite pivot, 8 the same for a root inverse, 16 for the
+// bordered inverse.  The host turns 2 into KIGPU_ERR_RANK and 4 | 8 | 16
+// into KIGPU_ERR_SINGULAR.
+constexpr int kStatusSingular = 4 | 8 | 16;
+
 // one box to compress at a level
 struct IdWork {
   int node;

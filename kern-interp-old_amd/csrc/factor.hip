@@ -620,7 +620,7 @@ __device__ __forceinline__ void id_box(const IdWork& w, unsigned char* smem, Geo
     }
     if (wv == 0) {
       double best = -1.0;
-      int bi = n;
+      int bi = i;  // NaN norms compare false: the column order is kept
       for (int j = i + lane; j < n; j += kWave) {
         double v = vn1[j];
         if (v > best) {
@@ -1505,7 +1505,15 @@ void factor_tree(Context& c, TreeData& t, const kigpu_factor_opts& o) {
                {{t.act_off.p, 0, b8}, {t.rank.p, b8, b4}, {t.nact.p, b8 + b4, b4},
                 {t.status.p, b8 + 2 * b4, sizeof(int)}});
   stream_wait(s);
-  if (pi[2 * nn] & 2) KI_FAIL(KIGPU_ERR_RANK, "ID matrix with fewer rows than columns");
+  const int st = pi[2 * nn];
+  if (st & 2) KI_FAIL(KIGPU_ERR_RANK, "ID matrix with fewer rows than columns");
+  if (st & kStatusSingular) {
+    std::string what = "zero or non-finite pivot in";
+    if (st & 4) what += " an X_rr inverse";
+    if (st & 8) what += " the root inverse";
+    if (st & 16) what += " the bordered inverse";
+    KI_FAIL(KIGPU_ERR_SINGULAR, what);
+  }
   t.h_act_off.assign(pin, pin + nn);
   t.h_rank.assign(pi, pi + nn);
   t.h_nact.assign(pi + nn, pi + 2 * nn);

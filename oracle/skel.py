@@ -201,6 +201,7 @@ class SkelFactorization:
         S[ns:, :ns] = mPsi[:, allskel]
         S[:ns, ns:] = mU[allskel, :]
         S[ns:, ns:] = -np.eye(h) - mPsi[:, red] @ DinvC
+        tree.S_mat = S
         tree.S_lu, tree.S_piv = lu_factor(S)
 
     # -- solve ----------------------------------------------------------------

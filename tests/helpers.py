@@ -22,3 +22,34 @@ def gpu_tree(ctx, b, sd=1, leaf=64):
 
 def pde_sd(pde):
     return 2 if pde == K.STOKES else 1
+
+
+def stokeslet_velocity(targets, src=(2.0, 1.0), force=(1.0, 0.0)):
+    """Velocity of a Stokeslet (the BC of oracle.geometry.bc_stokeslet) at targets."""
+    xy = np.asarray(targets).reshape(-1, 2)
+    rx, ry = xy[:, 0] - src[0], xy[:, 1] - src[1]
+    r2 = rx * rx + ry * ry
+    lg = -0.5 * np.log(r2)
+    fx, fy = force
+    ux = (lg * fx + (rx * rx * fx + rx * ry * fy) / r2) / (4 * np.pi)
+    uy = (lg * fy + (ry * rx * fx + ry * ry * fy) / r2) / (4 * np.pi)
+    return np.stack([ux, uy], 1).reshape(-1)
+
+
+def holed_disc_targets(b, n, seed, margin=0.03, radius=0.9):
+    """Up to n targets uniform in |x| < radius, at least margin outside every hole."""
+    u = G.uniform01(seed, 8 * n).reshape(-1, 2)
+    pts = []
+    for a, c in u:
+        r, ph = radius * np.sqrt(a), 2 * np.pi * c
+        x, y = r * np.cos(ph), r * np.sin(ph)
+        if any(np.hypot(x - h.center[0], y - h.center[1]) < h.radius + margin for h in b.holes):
+            continue
+        pts.append((x, y))
+        if len(pts) == n:
+            break
+    return np.array(pts).reshape(-1)
+
+
+def oracle_ranks(tree):
+    return np.array([len(n.skel) if n.compressed else -1 for n in tree.all_nodes()], np.int32)

@@ -62,9 +62,10 @@ def test_annulus_laplace_parity(ctx):
 
 
 @pytest.mark.gpu
-@pytest.mark.skip(reason="faulted the GPU twice while an experimental register-resident root "
-                         "Gauss-Jordan was built in (since removed); the cause was not found, "
-                         "so the case stays off the GPU until it is")
+@pytest.mark.skip(reason="faulted the GPU twice while an uncommitted register-resident root "
+                         "Gauss-Jordan was built in (since removed); no cause in the code that "
+                         "remains has been shown (the CPU emulation of its inverses on this "
+                         "case's matrices stays in bounds), so the case stays off the GPU")
 def test_annulus_laplace_analytic(ctx):
     # kern_interp_test.cpp:413-430 (id_tol 1e-13, N = 2^12): point-source field
     print("[border] annulus laplace N=4096 tol 1e-13", flush=True)

@@ -218,3 +218,21 @@ def test_apply_matches_dense(ctx, pde):
     u = t.forward(tgt, mu, pde)
     ref = kd.forward() @ mu
     assert np.allclose(u, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+@pytest.mark.gpu
+def test_singular_factor_reported(ctx):
+    # A NaN weight makes a source column of K NaN.  The pivot searches keep
+    # their indices inside the matrix (their first candidate is the default),
+    # the root inverse reports the unusable pivot, and kigpu_factor fails with
+    # KIGPU_ERR_SINGULAR instead of returning an inf/NaN factorization
+    # (the reference leaves dgetrf's info unchecked, ki_mat.cpp:431-432).
+    print("[factor] star N=512 with a NaN weight: singular status", flush=True)
+    b = star_problem(512)
+    w = b.weights.copy()
+    w[100] = np.nan
+    t = ctx.build_tree(b.points, b.normals, w, b.curvatures, 1, 64)
+    with pytest.raises(kigpu.KigpuError) as e:
+        t.factor(K.LAPLACE, 1e-6, kigpu.FP64)
+    assert e.value.code == 9
+    t.close()
