@@ -213,6 +213,19 @@ kigpu_status kigpu_solve_device(kigpu_ctx* ctx, kigpu_tree* tree, const double* 
                                 int nrhs, double* d_x, double* d_alpha,
                                 const kigpu_refine_opts* refine, kigpu_solve_info* info);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same x = F^{-1} b as kigpu_solve_device with
+ * refine->max_iters = 0, through the same kernels, allocating nothing.  An
+ * FP64 factor solves in place in d_x; an FP32_REFINE factor stages x in
+ * FP32 in a per-GPU static device buffer of 4 194 304 floats (16 MiB, set up
+ * by kigpu_context_create), so ndof * nrhs above that returns
+ * KIGPU_ERR_ARG.  Trees with a border return KIGPU_ERR_NOTSUPP.  Every call
+ * on one GPU shares that buffer: calls that overlap on one GPU race.
+ * Waits for the result like kigpu_solve_device. */
+kigpu_status kigpu_solve_device_noalloc(kigpu_ctx* ctx, kigpu_tree* tree, const double* d_b,
+                                        int nrhs, double* d_x);
+
 /* ---- dense kernel blocks ------------------------------------------------ */
 typedef struct kigpu_geom kigpu_geom;
 /* Device copies of a Kernel's arrays (kern_interp/kernel/kernel.h:22-37):

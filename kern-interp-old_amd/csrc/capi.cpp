@@ -161,6 +161,7 @@ kigpu_status kigpu_context_create(int device, kigpu_ctx** out) {
       ring[kProxyPoints + i] = std::sin(ang);
     }
     c.ring.upload(ring, c.stream);
+    noalloc_init(device);  // the no-allocation solve's scratch address on this device
     stream_wait(c.stream);
   }, &ctx->c);
   if (st == KIGPU_OK) *out = ctx.release();
@@ -546,6 +547,17 @@ kigpu_status kigpu_solve_device(kigpu_ctx* ctx, kigpu_tree* tree, const double* 
   }
   return guard([&] { solve_tree(ctx->c, tree->t, d_b, nrhs, d_x, d_alpha, refine, info); },
                &ctx->c);
+}
+
+kigpu_status kigpu_solve_device_noalloc(kigpu_ctx* ctx, kigpu_tree* tree, const double* d_b,
+                                        int nrhs, double* d_x) {
+  KI_REQUIRE(ctx && tree && d_b && d_x, "null argument");
+  KI_REQUIRE(nrhs > 0, "nrhs must be positive");
+  if (!tree->t.factored) {
+    set_error("solve before factor");
+    return KIGPU_ERR_STATE;
+  }
+  return guard([&] { solve_tree_noalloc(ctx->c, tree->t, d_b, nrhs, d_x); });
 }
 
 kigpu_status kigpu_solve(kigpu_ctx* ctx, kigpu_tree* tree, const double* b, int ldb, int nrhs,

@@ -65,6 +65,12 @@ template <typename T>
 void border_factor(Context& c, TreeData& t);
 template <typename T>
 void border_step(Context& c, TreeData& t, T* x, int ldx, int nrhs, const double* g);
+// Largest ndof * nrhs of solve_tree_noalloc on an FP32 factor: the
+// per-GPU static scratch holds that many floats (16 MiB).
+constexpr int kNoAllocMaxEntries = 1 << 22;
+// the scratch's device address on `device`, once per device (context creation)
+void noalloc_init(int device);
+void solve_tree_noalloc(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_x);
 void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_x,
                 double* d_alpha, const kigpu_refine_opts* ro, kigpu_solve_info* info,
                 bool trans = false);

@@ -17,6 +17,8 @@
 #include <algorithm>
 #include <cmath>
 
+#include <mutex>
+
 #include "device_util.h"
 #include "entries.h"
 #include "factor.h"
@@ -1330,6 +1332,63 @@ static void residual_sums(Context& c, const TreeData& t, const double* f, const 
       (*rr)[q] += h[2 * (q * nb + i)];
       (*ff)[q] += h[2 * (q * nb + i) + 1];
     }
+}
+
+// ---------------------------------------------------------------------------
+// solve without device allocation
+// ---------------------------------------------------------------------------
+// FP32 staging copy of the right-hand sides for solve_tree_noalloc, one per
+// GPU, held until the process ends.  In host code `scratch` names a host
+// stand-in, not device memory: kernels get the address hipGetSymbolAddress
+// returns for the current device (noalloc_init).
+static __device__ __align__(256) float scratch[kNoAllocMaxEntries];
+
+namespace {
+std::mutex g_noalloc_mu;
+std::vector<float*> g_noalloc_ptr;  // per device, set by noalloc_init
+}  // namespace
+
+void noalloc_init(int device) {
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if ((int)g_noalloc_ptr.size() <= device) g_noalloc_ptr.resize(device + 1, nullptr);
+  if (g_noalloc_ptr[device]) return;
+  void* p = nullptr;
+  KI_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_noalloc_ptr[device] = static_cast<float*>(p);
+}
+
+// x = F^{-1} b with the factorization's own arithmetic and no refinement,
+// the same kernels in the same order as solve_tree with max_iters = 0, but
+// no device memory is allocated: the FP32 factor's staging copy of x lives
+// in `scratch` (at most kNoAllocMaxEntries = ndof * nrhs values), the FP64
+// factor works in place in d_x.  Trees with a border are rejected (the
+// bordered step keeps per-call buffers).  Calls that overlap on one GPU
+// share its scratch and race.
+void solve_tree_noalloc(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_x) {
+  hipStream_t s = c.stream;
+  if (t.nborder > 0) KI_FAIL(KIGPU_ERR_NOTSUPP, "the no-allocation solve takes no border");
+  const size_t len = (size_t)t.ndof * nrhs;
+  if (t.opts.precision == KIGPU_FP64) {
+    if (d_x != d_b)
+      KI_HIP(hipMemcpyAsync(d_x, d_b, sizeof(double) * len, hipMemcpyDeviceToDevice, s));
+    sweeps<double>(c, t, d_x, t.ndof, nrhs, nullptr, false);
+    stream_wait(s);
+    return;
+  }
+  if (len > (size_t)kNoAllocMaxEntries)
+    KI_FAIL(KIGPU_ERR_ARG, "ndof * nrhs exceeds the no-allocation scratch (kNoAllocMaxEntries)");
+  float* x32 = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (c.device < (int)g_noalloc_ptr.size()) x32 = g_noalloc_ptr[c.device];
+  }
+  if (!x32) KI_FAIL(KIGPU_ERR_STATE, "no-allocation scratch not set up on this device");
+  const int g = ceil_div((long)len, 256);
+  k_convert<double, float><<<g, 256, 0, s>>>(d_b, x32, len);
+  sweeps<float>(c, t, x32, t.ndof, nrhs, nullptr, false);
+  k_convert<float, double><<<g, 256, 0, s>>>(x32, d_x, len);
+  KI_HIP(hipGetLastError());
+  stream_wait(s);
 }
 
 void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_x, double* d_alpha,

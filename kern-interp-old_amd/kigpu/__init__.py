@@ -58,7 +58,8 @@ EXPORTS = [
     "kigpu_last_timings", "kigpu_tree_build", "kigpu_tree_build_device", "kigpu_tree_destroy", "kigpu_tree_get_info",
     "kigpu_tree_export_nodes", "kigpu_tree_export_boxes", "kigpu_tree_export_neighbors",
     "kigpu_set_border", "kigpu_factor", "kigpu_factor_ranks", "kigpu_factor_export_lists",
-    "kigpu_factor_allskel", "kigpu_solve", "kigpu_solve_device", "kigpu_kernel_apply",
+    "kigpu_factor_allskel", "kigpu_solve", "kigpu_solve_device", "kigpu_solve_device_noalloc",
+    "kigpu_kernel_apply",
     "kigpu_kernel_apply_device", "kigpu_forward_apply", "kigpu_geometry_create",
     "kigpu_geometry_destroy", "kigpu_kernel_block", "kigpu_factor_export_active",
     "kigpu_solve_transpose", "kigpu_solve_transpose_device", "kigpu_kernel_apply_transpose",
@@ -109,6 +110,7 @@ def load() -> C.CDLL:
                                   C.POINTER(RefineOpts), C.POINTER(SolveInfo)]),
         "kigpu_solve_device": (C.c_int, [P, P, P, C.c_int, P, P, C.POINTER(RefineOpts),
                                          C.POINTER(SolveInfo)]),
+        "kigpu_solve_device_noalloc": (C.c_int, [P, P, P, C.c_int, P]),
         "kigpu_kernel_apply": (C.c_int, [P, P, C.c_int, _f64p, C.c_int, _f64p]),
         "kigpu_kernel_apply_treecode": (C.c_int, [P, P, _f64p, C.c_int, _f64p]),
         "kigpu_set_refine_apply": (C.c_int, [P, C.c_int]),
