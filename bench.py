@@ -229,20 +229,103 @@ def pmc_traffic(name):
     return None
 
 
-def cpu_baseline(n, leaf, tol):
-    """The oracle (numpy/scipy restatement, single-threaded BLAS) on the same workload."""
-    from oracle import geometry as G
-    from oracle import kernel as K
-    from oracle import linear_solve as LS
-    b = G.star(n)
-    b.boundary_values = G.bc_point_source(b, (3.0, 3.0))
+CPU_BASELINE_CODE = r"""
+import json, os, sys, time
+sys.path.insert(0, sys.argv[1])
+from oracle import geometry as G
+from oracle import kernel as K
+from oracle import linear_solve as LS
+n, leaf, tol, workers, budget = int(sys.argv[2]), int(sys.argv[3]), float(sys.argv[4]), int(sys.argv[5]), float(sys.argv[6])
+b = G.star(n)
+b.boundary_values = G.bc_point_source(b, (3.0, 3.0))
+times = []
+t_all = time.perf_counter()
+while not times or (time.perf_counter() - t_all < budget and len(times) < 5):
     t0 = time.perf_counter()
-    tree, kern, fac = LS.factor(b, K.LAPLACE, tol, leaf)
+    tree, kern, fac = LS.factor(b, K.LAPLACE, tol, leaf, fact_threads=workers)
     mu, _ = LS.linear_solve(fac, tree, b.boundary_values)
-    dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "DOF/s", "cores": 1, "kind": "port",
-            "sample": f"one FP64 tree+factor+solve of the same star workload at N={n} "
-                      f"(numpy/scipy oracle, OpenBLAS 1 thread): {dt:.2f} s"}
+    times.append(time.perf_counter() - t0)
+print(json.dumps({"times": times}))
+"""
+
+
+def cpu_baseline(n, leaf, tol, budget=12.0):
+    """The oracle (numpy/scipy restatement of the reference path) on the same
+    workload, its tree levels' boxes spread over the host cores as the
+    reference spreads them over fact_threads OpenMP threads
+    (skel_factorization.cpp:97-113): one forked process per core, BLAS
+    single-threaded.  Runs in a child process (no GPU state), repeated
+    until about `budget` seconds have passed (at most 5 runs)."""
+    import subprocess
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    workers = max(1, min(16, ncpu))
+    env = dict(os.environ, OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1", OMP_NUM_THREADS="1")
+    r = subprocess.run([sys.executable, "-c", CPU_BASELINE_CODE, ROOT, str(n), str(leaf), str(tol),
+                        str(workers), str(budget)], capture_output=True, text=True, env=env,
+                       timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError("cpu baseline failed: " + r.stderr[-2000:])
+    times = json.loads(r.stdout.strip().splitlines()[-1])["times"]
+    dt = float(np.mean(times))
+    return {"value": n / dt, "unit": "DOF/s", "cores": workers, "kind": "port",
+            "sample": f"{len(times)} FP64 tree+factor+solve runs of the same star workload at "
+                      f"N={n} (numpy/scipy oracle, each tree level's boxes over {workers} "
+                      f"processes, OpenBLAS 1 thread each): {dt:.2f} s per run"}
+
+
+def direct_residual(kigpu, ctx, h, mu, f):
+    """||f - K mu|| / ||f|| with K applied by the FP64 direct sum (kigpu_kernel_apply)."""
+    y = np.zeros_like(mu)
+    kigpu._check(ctx.lib.kigpu_kernel_apply(ctx.h, h, kigpu.LAPLACE, np.ascontiguousarray(mu),
+                                            1, y), "kigpu_kernel_apply")
+    return float(np.linalg.norm(f - y) / np.linalg.norm(f))
+
+
+def bench_stokes(kigpu, W, ctx, steps=2, warmup=1):
+    """Stokes double layer in the unit disc with 8 seeded circular holes
+    (16384 nodes outside, 2048 per hole: 65536 DOFs, 24 border unknowns),
+    Stokeslet data from (2, 1): tree + factor + bordered solve per step, FP64
+    and FP32 + refinement to 1e-10.  Host arrays in and out (the border
+    matrices go to the device once per factorization)."""
+    (pts, nrm, w, kap), holes = W.holed_disc(16384, 2048, 8, 7)
+    f = W.stokeslet_bc(pts)
+    U, Psi = W.border_matrices(True, holes, pts, w)
+    out = {"workload": "stokes_holed_disc_16384+8x2048_leaf64_tol1e-06", "dofs": int(f.size),
+           "border_unknowns": int(U.shape[1])}
+    for name, prec in (("fp64", kigpu.FP64), ("fp32_refine", kigpu.FP32_REFINE)):
+        refine = 20 if prec == kigpu.FP32_REFINE else 0
+
+        def step():
+            t = ctx.build_tree(pts, nrm, w, kap, 2, 64)
+            t.set_border(U, Psi)
+            t.factor(kigpu.STOKES, 1e-6, prec)
+            mu = t.solve(f, refine_iters=refine, rel_residual=1e-10)
+            return t, mu
+        for _ in range(warmup):
+            t, _ = step()
+            t.close()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            if i:
+                t.close()
+            t, mu = step()
+        dt = (time.perf_counter() - t0) / steps
+        tim = ctx.timings()
+        alpha = t.last_alpha
+        r1 = t.apply(mu, kigpu.STOKES) + U @ alpha - f
+        r2 = Psi @ mu - alpha
+        bwd = float(np.sqrt(np.sum(r1 ** 2) + np.sum(r2 ** 2)) / np.linalg.norm(f))
+        out[name] = {"s_per_step": dt, "dof_per_s": f.size / dt,
+                     "direct_backward_error": bwd,
+                     "refine": dict(t.last_info),
+                     "phases_s": {k: tim[k] for k in ("t_tree", "t_lists", "t_id", "t_decouple",
+                                                       "t_root", "t_solve", "t_refine_apply")}}
+        t.close()
+    out["speedup_fp32_refine_vs_fp64"] = out["fp64"]["s_per_step"] / out["fp32_refine"]["s_per_step"]
+    return out
 
 
 def main():
@@ -265,32 +348,44 @@ def main():
     for name, prec in (("fp64", kigpu.FP64), ("fp32_refine", kigpu.FP32_REFINE)):
         dt, tims, info, h, mu = time_path(torch, kigpu, ctx, prob, prec, args)
         err = interior_error(kigpu, ctx, h, mu)
+        dres = direct_residual(kigpu, ctx, h, mu[:, 0], F[:, 0])
         ctx.lib.kigpu_tree_destroy(h)
         tm = {k: float(np.median([t[k] for t in tims])) for k in tims[0]}
         res[name] = {"s_per_step": dt, "dof_per_s": args.n / dt, "interior_rel_err": err,
+                     "direct_rel_residual": dres,
                      "refine": info, "phases_s": {k: tm[k] for k in (
                          "t_tree", "t_lists", "t_id", "t_decouple", "t_root", "t_solve",
                          "t_refine_apply")}, "timings": tm}
 
-    # roofline of the kernel that dominates the FP32+refinement step: the ID
-    # kernel k_id<double, float> (ID matrix generation, pivoted QR to the rank
-    # step and T = R11^-1 R12, in FP64), one launch per tree level, timed with
-    # HIP events on the library's stream.  Algorithmic flops per box (m rows,
-    # n columns, rank k, generation cost c_K per entry): c_K m n + 2 m n +
-    # sum_{i<=k} [2 (m - i) + 4 (m - i)(n - i - 1)] + k^2 (n - k), summed over
-    # the level's boxes.
+    # roofline of the elimination kernel k_decouple_wave<float> (with the
+    # workgroup kernel for boxes above 64 DOFs): one wavefront per box, the
+    # Schur-update products on the FP32 matrix cores, timed per tree level
+    # with HIP events on the library's stream around both streams' launches.
+    # Algorithmic flops per box (n active DOFs, rank k, r = n - k): c_K n^2
+    # for the K_BN entries + 6 k^2 r + 8 k r^2 + 2/3 r^3, summed over the
+    # level; measured against the FP32 matrix peak.
     tf = res["fp32_refine"]["timings"]
+    dec_s = tf["dec_kernel_s"] / max(tf["dec_launches"], 1)
+    dec_f = tf["dec_flops"] / max(tf["dec_launches"], 1)
+    dec_achieved = dec_f / dec_s / 1e12 if dec_s > 0 else 0.0
+    roofline = {"bound": "mfma", "achieved": dec_achieved, "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": dec_achieved / PEAK_FP32_TFLOPS,
+                "traffic": pmc_traffic("pmc_decouple_laplace_star_N65536.md"),
+                "kernel": "k_decouple_wave<float> (elimination and Schur update, one wavefront "
+                          "per box, products on v_mfma_f32_16x16x4f32; each box is a chain of "
+                          "dependent steps, so the launch is bound by latency)",
+                "avg_launch_us": dec_s * 1e6, "launches_per_step": tf["dec_launches"],
+                "flops_per_launch": dec_f}
+    # the ID kernel beside it: FP64 vector arithmetic (no matrix cores),
+    # measured against the FP64 vector peak
     id_s = tf["id_kernel_s"] / max(tf["id_launches"], 1)
     id_f = tf["id_flops"] / max(tf["id_launches"], 1)
-    id_achieved = id_f / id_s / 1e12 if id_s > 0 else 0.0
-    roofline = {"bound": "mfma", "achieved": id_achieved, "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": id_achieved / PEAK_FP64_TFLOPS,
-                "traffic": pmc_traffic("pmc_id_laplace_star_N65536.md"),
-                "kernel": "k_id<double,float> (ID matrix + pivoted QR + T in FP64, one launch "
-                          "per level; each box is a chain of dependent Householder steps, "
-                          "bound by latency, measured against the FP64 peak)",
-                "avg_launch_us": id_s * 1e6, "launches_per_step": tf["id_launches"],
-                "flops_per_launch": id_f}
+    id_kernel = {"kernel": "k_id<double,float> (ID matrix + pivoted QR + T in FP64)",
+                 "limit": "latency of dependent Householder steps on the FP64 vector units",
+                 "achieved": id_f / id_s / 1e12 if id_s > 0 else 0.0,
+                 "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                 "avg_launch_us": id_s * 1e6, "launches_per_step": tf["id_launches"],
+                 "traffic": pmc_traffic("pmc_id_laplace_star_N65536.md")}
     # the refinement's residual apply: the treecode (box series + direct near
     # field).  Algorithmic flops: 8 (P - 1) + 15 per (target, series box) for
     # the Horner evaluation of the P <= 48 terms the box needs for the
@@ -308,38 +403,31 @@ def main():
                     "series_terms": tf["tc_series_terms"],
                     "eval_achieved_tflops": tc_flops / tc_eval_s / 1e12 if tc_eval_s > 0 else 0.0,
                     "direct_sum_equivalent_pairs": float(args.n) ** 2}
-    # the elimination kernel beside it (FP32 path)
-    dec_s = tf["dec_kernel_s"] / max(tf["dec_launches"], 1)
-    dec_f = tf["dec_flops"] / max(tf["dec_launches"], 1)
-    dec_kernel = {"kernel": "k_decouple<float> (elimination and Schur update per box)",
-                  "achieved": dec_f / dec_s / 1e12 if dec_s > 0 else 0.0,
-                  "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "avg_launch_us": dec_s * 1e6,
-                  "launches_per_step": tf["dec_launches"]}
-
     fast = res["fp32_refine"]
     out = {
         "metric": "factor+solve s & DOF/s at N=65536 FP32+refine vs FP64 GPU, rel err<=1e-6",
         "value": fast["dof_per_s"], "unit": "DOF/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": fast["s_per_step"] * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 elimination/solve + f64 ID, root LU and refinement",
+        "dtype": "f32 elimination (matrix cores) and solve + f64 ID, root LU and refinement",
         "data": "synthetic",
         "config": {"workload": f"laplace_star_N{args.n}_leaf{args.leaf}_tol{args.tol:g}",
                    "N": args.n, "nrhs": args.nrhs, "leaf": args.leaf, "id_tol": args.tol,
                    "refine_to": 1e-10},
         "fp64": {k: res["fp64"][k] for k in ("s_per_step", "dof_per_s", "interior_rel_err",
-                                               "phases_s")},
+                                               "direct_rel_residual", "phases_s")},
         "fp32_refine": {k: fast[k] for k in ("s_per_step", "dof_per_s", "interior_rel_err",
-                                             "refine", "phases_s")},
+                                             "direct_rel_residual", "refine", "phases_s")},
         "speedup_fp32_refine_vs_fp64": res["fp64"]["s_per_step"] / fast["s_per_step"],
         "roofline": roofline,
+        "id_kernel": id_kernel,
         "refine_apply": refine_apply,
-        "dec_kernel": dec_kernel,
     }
     if not args.no_extra:
         out["multi_rhs"] = bench_multi_rhs(torch, kigpu, W, ctx, args)
         out["batch"] = bench_batch(torch, kigpu, W, ctx, 2, 1)
         out["adjoint"] = bench_adjoint(kigpu, ctx, 2)
+        out["stokes_holes"] = bench_stokes(kigpu, W, ctx)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.leaf, args.tol)
     print(json.dumps(out))

@@ -26,10 +26,12 @@ FLAGS = [
 ]
 
 
-# per-file additions: solve.hip's matrix-core apply keeps its loop-carried
-# accumulators in the VGPR form of the MFMA instructions (the default
-# heuristic put them in AGPRs and copied them back and forth every step)
-FILE_FLAGS = {"solve.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+# per-file additions: the matrix-core loops of solve.hip (kernel apply) and
+# factor.hip (elimination products) keep their loop-carried accumulators in
+# the VGPR form of the MFMA instructions (the default heuristic put them in
+# AGPRs and copied them back and forth every step)
+FILE_FLAGS = {"solve.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+              "factor.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
 def _compile(src: str) -> str:

@@ -460,6 +460,138 @@ __device__ bool block_gj_inverse_perm(T* __restrict__ A, int n, int* __restrict_
   return ok;
 }
 
+// argmax over (value, index) of a wavefront in the value's own precision:
+// larger value wins, ties go to the smaller index; every lane gets the result
+template <typename V>
+__device__ __forceinline__ void wave_argmax_t(V& v, int& idx) {
+  auto step = [&](V v2, int i2) {
+    if (v2 > v || (v2 == v && i2 < idx)) {
+      v = v2;
+      idx = i2;
+    }
+  };
+  step(dpp::mov<dpp::kXor1>(v), dpp::mov<dpp::kXor1>(idx));
+  step(dpp::mov<dpp::kXor2>(v), dpp::mov<dpp::kXor2>(idx));
+  step(dpp::mov<dpp::kRor4>(v), dpp::mov<dpp::kRor4>(idx));
+  step(dpp::mov<dpp::kRor8>(v), dpp::mov<dpp::kRor8>(idx));
+  V bv = dpp::lane_value(v, 0);
+  int bi = dpp::lane_value(idx, 0);
+#pragma unroll
+  for (int l = 16; l < 64; l += 16) {
+    const V v2 = dpp::lane_value(v, l);
+    const int i2 = dpp::lane_value(idx, l);
+    if (v2 > bv || (v2 == bv && i2 < bi)) {
+      bv = v2;
+      bi = i2;
+    }
+  }
+  v = bv;
+  idx = bi;
+}
+
+// 1/d by the hardware reciprocal and Newton steps (one for FP32, two for
+// FP64: within an ulp or two of the correctly rounded quotient, measured
+// for FP64 on gfx950), for the pivots of the register Gauss-Jordan.
+__device__ __forceinline__ float fast_recip(float d) {
+  const float r = __builtin_amdgcn_rcpf(d);
+  return fmaf(r, fmaf(-d, r, 1.0f), r);
+}
+__device__ __forceinline__ double fast_recip(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  return fma(r, fma(-d, r, 1.0), r);
+}
+
+// X^{-1} of the r x r matrix X (LDS, column-major, ld r), r <= R <= 64, by
+// Gauss-Jordan exchange steps with partial pivoting.  Lane i holds row i in
+// registers (rows r .. R-1 are an identity border that never pivots while
+// a real row is left); the current column is always register 0: each step
+// writes the updated column j into register j - 1 and the pivot column's new
+// values into register R - 1, so after r steps register j holds column
+// (j + r) mod R.  Rows are not interchanged: step c picks the largest
+// |X(i, c)| over the rows not yet used (near-ties to 2^-17 go to the first
+// physical row, so the order can differ from dgetrf's interchanged one),
+// and X^{-1}(c', p(c)) ends up in the row of lane p(c') (p = the step's
+// pivot row).  Z (r x r, ld r) receives X^{-1}; piv_s (LDS, >= r ints)
+// the pivot row of each step; prow (LDS, R values) carries the pivot row
+// from its lane to the others.  Returns false on a zero or non-finite
+// pivot.
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return fma(a, b, c); }
+
+template <typename T, int R>
+__device__ bool wave_gj_reg(const T* X, int r, T* Z, int* piv_s, T* prow) {
+  const int lane = threadIdx.x & 63;
+  T a[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j)
+    a[j] = (lane < r && j < r) ? X[lane + (size_t)j * r] : (lane == j ? T(1) : T(0));
+  bool used = lane >= R;  // lanes past the border hold no row
+  bool ok = true;
+  int pmine = -1;  // the step at which this lane's row pivoted
+  for (int c = 0; c < r; ++c) {
+    // pivot search on 32-bit keys: the FP32 magnitude's bits with the low 6
+    // replaced by 63 - lane, so one unsigned wave maximum finds the largest
+    // candidate and, among candidates within 2^-17 of each other, the first
+    // row (partial pivoting needs no more than that); used rows key 0, a
+    // NaN candidate keys below every number
+    unsigned key = 0;
+    if (!used) {
+      const float m = fabsf((float)a[0]);
+      key = (m == m ? (__float_as_uint(m) & ~63u) : 0u) | (unsigned)(63 - lane);
+    }
+    key = max(key, (unsigned)dpp::mov<dpp::kXor1>((int)key));
+    key = max(key, (unsigned)dpp::mov<dpp::kXor2>((int)key));
+    key = max(key, (unsigned)dpp::mov<dpp::kRor4>((int)key));
+    key = max(key, (unsigned)dpp::mov<dpp::kRor8>((int)key));
+    const unsigned k0 = (unsigned)__builtin_amdgcn_readlane((int)key, 0);
+    const unsigned k1 = (unsigned)__builtin_amdgcn_readlane((int)key, 16);
+    const unsigned k2 = (unsigned)__builtin_amdgcn_readlane((int)key, 32);
+    const unsigned k3 = (unsigned)__builtin_amdgcn_readlane((int)key, 48);
+    const int p = 63 - (int)(max(max(k0, k1), max(k2, k3)) & 63u);
+    if (lane == p) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) prow[j] = a[j];
+    }
+    wave_sync();
+    const T pv = prow[0];
+    if (!pivot_usable(pv)) ok = false;
+    const T inv = fast_recip(pv);
+    const T f = a[0];
+    const T tc = lane == p ? inv - T(1) : -f * inv;
+    // the pivot row read 16 values at a time (the scheduling barriers keep
+    // all R of them from being held in registers at once)
+#pragma unroll
+    for (int j0 = 1; j0 < R; j0 += 16) {
+#pragma unroll
+      for (int j = j0; j < j0 + 16 && j < R; ++j) a[j - 1] = fma_t(tc, prow[j], a[j]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    a[R - 1] = lane == p ? inv : -f * inv;
+    if (lane == p) {
+      used = true;
+      pmine = c;
+    }
+    if (lane == 0) piv_s[c] = p;
+    wave_sync();  // every lane has read prow before the next step rewrites it
+  }
+  // (a border row pivots only when a column has no usable entry; the
+  // result is then flagged, and nothing is written outside Z)
+  if (pmine >= 0 && lane < r) {
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      int col = j + r;
+      if (col >= R) col -= R;
+      if (col < r) {
+        const int q = piv_s[col];
+        if (q < r) Z[pmine + (size_t)q * r] = a[j];
+      }
+    }
+  }
+  wave_sync();
+  return ok;
+}
+
 }  // namespace kigpu
 
 namespace kigpu {

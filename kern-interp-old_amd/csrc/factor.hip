@@ -862,7 +862,6 @@ __host__ __device__ inline size_t slab_elems(int n) { return ((size_t)n * n * 5 
 
 constexpr int kDecTB = 1024;
 constexpr int kDecNW = kDecTB / kWave;
-constexpr int kDecPair = 48;  // |active_box| up to which two eliminations share a CU (75 KiB LDS)
 
 // At most 64 VGPRs (8 waves per SIMD) so that two boxes share a CU when
 // their LDS fits: the elimination of one box is a chain of latency-bound
@@ -1036,6 +1035,378 @@ void k_decouple(const int* nodes, int a, GeomView g, int pde, const int* is_leaf
     Acc[i + (size_t)j * k] = s;
   });
   if (tid == 0) {
+    osize[x] = k;
+    acc_ptr[x] = reinterpret_cast<uint64_t>(Acc);
+    acc_ld[x] = k;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// elimination with one wavefront per box (|active_box| <= 64): the same
+// steps as k_decouple, with the products on the matrix cores
+// (v_mfma_f32_16x16x4f32 for the FP32 factor, v_mfma_f64_16x16x4f64 for
+// FP64) and X_rr^{-1} by a Gauss-Jordan held in registers, row i in lane i,
+// so no step waits on a workgroup barrier.
+// ---------------------------------------------------------------------------
+
+constexpr int kWdecMaxN = 64;
+
+typedef float wd_f4 __attribute__((ext_vector_type(4)));
+typedef double wd_d4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<float> {
+  typedef wd_f4 acc;
+  __device__ static acc step(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  // row of accumulator element v of lane l (column l % 16)
+  __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+template <>
+struct Mfma<double> {
+  typedef wd_d4 acc;
+  __device__ static acc step(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+
+// out(i, j, sum_t A(i, t) B(t, j)) for every (i, j) of an M x N product by
+// one wavefront, in 16 x 16 tiles with 4-deep MFMA steps.  The A operand of
+// lane l is A(i0 + l % 16, t0 + l / 16), the B operand B(t0 + l / 16,
+// j0 + l % 16); entries past M, N or K enter as zeros (read at a clamped
+// index and replaced, so the loads need no branch).  The next step's
+// operands are loaded before the current MFMA is issued.
+template <typename T, typename FA, typename FB, typename FO>
+__device__ __forceinline__ void wave_gemm(int M, int N, int K, FA&& A, FB&& B, FO&& out) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int l16 = lane & 15, l4 = lane >> 4;
+  for (int i0 = 0; i0 < M; i0 += 16)
+    for (int j0 = 0; j0 < N; j0 += 16) {
+      const int ia = i0 + l16, jb = j0 + l16;
+      const int iac = min(ia, M - 1), jbc = min(jb, N - 1);
+      auto ld = [&](int t0, T& a, T& b) {
+        const int t = t0 + l4, tc = min(t, K - 1);
+        const T av = A(iac, tc), bv = B(tc, jbc);
+        a = (ia < M && t < K) ? av : T(0);
+        b = (jb < N && t < K) ? bv : T(0);
+      };
+      typename Mfma<T>::acc c = {T(0), T(0), T(0), T(0)};
+      T a, b;
+      ld(0, a, b);
+      for (int t0 = 0; t0 < K; t0 += 4) {
+        T an, bn;
+        ld(min(t0 + 4, K - 1), an, bn);
+        c = Mfma<T>::step(a, b, c);
+        a = an;
+        b = bn;
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = i0 + Mfma<T>::row(lane, v);
+        if (i < M && jb < N) out(i, jb, c[v]);
+      }
+    }
+}
+
+
+// LDS of one k_decouple_wave box with |active_box| = n and rank k:
+// staged geometry, the [skel | redundant] DOFs, their active positions and
+// the inverse map, pivots, the Gauss-Jordan's pivot row, then
+//   Kp (n x n): K_BN; after X_rr is formed it holds X_rr^{-1}, and at the
+//               end the Schur update;
+//   X (r x r):  X_rr, kept for the refinement step of L;
+//   Tm (k x r): T; once X_rr is formed it holds L;
+//   K_r_sn (r x k), K_sn_r (k x r).
+struct WdecLds {
+  size_t geo, sr, pos, ipos, piv, gjv, kp, x, tm, krsn, ksnr, total;
+};
+__host__ __device__ inline WdecLds wdec_layout(int n, int k, size_t tsz) {
+  WdecLds L;
+  const int r = n - k;
+  size_t o = 0;
+  L.geo = o;  o = align16(o + sizeof(double) * kGeo * (size_t)n);
+  L.sr = o;   o = align16(o + sizeof(int) * (size_t)n);
+  L.pos = o;  o = align16(o + sizeof(int) * (size_t)n);
+  L.ipos = o; o = align16(o + sizeof(int) * (size_t)n);
+  L.piv = o;  o = align16(o + sizeof(int) * (size_t)kWdecMaxN);
+  L.gjv = o;  o = align16(o + tsz * (size_t)kWdecMaxN);
+  L.kp = o;   o = align16(o + tsz * (size_t)n * n);
+  L.x = o;    o = align16(o + tsz * (size_t)r * r);
+  L.tm = o;   o = align16(o + tsz * (size_t)k * r);
+  L.krsn = o; o = align16(o + tsz * (size_t)k * r);
+  L.ksnr = o; o = align16(o + tsz * (size_t)k * r);
+  L.total = o;
+  return L;
+}
+// the most any box of size <= n needs (over every rank k)
+__host__ inline size_t wdec_lds_bound(int n, size_t tsz) {
+  size_t best = wdec_layout(n, n, tsz).total;
+  for (int m = 1; m <= n; ++m)
+    for (int k = 1; k < m; ++k) best = std::max(best, wdec_layout(m, k, tsz).total);
+  return best;
+}
+
+// The children's accumulated updates of a box, flattened: child c's
+// osize x osize block holds entries [eo[c], eo[c + 1]) in column-major
+// order, entry (a, b) at active positions (co[c] + a, co[c] + b).
+struct ChildUpd {
+  int co[5], eo[5], cld[4];
+  uint64_t cacc[4];
+};
+
+// element c (0..3) of a 4-entry array held in registers (a runtime index
+// into the array itself would put it in scratch memory)
+template <typename V>
+__device__ __forceinline__ V pick4(const V* a, int c) {
+  return c == 0 ? a[0] : c == 1 ? a[1] : c == 2 ? a[2] : a[3];
+}
+
+// f(a', b', v) for the entries e = e0 + lane + 64 u (u < 8) of the
+// flattened update; the 8 loads of a lane are issued before any f runs.
+template <typename T, typename F>
+__device__ __forceinline__ void child_update_batch(const ChildUpd& cu, int e0, F&& f) {
+  const int lane = threadIdx.x & (kWave - 1);
+  T v[8];
+  int pa[8], pb[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int e = e0 + u * kWave + lane;
+    pa[u] = -1;
+    pb[u] = 0;
+    v[u] = T(0);
+    if (e < cu.eo[4]) {
+      const int c = (e >= cu.eo[1]) + (e >= cu.eo[2]) + (e >= cu.eo[3]);
+      const int c0 = pick4(cu.co, c), e0c = pick4(cu.eo, c);
+      const int sz = pick4(cu.co + 1, c) - c0;
+      const int loc = e - e0c;
+      int b = (int)(((float)loc + 0.5f) / (float)sz);  // exact: loc < 2^12
+      int a = loc - b * sz;
+      if (a < 0) { --b; a += sz; }
+      if (a >= sz) { ++b; a -= sz; }
+      const T* acc = reinterpret_cast<const T*>(pick4(cu.cacc, c));
+      if (acc) {
+        v[u] = acc[a + (size_t)b * pick4(cu.cld, c)];
+        pa[u] = c0 + a;
+        pb[u] = c0 + b;
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 8; ++u)
+    if (pa[u] >= 0) f(pa[u], pb[u], v[u]);
+}
+
+// RMAX: the largest r the launch's boxes can have (n <= RMAX + 1); the
+// register Gauss-Jordan is instantiated up to that width only.
+template <typename T, int RMAX>
+__global__ __launch_bounds__(kWave) void k_decouple_wave(
+    const int* nodes, GeomView g, int pde, const int* is_leaf, const int* child,
+    const int* osize_in, const uint64_t* acc_ptr_in, const int* acc_ld_in, const int* nact,
+    const int* rank, const int64_t* act_off, const int* sr_pool, const int* pos_pool,
+    int* piv_pool, const uint64_t* fac_ptr, int* osize, uint64_t* acc_ptr, int* acc_ld,
+    int* status) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int lane = threadIdx.x;
+  const int x = nodes[blockIdx.x];
+  // loads that need only x, issued together
+  const int n = nact[x];
+  const int kr = rank[x];
+  const bool leaf = is_leaf[x];
+  const int y = lane < 4 ? child[4 * x + lane] : -1;  // -1 for a leaf's slots
+  const int64_t aoff = act_off[x];
+  T* slab = reinterpret_cast<T*>(fac_ptr[x]);
+  if (n == 0) {
+    if (lane == 0) {
+      osize[x] = 0;
+      acc_ptr[x] = 0;
+      acc_ld[x] = 0;
+    }
+    return;
+  }
+  const int k = kr < 0 ? n : kr, r = n - k;
+  // the second round: children (lanes 0..3), the [skel | redundant] lists
+  // (one DOF per lane, n <= 64) and T (k x r, 8 values per lane per batch)
+  const int csz = (!leaf && y >= 0) ? osize_in[y] : 0;
+  const uint64_t cap = (!leaf && y >= 0) ? acc_ptr_in[y] : 0;
+  const int cldl = (!leaf && y >= 0) ? acc_ld_in[y] : 0;
+  int md = 0, ps = 0;
+  if (kr >= 0 && lane < n) {
+    md = sr_pool[aoff + lane];
+    ps = pos_pool[aoff + lane];
+  }
+  const int kt = kr >= 0 ? k * r : 0;
+  T tv[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int e = u * kWave + lane;
+    tv[u] = e < kt ? slab[e] : T(0);
+  }
+  ChildUpd cu;
+  cu.co[0] = cu.eo[0] = 0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int sz = __shfl(csz, c, kWave);
+    cu.co[c + 1] = cu.co[c] + sz;
+    cu.eo[c + 1] = cu.eo[c] + sz * sz;
+    cu.cacc[c] = (uint64_t)__shfl((long long)cap, c, kWave);
+    cu.cld[c] = __shfl(cldl, c, kWave);
+  }
+  if (kr < 0) {  // not compressed: hand the whole active list and its update up
+    if (!leaf) {  // lane a writes row a, each entry once
+      const int ca = (lane >= cu.co[1]) + (lane >= cu.co[2]) + (lane >= cu.co[3]);
+      const T* acca = reinterpret_cast<const T*>(pick4(cu.cacc, ca));
+      const int coa = pick4(cu.co, ca), lda = pick4(cu.cld, ca);
+      for (int b0 = 0; b0 < n; b0 += 8) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int b = b0 + u;
+          const int cb = (b >= cu.co[1]) + (b >= cu.co[2]) + (b >= cu.co[3]);
+          const bool in = lane < n && b < n && cb == ca && b < cu.co[4] && lane < cu.co[4] &&
+                          acca;
+          v[u] = in ? acca[(lane - coa) + (size_t)(b - coa) * lda] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (lane < n && b0 + u < n) slab[lane + (size_t)(b0 + u) * n] = v[u];
+      }
+    }
+    if (lane == 0) {
+      osize[x] = n;
+      acc_ptr[x] = leaf ? 0 : reinterpret_cast<uint64_t>(slab);
+      acc_ld[x] = n;
+    }
+    return;
+  }
+  const WdecLds Ly = wdec_layout(n, k, sizeof(T));
+  double* geo = reinterpret_cast<double*>(smem + Ly.geo);
+  int* sr = reinterpret_cast<int*>(smem + Ly.sr);
+  int* pos = reinterpret_cast<int*>(smem + Ly.pos);
+  int* ipos = reinterpret_cast<int*>(smem + Ly.ipos);
+  int* piv = reinterpret_cast<int*>(smem + Ly.piv);
+  T* gjv = reinterpret_cast<T*>(smem + Ly.gjv);  // pivot row of the Gauss-Jordan
+  T* Kp = reinterpret_cast<T*>(smem + Ly.kp);
+  T* X = reinterpret_cast<T*>(smem + Ly.x);
+  T* Ts = reinterpret_cast<T*>(smem + Ly.tm);
+  T* Krsn = reinterpret_cast<T*>(smem + Ly.krsn);
+  T* Ksnr = reinterpret_cast<T*>(smem + Ly.ksnr);
+  // the third round: this lane's DOF geometry (kept in registers for its row)
+  double mg[kGeo] = {0, 0, 0, 0, 0, 0};
+  if (lane < n) stage_geo(g, md, mg);
+  if (lane < n) {
+    sr[lane] = md;
+    pos[lane] = ps;
+    ipos[ps] = lane;
+#pragma unroll
+    for (int q = 0; q < kGeo; ++q) geo[kGeo * lane + q] = mg[q];
+  }
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int e = u * kWave + lane;
+    if (e < kt) Ts[e] = tv[u];
+  }
+  for (int e0 = 8 * kWave; e0 < kt; e0 += kWave) {  // T past the first batch
+    const int e = e0 + lane;
+    if (e < kt) Ts[e] = slab[e];
+  }
+  wave_sync();
+  // K_BN = K(BN, BN) - update, permuted to [s r] (skel_factorization.cpp:66-78):
+  // 64 / n lanes per row, lane l on row l % n and every (64 / n)-th column
+  // from l / n, four columns in flight
+  {
+    const int npl = kWave / n;
+    if (lane < npl * n) {
+      const int i = lane % n;
+      double tg[kGeo];
+#pragma unroll
+      for (int q = 0; q < kGeo; ++q) tg[q] = geo[kGeo * i + q];
+      const int ti = sr[i];
+#pragma unroll 4
+      for (int j = lane / n; j < n; j += npl)
+        Kp[i + (size_t)j * n] = (T)staged_entry(pde, ti, tg, sr[j], geo + kGeo * j);
+    }
+  }
+  wave_sync();
+  if (!leaf) {
+    for (int e0 = 0; e0 < cu.eo[4]; e0 += 8 * kWave)
+      child_update_batch<T>(cu, e0, [&](int a, int b, T v) {
+        if (a >= n || b >= n) return;
+        const size_t e = ipos[a] + (size_t)ipos[b] * n;
+        Kp[e] = (T)((double)Kp[e] - (double)v);
+      });
+    wave_sync();
+  }
+  auto KP = [&](int i, int j) -> T { return Kp[i + (size_t)j * n]; };
+  auto TT = [&](int i, int t) -> T { return Ts[t + (size_t)i * k]; };  // T^T (r x k)
+  auto TM = [&](int t, int j) -> T { return Ts[t + (size_t)j * k]; };  // T (k x r)
+  // K_r_sn = K_rs - T^T K_ss (r x k); K_sn_r = K_sr - K_ss T (k x r)
+  wave_gemm<T>(r, k, k, TT, [&](int t, int j) { return KP(t, j); },
+               [&](int i, int j, T v) { Krsn[i + (size_t)j * r] = KP(k + i, j) - v; });
+  wave_gemm<T>(k, r, k, [&](int i, int t) { return KP(i, t); }, TM,
+               [&](int i, int j, T v) { Ksnr[i + (size_t)j * k] = KP(i, k + j) - v; });
+  // X_rr = K_rr - T^T K_sr - K_r_sn T (r x r), in two products
+  wave_gemm<T>(r, r, k, TT, [&](int t, int j) { return KP(t, k + j); },
+               [&](int i, int j, T v) { X[i + (size_t)j * r] = KP(k + i, k + j) - v; });
+  wave_sync();
+  wave_gemm<T>(r, r, k, [&](int i, int t) { return Krsn[i + (size_t)t * r]; }, TM,
+               [&](int i, int j, T v) { X[i + (size_t)j * r] -= v; });
+  wave_sync();
+  // K_BN and T are dead: X_rr^{-1} goes where K_BN was, L where T was
+  T* Z = Kp;
+  T* Ls = Ts;
+  bool ok;
+  if (r <= 16) ok = wave_gj_reg<T, 16>(X, r, Z, piv, gjv);
+  else if (RMAX <= 32 || r <= 32) ok = wave_gj_reg<T, 32>(X, r, Z, piv, gjv);
+  else ok = wave_gj_reg<T, (RMAX > 32 ? 64 : 32)>(X, r, Z, piv, gjv);
+  if (!ok && lane == 0) flag(status, 4);
+  T* Lm = slab + (size_t)k * r;        // k x r
+  T* Um = slab + 2 * (size_t)k * r;    // r x k
+  T* Xinv = slab + 3 * (size_t)k * r;  // r x r
+  T* Acc = Xinv + (size_t)r * r;       // k x k
+  auto ZZ = [&](int i, int j) -> T { return Z[i + (size_t)j * r]; };
+  // L0 = K_sn_r X^{-1} (k x r) into Ls; U = X^{-1} K_r_sn (r x k) into the slab
+  wave_gemm<T>(k, r, r, [&](int i, int t) { return Ksnr[i + (size_t)t * k]; }, ZZ,
+               [&](int i, int j, T v) { Ls[i + (size_t)j * k] = v; });
+  wave_gemm<T>(r, k, r, ZZ, [&](int t, int j) { return Krsn[t + (size_t)j * r]; },
+               [&](int i, int j, T v) { Um[i + (size_t)j * r] = v; });
+  for (int e = lane; e < r * r; e += kWave) Xinv[e] = Z[e];
+  if (lane < r) piv_pool[aoff + lane] = piv[lane];
+  wave_sync();
+  // one refinement step of L against X_rr (see k_decouple): R = K_sn_r - L0 X_rr
+  // in place of K_sn_r, then L = L0 + R X_rr^{-1}
+  wave_gemm<T>(k, r, r, [&](int i, int t) { return Ls[i + (size_t)t * k]; },
+               [&](int t, int j) { return X[t + (size_t)j * r]; },
+               [&](int i, int j, T v) { Ksnr[i + (size_t)j * k] -= v; });
+  wave_sync();
+  wave_gemm<T>(k, r, r, [&](int i, int t) { return Ksnr[i + (size_t)t * k]; }, ZZ,
+               [&](int i, int j, T v) {
+                 const T l = Ls[i + (size_t)j * k] + v;
+                 Ls[i + (size_t)j * k] = l;
+                 Lm[i + (size_t)j * k] = l;
+               });
+  wave_sync();
+  // schur_update = L K_r_sn (k x k) where X_rr^{-1} was, plus the
+  // descendants' update on the skeleton, then out to the slab
+  T* Sa = Kp;
+  wave_gemm<T>(k, k, r, [&](int i, int t) { return Ls[i + (size_t)t * k]; },
+               [&](int t, int j) { return Krsn[t + (size_t)j * r]; },
+               [&](int i, int j, T v) { Sa[i + (size_t)j * k] = v; });
+  wave_sync();
+  if (!leaf) {
+    for (int e0 = 0; e0 < cu.eo[4]; e0 += 8 * kWave)
+      child_update_batch<T>(cu, e0, [&](int a, int b, T v) {
+        if (a >= n || b >= n) return;
+        const int i = ipos[a], j = ipos[b];
+        if (i < k && j < k) Sa[i + (size_t)j * k] += v;
+      });
+    wave_sync();
+  }
+  for (int e = lane; e < k * k; e += kWave) Acc[e] = Sa[e];
+  if (lane == 0) {
     osize[x] = k;
     acc_ptr[x] = reinterpret_cast<uint64_t>(Acc);
     acc_ld[x] = k;
@@ -1265,18 +1636,40 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     std::vector<IdWork> wl(wl_glb);
     wl.insert(wl.end(), wl_lds.begin(), wl_lds.end());
     const int nwl = (int)wl.size();
-    // elimination: boxes whose LDS lets two share a CU, and the larger ones
-    std::vector<int> dec_small, dec_large;
-    for (int i = 0; i < cn; ++i) (h_n[i] <= kDecPair ? dec_small : dec_large).push_back(a + i);
+    // elimination: boxes of at most kWdecMaxN active DOFs one wavefront
+    // each, in two size classes (their LDS differs fourfold), largest
+    // first; the larger boxes a workgroup each
+    std::vector<int> dec_small, dec_mid, dec_large;
+    int nmax_small = 0, nmax_mid = 0, nmax_large = 0;
+    {
+      std::vector<int> ord(cn);
+      for (int i = 0; i < cn; ++i) ord[i] = i;
+      std::stable_sort(ord.begin(), ord.end(), [&](int p, int q) { return h_n[p] > h_n[q]; });
+      for (int i : ord) {
+        const int nb = h_n[i];
+        if (nb <= kWdecMaxN / 2) {
+          dec_small.push_back(a + i);
+          nmax_small = std::max(nmax_small, nb);
+        } else if (nb <= kWdecMaxN) {
+          dec_mid.push_back(a + i);
+          nmax_mid = std::max(nmax_mid, nb);
+        } else {
+          dec_large.push_back(a + i);
+          nmax_large = std::max(nmax_large, nb);
+        }
+      }
+    }
     // ---- the level's lists in one copy
     UploadPack pk;
     const size_t o_soff = pk.add(soff.data(), sizeof(int64_t) * cn);
     const size_t o_roff = pk.add(rows_off.data(), sizeof(int64_t) * cn);
     const size_t o_wl = pk.add(wl.data(), sizeof(IdWork) * wl.size());
     const size_t o_ds = pk.add(dec_small.data(), sizeof(int) * dec_small.size());
+    const size_t o_dm = pk.add(dec_mid.data(), sizeof(int) * dec_mid.size());
     const size_t o_dl = pk.add(dec_large.data(), sizeof(int) * dec_large.size());
     unsigned char* dpk = pk.send(c.pin_up, lw->pack, s);
     const int* d_dec_small = reinterpret_cast<const int*>(dpk + o_ds);
+    const int* d_dec_mid = reinterpret_cast<const int*>(dpk + o_dm);
     const int* d_dec_large = reinterpret_cast<const int*>(dpk + o_dl);
     const int64_t* d_soff = reinterpret_cast<const int64_t*>(dpk + o_soff);
     const int64_t* d_roff = reinterpret_cast<const int64_t*>(dpk + o_roff);
@@ -1333,39 +1726,58 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     // ---- elimination for every node of the level
     E.dec0 = mark();
     {
-      DecLds L = dec_layout(std::max(nmax, 1), sizeof(T));
-      auto dec_launch = [&](const int* nodes, int count, size_t lds, hipStream_t st) {
-        allow_lds(k_decouple<T, true>, lds);
-        k_decouple<T, true><<<count, kDecTB, lds, st>>>(
-            nodes, a, g, pde, t.is_leaf.p, t.child.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p,
-            t.nact.p, t.rank.p, t.act_off.p, t.act_pool.p, t.sr_pool.p, t.pos_pool.p,
-            t.piv_pool.p, t.fac_ptr.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, (T*)nullptr, 0,
-            t.status.p);
+      auto wave_launch = [&](const int* nodes, int count, int nmx, hipStream_t st) {
+        const size_t lds = wdec_lds_bound(std::max(nmx, 1), sizeof(T));
+        auto kern = nmx <= kWdecMaxN / 2 ? k_decouple_wave<T, kWdecMaxN / 2>
+                                         : k_decouple_wave<T, kWdecMaxN>;
+        allow_lds(kern, lds);
+        kern<<<count, kWave, lds, st>>>(
+            nodes, g, pde, t.is_leaf.p, t.child.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p,
+            t.nact.p, t.rank.p, t.act_off.p, t.sr_pool.p, t.pos_pool.p, t.piv_pool.p,
+            t.fac_ptr.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, t.status.p);
       };
-      if (L.total <= lds_cap && !dec_large.empty() && !dec_small.empty()) {
-        // the large boxes on the side stream, the small ones two per CU
-        KI_HIP(hipEventRecord(c.ev_fork, s));
-        KI_HIP(hipStreamWaitEvent(c.side, c.ev_fork, 0));
-        dec_launch(d_dec_large, (int)dec_large.size(), L.total, c.side);
-        dec_launch(d_dec_small, (int)dec_small.size(), dec_layout(kDecPair, sizeof(T)).total, s);
-        KI_HIP(hipEventRecord(c.ev_join, c.side));
-        KI_HIP(hipStreamWaitEvent(s, c.ev_join, 0));
-      } else if (L.total <= lds_cap) {
-        dec_launch(nullptr, cn, L.total, s);
-      } else {
+      auto large_launch = [&](hipStream_t st) {
+        const int nl = (int)dec_large.size();
+        DecLds L = dec_layout(nmax_large, sizeof(T));
+        if (L.total <= lds_cap) {
+          allow_lds(k_decouple<T, true>, L.total);
+          k_decouple<T, true><<<nl, kDecTB, L.total, st>>>(
+              d_dec_large, a, g, pde, t.is_leaf.p, t.child.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p,
+              t.nact.p, t.rank.p, t.act_off.p, t.act_pool.p, t.sr_pool.p, t.pos_pool.p,
+              t.piv_pool.p, t.fac_ptr.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, (T*)nullptr, 0,
+              t.status.p);
+          return;
+        }
         size_t stride = (L.total + sizeof(T) - 1) / sizeof(T);
         T* ws;
         if (sizeof(T) == 8) {
-          lw->dws64.alloc(stride * cn);
+          lw->dws64.alloc(stride * nl);
           ws = reinterpret_cast<T*>(lw->dws64.p);
         } else {
-          lw->dws32.alloc(stride * cn);
+          lw->dws32.alloc(stride * nl);
           ws = reinterpret_cast<T*>(lw->dws32.p);
         }
-        k_decouple<T, false><<<cn, kDecTB, 0, s>>>(
-            nullptr, a, g, pde, t.is_leaf.p, t.child.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, t.nact.p,
-            t.rank.p, t.act_off.p, t.act_pool.p, t.sr_pool.p, t.pos_pool.p, t.piv_pool.p,
-            t.fac_ptr.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, ws, stride, t.status.p);
+        k_decouple<T, false><<<nl, kDecTB, 0, st>>>(
+            d_dec_large, a, g, pde, t.is_leaf.p, t.child.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p,
+            t.nact.p, t.rank.p, t.act_off.p, t.act_pool.p, t.sr_pool.p, t.pos_pool.p,
+            t.piv_pool.p, t.fac_ptr.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, ws, stride,
+            t.status.p);
+      };
+      // the middle size class on the side stream beside the small boxes and
+      // then the large ones on the main stream
+      const bool side = !dec_mid.empty() && (!dec_small.empty() || !dec_large.empty());
+      if (side) {
+        KI_HIP(hipEventRecord(c.ev_fork, s));
+        KI_HIP(hipStreamWaitEvent(c.side, c.ev_fork, 0));
+        wave_launch(d_dec_mid, (int)dec_mid.size(), nmax_mid, c.side);
+      } else if (!dec_mid.empty()) {
+        wave_launch(d_dec_mid, (int)dec_mid.size(), nmax_mid, s);
+      }
+      if (!dec_small.empty()) wave_launch(d_dec_small, (int)dec_small.size(), nmax_small, s);
+      if (!dec_large.empty()) large_launch(s);
+      if (side) {
+        KI_HIP(hipEventRecord(c.ev_join, c.side));
+        KI_HIP(hipStreamWaitEvent(s, c.ev_join, 0));
       }
       KI_HIP(hipGetLastError());
     }

@@ -78,7 +78,7 @@ def initialize_Psi(pde, holes, boundary):
     return Psi
 
 
-def factor(boundary, pde, id_tol, leaf_dofs=32):
+def factor(boundary, pde, id_tol, leaf_dofs=32, fact_threads=1):
     """Tree build + ``skeletonize`` with the hole border set (linear_solve.cpp:154-163)."""
     sd = 2 if pde == kmod.STOKES else 1
     tree = QuadTree(boundary.points, sd, leaf_dofs)
@@ -86,7 +86,7 @@ def factor(boundary, pde, id_tol, leaf_dofs=32):
     U = initialize_U(pde, boundary.holes, boundary.points)
     tree.U = U if U.size else None
     tree.Psi = initialize_Psi(pde, boundary.holes, boundary) if U.size else None
-    fac = SkelFactorization(id_tol)
+    fac = SkelFactorization(id_tol, fact_threads)
     fac.skeletonize(kern, tree)
     return tree, kern, fac
 

@@ -144,16 +144,22 @@ class SkelFactorization:
     def skeletonize(self, kernel, tree):
         """``skeletonize`` (skel_factorization.cpp:94-243)."""
         nl = len(tree.levels)
+
+        def box(level, node):
+            if node.compressed or len(node.active_box) < MIN_DOFS_TO_COMPRESS:
+                return None
+            if self.id_compress(kernel, tree, node) == 0:
+                return None
+            self.decouple(kernel, node)
+            return (level, node.id, len(node.active_box), node.m_rows, node.T.shape[0])
+
         for level in range(nl - 1, 0, -1):
             tree.refresh_level(level)
-            for node in tree.levels[level]:
-                if node.compressed or len(node.active_box) < MIN_DOFS_TO_COMPRESS:
-                    continue
-                if self.id_compress(kernel, tree, node) == 0:
-                    continue
-                self.decouple(kernel, node)
-                self.stats.append((level, node.id, len(node.active_box), node.m_rows,
-                                   node.T.shape[0]))
+            nodes = tree.levels[level]
+            if self.fact_threads > 1 and len(nodes) >= 2 * self.fact_threads:
+                self.stats.extend(_level_in_processes(box, level, nodes, self.fact_threads))
+            else:
+                self.stats.extend(r for r in (box(level, nd) for nd in nodes) if r is not None)
         tree.refresh_all()
         allskel = list(tree.root.active_box)
         tree.allskel = allskel
@@ -312,6 +318,45 @@ class SkelFactorization:
                 sweep(-nd.U, mu, nd.skel, nd.redundant, False)
                 sweep(-nd.T, mu, nd.redundant, nd.skel, False)
         return mu[:, 0], alpha[:, 0]
+
+
+_FORK_STATE = {}
+_BOX_FIELDS = ("T", "permutation", "skel", "redundant", "m_rows", "X_lu", "X_piv", "L", "U",
+               "schur", "compressed")
+
+
+def _fork_worker(args):
+    """Runs the boxes [a, b) of the level in a forked copy of the state."""
+    a, b = args
+    box, level, nodes = _FORK_STATE["job"]
+    out = []
+    for i in range(a, b):
+        st = box(level, nodes[i])
+        if st is not None:
+            nd = nodes[i]
+            out.append((i, st, {f: getattr(nd, f) for f in _BOX_FIELDS}))
+    return out
+
+
+def _level_in_processes(box, level, nodes, workers):
+    """The boxes of one level in `workers` forked processes (the
+    reference runs them on `fact_threads` OpenMP threads,
+    skel_factorization.cpp:97-113); each child works on a copy-on-write image
+    of the tree, and the compressed boxes' results are copied back."""
+    import multiprocessing as mp
+    _FORK_STATE["job"] = (box, level, nodes)
+    n = len(nodes)
+    cuts = [n * w // workers for w in range(workers + 1)]
+    with mp.get_context("fork").Pool(workers) as pool:
+        parts = pool.map(_fork_worker, [(cuts[w], cuts[w + 1]) for w in range(workers)])
+    stats = []
+    for part in parts:
+        for i, st, fields in part:
+            for f, v in fields.items():
+                setattr(nodes[i], f, v)
+            stats.append(st)
+    _FORK_STATE.clear()
+    return stats
 
 
 def sweep(M, vec, a, b, transpose):

@@ -138,11 +138,21 @@ inline unsigned long long __ballot(int pred) {
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 
 inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }
+inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
+inline void __builtin_amdgcn_sched_barrier(int) {}
 inline float __int_as_float(int v) {
   float f;
   std::memcpy(&f, &v, 4);
   return f;
 }
+inline unsigned __float_as_uint(float f) {
+  unsigned v;
+  std::memcpy(&v, &f, 4);
+  return v;
+}
+inline unsigned max(unsigned a, unsigned b) { return a > b ? a : b; }
+inline int max(int a, int b) { return a > b ? a : b; }
+inline int min(int a, int b) { return a < b ? a : b; }
 inline int __float_as_int(float f) {
   int v;
   std::memcpy(&v, &f, 4);
@@ -160,4 +170,7 @@ inline double __longlong_as_double(long long v) {
 }
 inline int atomicOr(int* p, int v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
 using std::fabs;
+using std::fma;
+using std::fmaf;
+inline float fabsf(float x) { return std::fabs(x); }
 using std::sqrt;

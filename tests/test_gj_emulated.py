@@ -15,6 +15,10 @@ pivot order, so the checks are:
   is numerically singular there (two pivots at roundoff level, cond ~4e17),
   and the routines must stay inside their arrays and give an S inverse to
   the accuracy cond(S) allows;
+* the register-resident Gauss-Jordan of the one-wavefront elimination
+  (wave_gj_reg: row i in lane i, no row interchanges, the pivot row
+  broadcast through LDS) in FP64 and FP32 at every register width it has
+  (16, 32, 64 rows), against LAPACK's inverse;
 * a column without a usable pivot (all zero, or NaN): the routine reports
   it and every index stays in bounds.  With the first-candidate search of
   the earlier code (best index starting at n) a zero pivot turned the next
@@ -34,7 +38,7 @@ from oracle import kernel as K
 from oracle import linear_solve as LS
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-PERM, BLOCK, WAVE = 0, 1, 2
+PERM, BLOCK, WAVE, REG64, REG32 = 0, 1, 2, 3, 4
 
 
 @pytest.fixture(scope="module")
@@ -127,3 +131,28 @@ def test_largest_root_size(emu):
     ok, piv, inv = run(emu, A, PERM, 256)
     assert ok
     assert np.abs(inv @ A - np.eye(n)).max() <= 1e-11
+
+
+@pytest.mark.parametrize("variant", [REG64, REG32])
+@pytest.mark.parametrize("r", [1, 7, 16, 17, 25, 32, 33, 59, 64])
+def test_register_gauss_jordan(emu, variant, r):
+    rng = np.random.default_rng(100 + r)
+    X = rng.standard_normal((r, r)) + 3 * np.eye(r)
+    ok, piv, Z = run(emu, X, variant, 64)
+    assert ok
+    assert sorted(piv) == list(range(r))   # each row pivots once
+    ref = np.linalg.inv(X)
+    tol = 1e-12 if variant == REG64 else 2e-5
+    assert np.abs(Z - ref).max() <= tol * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("variant", [REG64, REG32])
+@pytest.mark.parametrize("bad", ["zero", "nan"])
+def test_register_gauss_jordan_unusable_pivot(emu, variant, bad):
+    rng = np.random.default_rng(9)
+    r = 40
+    X = rng.standard_normal((r, r))
+    X[:, 7] = 0.0 if bad == "zero" else np.nan
+    ok, piv, _ = run(emu, X, variant, 64)
+    assert not ok
+    assert np.all((piv >= 0) & (piv < 64))

@@ -28,3 +28,19 @@ def test_batch_sizes_range():
     n = W.batch_sizes(256, 4)
     assert n.min() >= 1024 and n.max() <= 16384 and np.all(n % 64 == 0)
     assert 6000 < n.mean() < 11000
+
+
+def test_holed_disc_and_border_match():
+    from oracle import kernel as K
+    from oracle import linear_solve as LS
+    (pts, nrm, w, kap), holes = W.holed_disc(4096, 512, 8, 7)
+    b = G.multi_hole_disc(4096, 512, 8, 7)
+    for x, y in zip((pts, nrm, w, kap), (b.points, b.normals, b.weights, b.curvatures)):
+        assert np.array_equal(x, y)
+    assert [(h[0], h[1], h[2]) for h in holes] == [(h.center[0], h.center[1], h.radius)
+                                                  for h in b.holes]
+    for stokes, pde in ((False, K.LAPLACE), (True, K.STOKES)):
+        U, Psi = W.border_matrices(stokes, holes, pts, w)
+        assert np.array_equal(U, LS.initialize_U(pde, b.holes, b.points))
+        assert np.array_equal(Psi, LS.initialize_Psi(pde, b.holes, b))
+    assert np.array_equal(W.stokeslet_bc(pts), G.bc_stokeslet(b))
