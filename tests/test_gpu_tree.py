@@ -59,3 +59,29 @@ def test_tree_bit_exact(ctx, name, make, sd, leaf):
     assert np.array_equal(nbrs, ref["nbrs"])
     info = t.info()
     assert info["min"] == float(np.min(b.points)) and info["max"] == float(np.max(b.points))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_tree_hand_traced(ctx, case):
+    # the four-point tree traced by hand through quadtree.cpp:27-285
+    # (tests/golden/tree_trace_4pts.md)
+    import json
+    import os
+    print(f"[tree] hand-traced fixture case {case}", flush=True)
+    with open(os.path.join(os.path.dirname(__file__), "golden", "tree_trace_4pts.json")) as fh:
+        fx = json.load(fh)
+    sd, leaf = fx["cases"][case]["sd"], fx["cases"][case]["leaf_dofs"]
+    pts = np.array(fx["points"])
+    n = pts.size // 2
+    t = ctx.build_tree(pts, np.tile([1.0, 0.0], n), np.full(n, 1.0 / n), np.zeros(n), sd, leaf)
+    got = t.export_nodes()
+    assert got["level_offsets"].tolist() == fx["level_offsets"]
+    assert got["parent"].tolist() == fx["parent"]
+    assert got["children"].tolist() == fx["children"]
+    assert got["is_leaf"].tolist() == fx["is_leaf"]
+    off, dofs = t.export_boxes()
+    boxes = [[sd * p + c for p in b for c in range(sd)] for b in fx["original_box_points"]]
+    assert [dofs[off[i]:off[i + 1]].tolist() for i in range(len(boxes))] == boxes
+    noff, nbrs = t.export_neighbors()
+    assert [nbrs[noff[i]:noff[i + 1]].tolist() for i in range(len(boxes))] == fx["neighbors"]

@@ -81,3 +81,47 @@ def test_stokes_rigid_rotation():
     r = tgt.reshape(-1, 2) - 0.5
     truth = 4.0 * np.stack([-r[:, 1], r[:, 0]], 1)
     assert np.linalg.norm(u - truth) / np.linalg.norm(truth) < 1e-10
+
+
+def test_neumann_circle_closed_form():
+    # LAPLACE_NEUMANN on a circle of radius R (kernel.cpp:23-33): with
+    # r = t - s and n_t = (t - c)/R, (r . n_t)/|r|^2 = 1/(2R) for every pair,
+    # so the off-diagonal entries are w + w/(4 pi R) and the diagonal
+    # (kappa = 1/R) w - 1/2 + w/(4 pi R): K = -I/2 + (w + w/(4 pi R)) 1 1^T
+    b = G.circle(256)
+    R = 0.25
+    w = b.weights[0]
+    kern = K.Kernel(1, K.LAPLACE_NEUMANN, b)
+    A = kern(np.arange(256), np.arange(256))
+    c = w + w / (4 * math.pi * R)
+    ref = -0.5 * np.eye(256) + c
+    assert np.max(np.abs(A - ref)) <= 1e-11 * c
+
+
+def test_neumann_solve_sherman_morrison():
+    # the factor/solve of the Neumann kernel on the circle against the inverse
+    # of -I/2 + c 1 1^T: (1/a)(I - c 1 1^T / (a + c N)), a = -1/2
+    n = 1024
+    b = G.circle(n)
+    b.boundary_values = G.bc_single_electron(b)
+    R = 0.25
+    w = b.weights[0]
+    c = w + w / (4 * math.pi * R)
+    tree, kern, fac = LS.factor(b, K.LAPLACE_NEUMANN, 1e-12, 32)
+    mu, _ = LS.linear_solve(fac, tree, b.boundary_values)
+    f = b.boundary_values
+    a = -0.5
+    exact = (f - c * f.sum() / (a + c * n)) / a
+    assert np.linalg.norm(mu - exact) / np.linalg.norm(exact) < 1e-9
+
+
+def test_neumann_forward_single_layer_of_constant():
+    # the Neumann forward entries w/(2 pi) log|x - s| (kernel.cpp:24-27) sum a
+    # trapezoidal single layer; for density 1 on a circle of radius R the
+    # potential inside is R log R (mean value of log|x - s| over the circle)
+    b = G.circle(512)
+    R = 0.25
+    tgt = G.interior_disc_targets(30, 5, radius=0.15) + 0.5
+    kd = K.Kernel(1, K.LAPLACE_NEUMANN, b, tgt)
+    u = kd.forward() @ np.ones(512)
+    assert np.max(np.abs(u - R * math.log(R))) < 1e-12

@@ -117,3 +117,54 @@ def test_same_level_neighbors_geometric(name, make, sd, leaf):
                 if dx <= a.side * 1.000001 and dy <= a.side * 1.000001:
                     want.add(id(c))
             assert got == want
+
+
+def _fixture():
+    import json
+    import os
+    p = os.path.join(os.path.dirname(__file__), "golden", "tree_trace_4pts.json")
+    with open(p) as fh:
+        return json.load(fh)
+
+
+def _expected_boxes(fx, sd):
+    return [[sd * p + c for p in pts for c in range(sd)] for pts in fx["original_box_points"]]
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_hand_traced_tree(case):
+    # node order (subdivision events, not slot paths), DOF lists and neighbor
+    # lists -- including the coarse leaves that get_descendent_neighbors
+    # pushes and the contacts of the `y < top && y < bottom` test
+    # (quadtree.cpp:143) -- against the trace in tests/golden
+    fx = _fixture()
+    sd, leaf = fx["cases"][case]["sd"], fx["cases"][case]["leaf_dofs"]
+    ex = QuadTree(np.array(fx["points"]), sd, leaf).export()
+    assert ex["level_offsets"].tolist() == fx["level_offsets"]
+    assert ex["parent"].tolist() == fx["parent"]
+    assert ex["children"].tolist() == fx["children"]
+    assert ex["is_leaf"].tolist() == fx["is_leaf"]
+    boxes = _expected_boxes(fx, sd)
+    off, dofs = ex["box_offsets"], ex["box_dofs"]
+    assert [dofs[off[i]:off[i + 1]].tolist() for i in range(len(boxes))] == boxes
+    noff, nbrs = ex["nbr_offsets"], ex["nbrs"]
+    assert [nbrs[noff[i]:noff[i + 1]].tolist() for i in range(len(boxes))] == fx["neighbors"]
+
+
+@pytest.mark.parametrize("name,make,sd,leaf", CASES, ids=[c[0] for c in CASES])
+def test_child_boxes_contain_their_points(name, make, sd, leaf):
+    # child slots checked by containment in the child's own corners
+    # (half-open: a point on a midline belongs to the right / upper child,
+    # quadtree.cpp:177-185), and each box is the union of its children's
+    b = make()
+    tree = QuadTree(b.points, sd, leaf)
+    xy = b.points.reshape(-1, 2)
+    for n in tree.all_nodes():
+        c = n.corners
+        for d in n.original_box[::sd]:
+            x, y = xy[d // sd]
+            assert c[0] <= x and c[1] <= y
+            assert (x < c[4] or n.parent is None) and (y < c[3] or n.parent is None)
+        if not n.is_leaf:
+            kids = sorted(d for ch in n.children for d in ch.original_box)
+            assert kids == sorted(n.original_box)
