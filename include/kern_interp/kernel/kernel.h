@@ -36,6 +36,9 @@ struct Kernel {
   std::vector<double> boundary_points, boundary_normals, boundary_weights, boundary_curvatures,
       domain_points;
 
+  // one Laplace / Laplace-Neumann entry into ret->mat[mat_idx] (kernel.cpp:11-36)
+  void laplace(int mat_idx, ki_Mat* ret, double r1, double r2, double sw, double sc, double sn1,
+               double sn2, double tn1, double tn2, bool forward = false) const;
   ki_Mat operator()(const std::vector<int>& tgt_inds, const std::vector<int>& src_inds,
                     bool forward = false) const;
   void update_boundary(Boundary* boundary);
@@ -47,6 +50,11 @@ struct Kernel {
   ki_Mat get_id_mat(const QuadTree* tree, const QuadTreeNode* node) const;
   ki_Mat get_proxy_mat(double cntr_x, double cntr_y, double r, const QuadTree* tree,
                        const std::vector<int>& box_inds) const;
+  // proxy rows against explicit proxy points (kernel.cpp:268-334), on the GPU
+  ki_Mat laplace_proxy_get(const std::vector<double>& pxy_p, const std::vector<double>& pxy_n,
+                           double pxy_w, const std::vector<int>& box_inds) const;
+  ki_Mat stokes_proxy_get(const std::vector<double>& pxy_p, const std::vector<double>& pxy_n,
+                          double pxy_w, const std::vector<int>& box_inds) const;
   ki_Mat forward() const;
 
  private:

@@ -77,6 +77,14 @@ class QuadTree {
   void remove_inactive_dofs_at_all_boxes();
   void remove_inactive_dofs_at_box(QuadTreeNode* node);
 
+  // Host construction steps on these node structs (quadtree.cpp:27-285),
+  // with leaf_dofs as the leaf capacity.  initialize_tree does not use them:
+  // it builds the same tree on the GPU.
+  void compute_neighbor_lists();
+  void recursive_add(QuadTreeNode* node, double x, double y, int point_ind);
+  void get_descendent_neighbors(QuadTreeNode* big, QuadTreeNode* small);
+  void node_subdivide(QuadTreeNode* node);
+
   // device handle (geometry, tree, factorization)
   kigpu_tree* gpu() const { return gpu_; }
   // node ids in level order, as the C ABI numbers them

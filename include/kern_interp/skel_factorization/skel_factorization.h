@@ -1,7 +1,9 @@
 // Drop-in for kern_interp/skel_factorization/skel_factorization.h
 // (reference skel_factorization.h:19-62).  skeletonize/solve run on the GPU
-// through kigpu_factor / kigpu_solve; the per-box building blocks
-// (id_compress, decouple, get_*_updates) are internal to those kernels.
+// through kigpu_factor / kigpu_solve (the factorization stays on the device).
+// The per-box building blocks work on the host node structs like the
+// reference's: id_compress (ID on the GPU, kigpu_interp_decomp), decouple
+// (kigpu_schur_eliminate) and the get_*_updates bookkeeping.
 #ifndef KERN_INTERP_SKEL_FACTORIZATION_SKEL_FACTORIZATION_H_
 #define KERN_INTERP_SKEL_FACTORIZATION_SKEL_FACTORIZATION_H_
 
@@ -28,6 +30,15 @@ class SkelFactorization {
   SkelFactorization() {}
   SkelFactorization(double id_tol, int fact_threads);
   ~SkelFactorization() {}
+
+  void decouple(const Kernel& K, QuadTreeNode* node);
+  int id_compress(const Kernel& K, const QuadTree* tree, QuadTreeNode* node);
+
+  void get_all_schur_updates(ki_Mat* updates, const std::vector<int>& BN,
+                             const QuadTreeNode* node) const;
+  void get_descendents_updates(ki_Mat* updates, const std::vector<int>& BN,
+                               const QuadTreeNode* node) const;
+  void get_update(ki_Mat* updates, const std::vector<int>& BN, const QuadTreeNode* node) const;
 
   void skeletonize(const Kernel& K, QuadTree* tree);
 

@@ -242,6 +242,32 @@ kigpu_status kigpu_kernel_block(kigpu_ctx* ctx, const kigpu_geom* geom, int pde,
                                 const int* tgt, int ntgt, const int* src, int nsrc, int forward,
                                 double* out);
 
+/* Kernel::laplace_proxy_get / stokes_proxy_get (kernel.cpp:268-334): the
+ * rows [proxy <- box; box <- proxy] of the box DOFs against np explicit proxy
+ * points (pxy_p, pxy_n: 2*np interleaved) of quadrature weight pxy_w.
+ * out: (2*np for Laplace, 4*np for Stokes) x nbox, column-major. */
+kigpu_status kigpu_proxy_block(kigpu_ctx* ctx, const kigpu_geom* geom, int pde,
+                               const double* pxy_p, const double* pxy_n, int np, double pxy_w,
+                               const int* box, int nbox, double* out);
+/* ki_Mat::id (ki_mat.cpp:505-541) of a host matrix A (m x n, column-major) by
+ * the pivoted QR of the factorization: k = the first i >= 1 with
+ * |R_ii| < tol |R_00| (0: not compressible), perm[n] the column pivots
+ * (0-based; skeleton first), T (k x (n - k)) = R11^{-1} R12.  perm and T are
+ * written only when k > 0.  m < n returns KIGPU_ERR_RANK. */
+kigpu_status kigpu_interp_decomp(kigpu_ctx* ctx, const double* A, int m, int n, double tol,
+                                 int* k, int* perm, double* T);
+/* The arithmetic of SkelFactorization::decouple (skel_factorization.cpp:55-91)
+ * for one box: K_BN (n x n, active order, the descendants' updates already
+ * subtracted), the ID's perm (skeleton positions first) and T (k x (n - k)).
+ * Out: X_rr, its dgetrf factors X_rr_lu and X_rr_piv (1-based), L (k x r) =
+ * K_sn_r X_rr^{-1}, U (r x k) = X_rr^{-1} K_r_sn, schur_update (k x k) =
+ * L K_r_sn, r = n - k.  Any output may be NULL.  A zero or non-finite pivot
+ * fills the outputs and returns KIGPU_ERR_SINGULAR. */
+kigpu_status kigpu_schur_eliminate(kigpu_ctx* ctx, const double* K_BN, int n, int k,
+                                   const int* perm, const double* T, double* X_rr,
+                                   double* X_rr_lu, int* X_rr_piv, double* L, double* U,
+                                   double* schur_update);
+
 /* ---- matrix-free kernel apply ----------------------------------------- */
 /* y = K(all, all) mu in FP64 on the fly, Kernel::operator() over all DOFs
  * (kernel.cpp:99-150, 47-96).  mu, y: ndof x nrhs. */

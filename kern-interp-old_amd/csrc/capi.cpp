@@ -648,6 +648,53 @@ kigpu_status kigpu_kernel_block(kigpu_ctx* ctx, const kigpu_geom* geom, int pde,
   });
 }
 
+kigpu_status kigpu_proxy_block(kigpu_ctx* ctx, const kigpu_geom* geom, int pde,
+                               const double* pxy_p, const double* pxy_n, int np, double pxy_w,
+                               const int* box, int nbox, double* out) {
+  KI_REQUIRE(ctx && geom && pxy_p && pxy_n && out && np > 0 && nbox >= 0, "invalid argument");
+  KI_REQUIRE(nbox == 0 || box, "null index list");
+  KI_REQUIRE(pde >= KIGPU_LAPLACE && pde <= KIGPU_STOKES, "unknown pde");
+  KI_REQUIRE(nbox <= 65535, "at most 65535 box DOFs");
+  const int sd = pde == KIGPU_STOKES ? 2 : 1;
+  for (int j = 0; j < nbox; ++j)
+    KI_REQUIRE(box[j] >= 0 && box[j] < geom->npts * sd, "box index out of range");
+  return guard([&] {
+    const GeomView g{geom->px.p, geom->py.p, geom->nx.p, geom->ny.p, geom->w.p,
+                     geom->kappa.p, geom->npts, sd};
+    proxy_block(ctx->c, g, pde, pxy_p, pxy_n, np, pxy_w, box, nbox, out);
+  }, &ctx->c);
+}
+
+kigpu_status kigpu_interp_decomp(kigpu_ctx* ctx, const double* A, int m, int n, double tol,
+                                 int* k, int* perm, double* T) {
+  KI_REQUIRE(ctx && A && k && m > 0 && n > 0 && tol > 0, "invalid argument");
+  KI_REQUIRE((int64_t)m * n < (1ll << 31), "matrix too large");
+  return guard([&] { interp_decomp(ctx->c, A, m, n, tol, k, perm, T); }, &ctx->c);
+}
+
+kigpu_status kigpu_schur_eliminate(kigpu_ctx* ctx, const double* K_BN, int n, int k,
+                                   const int* perm, const double* T, double* X_rr,
+                                   double* X_rr_lu, int* X_rr_piv, double* L, double* U,
+                                   double* schur_update) {
+  KI_REQUIRE(ctx && K_BN && perm && T && n > 1 && k >= 1 && k < n, "invalid argument");
+  KI_REQUIRE(n - k <= 1024, "at most 1024 redundant DOFs");
+  std::vector<int> seen(n, 0);
+  for (int i = 0; i < n; ++i) {
+    KI_REQUIRE(perm[i] >= 0 && perm[i] < n && !seen[perm[i]], "perm is not a permutation");
+    seen[perm[i]] = 1;
+  }
+  int singular = 0;
+  kigpu_status st = guard([&] {
+    schur_eliminate(ctx->c, K_BN, n, k, perm, T, X_rr, X_rr_lu, X_rr_piv, L, U, schur_update,
+                    &singular);
+  }, &ctx->c);
+  if (st == KIGPU_OK && singular) {
+    set_error("zero or non-finite pivot in the LU of X_rr");
+    return KIGPU_ERR_SINGULAR;
+  }
+  return st;
+}
+
 kigpu_status kigpu_kernel_apply_device(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
                                        const double* d_mu, int nrhs, double* d_y) {
   KI_REQUIRE(ctx && tree && d_mu && d_y && nrhs > 0, "invalid argument");

@@ -60,6 +60,17 @@ struct Context {
 };
 
 void factor_tree(Context& c, TreeData& t, const kigpu_factor_opts& o);
+// ID of one host matrix (m x n, column-major) by the factorization's pivoted
+// QR: rank k (0 = not compressed), pivots perm[n], T (k x (n - k)).
+void interp_decomp(Context& c, const double* h_A, int m, int n, double tol, int* k,
+                   int* perm, double* h_T);
+// the proxy block of a box against explicit proxy points (blocks.hip)
+void proxy_block(Context& c, const GeomView& g, int pde, const double* h_pp, const double* h_pn,
+                 int np, double pw, const int* h_box, int nbox, double* h_out);
+// the elimination of one box from its K_BN (blocks.hip)
+void schur_eliminate(Context& c, const double* h_K, int n, int k, const int* h_perm,
+                     const double* h_T, double* X_rr, double* X_lu, int* piv, double* L,
+                     double* U, double* schur, int* singular);
 // bordered system (border.hip)
 template <typename T>
 void border_factor(Context& c, TreeData& t);

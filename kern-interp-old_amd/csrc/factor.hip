@@ -538,12 +538,15 @@ __device__ __forceinline__ void id_qr_reg(const IdWork& w, T* A, int lda, int m,
   id_finish<T, TS>(w, A, lda, n, k, perm, s_act, fac_ptr, rank, sr_pool, pos_pool, aoff);
 }
 
-template <typename T, typename TS, bool LDS>
+// GIVEN: the ID matrix is read from `given` (m x n, column-major) instead of
+// generated, for the ID of an arbitrary matrix (ki_Mat::id, ki_mat.cpp:505-541).
+template <typename T, typename TS, bool LDS, bool GIVEN = false>
 __device__ __forceinline__ void id_box(const IdWork& w, unsigned char* smem, GeomView g, int pde,
                                        double tol, const int* rows_pool, const int* act_pool,
                                        const int64_t* act_off, const double* ring, T* ws,
                                        int* rank, int* sr_pool, int* pos_pool,
-                                       const uint64_t* fac_ptr, int* status) {
+                                       const uint64_t* fac_ptr, int* status,
+                                       const double* given = nullptr) {
   const int n = w.n, q = w.q, m = w.m;
   const IdLds Ly = id_lds_layout(m, n, q, LDS, sizeof(T));
   T* A = LDS ? reinterpret_cast<T*>(smem + Ly.a) : ws + w.a_off;
@@ -560,32 +563,41 @@ __device__ __forceinline__ void id_box(const IdWork& w, unsigned char* smem, Geo
   const int lda = m;
   const int64_t aoff = act_off[w.node];
 
-  for (int i = tid; i < q; i += kIdTB) {
-    const int d = rows_pool[w.rows_off + i];
-    s_rows[i] = d;
-    stage_geo(g, d, rgeo + kGeo * i);
-  }
-  for (int j = tid; j < n; j += kIdTB) {
-    const int d = act_pool[aoff + j];
-    s_act[j] = d;
-    perm[j] = j;
-    stage_geo(g, d, cgeo + kGeo * j);
-  }
-  __syncthreads();
+  if (GIVEN) {
+    for (int j = tid; j < n; j += kIdTB) {
+      s_act[j] = act_pool[aoff + j];
+      perm[j] = j;
+    }
+    if (LDS) for2d<kIdNW>(m, n, [&](int i, int j) { A[i + (size_t)j * lda] = (T)given[i + (size_t)j * m]; });
+    __syncthreads();
+  } else {
+    for (int i = tid; i < q; i += kIdTB) {
+      const int d = rows_pool[w.rows_off + i];
+      s_rows[i] = d;
+      stage_geo(g, d, rgeo + kGeo * i);
+    }
+    for (int j = tid; j < n; j += kIdTB) {
+      const int d = act_pool[aoff + j];
+      s_act[j] = d;
+      perm[j] = j;
+      stage_geo(g, d, cgeo + kGeo * j);
+    }
+    __syncthreads();
 
-  // ---- ID matrix [K(rows, B); K(B, rows)^T; proxy] (kernel.cpp:153-235)
-  const double* ring_c = ring;
-  const double* ring_s = ring + kProxyPoints;
-  for2d<kIdNW>(m, n, [&](int i, int j) {
-    const int bd = s_act[j];
-    const double* bg = cgeo + kGeo * j;
-    double v;
-    if (i < q) v = staged_entry(pde, s_rows[i], rgeo + kGeo * i, bd, bg);
-    else if (i < 2 * q) v = staged_entry(pde, bd, bg, s_rows[i - q], rgeo + kGeo * (i - q));
-    else v = staged_proxy_entry(pde, i - 2 * q, bd, bg, w.cx, w.cy, w.rad, w.pw, ring_c, ring_s);
-    A[i + (size_t)j * lda] = (T)v;
-  });
-  __syncthreads();
+    // ---- ID matrix [K(rows, B); K(B, rows)^T; proxy] (kernel.cpp:153-235)
+    const double* ring_c = ring;
+    const double* ring_s = ring + kProxyPoints;
+    for2d<kIdNW>(m, n, [&](int i, int j) {
+      const int bd = s_act[j];
+      const double* bg = cgeo + kGeo * j;
+      double v;
+      if (i < q) v = staged_entry(pde, s_rows[i], rgeo + kGeo * i, bd, bg);
+      else if (i < 2 * q) v = staged_entry(pde, bd, bg, s_rows[i - q], rgeo + kGeo * (i - q));
+      else v = staged_proxy_entry(pde, i - 2 * q, bd, bg, w.cx, w.cy, w.rad, w.pw, ring_c, ring_s);
+      A[i + (size_t)j * lda] = (T)v;
+    });
+    __syncthreads();
+  }
 
   // ---- column norms (dgeqp3: DNRM2 of each column), FP64
   for (int j = wv; j < n; j += kIdNW) {
@@ -826,6 +838,24 @@ __global__ __launch_bounds__(kIdTB) void k_id(const IdWork* work, GeomView g, in
   else
     id_box<T, TS, false>(w, smem, g, pde, tol, rows_pool, act_pool, act_off, ring, ws, rank,
                          sr_pool, pos_pool, fac_ptr, status);
+}
+
+// The same pivoted QR and T on one given m x n matrix (one block): in LDS
+// when it fits (w.in_lds), else in place in `given` (global memory).
+// Outputs as for a tree box: rank[0] (-1 when not compressed), the pivots in
+// pos_pool[0..n), T into *fac_ptr[0].
+__global__ __launch_bounds__(kIdTB) void k_id_given(IdWork w, double* given, double tol,
+                                                    const int* iota, const int64_t* zero_off,
+                                                    int* rank, int* sr_pool, int* pos_pool,
+                                                    const uint64_t* fac_ptr, int* status) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const GeomView g{};
+  if (w.in_lds)
+    id_box<double, double, true, true>(w, smem, g, 0, tol, nullptr, iota, zero_off, nullptr,
+                                       given, rank, sr_pool, pos_pool, fac_ptr, status, given);
+  else
+    id_box<double, double, false, true>(w, smem, g, 0, tol, nullptr, iota, zero_off, nullptr,
+                                        given, rank, sr_pool, pos_pool, fac_ptr, status, given);
 }
 
 // ---------------------------------------------------------------------------
@@ -1871,6 +1901,52 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     }
   }
   c.tim.t_root += el(e_root0, e_root1);
+}
+
+void interp_decomp(Context& c, const double* h_A, int m, int n, double tol, int* k_out,
+                   int* perm, double* h_T) {
+  hipStream_t s = c.stream;
+  DBuf<double> A, T;
+  DBuf<int> iota, rank, srp, posp, status;
+  DBuf<int64_t> zero;
+  DBuf<uint64_t> fp;
+  A.upload(h_A, (size_t)m * n, s);
+  std::vector<int> io(n);
+  for (int j = 0; j < n; ++j) io[j] = j;
+  iota.upload(io, s);
+  const int64_t z = 0;
+  zero.upload(&z, 1, s);
+  T.alloc((size_t)n * n);
+  const uint64_t tp = reinterpret_cast<uint64_t>(T.p);
+  fp.upload(&tp, 1, s);
+  rank.alloc(1);
+  srp.alloc(n);
+  posp.alloc(n);
+  status.alloc(1);
+  status.zero(s);
+  IdWork w{};
+  w.node = 0;
+  w.n = n;
+  w.q = 0;
+  w.m = m;
+  const size_t in_l = id_lds_layout(m, n, 0, true, sizeof(double)).total;
+  w.in_lds = in_l <= c.lds_max;
+  const size_t lds = w.in_lds ? in_l : id_lds_layout(m, n, 0, false, sizeof(double)).total;
+  allow_lds(k_id_given, lds);
+  k_id_given<<<1, kIdTB, lds, s>>>(w, A.p, tol, iota.p, zero.p, rank.p, srp.p, posp.p, fp.p,
+                                   status.p);
+  KI_HIP(hipGetLastError());
+  int rk = -1, st = 0;
+  KI_HIP(hipMemcpyAsync(&rk, rank.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  KI_HIP(hipMemcpyAsync(&st, status.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  stream_wait(s);
+  if (st & 2) KI_FAIL(KIGPU_ERR_RANK, "ID matrix with fewer rows than columns");
+  *k_out = rk > 0 ? rk : 0;
+  if (*k_out > 0) {
+    if (perm) posp.download(perm, n, s);
+    if (h_T) T.download(h_T, (size_t)*k_out * (n - *k_out), s);
+    stream_wait(s);
+  }
 }
 
 // Algorithmic flops of the ID kernel for one box, as executed: entry

@@ -7,6 +7,7 @@ loading fails loudly when the library has not been built.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -59,6 +60,7 @@ EXPORTS = [
     "kigpu_tree_export_nodes", "kigpu_tree_export_boxes", "kigpu_tree_export_neighbors",
     "kigpu_set_border", "kigpu_factor", "kigpu_factor_ranks", "kigpu_factor_export_lists",
     "kigpu_factor_allskel", "kigpu_solve", "kigpu_solve_device", "kigpu_solve_device_noalloc",
+    "kigpu_proxy_block", "kigpu_interp_decomp", "kigpu_schur_eliminate",
     "kigpu_kernel_apply",
     "kigpu_kernel_apply_device", "kigpu_forward_apply", "kigpu_geometry_create",
     "kigpu_geometry_destroy", "kigpu_kernel_block", "kigpu_factor_export_active",
@@ -111,6 +113,17 @@ def load() -> C.CDLL:
         "kigpu_solve_device": (C.c_int, [P, P, P, C.c_int, P, P, C.POINTER(RefineOpts),
                                          C.POINTER(SolveInfo)]),
         "kigpu_solve_device_noalloc": (C.c_int, [P, P, P, C.c_int, P]),
+        "kigpu_geometry_create": (C.c_int, [P, _f64p, _f64p, _f64p, _f64p, C.c_int, P, C.c_int,
+                                            C.POINTER(P)]),
+        "kigpu_geometry_destroy": (C.c_int, [P]),
+        "kigpu_kernel_block": (C.c_int, [P, P, C.c_int, C.c_int, _i32p, C.c_int, _i32p, C.c_int,
+                                         C.c_int, _f64p]),
+        "kigpu_proxy_block": (C.c_int, [P, P, C.c_int, _f64p, _f64p, C.c_int, C.c_double, _i32p,
+                                        C.c_int, _f64p]),
+        "kigpu_interp_decomp": (C.c_int, [P, _f64p, C.c_int, C.c_int, C.c_double,
+                                          C.POINTER(C.c_int), _i32p, _f64p]),
+        "kigpu_schur_eliminate": (C.c_int, [P, _f64p, C.c_int, C.c_int, _i32p, _f64p, _f64p,
+                                            _f64p, _i32p, _f64p, _f64p, _f64p]),
         "kigpu_kernel_apply": (C.c_int, [P, P, C.c_int, _f64p, C.c_int, _f64p]),
         "kigpu_kernel_apply_treecode": (C.c_int, [P, P, _f64p, C.c_int, _f64p]),
         "kigpu_set_refine_apply": (C.c_int, [P, C.c_int]),
@@ -182,6 +195,49 @@ class Context:
         _check(self.lib.kigpu_last_timings(self.h, C.byref(t)), "kigpu_last_timings")
         return {n: getattr(t, n) for n, _ in Timings._fields_}
 
+    def geometry(self, points, normals, weights, curvatures):
+        return Geometry(self, points, normals, weights, curvatures)
+
+    def interp_decomp(self, A, tol):
+        """ki_Mat::id of a host matrix on the device: (k, perm, T); k = 0 means
+        not compressible (perm, T None)."""
+        A = np.asarray(A, np.float64)
+        m, n = A.shape
+        flat = np.ascontiguousarray(A.T).reshape(-1)   # column-major
+        k = C.c_int(0)
+        perm = np.zeros(n, np.int32)
+        T = np.zeros(n * n, np.float64)
+        _check(self.lib.kigpu_interp_decomp(self.h, flat, m, n, float(tol), C.byref(k), perm, T),
+               "kigpu_interp_decomp")
+        k = k.value
+        if k == 0:
+            return 0, None, None
+        return k, perm.astype(np.int64), T[:k * (n - k)].reshape(n - k, k).T.copy()
+
+    def schur_eliminate(self, K_BN, k, perm, T):
+        """The arithmetic of SkelFactorization::decouple for one box:
+        dict of X_rr, X_rr_lu, X_rr_piv (1-based), L, U, schur_update and the
+        status code (0, or KIGPU_ERR_SINGULAR = 9)."""
+        K_BN = np.asarray(K_BN, np.float64)
+        n = K_BN.shape[0]
+        r = n - k
+        kf = np.ascontiguousarray(K_BN.T).reshape(-1)
+        pm = np.ascontiguousarray(perm, np.int32)
+        Tf = np.ascontiguousarray(np.asarray(T, np.float64).T).reshape(-1)
+        out = {"X_rr": np.zeros(r * r), "X_rr_lu": np.zeros(r * r), "L": np.zeros(k * r),
+               "U": np.zeros(r * k), "schur_update": np.zeros(k * k)}
+        piv = np.zeros(max(r, 1), np.int32)
+        st = self.lib.kigpu_schur_eliminate(self.h, kf, n, k, pm, Tf, out["X_rr"], out["X_rr_lu"],
+                                            piv, out["L"], out["U"], out["schur_update"])
+        if st not in (0, 9):
+            _check(st, "kigpu_schur_eliminate")
+        shapes = {"X_rr": (r, r), "X_rr_lu": (r, r), "L": (k, r), "U": (r, k),
+                  "schur_update": (k, k)}
+        res = {a: out[a].reshape(shapes[a][1], shapes[a][0]).T.copy() for a in out}
+        res["X_rr_piv"] = piv[:r].copy()
+        res["status"] = st
+        return res
+
     def build_tree(self, points, normals, weights, curvatures, sd=1, leaf_dofs=32):
         return Tree(self, points, normals, weights, curvatures, sd, leaf_dofs)
 
@@ -190,6 +246,57 @@ class Context:
         (points, normals, weights, curvatures) tuples, points/normals
         interleaved (2n,) or (n, 2)."""
         return Tree(self, None, None, None, None, sd, leaf_dofs, batch=boundaries)
+
+
+class Geometry:
+    """Boundary arrays on the device (kigpu_geom): the dense blocks
+    Kernel::operator() and the proxy rows of Kernel::get_proxy_mat."""
+
+    def __init__(self, ctx, points, normals, weights, curvatures):
+        self.ctx, self.lib = ctx, ctx.lib
+        p, nm = _f64(points).reshape(-1), _f64(normals).reshape(-1)
+        w, c = _f64(weights).reshape(-1), _f64(curvatures).reshape(-1)
+        self.npts = w.size
+        h = C.c_void_p()
+        _check(self.lib.kigpu_geometry_create(ctx.h, p, nm, w, c, self.npts, None, 0, C.byref(h)),
+               "kigpu_geometry_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.kigpu_geometry_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def block(self, pde, tgt, src):
+        sd = 2 if pde == STOKES else 1
+        t = np.ascontiguousarray(tgt, np.int32)
+        s_ = np.ascontiguousarray(src, np.int32)
+        out = np.zeros(t.size * s_.size)
+        _check(self.lib.kigpu_kernel_block(self.ctx.h, self.h, pde, sd, t, t.size, s_, s_.size, 0,
+                                           out), "kigpu_kernel_block")
+        return out.reshape(s_.size, t.size).T.copy()
+
+    def proxy_block(self, pde, cx, cy, r, box, np_=128):
+        """Rows [proxy <- box; box <- proxy] of Kernel::get_proxy_mat for the
+        ring of np_ points of radius r about (cx, cy), weight 2 pi r / np_."""
+        ang = [2 * np.pi * i * (1.0 / np_) for i in range(np_)]
+        cs = np.array([math.cos(a) for a in ang])
+        sn = np.array([math.sin(a) for a in ang])
+        pp = np.ascontiguousarray(np.stack([cx + r * cs, cy + r * sn], 1).reshape(-1))
+        pn = np.ascontiguousarray(np.stack([cs, sn], 1).reshape(-1))
+        b = np.ascontiguousarray(box, np.int32)
+        rows = (4 if pde == STOKES else 2) * np_
+        out = np.zeros(rows * b.size)
+        _check(self.lib.kigpu_proxy_block(self.ctx.h, self.h, pde, pp, pn, np_,
+                                          2.0 * np.pi * r / np_, b, b.size, out),
+               "kigpu_proxy_block")
+        return out.reshape(b.size, rows).T.copy()
 
 
 class Tree:

@@ -1,6 +1,6 @@
 // Kernel of the drop-in API (interface of the reference's kernel.h:15-69).
-// Blocks K(tgt, src) come from the GPU (kigpu_kernel_block); the proxy rows
-// and the ID-matrix assembly follow kernel.cpp:153-334.
+// Blocks K(tgt, src) and proxy rows come from the GPU (kigpu_kernel_block,
+// kigpu_proxy_block); the ID-matrix assembly follows kernel.cpp:153-235.
 #include "kern_interp/kernel/kernel.h"
 
 #include <cmath>
@@ -59,45 +59,64 @@ ki_Mat Kernel::forward() const {
   return (*this)(tgt, src, true);
 }
 
-// proxy rows against box DOFs (kernel.cpp:238-334): 128 points on a circle of
-// radius r, rows [proxy <- box; box <- proxy]
+// kernel.cpp:11-36, one entry: the double-layer entry for LAPLACE, the
+// Neumann entry (or its single-layer log form when forward) for
+// LAPLACE_NEUMANN; r = target - source, diagonal when r is exactly zero.
+void Kernel::laplace(int mat_idx, ki_Mat* ret, double r1, double r2, double sw, double sc,
+                     double sn1, double sn2, double tn1, double tn2, bool forward) const {
+  const double inv2pi = 1.0 / (2 * M_PI);
+  double v = 0;
+  if (pde == LAPLACE) {
+    if (r1 == 0. && r2 == 0.) v = 0.5 + 0.5 * sc * sw * inv2pi;
+    else v = -sw * inv2pi * (r1 * sn1 + r2 * sn2) / (r1 * r1 + r2 * r2);
+  } else if (pde == LAPLACE_NEUMANN) {
+    if (forward) v = sw * inv2pi * std::log(std::sqrt(r1 * r1 + r2 * r2));
+    else if (r1 == 0. && r2 == 0.) v = sw - 0.5 + 0.5 * sc * sw * inv2pi;
+    else v = sw + sw * inv2pi * (r1 * tn1 + r2 * tn2) / (r1 * r1 + r2 * r2);
+  } else {
+    return;
+  }
+  ret->mat[mat_idx] = v;
+}
+
+static ki_Mat proxy_rows(kigpu_geom* g, int pde, const std::vector<double>& pp,
+                         const std::vector<double>& pn, double pw, const std::vector<int>& box) {
+  const int np = pp.size() / 2;
+  ki_Mat ret((pde == Kernel::STOKES ? 4 : 2) * np, box.size());
+  if (np == 0 || box.empty()) return ret;
+  detail::check(kigpu_proxy_block(detail::context(), g, pde, pp.data(), pn.data(), np, pw,
+                                  box.data(), (int)box.size(), ret.mat),
+                "kigpu_proxy_block");
+  return ret;
+}
+
+ki_Mat Kernel::laplace_proxy_get(const std::vector<double>& pxy_p,
+                                 const std::vector<double>& pxy_n, double pxy_w,
+                                 const std::vector<int>& box_inds) const {
+  return proxy_rows(geom(), pde == LAPLACE_NEUMANN ? LAPLACE_NEUMANN : LAPLACE, pxy_p, pxy_n,
+                    pxy_w, box_inds);
+}
+
+ki_Mat Kernel::stokes_proxy_get(const std::vector<double>& pxy_p,
+                                const std::vector<double>& pxy_n, double pxy_w,
+                                const std::vector<int>& box_inds) const {
+  return proxy_rows(geom(), STOKES, pxy_p, pxy_n, pxy_w, box_inds);
+}
+
+// kernel.cpp:236-265: NUM_PROXY_POINTS points on a circle of radius r, unit
+// outward normals, weight 2 pi r / NUM_PROXY_POINTS
 ki_Mat Kernel::get_proxy_mat(double cx, double cy, double r, const QuadTree*,
                              const std::vector<int>& box) const {
-  const int P = NUM_PROXY_POINTS;
-  const double pw = 2.0 * M_PI * r / P;
-  const double inv2pi = 1.0 / (2 * M_PI), invpi = 1.0 / M_PI;
-  const bool stokes = pde == STOKES;
-  ki_Mat ret((stokes ? 4 : 2) * P, box.size());
-  for (size_t j = 0; j < box.size(); ++j) {
-    const int d = box[j], pt = stokes ? d / 2 : d, comp = d % 2;
-    const double bx = boundary_points[2 * pt], by = boundary_points[2 * pt + 1];
-    const double bnx = boundary_normals[2 * pt], bny = boundary_normals[2 * pt + 1];
-    const double bw = boundary_weights[pt];
-    for (int i = 0; i < P; ++i) {
-      const double ang = 2 * M_PI * i * (1.0 / P);
-      const double pc = std::cos(ang), ps = std::sin(ang);
-      const double r1 = (cx + r * pc) - bx, r2 = (cy + r * ps) - by;
-      const double rr = r1 * r1 + r2 * r2;
-      if (!stokes) {
-        if (pde == LAPLACE) {
-          ret.set(i, j, -bw * inv2pi * (r1 * bnx + r2 * bny) / rr);
-          ret.set(P + i, j, -pw * inv2pi * ((-r1) * pc + (-r2) * ps) / rr);
-        } else {
-          ret.set(i, j, bw + bw * inv2pi * (r1 * pc + r2 * ps) / rr);
-          ret.set(P + i, j, pw + pw * inv2pi * ((-r1) * bnx + (-r2) * bny) / rr);
-        }
-      } else {
-        const double rv[2] = {r1, r2}, sn[2] = {bnx, bny}, tn[2] = {pc, ps};
-        const double pot = bw * invpi * (r1 * bnx + r2 * bny) / (rr * rr);
-        const double pot2 = -pw * invpi * (r1 * pc + r2 * ps) / (rr * rr);
-        for (int a = 0; a < 2; ++a) {
-          ret.set(2 * i + a, j, pot * rv[a] * rv[comp] + bw * tn[a] * sn[comp]);
-          ret.set(2 * P + 2 * i + a, j, pot2 * rv[a] * rv[comp] + pw * sn[comp] * tn[a]);
-        }
-      }
-    }
+  const double pw = 2.0 * M_PI * r / NUM_PROXY_POINTS;
+  std::vector<double> pp, pn;
+  for (int i = 0; i < NUM_PROXY_POINTS; ++i) {
+    const double ang = 2 * M_PI * i * (1.0 / NUM_PROXY_POINTS);
+    pp.push_back(cx + r * std::cos(ang));
+    pp.push_back(cy + r * std::sin(ang));
+    pn.push_back(std::cos(ang));
+    pn.push_back(std::sin(ang));
   }
-  return ret;
+  return pde == STOKES ? stokes_proxy_get(pp, pn, pw, box) : laplace_proxy_get(pp, pn, pw, box);
 }
 
 // kernel.cpp:153-235: [K(rows, B); K(B, rows)^T (; proxy)] from the node's lists

@@ -3,6 +3,8 @@
 // BLAS/LAPACK dependency); the factor/solve path runs on the GPU.
 #include "kern_interp/ki_mat.h"
 
+#include "runtime.h"
+
 #include <algorithm>
 #include <cassert>
 #include <cmath>
@@ -322,85 +324,24 @@ void ki_Mat::right_multiply_inverse(const ki_Mat& K, const std::vector<lapack_in
   *L = kt.transpose();
 }
 
+// ki_mat.cpp:505-541: pivoted QR to the first |R_ii| < tol |R_00| (i >= 1),
+// then Z = R11^{-1} R12; run on the GPU by the factorization's own routine
+// (kigpu_interp_decomp).  Returns the number of skeleton columns, 0 when no
+// |R_ii| falls below the threshold.
 int ki_Mat::id(std::vector<int>* p, ki_Mat* Z, double tol) const {
   assert(height() != 0 && width() != 0);
   const int m = height_, n = width_;
-  ki_Mat A(*this);
+  int k = 0;
   std::vector<int> perm(n);
-  std::vector<double> vn1(n), vn2(n);
-  for (int j = 0; j < n; ++j) {
-    perm[j] = j;
-    double s = 0;
-    for (int i = 0; i < m; ++i) s += A.get(i, j) * A.get(i, j);
-    vn1[j] = vn2[j] = std::sqrt(s);
-  }
-  const double tol3z = std::sqrt(2.220446049250313e-16);
-  std::vector<double> rdiag(std::min(m, n));
-  for (int i = 0; i < std::min(m, n); ++i) {
-    int pv = i;
-    for (int j = i + 1; j < n; ++j)
-      if (vn1[j] > vn1[pv]) pv = j;
-    if (pv != i) {
-      for (int r = 0; r < m; ++r) std::swap(A.mat[r + (size_t)m * i], A.mat[r + (size_t)m * pv]);
-      std::swap(perm[i], perm[pv]);
-      vn1[pv] = vn1[i];
-      vn2[pv] = vn2[i];
-    }
-    double* v = A.mat + (size_t)m * i;
-    double xn = 0;
-    for (int r = i + 1; r < m; ++r) xn += v[r] * v[r];
-    xn = std::sqrt(xn);
-    const double alpha = v[i];
-    double beta = alpha, tau = 0;
-    if (xn != 0.0) {
-      beta = -std::copysign(std::hypot(alpha, xn), alpha);
-      tau = (beta - alpha) / beta;
-      const double sc = 1.0 / (alpha - beta);
-      for (int r = i + 1; r < m; ++r) v[r] *= sc;
-    }
-    v[i] = beta;
-    rdiag[i] = beta;
-    for (int j = i + 1; j < n; ++j) {
-      double* cj = A.mat + (size_t)m * j;
-      double d = cj[i];
-      for (int r = i + 1; r < m; ++r) d += v[r] * cj[r];
-      d *= tau;
-      cj[i] -= d;
-      for (int r = i + 1; r < m; ++r) cj[r] -= d * v[r];
-      if (vn1[j] != 0.0) {
-        double tt = std::fabs(cj[i]) / vn1[j];
-        double temp = std::max(1.0 - tt * tt, 0.0);
-        double rr = vn1[j] / vn2[j];
-        if (temp * rr * rr <= tol3z) {
-          double s = 0;
-          for (int r = i + 1; r < m; ++r) s += cj[r] * cj[r];
-          vn1[j] = vn2[j] = i + 1 < m ? std::sqrt(s) : 0.0;
-        } else {
-          vn1[j] *= std::sqrt(temp);
-        }
-      }
-    }
-  }
-  const double thresh = std::fabs(tol * rdiag[0]);
-  int skel = 0;
-  for (int i = 1; i < n; ++i) {
-    if (i >= m) throw std::runtime_error("ki_Mat::id: fewer rows than columns");
-    if (std::fabs(rdiag[i]) < thresh) {
-      skel = i;
-      break;
-    }
-  }
-  if (skel == 0) return 0;
-  p->assign(perm.begin(), perm.end());
-  const int red = n - skel;
-  *Z = ki_Mat(skel, red);
-  for (int c = 0; c < red; ++c)
-    for (int row = skel - 1; row >= 0; --row) {
-      double s = A.get(row, skel + c);
-      for (int j = row + 1; j < skel; ++j) s -= A.get(row, j) * Z->get(j, c);
-      Z->set(row, c, s / A.get(row, row));
-    }
-  return skel;
+  std::vector<double> t((size_t)n * n);
+  detail::check(kigpu_interp_decomp(detail::context(), mat, m, n, tol, &k, perm.data(), t.data()),
+                "kigpu_interp_decomp");
+  if (k == 0) return 0;
+  for (int j = 0; j < n; ++j) p->push_back(perm[j]);
+  *Z = ki_Mat(k, n - k);
+  for (int j = 0; j < n - k; ++j)
+    for (int i = 0; i < k; ++i) Z->set(i, j, t[i + (size_t)j * k]);
+  return k;
 }
 
 }  // namespace kern_interp

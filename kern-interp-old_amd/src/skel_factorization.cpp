@@ -15,6 +15,87 @@ SkelFactorization::SkelFactorization(double tol, int threads) {
   fact_threads = threads;
 }
 
+// skel_factorization.cpp:32-52: ID of the node's ID matrix; returns the
+// number of redundant DOFs (0: not compressed)
+int SkelFactorization::id_compress(const Kernel& kernel, const QuadTree* tree,
+                                   QuadTreeNode* node) {
+  assert(node != nullptr && "InterpolativeDecomposition fails on null node.");
+  assert(node->dof_lists.active_box.size() > 0 &&
+         "Num of DOFs must be positive in InterpolativeDecomposition.");
+  ki_Mat pxy = kernel.get_id_mat(tree, node);
+  if (pxy.height() == 0) return 0;
+  std::vector<int> p;
+  const int numskel = pxy.id(&p, &node->T, id_tol);
+  if (numskel == 0) return 0;
+  node->dof_lists.set_rs_ranges(p, node->T.height(), node->T.width());
+  node->dof_lists.set_skelnear_range();
+  return node->T.width();
+}
+
+// skel_factorization.cpp:55-91: K_BN minus the descendants' updates, then
+// X_rr, its LU, L, U and the Schur update (kigpu_schur_eliminate)
+void SkelFactorization::decouple(const Kernel& kernel, QuadTreeNode* node) {
+  const int nr = node->T.width(), ns = node->T.height();
+  const std::vector<int>& BN = node->dof_lists.active_box;
+  const int n = BN.size();
+  assert(ns + nr == n && (int)node->dof_lists.permutation.size() == n);
+  ki_Mat update(n, n);
+  get_all_schur_updates(&update, BN, node);
+  ki_Mat K_BN = kernel(BN, BN) - update;
+  node->X_rr = ki_Mat(nr, nr);
+  node->X_rr_lu = ki_Mat(nr, nr);
+  node->X_rr_piv.assign(nr, 0);
+  node->L = ki_Mat(ns, nr);
+  node->U = ki_Mat(nr, ns);
+  node->schur_update = ki_Mat(ns, ns);
+  std::vector<int> piv(nr);
+  const kigpu_status st = kigpu_schur_eliminate(
+      detail::context(), K_BN.mat, n, ns, node->dof_lists.permutation.data(), node->T.mat,
+      node->X_rr.mat, node->X_rr_lu.mat, piv.data(), node->L.mat, node->U.mat,
+      node->schur_update.mat);
+  // a singular X_rr leaves its factors as computed (the reference does not
+  // check dgetrf's info, ki_mat.cpp:431-432)
+  if (st != KIGPU_ERR_SINGULAR) detail::check(st, "kigpu_schur_eliminate");
+  for (int i = 0; i < nr; ++i) node->X_rr_piv[i] = piv[i];
+  node->X_rr_is_LU_factored = true;
+  node->compressed = true;
+}
+
+// skel_factorization.cpp:246-294: the Schur updates of every compressed
+// descendant, entered at the positions of its skelnear DOFs in BN
+void SkelFactorization::get_all_schur_updates(ki_Mat* updates, const std::vector<int>& BN,
+                                              const QuadTreeNode* node) const {
+  assert(node != nullptr && "get_all_schur_updates fails on null node.");
+  assert(BN.size() > 0 && "get_all_schur_updates needs positive num of DOFs");
+  if (!node->is_leaf) get_descendents_updates(updates, BN, node);
+}
+
+void SkelFactorization::get_descendents_updates(ki_Mat* updates, const std::vector<int>& BN,
+                                                const QuadTreeNode* node) const {
+  assert(node != nullptr && !node->is_leaf &&
+         "get_descendents_updates must be called on non-leaf.");
+  for (QuadTreeNode* child : node->children) {
+    if (child->compressed) get_update(updates, BN, child);
+    if (!child->is_leaf) get_descendents_updates(updates, BN, child);
+  }
+}
+
+void SkelFactorization::get_update(ki_Mat* update, const std::vector<int>& BN,
+                                   const QuadTreeNode* node) const {
+  // positions of the node's skelnear DOFs that BN holds, in skelnear order
+  std::vector<int> at_bn, at_sn;
+  const std::vector<int>& sn = node->dof_lists.skelnear;
+  for (size_t a = 0; a < sn.size(); ++a)
+    for (size_t b = 0; b < BN.size(); ++b)
+      if (BN[b] == sn[a]) {
+        at_sn.push_back(a);
+        at_bn.push_back(b);
+      }
+  for (size_t i = 0; i < at_bn.size(); ++i)
+    for (size_t j = 0; j < at_bn.size(); ++j)
+      update->addset(at_bn[i], at_bn[j], node->schur_update.get(at_sn[i], at_sn[j]));
+}
+
 void SkelFactorization::skeletonize(const Kernel& K, QuadTree* tree) {
   kigpu_tree* g = tree->gpu();
   assert(g && "initialize_tree before skeletonize");

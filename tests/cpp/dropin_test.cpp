@@ -1,8 +1,11 @@
 // The drop-in C++ API on the GPU, exercised the way the reference's own test
 // suite uses it (kern_interp_test.cpp:26-95 backward error, :297-316 and
 // :392-410 analytic agreement): same headers, classes and calls.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -57,7 +60,172 @@ static double backward_error(const Kernel& kernel, Boundary* b, int leaf, double
   return std::sqrt(e2) / b->boundary_values.vec_two_norm();
 }
 
+// QuadTree's host construction members (recursive_add, node_subdivide,
+// compute_neighbor_lists, get_descendent_neighbors) driven the way
+// initialize_tree drives them (quadtree.cpp:72-114), against the tree
+// initialize_tree builds on the GPU: level order, corners (bit for bit), leaf
+// flags, DOF lists and neighbor lists in order.
+static void host_tree_matches_gpu(const std::string& name, Boundary* b, int sd, int leaf) {
+  QuadTree g;
+  g.leaf_dofs = leaf;
+  g.initialize_tree(b, sd, 2);
+  QuadTree h;
+  h.leaf_dofs = leaf;
+  h.boundary_points = b->points;
+  h.solution_dimension = sd;
+  h.domain_dimension = 2;
+  double lo = b->points[0], hi = b->points[0];
+  for (double v : b->points) lo = std::min(lo, v), hi = std::max(hi, v);
+  lo -= 0.01;
+  hi += 0.01;
+  QuadTreeNode* r = new QuadTreeNode();
+  r->level = 0;
+  r->parent = nullptr;
+  const double box[8] = {lo, lo, lo, hi, hi, hi, hi, lo};
+  for (int j = 0; j < 8; ++j) r->corners[j] = box[j];
+  r->side_length = hi - lo;
+  h.root = r;
+  h.levels.push_back(new QuadTreeLevel());
+  h.levels[0]->nodes.push_back(r);
+  for (size_t i = 0; i < b->points.size(); i += 2) h.recursive_add(r, b->points[i], b->points[i + 1], i / 2);
+  h.compute_neighbor_lists();
+
+  auto slot = [](const QuadTree& t) {
+    std::map<const QuadTreeNode*, std::pair<int, int>> m;
+    for (size_t l = 0; l < t.levels.size(); ++l)
+      for (size_t k = 0; k < t.levels[l]->nodes.size(); ++k) m[t.levels[l]->nodes[k]] = {l, k};
+    return m;
+  };
+  auto hs = slot(h), gs = slot(g);
+  long bad = 0, nodes = 0;
+  if (h.levels.size() != g.levels.size()) ++bad;
+  for (size_t l = 0; l < std::min(h.levels.size(), g.levels.size()); ++l) {
+    const auto& hn = h.levels[l]->nodes;
+    const auto& gn = g.levels[l]->nodes;
+    if (hn.size() != gn.size()) {
+      ++bad;
+      continue;
+    }
+    for (size_t k = 0; k < hn.size(); ++k, ++nodes) {
+      const QuadTreeNode* x = hn[k];
+      const QuadTreeNode* y = gn[k];
+      bool same = x->is_leaf == y->is_leaf && x->level == y->level &&
+                  std::memcmp(x->corners, y->corners, sizeof x->corners) == 0 &&
+                  x->dof_lists.original_box == y->dof_lists.original_box &&
+                  x->neighbors.size() == y->neighbors.size();
+      for (size_t e = 0; same && e < x->neighbors.size(); ++e)
+        same = hs[x->neighbors[e]] == gs[y->neighbors[e]];
+      bad += !same;
+    }
+  }
+  std::printf("  %s: %zu levels, %ld nodes\n", name.c_str(), h.levels.size(), nodes);
+  report("host_tree_vs_gpu_" + name, (double)bad, 0);
+}
+
+// x = A^{-1} b by the reference's sweeps (skel_factorization.cpp:343-402)
+// over the per-box factors that id_compress/decouple left in the nodes
+static ki_Mat host_sweep_solve(const SkelFactorization& f, const QuadTree& t,
+                               const ki_Mat& root_lu, const std::vector<lapack_int>& root_piv,
+                               const ki_Mat& b) {
+  ki_Mat x = b;
+  const int L = t.levels.size();
+  for (int l = L - 1; l >= 0; --l)
+    for (QuadTreeNode* nd : t.levels[l]->nodes) {
+      if (!nd->compressed) continue;
+      f.apply_sweep_matrix(-nd->T, &x, nd->dof_lists.skel, nd->dof_lists.redundant, true);
+      f.apply_sweep_matrix(-nd->L, &x, nd->dof_lists.redundant, nd->dof_lists.skelnear, false);
+    }
+  for (int l = L - 1; l >= 0; --l)
+    for (QuadTreeNode* nd : t.levels[l]->nodes)
+      if (nd->compressed && !nd->dof_lists.redundant.empty())
+        f.apply_diag_inv_matrix(nd->X_rr_lu, nd->X_rr_piv, &x, nd->dof_lists.redundant);
+  if (!t.root->dof_lists.active_box.empty())
+    f.apply_diag_inv_matrix(root_lu, root_piv, &x, t.root->dof_lists.active_box);
+  for (int l = 0; l < L; ++l) {
+    const auto& nodes = t.levels[l]->nodes;
+    for (int k = (int)nodes.size() - 1; k >= 0; --k) {
+      QuadTreeNode* nd = nodes[k];
+      if (!nd->compressed) continue;
+      f.apply_sweep_matrix(-nd->U, &x, nd->dof_lists.skelnear, nd->dof_lists.redundant, false);
+      f.apply_sweep_matrix(-nd->T, &x, nd->dof_lists.redundant, nd->dof_lists.skel, false);
+    }
+  }
+  return x;
+}
+
+// SkelFactorization's per-box members (id_compress, decouple,
+// get_all_schur_updates) in the reference's skeletonize loop
+// (skel_factorization.cpp:94-120), against the GPU's batched skeletonize:
+// same rank and ordered skeleton per box, same root skeleton, and the
+// reference sweeps over the per-box factors give the GPU solve's density.
+static void per_box_matches_batched(const std::string& name, const Kernel& k, Boundary* b,
+                                    int leaf, double tol) {
+  const int sd = k.solution_dimension;
+  QuadTree g, h;
+  g.leaf_dofs = h.leaf_dofs = leaf;
+  g.initialize_tree(b, sd, 2);
+  h.initialize_tree(b, sd, 2);
+  SkelFactorization fg(tol, 1), fh(tol, 1);
+  fg.skeletonize(k, &g);
+  ki_Mat mu_gpu(b->boundary_values.height(), 1);
+  fg.solve(g, &mu_gpu, b->boundary_values);
+
+  for (int l = (int)h.levels.size() - 1; l > 0; --l) {
+    h.remove_inactive_dofs_at_level(l);
+    for (QuadTreeNode* nd : h.levels[l]->nodes) {
+      if (nd->compressed || nd->dof_lists.active_box.size() < MIN_DOFS_TO_COMPRESS) continue;
+      if (fh.id_compress(k, &h, nd) == 0) continue;
+      fh.decouple(k, nd);
+    }
+  }
+  h.remove_inactive_dofs_at_all_boxes();
+  const std::vector<int> allskel = h.root->dof_lists.active_box;
+  ki_Mat upd(allskel.size(), allskel.size());
+  if (!allskel.empty()) fh.get_all_schur_updates(&upd, allskel, h.root);
+  ki_Mat root = k(allskel, allskel) - upd, root_lu;
+  std::vector<lapack_int> root_piv;
+  root.LU_factorize(&root_lu, &root_piv);
+
+  long bad = 0, boxes = 0;
+  for (size_t l = 0; l < h.levels.size(); ++l)
+    for (size_t x = 0; x < h.levels[l]->nodes.size(); ++x) {
+      const QuadTreeNode* p = h.levels[l]->nodes[x];
+      const QuadTreeNode* q = g.levels[l]->nodes[x];
+      boxes += p->compressed;
+      bad += p->compressed != q->compressed || p->dof_lists.skel != q->dof_lists.skel ||
+             p->dof_lists.redundant != q->dof_lists.redundant;
+    }
+  bad += allskel != g.root->dof_lists.active_box;
+  std::printf("  %s: %ld compressed boxes, |allskel| = %zu\n", name.c_str(), boxes,
+              allskel.size());
+  report("per_box_ranks_vs_batched_" + name, (double)bad, 0);
+  ki_Mat mu_host = host_sweep_solve(fh, h, root_lu, root_piv, b->boundary_values);
+  report("per_box_solve_vs_batched_" + name, (mu_host - mu_gpu).vec_two_norm() / mu_gpu.vec_two_norm(),
+         1e-10);
+}
+
 int main() {
+  {
+    StarCurve s;
+    s.initialize(16384, DEFAULT);
+    host_tree_matches_gpu("star16384_leaf64", &s, 1, 64);
+    Circle c;
+    c.initialize(2048, TANGENT_VEC);
+    host_tree_matches_gpu("stokes_circle2048_leaf32", &c, 2, 32);
+    Annulus a;
+    a.initialize(4096, SINGLE_ELECTRON);
+    host_tree_matches_gpu("annulus4096_leaf32", &a, 1, 32);
+  }
+  {
+    StarCurve s;
+    s.initialize(4096, DEFAULT);
+    Kernel k(1, 2, Kernel::LAPLACE, &s, {});
+    per_box_matches_batched("laplace_star4096", k, &s, 64, 1e-6);
+    StarCurve st;
+    st.initialize(2048, TANGENT_VEC);
+    Kernel ks(2, 2, Kernel::STOKES, &st, {});
+    per_box_matches_batched("stokes_star2048", ks, &st, 64, 1e-6);
+  }
   {
     Circle c;
     c.initialize(1024, SINGLE_ELECTRON);
