@@ -12,6 +12,7 @@ class Circle : public Boundary {
  public:
   void initialize(int n, BoundaryCondition bc) override;
   bool is_in_domain(const Vec2& a) const override;
+  DomainRule domain_rule() const override;
   std::unique_ptr<Boundary> clone() const override {
     return std::unique_ptr<Boundary>(new Circle(*this));
   }

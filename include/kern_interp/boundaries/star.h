@@ -19,6 +19,7 @@ class StarCurve : public Boundary {
   Vec2 source = Vec2(3.0, 3.0);
   void initialize(int n, BoundaryCondition bc) override;
   bool is_in_domain(const Vec2& a) const override;
+  DomainRule domain_rule() const override;
   double radius(double t) const;
   std::unique_ptr<Boundary> clone() const override {
     return std::unique_ptr<Boundary>(new StarCurve(*this));
@@ -31,6 +32,7 @@ class HoledDisc : public Boundary {
   int hole_nodes = 256;
   void initialize(int n, BoundaryCondition bc) override;
   bool is_in_domain(const Vec2& a) const override;
+  DomainRule domain_rule() const override;
   std::unique_ptr<Boundary> clone() const override {
     return std::unique_ptr<Boundary>(new HoledDisc(*this));
   }

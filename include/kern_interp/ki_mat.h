@@ -76,6 +76,11 @@ struct ki_Mat {
   // interpolative decomposition by Householder QR with column pivoting;
   // returns the number of skeleton columns (0: no compression)
   int id(std::vector<int>* p, ki_Mat* Z, double tol) const;
+
+  // eigenvalues of a square matrix whose imaginary part is below 1e-14
+  // (reference ki_mat.cpp:544-560, LAPACK dgeev there), by Hessenberg
+  // reduction and Francis double-shift QR; the matrix is left unchanged
+  std::vector<double> real_eigenvalues();
 };
 
 }  // namespace kern_interp

@@ -294,6 +294,51 @@ kigpu_status kigpu_forward_apply(kigpu_ctx* ctx, const kigpu_tree* tree, int pde
                                  const double* targets, int ntgt, const double* mu,
                                  int nrhs, double* u);
 
+/* ---- in-domain test and domain evaluation ------------------------------ */
+/* Boundary::is_in_domain as data (one rule per geometry family):
+ *  KIGPU_REGION_DISC    Circle, Annulus, Donut (circle.cpp:34-42,
+ *                       annulus.cpp:66-78, donut.cpp:67-79): outside when
+ *                       |t - c| + eps > radius, or |t - h_k| - eps < r_k for
+ *                       a hole k (holes: x, y, r per hole);
+ *  KIGPU_REGION_WINDING CubicBoundary (cubic_boundary.cpp:182-230): crossing
+ *                       counts of the node polygon, nodes = the boundary
+ *                       points (2 per node), num_outer outer nodes first, then
+ *                       hole_nodes[k] nodes per hole; the outer count must be
+ *                       odd, a hole's count of 1 excludes, a node closer than
+ *                       eps excludes;
+ *  KIGPU_REGION_STAR    a Fourier star about the origin (no counterpart in
+ *                       the reference): inside when
+ *                       |t| + eps < 1 + sum a_k cos(f_k atan2(t) + phi_k). */
+enum { KIGPU_REGION_DISC = 0, KIGPU_REGION_WINDING = 1, KIGPU_REGION_STAR = 2 };
+typedef struct {
+  int kind;
+  double eps;                 /* 1e-2 in every reference geometry */
+  double cx, cy, radius;      /* DISC */
+  int nholes;
+  const double* holes;        /* DISC: 3 * nholes */
+  const double* nodes;        /* WINDING */
+  int num_outer;              /* WINDING */
+  const int* hole_nodes;      /* WINDING: nholes */
+  int nterms;                 /* STAR */
+  const double* amps;
+  const double* freqs;
+  const double* phases;
+} kigpu_region;
+/* inside[i] = 1 when target i (2*ntgt interleaved) is in the region. */
+kigpu_status kigpu_domain_mask(kigpu_ctx* ctx, const kigpu_region* region, const double* targets,
+                               int ntgt, unsigned char* inside);
+/* The domain solution of boundary_integral_solve (linear_solve.cpp:143-180):
+ * u = Kernel::forward() mu + U_forward alpha at the targets, zero at targets
+ * outside region (NULL: no mask).  U_forward is initialize_U_mat at the
+ * targets (linear_solve.cpp:18-64) for holes centred at hole_centers (2 per
+ * hole); alpha ((3 for Stokes, else 1) * nholes) x nrhs, or NULL for no hole
+ * terms (simply connected, or LAPLACE_NEUMANN).  mu: ndof x nrhs;
+ * u: (sd*ntgt) x nrhs. */
+kigpu_status kigpu_domain_solution(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
+                                   const double* targets, int ntgt, const double* mu,
+                                   const double* alpha, const double* hole_centers, int nholes,
+                                   int nrhs, const kigpu_region* region, double* u);
+
 /* ---- adjoint ------------------------------------------------------------ */
 /* No counterpart in the reference, whose experiments differentiate by finite
  * differences of whole solves (experiments/experiment_three.cpp:22-56,

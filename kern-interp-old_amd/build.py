@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libkigpu.so")
 
-SOURCES = ["alloc.cpp", "scan.hip", "tree.hip", "factor.hip", "border.hip", "solve.hip", "treecode.hip", "blocks.hip", "adjoint.hip", "capi.cpp"]
+SOURCES = ["alloc.cpp", "scan.hip", "tree.hip", "factor.hip", "border.hip", "solve.hip", "treecode.hip", "blocks.hip", "adjoint.hip", "domain.hip", "capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-std=c++17", "-O3", "-fPIC", "--offload-arch=gfx950",
@@ -46,7 +46,7 @@ def _compile(src: str) -> str:
     return obj
 
 
-API_SOURCES = ["runtime.cpp", "ki_mat.cpp", "boundary.cpp", "quadtree.cpp", "kernel.cpp",
+API_SOURCES = ["runtime.cpp", "ki_mat.cpp", "boundary.cpp", "cubic_boundary.cpp", "quadtree.cpp", "kernel.cpp",
                "skel_factorization.cpp", "linear_solve.cpp"]
 API_LIB = os.path.join(HERE, "libkern_interp.so")
 DROPIN_TEST = os.path.join(BUILD, "dropin_test")

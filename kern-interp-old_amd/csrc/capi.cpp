@@ -766,6 +766,84 @@ kigpu_status kigpu_forward_apply(kigpu_ctx* ctx, const kigpu_tree* tree, int pde
   }, &ctx->c);
 }
 
+static const char* check_region(const kigpu_region* r) {
+  if (!(r->eps >= 0)) return "region eps must be >= 0";
+  if (r->nholes < 0) return "negative hole count";
+  switch (r->kind) {
+    case KIGPU_REGION_DISC:
+      if (r->nholes && !r->holes) return "disc region: holes missing";
+      return nullptr;
+    case KIGPU_REGION_WINDING:
+      if (!r->nodes || r->num_outer < 1) return "winding region: nodes missing";
+      if (r->nholes && !r->hole_nodes) return "winding region: hole_nodes missing";
+      for (int k = 0; k < r->nholes; ++k)
+        if (r->hole_nodes[k] < 1) return "winding region: empty hole section";
+      return nullptr;
+    case KIGPU_REGION_STAR:
+      if (r->nterms < 0 || (r->nterms && !(r->amps && r->freqs && r->phases)))
+        return "star region: terms missing";
+      return nullptr;
+    default:
+      return "unknown region kind";
+  }
+}
+
+kigpu_status kigpu_domain_mask(kigpu_ctx* ctx, const kigpu_region* region, const double* targets,
+                               int ntgt, unsigned char* inside) {
+  KI_REQUIRE(ctx && region && ntgt >= 0 && (ntgt == 0 || (targets && inside)),
+             "invalid argument");
+  if (const char* e = check_region(region)) {
+    set_error(e);
+    return KIGPU_ERR_ARG;
+  }
+  if (ntgt == 0) return KIGPU_OK;
+  return guard([&] {
+    Context& c = ctx->c;
+    DBuf<double> dt;
+    DBuf<unsigned char> din;
+    dt.upload(targets, 2 * (size_t)ntgt, c.stream);
+    din.alloc(ntgt);
+    domain_mask(c, *region, dt.p, ntgt, din.p);
+    din.download(inside, ntgt, c.stream);
+    stream_wait(c.stream);
+  }, &ctx->c);
+}
+
+kigpu_status kigpu_domain_solution(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,
+                                   const double* targets, int ntgt, const double* mu,
+                                   const double* alpha, const double* hole_centers, int nholes,
+                                   int nrhs, const kigpu_region* region, double* u) {
+  KI_REQUIRE(ctx && tree && targets && mu && u && ntgt > 0 && nrhs > 0 && nholes >= 0,
+             "invalid argument");
+  KI_REQUIRE(!alpha || nholes == 0 || hole_centers, "hole centres missing");
+  KI_REQUIRE(pde_matches(tree, pde), "pde does not match the tree's solution dimension");
+  if (region) {
+    if (const char* e = check_region(region)) {
+      set_error(e);
+      return KIGPU_ERR_ARG;
+    }
+  }
+  if (tree->t.ninst > 1) {
+    set_error("domain targets need a single instance");
+    return KIGPU_ERR_NOTSUPP;
+  }
+  return guard([&] {
+    Context& c = ctx->c;
+    const TreeData& t = tree->t;
+    const int sd = pde == KIGPU_STOKES ? 2 : 1;
+    const size_t na = (size_t)(sd == 2 ? 3 : 1) * nholes;
+    DBuf<double> dt, dm, da, du;
+    dt.upload(targets, 2 * (size_t)ntgt, c.stream);
+    dm.upload(mu, (size_t)t.ndof * nrhs, c.stream);
+    if (alpha && nholes) da.upload(alpha, na * nrhs, c.stream);
+    du.alloc((size_t)sd * ntgt * nrhs);
+    domain_solution(c, t, pde, dt.p, ntgt, dm.p, nrhs, alpha && nholes ? da.p : nullptr,
+                    hole_centers, nholes, region, du.p);
+    du.download(u, (size_t)sd * ntgt * nrhs, c.stream);
+    stream_wait(c.stream);
+  }, &ctx->c);
+}
+
 kigpu_status kigpu_solve_transpose_device(kigpu_ctx* ctx, kigpu_tree* tree, const double* d_b,
                                           int nrhs, double* d_x, const kigpu_refine_opts* refine,
                                           kigpu_solve_info* info) {

@@ -112,6 +112,12 @@ void forward_sensitivity(Context& c, const TreeData& t, const double* d_tgt, int
 void kernel_block(Context& c, const GeomView& g, const double* dpx, const double* dpy, int pde,
                   int sd, const int* h_tgt, int ntgt, const int* h_src, int nsrc, int forward,
                   double* h_out);
+// in-domain mask and domain evaluation (domain.hip)
+void domain_mask(Context& c, const kigpu_region& r, const double* d_txy, int ntgt,
+                 unsigned char* d_inside);
+void domain_solution(Context& c, const TreeData& t, int pde, const double* d_txy, int ntgt,
+                     const double* d_mu, int nrhs, const double* d_alpha, const double* h_hc,
+                     int nholes, const kigpu_region* region, double* d_u);
 void forward_apply(Context& c, const TreeData& t, int pde, const double* d_tgt, int ntgt,
                    const double* d_mu, int nrhs, double* d_u);
 

@@ -61,6 +61,7 @@ EXPORTS = [
     "kigpu_set_border", "kigpu_factor", "kigpu_factor_ranks", "kigpu_factor_export_lists",
     "kigpu_factor_allskel", "kigpu_solve", "kigpu_solve_device", "kigpu_solve_device_noalloc",
     "kigpu_proxy_block", "kigpu_interp_decomp", "kigpu_schur_eliminate",
+    "kigpu_domain_mask", "kigpu_domain_solution",
     "kigpu_kernel_apply",
     "kigpu_kernel_apply_device", "kigpu_forward_apply", "kigpu_geometry_create",
     "kigpu_geometry_destroy", "kigpu_kernel_block", "kigpu_factor_export_active",
@@ -124,6 +125,9 @@ def load() -> C.CDLL:
                                           C.POINTER(C.c_int), _i32p, _f64p]),
         "kigpu_schur_eliminate": (C.c_int, [P, _f64p, C.c_int, C.c_int, _i32p, _f64p, _f64p,
                                             _f64p, _i32p, _f64p, _f64p, _f64p]),
+        "kigpu_domain_mask": (C.c_int, [P, C.POINTER(Region), _f64p, C.c_int, P]),
+        "kigpu_domain_solution": (C.c_int, [P, P, C.c_int, _f64p, C.c_int, _f64p, P, P, C.c_int,
+                                            C.c_int, P, _f64p]),
         "kigpu_kernel_apply": (C.c_int, [P, P, C.c_int, _f64p, C.c_int, _f64p]),
         "kigpu_kernel_apply_treecode": (C.c_int, [P, P, _f64p, C.c_int, _f64p]),
         "kigpu_set_refine_apply": (C.c_int, [P, C.c_int]),
@@ -146,6 +150,47 @@ def load() -> C.CDLL:
         f.argtypes = args
     _lib = lib
     return lib
+
+
+class Region(C.Structure):
+    """kigpu_region: Boundary::is_in_domain as data (kind DISC, WINDING or STAR)."""
+    _fields_ = [("kind", C.c_int), ("eps", C.c_double), ("cx", C.c_double), ("cy", C.c_double),
+                ("radius", C.c_double), ("nholes", C.c_int), ("holes", C.c_void_p),
+                ("nodes", C.c_void_p), ("num_outer", C.c_int), ("hole_nodes", C.c_void_p),
+                ("nterms", C.c_int), ("amps", C.c_void_p), ("freqs", C.c_void_p),
+                ("phases", C.c_void_p)]
+
+
+REGION_DISC, REGION_WINDING, REGION_STAR = 0, 1, 2
+
+
+def _keep(region, *arrays):
+    region._arrays = arrays   # the structure holds raw pointers into these
+    return region
+
+
+def disc_region(center, radius, holes=(), eps=1e-2):
+    """Circle / Annulus / Donut rule; holes: (cx, cy, r) triples."""
+    h = np.ascontiguousarray(np.asarray(holes, np.float64).reshape(-1))
+    r = Region(REGION_DISC, eps, float(center[0]), float(center[1]), float(radius),
+               h.size // 3, h.ctypes.data if h.size else None, None, 0, None, 0, None, None, None)
+    return _keep(r, h)
+
+
+def winding_region(points, num_outer, hole_nodes=(), eps=1e-2):
+    """CubicBoundary rule over the boundary points (outer section first)."""
+    p = _f64(points).reshape(-1)
+    hn = np.ascontiguousarray(hole_nodes, np.int32)
+    r = Region(REGION_WINDING, eps, 0.0, 0.0, 0.0, hn.size, None, p.ctypes.data, int(num_outer),
+               hn.ctypes.data if hn.size else None, 0, None, None, None)
+    return _keep(r, p, hn)
+
+
+def star_region(amps, freqs, phases, eps=1e-2):
+    a, f, ph = (_f64(v).reshape(-1) for v in (amps, freqs, phases))
+    r = Region(REGION_STAR, eps, 0.0, 0.0, 0.0, 0, None, None, 0, None, a.size, a.ctypes.data,
+               f.ctypes.data, ph.ctypes.data)
+    return _keep(r, a, f, ph)
 
 
 class KigpuError(RuntimeError):
@@ -237,6 +282,15 @@ class Context:
         res["X_rr_piv"] = piv[:r].copy()
         res["status"] = st
         return res
+
+    def domain_mask(self, region, targets):
+        """Boundary::is_in_domain of every target (2n interleaved or (n, 2))."""
+        t = _f64(targets).reshape(-1)
+        n = t.size // 2
+        out = np.zeros(max(n, 1), np.uint8)
+        _check(self.lib.kigpu_domain_mask(self.h, C.byref(region), t, n, out.ctypes.data),
+               "kigpu_domain_mask")
+        return out[:n].astype(bool)
 
     def build_tree(self, points, normals, weights, curvatures, sd=1, leaf_dofs=32):
         return Tree(self, points, normals, weights, curvatures, sd, leaf_dofs)
@@ -538,6 +592,34 @@ class Tree:
                "kigpu_kernel_apply")
         Y = y.reshape(nrhs, self.ndof).T
         return Y[:, 0].copy() if one else np.array(Y)
+
+    def domain_solution(self, targets, mu, alpha=None, hole_centers=None, region=None, pde=None):
+        """boundary_integral_solve's domain values: forward(mu) + U_forward alpha,
+        zero outside region (None: no mask)."""
+        pde = self.pde if pde is None else pde
+        t = _f64(targets).reshape(-1)
+        ntgt = t.size // 2
+        mu = np.asarray(mu, np.float64)
+        one = mu.ndim == 1
+        M = mu.reshape(self.ndof, -1)
+        nrhs = M.shape[1]
+        flat = np.ascontiguousarray(M.T).reshape(-1)
+        nh = 0
+        a_ptr = hc_ptr = None
+        if alpha is not None:
+            hc = _f64(hole_centers).reshape(-1)
+            nh = hc.size // 2
+            A = np.asarray(alpha, np.float64).reshape(-1, nrhs)
+            af = np.ascontiguousarray(A.T).reshape(-1)
+            a_ptr, hc_ptr = af.ctypes.data, hc.ctypes.data
+        sd = 2 if pde == STOKES else 1
+        u = np.zeros(sd * ntgt * nrhs)
+        _check(self.lib.kigpu_domain_solution(self.ctx.h, self.h, pde, t, ntgt, flat, a_ptr,
+                                              hc_ptr, nh, nrhs,
+                                              C.addressof(region) if region is not None else None, u),
+               "kigpu_domain_solution")
+        U = u.reshape(nrhs, sd * ntgt).T
+        return U[:, 0].copy() if one else np.array(U)
 
     def forward(self, targets, mu, pde=None):
         pde = self.pde if pde is None else pde

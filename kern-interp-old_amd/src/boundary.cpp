@@ -87,6 +87,30 @@ void Boundary::apply_boundary_condition(int s, int e, BoundaryCondition bc) {
   }
 }
 
+// the disc rules of circle.cpp:34-42, annulus.cpp:66-78, donut.cpp:67-79:
+// outside when |a - c| + 1e-2 > R or |a - h| - 1e-2 < r_h for a hole
+static double dist_to(const Vec2& a, double cx, double cy) {
+  const double x = a.a[0] - cx, y = a.a[1] - cy;
+  return std::sqrt(x * x + y * y);
+}
+
+static bool in_disc(const Vec2& a, double cx, double cy, double R, const std::vector<Hole>& holes) {
+  if (dist_to(a, cx, cy) + 1e-2 > R) return false;
+  for (const Hole& h : holes)
+    if ((a - h.center).norm() - 1e-2 < h.radius) return false;
+  return true;
+}
+
+static DomainRule disc_rule(double cx, double cy, double R, const std::vector<Hole>& holes) {
+  DomainRule r;
+  r.kind = DomainRule::kDisc;
+  r.cx = cx;
+  r.cy = cy;
+  r.radius = R;
+  for (const Hole& h : holes) r.holes.insert(r.holes.end(), {h.center.a[0], h.center.a[1], h.radius});
+  return r;
+}
+
 static void push_ring(Boundary* b, double cx, double cy, double rad, int n, bool inward) {
   for (int i = 0; i < n; ++i) {
     const double ang = i * 2.0 * M_PI / n;
@@ -119,9 +143,12 @@ void Circle::initialize(int n, BoundaryCondition bc) {
   }
 }
 
+// circle.cpp:34-42: outside when |a - (0.5, 0.5)| + 1e-2 > 0.25
 bool Circle::is_in_domain(const Vec2& a) const {
-  return std::hypot(a.a[0] - 0.5, a.a[1] - 0.5) + 1e-2 <= 0.25;
+  return !(dist_to(a, 0.5, 0.5) + 1e-2 > 0.25);
 }
+
+DomainRule Circle::domain_rule() const { return disc_rule(0.5, 0.5, 0.25, {}); }
 
 static void ring_with_holes(Boundary* b, int n, double cx, double cy, double rad) {
   clear_arrays(b);
@@ -148,12 +175,10 @@ void Annulus::initialize(int n, BoundaryCondition bc) {
   }
 }
 
-bool Annulus::is_in_domain(const Vec2& a) const {
-  if (std::hypot(a.a[0] - 0.5, a.a[1] - 0.5) + 1e-2 > 1) return false;
-  for (const Hole& h : holes)
-    if ((a - h.center).norm() - 1e-2 < h.radius) return false;
-  return true;
-}
+// annulus.cpp:66-78
+bool Annulus::is_in_domain(const Vec2& a) const { return in_disc(a, 0.5, 0.5, 1.0, holes); }
+
+DomainRule Annulus::domain_rule() const { return disc_rule(0.5, 0.5, 1.0, holes); }
 
 void Donut::initialize(int n, BoundaryCondition bc) {
   if (holes.empty()) holes.push_back(Hole{Vec2(0.5, 0.5), 0.5, n / 5});
@@ -170,10 +195,10 @@ void Donut::initialize(int n, BoundaryCondition bc) {
   }
 }
 
-bool Donut::is_in_domain(const Vec2& a) const {
-  const double r = std::hypot(a.a[0] - 0.5, a.a[1] - 0.5);
-  return r + 1e-2 <= 1 && r - 1e-2 >= holes[0].radius;
-}
+// donut.cpp:67-79
+bool Donut::is_in_domain(const Vec2& a) const { return in_disc(a, 0.5, 0.5, 1.0, holes); }
+
+DomainRule Donut::domain_rule() const { return disc_rule(0.5, 0.5, 1.0, holes); }
 
 double StarCurve::radius(double t) const {
   double r = 1.0;
@@ -220,6 +245,15 @@ bool StarCurve::is_in_domain(const Vec2& a) const {
   return a.norm() + 1e-2 < radius(t);
 }
 
+DomainRule StarCurve::domain_rule() const {
+  DomainRule r;
+  r.kind = DomainRule::kStar;
+  r.amps = amps;
+  r.freqs = freqs;
+  r.phases = phases;
+  return r;
+}
+
 void HoledDisc::initialize(int n, BoundaryCondition bc) {
   for (auto& h : holes) h.num_nodes = hole_nodes;
   ring_with_holes(this, n, 0.0, 0.0, 1.0);
@@ -228,11 +262,8 @@ void HoledDisc::initialize(int n, BoundaryCondition bc) {
   apply_boundary_condition(0, weights.size(), bc);
 }
 
-bool HoledDisc::is_in_domain(const Vec2& a) const {
-  if (a.norm() + 1e-2 > 1) return false;
-  for (const Hole& h : holes)
-    if ((a - h.center).norm() - 1e-2 < h.radius) return false;
-  return true;
-}
+bool HoledDisc::is_in_domain(const Vec2& a) const { return in_disc(a, 0.0, 0.0, 1.0, holes); }
+
+DomainRule HoledDisc::domain_rule() const { return disc_rule(0.0, 0.0, 1.0, holes); }
 
 }  // namespace kern_interp

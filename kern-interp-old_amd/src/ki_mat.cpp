@@ -344,4 +344,104 @@ int ki_Mat::id(std::vector<int>* p, ki_Mat* Z, double tol) const {
   return k;
 }
 
+
+// Householder reduction to upper Hessenberg form, then Francis double-shift
+// QR steps on the active block (Golub & Van Loan, Matrix Computations,
+// algorithms 7.4.2 and 7.5.2), deflating at negligible subdiagonals; only the
+// block is updated, as no Schur vectors are wanted.
+std::vector<double> ki_Mat::real_eigenvalues() {
+  assert(height_ == width_);
+  const int n = width_;
+  std::vector<double> H((size_t)n * n);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) H[i + (size_t)n * j] = get(i, j);
+  auto h = [&](int i, int j) -> double& { return H[i + (size_t)n * j]; };
+  // v^T-reflect rows r0..r0+m-1 over columns c0..c1, then columns over rows q0..q1
+  auto reflect = [&](const double* v, int m, int r0, int c0, int c1, int q0, int q1) {
+    double vv = 0;
+    for (int a = 0; a < m; ++a) vv += v[a] * v[a];
+    if (vv == 0) return;
+    for (int j = c0; j <= c1; ++j) {
+      double s = 0;
+      for (int a = 0; a < m; ++a) s += v[a] * h(r0 + a, j);
+      s *= 2 / vv;
+      for (int a = 0; a < m; ++a) h(r0 + a, j) -= s * v[a];
+    }
+    for (int i = q0; i <= q1; ++i) {
+      double s = 0;
+      for (int a = 0; a < m; ++a) s += h(i, r0 + a) * v[a];
+      s *= 2 / vv;
+      for (int a = 0; a < m; ++a) h(i, r0 + a) -= s * v[a];
+    }
+  };
+  auto house = [](double* v, int m) {  // v <- x + sign(x0) |x| e0
+    double nx = 0;
+    for (int a = 0; a < m; ++a) nx += v[a] * v[a];
+    nx = std::sqrt(nx);
+    v[0] += v[0] >= 0 ? nx : -nx;
+  };
+  std::vector<double> v(std::max(n, 3));
+  for (int k = 0; k + 2 < n; ++k) {
+    const int m = n - k - 1;
+    for (int a = 0; a < m; ++a) v[a] = h(k + 1 + a, k);
+    house(v.data(), m);
+    reflect(v.data(), m, k + 1, 0, n - 1, 0, n - 1);
+  }
+  std::vector<double> re;
+  const double tiny = 1e-14, ulp = 2.220446049250313e-16;
+  int hi = n - 1, iter = 0;
+  while (hi >= 0) {
+    int l = hi;
+    while (l > 0 && std::fabs(h(l, l - 1)) > ulp * (std::fabs(h(l, l)) + std::fabs(h(l - 1, l - 1))))
+      --l;
+    if (l == hi) {
+      re.push_back(h(hi, hi));
+      --hi;
+      iter = 0;
+      continue;
+    }
+    if (l == hi - 1) {
+      const double a = h(l, l), b = h(l, hi), c = h(hi, l), d = h(hi, hi);
+      const double p = 0.5 * (a - d), disc = p * p + b * c;
+      if (disc >= 0) {
+        const double sq = std::sqrt(disc), m = 0.5 * (a + d);
+        const double e1 = m + (p >= 0 ? sq : -sq);
+        // the other root from the determinant, unless e1 is zero
+        const double e2 = e1 != 0 ? (a * d - b * c) / e1 : m - (p >= 0 ? sq : -sq);
+        re.push_back(e1);
+        re.push_back(e2);
+      } else if (std::sqrt(-disc) < tiny) {
+        re.push_back(0.5 * (a + d));
+        re.push_back(0.5 * (a + d));
+      }
+      hi -= 2;
+      iter = 0;
+      continue;
+    }
+    if (++iter > 60 * n) throw std::runtime_error("real_eigenvalues: QR iteration did not converge");
+    double s = h(hi - 1, hi - 1) + h(hi, hi);
+    double t = h(hi - 1, hi - 1) * h(hi, hi) - h(hi - 1, hi) * h(hi, hi - 1);
+    if (iter % 11 == 10) {  // exceptional shift
+      const double w = std::fabs(h(hi, hi - 1)) + std::fabs(h(hi - 1, hi - 2));
+      s = 1.5 * w;
+      t = w * w;
+    }
+    double x = h(l, l) * h(l, l) + h(l, l + 1) * h(l + 1, l) - s * h(l, l) + t;
+    double y = h(l + 1, l) * (h(l, l) + h(l + 1, l + 1) - s);
+    double z = h(l + 1, l) * h(l + 2, l + 1);
+    for (int k = l; k <= hi - 2; ++k) {
+      double w3[3] = {x, y, z};
+      house(w3, 3);
+      reflect(w3, 3, k, std::max(l, k - 1), hi, l, std::min(k + 3, hi));
+      x = h(k + 1, k);
+      y = h(k + 2, k);
+      if (k < hi - 2) z = h(k + 3, k);
+    }
+    double w2[2] = {x, y};
+    house(w2, 2);
+    reflect(w2, 2, hi - 1, hi - 2, hi, l, hi);
+  }
+  return re;
+}
+
 }  // namespace kern_interp

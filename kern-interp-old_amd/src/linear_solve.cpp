@@ -1,7 +1,8 @@
 // linear_solve of the drop-in API (interface of the reference's
 // linear_solve.h:17-35): hole border matrices and the end-to-end driver.
-// The domain evaluation is the GPU's matrix-free forward apply instead of
-// the reference's dense forward matrix (linear_solve.cpp:143-180).
+// The domain evaluation is one device call (kigpu_domain_solution): the
+// matrix-free forward apply instead of the reference's dense forward matrix,
+// the hole terms and the in-domain mask (linear_solve.cpp:143-180).
 #include "kern_interp/linear_solve.h"
 
 #include <cmath>
@@ -102,16 +103,44 @@ ki_Mat boundary_integral_solve(const Kernel& kernel, const Boundary& boundary, Q
   else linear_solve(fac, *tree, boundary.boundary_values, &mu, &alpha);
   const int ntgt = domain_points.size() / 2, sd = kernel.solution_dimension;
   ki_Mat sol(ntgt * sd, 1);
-  if (ntgt) {
-    detail::check(kigpu_forward_apply(detail::context(), tree->gpu(), (int)kernel.pde,
-                                      domain_points.data(), ntgt, mu.mat, 1, sol.mat),
-                  "kigpu_forward_apply");
-    if (U.width()) sol += initialize_U_mat(kernel.pde, boundary.holes, domain_points) * alpha;
+  if (!ntgt) return sol;
+  // forward apply, hole terms and the in-domain mask in one device call
+  const DomainRule rule = boundary.domain_rule();
+  kigpu_region reg{};
+  reg.kind = rule.kind;
+  reg.eps = rule.eps;
+  reg.cx = rule.cx;
+  reg.cy = rule.cy;
+  reg.radius = rule.radius;
+  if (rule.kind == DomainRule::kDisc) {
+    reg.nholes = rule.holes.size() / 3;
+    reg.holes = rule.holes.data();
+  } else if (rule.kind == DomainRule::kWinding) {
+    reg.nodes = boundary.points.data();
+    reg.num_outer = boundary.num_outer_nodes;
+    reg.nholes = rule.hole_nodes.size();
+    reg.hole_nodes = rule.hole_nodes.data();
+  } else if (rule.kind == DomainRule::kStar) {
+    reg.nterms = rule.amps.size();
+    reg.amps = rule.amps.data();
+    reg.freqs = rule.freqs.data();
+    reg.phases = rule.phases.data();
   }
-  for (int i = 0; i < ntgt; ++i) {
-    if (boundary.is_in_domain(Vec2(domain_points[2 * i], domain_points[2 * i + 1]))) continue;
-    for (int c = 0; c < sd; ++c) sol.set(sd * i + c, 0, 0.);
-  }
+  std::vector<double> hc;
+  for (const Hole& h : boundary.holes) hc.insert(hc.end(), {h.center.a[0], h.center.a[1]});
+  const bool with_holes = U.width() > 0;
+  detail::check(kigpu_domain_solution(detail::context(), tree->gpu(), (int)kernel.pde,
+                                      domain_points.data(), ntgt, mu.mat,
+                                      with_holes ? alpha.mat : nullptr, hc.data(),
+                                      with_holes ? (int)boundary.holes.size() : 0, 1,
+                                      rule.kind == DomainRule::kNone ? nullptr : &reg, sol.mat),
+                "kigpu_domain_solution");
+  // a Boundary subclass without a device rule: its own host is_in_domain
+  if (rule.kind == DomainRule::kNone)
+    for (int i = 0; i < ntgt; ++i) {
+      if (boundary.is_in_domain(Vec2(domain_points[2 * i], domain_points[2 * i + 1]))) continue;
+      for (int c = 0; c < sd; ++c) sol.set(sd * i + c, 0, 0.);
+    }
   return sol;
 }
 

@@ -1,5 +1,6 @@
 """The C++ drop-in API (include/kern_interp/*) through a C++ program that uses
-it exactly as the reference's test suite does (tests/cpp/dropin_test.cpp)."""
+it exactly as the reference's test suite does (tests/cpp/dropin_test.cpp),
+one process per group of cases."""
 import os
 import subprocess
 
@@ -10,10 +11,11 @@ BIN = os.path.join(ROOT, "kern-interp-old_amd", "_build", "dropin_test")
 
 
 @pytest.mark.gpu
-def test_cpp_dropin_api():
-    print("[dropin] tests/cpp/dropin_test.cpp", flush=True)
+@pytest.mark.parametrize("group", ["api", "matrix", "matrix_big", "analytic", "fp32"])
+def test_cpp_dropin_api(group):
+    print(f"[dropin] tests/cpp/dropin_test.cpp {group}", flush=True)
     assert os.path.exists(BIN), "build first (__graft_entry__.build())"
-    r = subprocess.run([BIN], capture_output=True, text=True, timeout=110)
+    r = subprocess.run([BIN, group], capture_output=True, text=True, timeout=110)
     print(r.stdout)
     print(r.stderr[-2000:])
     assert r.returncode == 0, r.stdout + r.stderr
