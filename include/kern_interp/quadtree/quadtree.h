@@ -1,7 +1,8 @@
 // Drop-in for kern_interp/quadtree/quadtree.h (reference quadtree.h:14-102).
 // initialize_tree builds the tree on the GPU (kigpu_tree_build) and mirrors
 // nodes, DOF lists and neighbor lists into these host structs; the factor
-// data stays device-resident in the handle (gpu()).
+// data stays device-resident in the handle (gpu()).  After host edits
+// (perturb, copy_into) the device tree is rebuilt from the structs.
 #ifndef KERN_INTERP_QUADTREE_QUADTREE_H_
 #define KERN_INTERP_QUADTREE_QUADTREE_H_
 
@@ -84,6 +85,20 @@ class QuadTree {
   void recursive_add(QuadTreeNode* node, double x, double y, int point_ind);
   void get_descendent_neighbors(QuadTreeNode* big, QuadTreeNode* small);
   void node_subdivide(QuadTreeNode* node);
+  void consolidate_node(QuadTreeNode* node);
+
+  // Incremental edits (quadtree.cpp:288-641) on the host structs; the device
+  // tree is then rebuilt from them (kigpu_tree_import).  perturb maps the DOF
+  // lists to the new boundary's numbering, adds and deletes points, marks
+  // the touched boxes, their neighbors and ancestors uncompressed, splits
+  // overfull leaves, merges sparse subtrees and recomputes the neighbor
+  // lists.  The next skeletonize factors the new device tree (every box: one
+  // batched launch per level costs about what the marked boxes alone would).
+  // copy_into deep-copies the structs and, when this tree is factored, gives
+  // the copy the same factorization.
+  void copy_into(QuadTree* new_tree) const;
+  void mark_neighbors_and_parents(QuadTreeNode* node);
+  void perturb(const Boundary& new_boundary);
 
   // device handle (geometry, tree, factorization)
   kigpu_tree* gpu() const { return gpu_; }
@@ -91,9 +106,15 @@ class QuadTree {
   const std::vector<QuadTreeNode*>& nodes_by_id() const { return by_id_; }
 
  private:
+  friend class SkelFactorization;
   void clear();
+  void rebuild_device();  // device tree from the host structs
   kigpu_tree* gpu_ = nullptr;
   std::vector<QuadTreeNode*> by_id_;
+  std::vector<double> normals_, weights_, curvatures_;  // the tree's boundary arrays
+  // the factorization on the device tree (pde < 0: none)
+  int fac_pde_ = -1, fac_precision_ = 0, fac_min_dofs_ = 0;
+  double fac_tol_ = 0;
 };
 
 }  // namespace kern_interp

@@ -128,6 +128,25 @@ kigpu_status kigpu_tree_build_device(kigpu_ctx* ctx, const double* d_px, const d
                                      const double* d_nx, const double* d_ny, const double* d_w,
                                      const double* d_kappa, int npts, int sd, int leaf_dofs,
                                      kigpu_tree** tree);
+/* A tree described on the host instead of built from the points: QuadTree's
+ * node structs after QuadTree::perturb or QuadTree::copy_into
+ * (quadtree.cpp:359-641).  Nodes numbered level by level: level_off[nlevels
+ * + 1] (level 0 = the root alone); parent[nn] (-1 for the root); child[4 nn]
+ * in the slots {BL, TL, TR, BR} (all four or none, -1); corners[8 nn]
+ * ({BL, TL, TR, BR} x, y) and side[nn]; each leaf's original_box DOF list
+ * box_dofs[box_off[x] .. box_off[x + 1]) (interior entries ignored: their
+ * points are the union of their leaves'); each node's neighbor list
+ * nbrs[nbr_off[x] .. nbr_off[x + 1]) in QuadTree order (coarser leaves, then
+ * same-level boxes, then finer boxes, which are not read); the root box's
+ * min / max (QuadTree::min/max).  Every point in exactly one leaf.  The
+ * factorization runs on the result as on a built tree. */
+kigpu_status kigpu_tree_import(kigpu_ctx* ctx, const double* points, const double* normals,
+                               const double* weights, const double* curvatures, int npts, int sd,
+                               int leaf_dofs, int nlevels, const int64_t* level_off,
+                               const int* parent, const int* child, const double* corners,
+                               const double* side, const int64_t* box_off, const int* box_dofs,
+                               const int64_t* nbr_off, const int* nbrs, double tree_min,
+                               double tree_max, kigpu_tree** tree);
 kigpu_status kigpu_tree_destroy(kigpu_tree* tree);
 /* A batch of independent boundaries (instance i: npts[i] points, stored one
  * after another in the arrays, interleaved like kigpu_tree_build) built as

@@ -191,7 +191,8 @@ kigpu_status kigpu_last_timings(kigpu_ctx* ctx, kigpu_timings* out) {
 static kigpu_status tree_build_common(kigpu_ctx* ctx, int npts, int sd, int leaf_dofs,
                                       kigpu_tree** out,
                                       const std::function<void(TreeData&, hipStream_t)>& load,
-                                      const std::vector<int64_t>* inst_pt = nullptr) {
+                                      const std::vector<int64_t>* inst_pt = nullptr,
+                                      const TreeImport* imported = nullptr) {
   KI_REQUIRE(ctx && out, "null argument");
   KI_REQUIRE(npts > 0, "number of boundary points to init tree cannot be 0");
   KI_REQUIRE(sd == 1 || sd == 2, "solution dimension must be 1 or 2");
@@ -216,7 +217,8 @@ static kigpu_status tree_build_common(kigpu_ctx* ctx, int npts, int sd, int leaf
     KI_HIP(hipEventRecord(c.ev[10], c.stream));
     load(t, c.stream);
     c.tim.syncs = 0;
-    build_tree(t, c.stream, &c.tim.syncs, c.pin_down, c.evpool);
+    if (imported) import_tree(t, *imported, c.stream);
+    else build_tree(t, c.stream, &c.tim.syncs, c.pin_down, c.evpool);
     KI_HIP(hipEventRecord(c.ev[11], c.stream));
     stream_wait(c.stream);
     float ms = 0;
@@ -247,6 +249,52 @@ kigpu_status kigpu_tree_build(kigpu_ctx* ctx, const double* points, const double
     t.kappa.upload(curvatures, npts, s);
     stream_wait(s);
   });
+}
+
+kigpu_status kigpu_tree_import(kigpu_ctx* ctx, const double* points, const double* normals,
+                               const double* weights, const double* curvatures, int npts, int sd,
+                               int leaf_dofs, int nlevels, const int64_t* level_off,
+                               const int* parent, const int* child, const double* corners,
+                               const double* side, const int64_t* box_off, const int* box_dofs,
+                               const int64_t* nbr_off, const int* nbrs, double tree_min,
+                               double tree_max, kigpu_tree** out) {
+  KI_REQUIRE(points && normals && weights && curvatures, "null geometry array");
+  KI_REQUIRE(nlevels >= 1 && level_off && parent && child && corners && side && box_off &&
+                 box_dofs && nbr_off,
+             "null tree array");
+  KI_REQUIRE(nbrs || nbr_off[level_off[nlevels]] == 0, "null neighbor array");
+  TreeImport in;
+  in.nlevels = nlevels;
+  in.level_off = level_off;
+  in.parent = parent;
+  in.child = child;
+  in.corners = corners;
+  in.side = side;
+  in.box_off = box_off;
+  in.box_dofs = box_dofs;
+  in.nbr_off = nbr_off;
+  in.nbrs = nbrs;
+  in.min = tree_min;
+  in.max = tree_max;
+  return tree_build_common(
+      ctx, npts, sd, leaf_dofs, out,
+      [&](TreeData& t, hipStream_t s) {
+        std::vector<double> a(npts), b(npts), na(npts), nb(npts);
+        for (int i = 0; i < npts; ++i) {
+          a[i] = points[2 * i];
+          b[i] = points[2 * i + 1];
+          na[i] = normals[2 * i];
+          nb[i] = normals[2 * i + 1];
+        }
+        t.px.upload(a, s);
+        t.py.upload(b, s);
+        t.nx.upload(na, s);
+        t.ny.upload(nb, s);
+        t.w.upload(weights, npts, s);
+        t.kappa.upload(curvatures, npts, s);
+        stream_wait(s);
+      },
+      nullptr, &in);
 }
 
 kigpu_status kigpu_tree_build_device(kigpu_ctx* ctx, const double* d_px, const double* d_py,

@@ -133,6 +133,26 @@ struct TreeData {
 // evs: an event pool (grown to the tree's depth bound) for the pipelined levels
 void build_tree(TreeData& t, hipStream_t s, int* syncs, PinnedHost& pin,
                 std::vector<hipEvent_t>& evs);
+// A quadtree described on the host, nodes numbered level by level
+// (tree_import.cpp): level_off[nlevels + 1]; parent[nn] (-1: root);
+// child[4 nn] (all -1 or all set); corners[8 nn]; side[nn]; every leaf's
+// DOF list box_dofs[box_off[x] .. box_off[x + 1]) (interior entries are not
+// read); neighbor lists nbrs[nbr_off[x] .. nbr_off[x + 1]) in QuadTree order:
+// coarser leaves, then same-level boxes, then finer boxes (not read).
+struct TreeImport {
+  int nlevels = 0;
+  const int64_t* level_off = nullptr;
+  const int* parent = nullptr;
+  const int* child = nullptr;
+  const double* corners = nullptr;
+  const double* side = nullptr;
+  const int64_t* box_off = nullptr;
+  const int* box_dofs = nullptr;
+  const int64_t* nbr_off = nullptr;
+  const int* nbrs = nullptr;
+  double min = 0, max = 0;
+};
+void import_tree(TreeData& t, const TreeImport& in, hipStream_t s);
 void ensure_tree_mirrors(TreeData& t);    // host copies used by the export calls
 void ensure_factor_mirrors(TreeData& t);
 

@@ -61,7 +61,7 @@ EXPORTS = [
     "kigpu_set_border", "kigpu_factor", "kigpu_factor_ranks", "kigpu_factor_export_lists",
     "kigpu_factor_allskel", "kigpu_solve", "kigpu_solve_device", "kigpu_solve_device_noalloc",
     "kigpu_proxy_block", "kigpu_interp_decomp", "kigpu_schur_eliminate",
-    "kigpu_domain_mask", "kigpu_domain_solution",
+    "kigpu_domain_mask", "kigpu_domain_solution", "kigpu_tree_import",
     "kigpu_kernel_apply",
     "kigpu_kernel_apply_device", "kigpu_forward_apply", "kigpu_geometry_create",
     "kigpu_geometry_destroy", "kigpu_kernel_block", "kigpu_factor_export_active",

@@ -108,6 +108,10 @@ void SkelFactorization::skeletonize(const Kernel& K, QuadTree* tree) {
   o.precision = fp32_refine ? KIGPU_FP32_REFINE : KIGPU_FP64;
   o.min_dofs = MIN_DOFS_TO_COMPRESS;
   detail::check(kigpu_factor(detail::context(), g, &o), "kigpu_factor");
+  tree->fac_pde_ = o.pde;
+  tree->fac_tol_ = o.id_tol;
+  tree->fac_precision_ = o.precision;
+  tree->fac_min_dofs_ = o.min_dofs;
   const auto& nodes = tree->nodes_by_id();
   const int nn = nodes.size();
   std::vector<int> rank(nn), nact(nn);

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -15,16 +16,26 @@
 #include "kern_interp/boundaries/circle.h"
 #include "kern_interp/boundaries/cubic_spline.h"
 #include "kern_interp/boundaries/donut.h"
+#include "kern_interp/boundaries/ex1boundary.h"
+#include "kern_interp/boundaries/ex3boundary.h"
 #include "kern_interp/boundaries/star.h"
 #include "kern_interp/kernel/kernel.h"
 #include "kern_interp/ki_mat.h"
 #include "kern_interp/linear_solve.h"
 #include "kern_interp/quadtree/quadtree.h"
 #include "kern_interp/skel_factorization/skel_factorization.h"
+#include "kigpu.h"
 
 using namespace kern_interp;
 
 static int g_fail = 0;
+
+static void ck(kigpu_status st, const char* what) {
+  if (st != KIGPU_OK) {
+    std::printf("FAIL %s: %s\n", what, kigpu_last_error());
+    std::exit(1);
+  }
+}
 
 static void report(const std::string& name, double err, double bound) {
   const bool ok = err <= bound;
@@ -370,11 +381,122 @@ static void per_box_api() {
   per_box_matches_batched("stokes_star2048", ks, &st, 64, 1e-6);
 }
 
+// every point in exactly one leaf, inside the leaf's box; parent/child and
+// level links consistent; the device tree rebuilt from the structs exports
+// the same boxes and neighbor lists
+static long tree_invariants(const QuadTree& t, int npts, int sd) {
+  long bad = 0;
+  std::vector<int> owner(npts, -1);
+  std::map<const QuadTreeNode*, int> id;
+  int x = 0;
+  for (size_t l = 0; l < t.levels.size(); ++l)
+    for (const QuadTreeNode* n : t.levels[l]->nodes) {
+      id[n] = x++;
+      bad += n->level != (int)l;
+      if (l) bad += n->parent == nullptr || n->parent->level != (int)l - 1;
+      if (!n->is_leaf)
+        for (const QuadTreeNode* c : n->children) bad += c == nullptr || c->parent != n;
+      if (!n->is_leaf) continue;
+      const std::vector<int>& ob = n->dof_lists.original_box;
+      for (size_t e = 0; e < ob.size(); e += sd) {
+        const int p = ob[e] / sd;
+        bad += owner[p] != -1;
+        owner[p] = 1;
+        const double px = t.boundary_points[2 * p], py = t.boundary_points[2 * p + 1];
+        const double mx = n->corners[0], my = n->corners[1], Mx = n->corners[4], My = n->corners[5];
+        // half-open boxes, except where a point sits on the root's far edges
+        bad += !(px >= mx && py >= my && px <= Mx && py <= My);
+      }
+    }
+  for (int p = 0; p < npts; ++p) bad += owner[p] != 1;
+  // the device copy against the structs
+  kigpu_tree* g = t.gpu();
+  const int nn = x;
+  std::vector<int64_t> boff(nn + 1), noff(nn + 1);
+  ck(kigpu_tree_export_boxes(g, boff.data(), nullptr), "export_boxes");
+  std::vector<int> dofs(boff[nn]);
+  ck(kigpu_tree_export_boxes(g, boff.data(), dofs.data()), "export_boxes");
+  ck(kigpu_tree_export_neighbors(g, noff.data(), nullptr), "export_neighbors");
+  std::vector<int> nbrs(noff[nn]);
+  ck(kigpu_tree_export_neighbors(g, noff.data(), nbrs.data()), "export_neighbors");
+  x = 0;
+  for (size_t l = 0; l < t.levels.size(); ++l)
+    for (const QuadTreeNode* n : t.levels[l]->nodes) {
+      if (n->is_leaf)
+        bad += std::vector<int>(dofs.begin() + boff[x], dofs.begin() + boff[x + 1]) !=
+               n->dof_lists.original_box;
+      std::vector<int> want;
+      for (const QuadTreeNode* nb : n->neighbors) want.push_back(id[nb]);
+      bad += std::vector<int>(nbrs.begin() + noff[x], nbrs.begin() + noff[x + 1]) != want;
+      ++x;
+    }
+  return bad;
+}
+
+// kern_interp_test.cpp:491-533 (Ex3UpdateLosesNoAcc) and :536-567
+// (TreeCopyGivesSameAnswer)
+static void incremental() {
+  const double id_tol = 1e-6;
+  const int N = 1 << 12;
+  {
+    std::unique_ptr<Boundary> b(new Ex3Boundary());
+    b->initialize(N, EX3B);
+    QuadTree qt;
+    qt.initialize_tree(b.get(), 2, 2);
+    std::vector<double> pts;
+    get_domain_points(20, &pts, qt.min, qt.max, qt.min, qt.max);
+    Kernel k(2, 2, Kernel::STOKES, b.get(), pts);
+    double worst = 0;
+    long bad = 0;
+    for (int frame = 0; frame < 10; ++frame) {
+      const double ang = (frame / 10.) * 2 * M_PI;
+      b->perturbation_parameters[0] = ang;
+      b->perturbation_parameters[1] = ang + M_PI;
+      b->initialize(N, EX3B);
+      qt.perturb(*b);
+      bad += tree_invariants(qt, b->weights.size(), 2);
+      k.update_boundary(b.get());
+      ki_Mat sol = boundary_integral_solve(k, *b, &qt, id_tol, 4, pts);
+      QuadTree fresh;
+      fresh.initialize_tree(b.get(), 2, 2);
+      ki_Mat ref = boundary_integral_solve(k, *b, &fresh, id_tol, 4, pts);
+      const double e = (ref - sol).vec_two_norm() / ref.vec_two_norm();
+      std::printf("  frame %d: %zu levels, |perturbed - fresh| / |fresh| = %.3e\n", frame,
+                  qt.levels.size(), e);
+      worst = std::max(worst, e);
+    }
+    report("Ex3PerturbedTreeInvariants", (double)bad, 0);
+    report("Ex3UpdateLosesNoAcc", worst, 20 * id_tol);
+  }
+  {
+    std::unique_ptr<Boundary> b(new Ex1Boundary());
+    b->initialize(N, DEFAULT);
+    QuadTree qt;
+    qt.initialize_tree(b.get(), 2, 2);
+    std::vector<double> pts;
+    get_domain_points(20, &pts, qt.min, qt.max, qt.min, qt.max);
+    Kernel k(2, 2, Kernel::STOKES, b.get(), pts);
+    ki_Mat sol = boundary_integral_solve(k, *b, &qt, id_tol, 4, pts);
+    QuadTree fresh;
+    qt.copy_into(&fresh);
+    report("TreeCopyInvariants", (double)tree_invariants(fresh, b->weights.size(), 2), 0);
+    // the copy is factored: it solves without a skeletonize of its own
+    SkelFactorization fac(id_tol, 4);
+    ki_Mat mu(b->boundary_values.height(), 1), alpha;
+    fac.multiply_connected_solve(fresh, &mu, &alpha, b->boundary_values);
+    ki_Mat mu0(b->boundary_values.height(), 1), alpha0;
+    fac.multiply_connected_solve(qt, &mu0, &alpha0, b->boundary_values);
+    report("TreeCopySolvesWithoutRefactor", (mu - mu0).vec_two_norm(), 0);
+    ki_Mat again = boundary_integral_solve(k, *b, &fresh, id_tol, 4, pts);
+    report("TreeCopyGivesSameAnswer", (again - sol).vec_two_norm(), 1e-15);
+  }
+}
+
 // usage: dropin_test [group ...]; groups: api matrix matrix_big analytic fp32
-// (default: all)
+// incremental (default: all)
 int main(int argc, char** argv) {
   std::vector<std::string> groups(argv + 1, argv + argc);
-  if (groups.empty()) groups = {"api", "matrix", "matrix_big", "analytic", "fp32"};
+  if (groups.empty()) groups = {"api", "matrix", "matrix_big", "analytic", "fp32", "incremental"};
   for (const std::string& g : groups) {
     std::printf("== %s\n", g.c_str());
     std::fflush(stdout);
@@ -383,6 +505,7 @@ int main(int argc, char** argv) {
     else if (g == "matrix_big") backward_error_matrix(1);
     else if (g == "analytic") analytic_agreement();
     else if (g == "fp32") fp32_refine_residual();
+    else if (g == "incremental") incremental();
     else {
       std::printf("unknown group %s\n", g.c_str());
       return 2;

@@ -11,7 +11,7 @@ BIN = os.path.join(ROOT, "kern-interp-old_amd", "_build", "dropin_test")
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("group", ["api", "matrix", "matrix_big", "analytic", "fp32"])
+@pytest.mark.parametrize("group", ["api", "matrix", "matrix_big", "analytic", "fp32", "incremental"])
 def test_cpp_dropin_api(group):
     print(f"[dropin] tests/cpp/dropin_test.cpp {group}", flush=True)
     assert os.path.exists(BIN), "build first (__graft_entry__.build())"
