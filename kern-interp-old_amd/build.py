@@ -47,9 +47,10 @@ def _compile(src: str) -> str:
 
 
 API_SOURCES = ["runtime.cpp", "ki_mat.cpp", "boundary.cpp", "cubic_boundary.cpp", "quadtree.cpp", "kernel.cpp",
-               "skel_factorization.cpp", "linear_solve.cpp"]
+               "skel_factorization.cpp", "linear_solve.cpp", "output_dump.cpp"]
 API_LIB = os.path.join(HERE, "libkern_interp.so")
 DROPIN_TEST = os.path.join(BUILD, "dropin_test")
+EXPERIMENTS = ["experiment_one", "experiment_two", "experiment_three"]
 CXX = os.environ.get("CXX", "g++")
 CXXFLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-I" + os.path.join(ROOT, "include")]
 
@@ -80,6 +81,14 @@ def build_api(jobs: int = 4) -> str:
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"dropin_test link failed:\n{r.stderr[-8000:]}")
+    # the reference's experiment drivers over the drop-in API
+    for ex in EXPERIMENTS:
+        cmd = [CXX, *CXXFLAGS, "-fopenmp", os.path.join(HERE, "experiments", ex + ".cpp"), "-o",
+               os.path.join(BUILD, ex), "-L" + HERE, "-lkern_interp", "-lkigpu",
+               "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link," + HERE]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"{ex} link failed:\n{r.stderr[-8000:]}")
     return API_LIB
 
 

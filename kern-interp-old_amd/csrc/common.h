@@ -6,11 +6,13 @@
 
 #include <cstring>
 #include <initializer_list>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
 
 #include "kigpu.h"
+#include "lds_grants.h"
 
 namespace kigpu {
 
@@ -220,6 +222,11 @@ void exclusive_scan_i32_to_i64_dev(const int* in, int64_t* out, const int64_t* d
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 
+inline LdsGrants& lds_grants() {
+  static LdsGrants g;
+  return g;
+}
+
 // allow a kernel to use `bytes` of dynamic LDS (above the default limit).
 // The attribute is set only when it grows for that kernel on the current
 // device: the per-level launches ask again and again, and each set costs a
@@ -227,23 +234,13 @@ inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 template <typename K>
 inline void allow_lds(K kern, size_t bytes) {
   if (bytes == 0) return;
-  struct Key {
-    const void* f;
-    int dev;
-  };
-  thread_local std::vector<std::pair<Key, size_t>> granted;
   int dev = 0;
   KI_HIP(hipGetDevice(&dev));
-  const void* f = reinterpret_cast<const void*>(kern);
-  for (auto& g : granted)
-    if (g.first.f == f && g.first.dev == dev) {
-      if (bytes <= g.second) return;
-      KI_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-      g.second = bytes;
-      return;
-    }
-  KI_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-  granted.push_back({Key{f, dev}, bytes});
+  lds_grants().allow(reinterpret_cast<const void*>(kern), dev, bytes,
+                     [](const void* f, size_t b) {
+                       KI_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                  (int)b));
+                     });
 }
 
 }  // namespace kigpu

@@ -37,6 +37,7 @@ void QuadTree::clear() {
 }
 
 void QuadTree::initialize_tree(Boundary* boundary, int sd, int dd) {
+  const detail::CtxRef lock = detail::context();  // the build and its exports
   assert(!boundary->points.empty() && "number of boundary_points to init tree cannot be 0.");
   clear();
   boundary_points = boundary->points;
@@ -386,6 +387,7 @@ void QuadTree::perturb(const Boundary& nb) {
 
 // quadtree.cpp:563-641
 void QuadTree::copy_into(QuadTree* dst) const {
+  const detail::CtxRef lock = detail::context();
   dst->clear();
   std::unordered_map<const QuadTreeNode*, QuadTreeNode*> to_new;
   to_new[nullptr] = nullptr;
@@ -460,6 +462,7 @@ void QuadTree::copy_into(QuadTree* dst) const {
 // the device tree from the host structs: nodes numbered level by level,
 // empty trailing levels (left by consolidate_node) dropped
 void QuadTree::rebuild_device() {
+  const detail::CtxRef lock = detail::context();
   while (!levels.empty() && levels.back()->nodes.empty()) {
     delete levels.back();
     levels.pop_back();

@@ -97,6 +97,7 @@ void SkelFactorization::get_update(ki_Mat* update, const std::vector<int>& BN,
 }
 
 void SkelFactorization::skeletonize(const Kernel& K, QuadTree* tree) {
+  const detail::CtxRef lock = detail::context();  // the whole factor and its exports
   kigpu_tree* g = tree->gpu();
   assert(g && "initialize_tree before skeletonize");
   const int h = tree->U.width();
@@ -165,6 +166,7 @@ void SkelFactorization::apply_diag_inv_matrix(const ki_Mat& mat, const std::vect
 
 static void gpu_solve(const SkelFactorization& f, const QuadTree& tree, ki_Mat* x, ki_Mat* alpha,
                       const ki_Mat& b) {
+  const detail::CtxRef lock = detail::context();
   kigpu_tree* g = tree.gpu();
   assert(g && x->height() == b.height());
   *x = ki_Mat(b.height(), b.width());
