@@ -103,6 +103,8 @@ typedef struct {
    * summed (each pair sums the terms its box needs for its leaf, <= 48) */
   int tc_launches;
   double tc_kernel_s, tc_eval_s, tc_series_pairs, tc_direct_pairs, tc_series_terms;
+  /* sweeps of the last solve replayed from a captured graph, and captured */
+  int graph_replays, graph_captures;
 } kigpu_timings;
 
 /* ---- context ---------------------------------------------------------- */
@@ -111,6 +113,13 @@ kigpu_status kigpu_context_destroy(kigpu_ctx* ctx);
 const char* kigpu_status_string(kigpu_status s);
 const char* kigpu_last_error(void);
 /* device-side phase timings of the last factor/solve on this context */
+/* Solve sweeps from HIP graphs (no counterpart in the reference): with on = 1,
+ * each tree captures the launches of its up/down sweeps and root (or
+ * bordered) step on the second solve with the same vector, shape and border,
+ * and replays the graph from then on (a new factorization drops them).  For
+ * callers that solve many times with one factorization; the results are those
+ * of the direct launches, bit for bit.  Default 0. */
+kigpu_status kigpu_set_graphs(kigpu_ctx* ctx, int on);
 kigpu_status kigpu_last_timings(kigpu_ctx* ctx, kigpu_timings* out);
 
 /* ---- geometry + tree -------------------------------------------------- */

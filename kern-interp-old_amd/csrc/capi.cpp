@@ -182,6 +182,12 @@ kigpu_status kigpu_context_destroy(kigpu_ctx* ctx) {
   return KIGPU_OK;
 }
 
+kigpu_status kigpu_set_graphs(kigpu_ctx* ctx, int on) {
+  KI_REQUIRE(ctx, "null context");
+  ctx->c.graphs = on != 0;
+  return KIGPU_OK;
+}
+
 kigpu_status kigpu_last_timings(kigpu_ctx* ctx, kigpu_timings* out) {
   KI_REQUIRE(ctx && out, "null argument");
   *out = ctx->c.tim;

@@ -49,6 +49,7 @@ struct Context {
   std::vector<hipEvent_t> evpool;  // per-level phase markers of a factorization
   kigpu_timings tim{};
   int refine_apply = 0;     // refinement residual: 0 treecode where it applies, 1 direct sum
+  bool graphs = false;      // replay the solve sweeps from captured HIP graphs
   DBuf<double> ring;        // proxy unit vectors cos | sin (host libm)
   size_t lds_max = 65536;
   PinnedHost pin_down;      // per-level downloads

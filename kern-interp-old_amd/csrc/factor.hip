@@ -1549,6 +1549,7 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
   t.pos_pool.alloc(pool);
   t.piv_pool.alloc(pool);
   t.act_pool_size = pool;
+  t.drop_graphs();  // they hold the previous factorization's launches
   t.work.clear();
   t.work.resize(t.nlevels);
   std::vector<int> h_n, h_q;
@@ -1972,6 +1973,8 @@ static double dec_flops(int n, int k, double c_gen) {
   const double r = n - k;
   return c_gen * n * n + 6.0 * k * k * r + 8.0 * k * r * r + 2.0 / 3.0 * r * r * r;
 }
+
+TreeData::~TreeData() { drop_graphs(); }
 
 void factor_tree(Context& c, TreeData& t, const kigpu_factor_opts& o) {
   c.tim.t_lists = c.tim.t_id = c.tim.t_decouple = c.tim.t_root = 0;

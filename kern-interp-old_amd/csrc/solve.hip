@@ -1051,10 +1051,61 @@ __global__ void k_axpy_f64(double* x, const double* d, size_t n) {
   if (i < n) x[i] += d[i];
 }
 
-// x <- F^{-1} x, or F^{-T} x with trans (no border then)
+template <typename T>
+void sweeps_launch(Context& c, TreeData& t, T* x, int ldx, int nrhs, const double* g, bool trans);
+
+// x <- F^{-1} x, or F^{-T} x with trans (no border then).  With graphs on,
+// the launches of a sweep are captured on its second call with the same
+// vector, shape and border vector, and replayed from then on: the first call
+// runs them directly (it sets the LDS limits and sizes the border scratch, so
+// the captured launches allocate nothing), and the sweeps hold no memset,
+// host wait or allocation.
 template <typename T>
 void sweeps(Context& c, TreeData& t, T* x, int ldx, int nrhs, const double* g = nullptr,
             bool trans = false) {
+  if (!c.graphs) {
+    sweeps_launch<T>(c, t, x, ldx, nrhs, g, trans);
+    return;
+  }
+  hipStream_t s = c.stream;
+  TreeData::SweepGraph* e = nullptr;
+  for (auto& q : t.sweep_graphs)
+    if (q.tsz == (int)sizeof(T) && q.x == x && q.ldx == ldx && q.nrhs == nrhs && q.g == g &&
+        q.trans == trans)
+      e = &q;
+  if (!e) {
+    if (t.sweep_graphs.size() >= 8) {  // keep a few: the oldest goes
+      if (t.sweep_graphs.front().exec) (void)hipGraphExecDestroy(t.sweep_graphs.front().exec);
+      t.sweep_graphs.erase(t.sweep_graphs.begin());
+    }
+    t.sweep_graphs.push_back({(int)sizeof(T), x, ldx, nrhs, g, trans, nullptr});
+    sweeps_launch<T>(c, t, x, ldx, nrhs, g, trans);
+    return;
+  }
+  if (!e->exec) {
+    hipGraph_t graph = nullptr;
+    KI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      sweeps_launch<T>(c, t, x, ldx, nrhs, g, trans);
+    } catch (...) {
+      (void)hipStreamEndCapture(s, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    KI_HIP(hipStreamEndCapture(s, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    KI_HIP(ie);
+    e->exec = exec;
+    ++c.tim.graph_captures;
+  }
+  KI_HIP(hipGraphLaunch(e->exec, s));
+  ++c.tim.graph_replays;
+}
+
+template <typename T>
+void sweeps_launch(Context& c, TreeData& t, T* x, int ldx, int nrhs, const double* g, bool trans) {
   hipStream_t s = c.stream;
   up_sweeps<T>(c, t, x, ldx, nrhs, trans ? 3 : 1);
   if (t.nborder > 0) {
@@ -1401,6 +1452,7 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
   c.tim.tc_launches = 0;
   c.tim.tc_kernel_s = c.tim.tc_eval_s = c.tim.tc_series_pairs = c.tim.tc_direct_pairs = 0;
   c.tim.tc_series_terms = 0;
+  c.tim.graph_replays = c.tim.graph_captures = 0;
   const size_t len = (size_t)t.ndof * nrhs;
   const int h = t.nborder;
   const size_t alen = (size_t)h * nrhs;
@@ -1410,9 +1462,18 @@ void solve_tree(Context& c, TreeData& t, const double* d_b, int nrhs, double* d_
   }
   KI_HIP(hipEventRecord(c.ev[6], s));
   if (t.opts.precision == KIGPU_FP64) {
-    if (d_x != d_b)
-      KI_HIP(hipMemcpyAsync(d_x, d_b, sizeof(double) * len, hipMemcpyDeviceToDevice, s));
-    sweeps<double>(c, t, d_x, t.ndof, nrhs, nullptr, trans);
+    if (c.graphs) {
+      // the captured sweeps run on the tree's own vector, whose address
+      // stays the same from solve to solve
+      t.gx.alloc(len);
+      KI_HIP(hipMemcpyAsync(t.gx.p, d_b, sizeof(double) * len, hipMemcpyDeviceToDevice, s));
+      sweeps<double>(c, t, t.gx.p, t.ndof, nrhs, nullptr, trans);
+      KI_HIP(hipMemcpyAsync(d_x, t.gx.p, sizeof(double) * len, hipMemcpyDeviceToDevice, s));
+    } else {
+      if (d_x != d_b)
+        KI_HIP(hipMemcpyAsync(d_x, d_b, sizeof(double) * len, hipMemcpyDeviceToDevice, s));
+      sweeps<double>(c, t, d_x, t.ndof, nrhs, nullptr, trans);
+    }
     if (h && d_alpha)
       KI_HIP(hipMemcpyAsync(d_alpha, t.alpha_dev.p, sizeof(double) * alen,
                             hipMemcpyDeviceToDevice, s));

@@ -69,6 +69,25 @@ struct TreeData {
   // ---- factorization ----
   bool factored = false;
   kigpu_factor_opts opts{};
+  // solve sweeps captured as HIP graphs (contexts with graphs on): one per
+  // (element size, vector, leading dimension, columns, border vector,
+  // transpose); exec == nullptr marks a sweep seen once and not captured yet
+  struct SweepGraph {
+    int tsz;
+    const void* x;
+    int ldx, nrhs;
+    const void* g;
+    bool trans;
+    hipGraphExec_t exec;
+  };
+  std::vector<SweepGraph> sweep_graphs;
+  DBuf<double> gx;  // FP64 solves with graphs: the sweeps' vector, kept at one address
+  void drop_graphs() {
+    for (auto& e : sweep_graphs)
+      if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    sweep_graphs.clear();
+  }
+  ~TreeData();  // factor.hip, where LevelWork is complete
   DBuf<int64_t> obox_off;  // original_box DOF list of each leaf in obox_pool
   DBuf<int> obox_pool;
   // per-node factor bookkeeping (device), node ids as above

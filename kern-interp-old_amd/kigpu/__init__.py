@@ -50,7 +50,8 @@ class Timings(C.Structure):
         ("dec_flops", C.c_double), ("apply_launches", C.c_int), ("apply_kernel_s", C.c_double),
         ("apply_pairs", C.c_double), ("tc_launches", C.c_int), ("tc_kernel_s", C.c_double),
         ("tc_eval_s", C.c_double), ("tc_series_pairs", C.c_double),
-        ("tc_direct_pairs", C.c_double), ("tc_series_terms", C.c_double)]
+        ("tc_direct_pairs", C.c_double), ("tc_series_terms", C.c_double),
+        ("graph_replays", C.c_int), ("graph_captures", C.c_int)]
 
 
 # every symbol include/kigpu.h declares
@@ -61,7 +62,7 @@ EXPORTS = [
     "kigpu_set_border", "kigpu_factor", "kigpu_factor_ranks", "kigpu_factor_export_lists",
     "kigpu_factor_allskel", "kigpu_solve", "kigpu_solve_device", "kigpu_solve_device_noalloc",
     "kigpu_proxy_block", "kigpu_interp_decomp", "kigpu_schur_eliminate",
-    "kigpu_domain_mask", "kigpu_domain_solution", "kigpu_tree_import",
+    "kigpu_domain_mask", "kigpu_domain_solution", "kigpu_tree_import", "kigpu_set_graphs",
     "kigpu_kernel_apply",
     "kigpu_kernel_apply_device", "kigpu_forward_apply", "kigpu_geometry_create",
     "kigpu_geometry_destroy", "kigpu_kernel_block", "kigpu_factor_export_active",
@@ -131,6 +132,7 @@ def load() -> C.CDLL:
         "kigpu_kernel_apply": (C.c_int, [P, P, C.c_int, _f64p, C.c_int, _f64p]),
         "kigpu_kernel_apply_treecode": (C.c_int, [P, P, _f64p, C.c_int, _f64p]),
         "kigpu_set_refine_apply": (C.c_int, [P, C.c_int]),
+        "kigpu_set_graphs": (C.c_int, [P, C.c_int]),
         "kigpu_kernel_apply_device": (C.c_int, [P, P, C.c_int, P, C.c_int, P]),
         "kigpu_forward_apply": (C.c_int, [P, P, C.c_int, _f64p, C.c_int, _f64p, C.c_int, _f64p]),
         "kigpu_solve_transpose": (C.c_int, [P, P, _f64p, C.c_int, C.c_int, _f64p, C.c_int,
@@ -228,6 +230,10 @@ class Context:
     def set_refine_apply(self, mode: int):
         """Residual apply of the FP64 refinement: 0 treecode (default), 1 direct sum."""
         _check(self.lib.kigpu_set_refine_apply(self.h, mode), "kigpu_set_refine_apply")
+
+    def set_graphs(self, on: bool):
+        """Replay the solve sweeps from captured HIP graphs (repeated solves)."""
+        _check(self.lib.kigpu_set_graphs(self.h, int(bool(on))), "kigpu_set_graphs")
 
     def __del__(self):
         try:
