@@ -32,6 +32,8 @@ import numpy as np  # noqa: E402
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+DEFAULT_TUNING = {"version": 1, "dec_wave": [1, 1, 1, 1], "mid_on_side": 1, "pair_sweeps": 1,
+                  "apply_split_cap": 64, "fwd_split_cap": 256}
 
 
 def parse():
@@ -47,6 +49,11 @@ def parse():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the batched-instances and adjoint measurements")
     ap.add_argument("--cpu-n", type=int, default=65536)
+    ap.add_argument("--tuning", default=os.path.join(ROOT, "kern-interp-old_amd", "tuning",
+                                                     "gfx950.json"),
+                    help="launch-tuning table to load (defaults when absent)")
+    ap.add_argument("--autotune", default=None, metavar="PATH",
+                    help="tune the launch choices on the workload first and write the table")
     return ap.parse_args()
 
 
@@ -344,8 +351,21 @@ def main():
     ctx = kigpu.Context(0)
     prob = DeviceProblem(torch, (pts, nrm, w, kap), F)
 
+    from kigpu import autotune
+    if args.autotune:
+        for prec in (kigpu.FP64, kigpu.FP32_REFINE):
+            t = ctx.build_tree(pts, nrm, w, kap, 1, args.leaf)
+            autotune.tune(ctx, t, args.autotune, kigpu.LAPLACE, args.tol, prec, reps=3,
+                          note=f"laplace star N={args.n} leaf {args.leaf}")
+            t.close()
+    table_path = args.autotune or args.tuning
     res = {}
+    tunings = {}
     for name, prec in (("fp64", kigpu.FP64), ("fp32_refine", kigpu.FP32_REFINE)):
+        ctx.set_tuning(DEFAULT_TUNING)
+        entry = autotune.load(ctx, table_path, args.n, prec) if os.path.exists(table_path) else None
+        tunings[name] = {"table": os.path.relpath(table_path, ROOT) if entry else None,
+                         **ctx.tuning()}
         dt, tims, info, h, mu = time_path(torch, kigpu, ctx, prob, prec, args)
         err = interior_error(kigpu, ctx, h, mu)
         dres = direct_residual(kigpu, ctx, h, mu[:, 0], F[:, 0])
@@ -419,6 +439,7 @@ def main():
         "fp32_refine": {k: fast[k] for k in ("s_per_step", "dof_per_s", "interior_rel_err",
                                              "direct_rel_residual", "refine", "phases_s")},
         "speedup_fp32_refine_vs_fp64": res["fp64"]["s_per_step"] / fast["s_per_step"],
+        "tuning": tunings,
         "roofline": roofline,
         "id_kernel": id_kernel,
         "refine_apply": refine_apply,

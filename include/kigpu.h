@@ -113,6 +113,36 @@ kigpu_status kigpu_context_destroy(kigpu_ctx* ctx);
 const char* kigpu_status_string(kigpu_status s);
 const char* kigpu_last_error(void);
 /* device-side phase timings of the last factor/solve on this context */
+/* ---- launch tuning ------------------------------------------------------
+ * No counterpart in the reference.  Launch choices of this library that do
+ * not change what is computed, only how it is cut into launches:
+ *   dec_wave[b]      elimination of boxes with 16 b + 1 .. 16 b + 16 active
+ *                    DOFs: 1 one wavefront per box (matrix-core products,
+ *                    register Gauss-Jordan), 0 one workgroup per box;
+ *   mid_on_side      the 33-64 wavefront class on the context's side stream,
+ *                    beside the other classes;
+ *   pair_sweeps      two tree levels per solve-sweep launch where LDS allows;
+ *   apply_split_cap  most source splits (blocks per target tile) of the
+ *                    direct K apply; fwd_split_cap the same for the forward
+ *                    and transposed applies.
+ * Results may differ in the last bits between settings (other summation
+ * orders), never more. */
+typedef struct {
+  int version; /* KIGPU_TUNING_VERSION */
+  int dec_wave[4];
+  int mid_on_side, pair_sweeps, apply_split_cap, fwd_split_cap;
+} kigpu_tuning;
+enum { KIGPU_TUNING_VERSION = 1 };
+kigpu_status kigpu_get_tuning(kigpu_ctx* ctx, kigpu_tuning* out);
+kigpu_status kigpu_set_tuning(kigpu_ctx* ctx, const kigpu_tuning* in);
+/* Measures the choices on this tree: factors it (opts) reps times per
+ * elimination choice and keeps the fastest per box-size class, then times
+ * solves and direct applies for the sweep and split choices; the context
+ * keeps the winners, *out gets them (the tree is left factored with them).
+ * Time per choice: the fastest of reps runs, from HIP events. */
+kigpu_status kigpu_autotune(kigpu_ctx* ctx, kigpu_tree* tree, const kigpu_factor_opts* opts,
+                            int reps, kigpu_tuning* out);
+
 /* Solve sweeps from HIP graphs (no counterpart in the reference): with on = 1,
  * each tree captures the launches of its up/down sweeps and root (or
  * bordered) step on the second solve with the same vector, shape and border,

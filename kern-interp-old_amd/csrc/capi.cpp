@@ -182,6 +182,117 @@ kigpu_status kigpu_context_destroy(kigpu_ctx* ctx) {
   return KIGPU_OK;
 }
 
+static void to_abi(const Tuning& t, kigpu_tuning* o) {
+  o->version = KIGPU_TUNING_VERSION;
+  for (int b = 0; b < 4; ++b) o->dec_wave[b] = t.dec_wave[b];
+  o->mid_on_side = t.mid_on_side;
+  o->pair_sweeps = t.pair_sweeps;
+  o->apply_split_cap = t.apply_split_cap;
+  o->fwd_split_cap = t.fwd_split_cap;
+}
+
+kigpu_status kigpu_get_tuning(kigpu_ctx* ctx, kigpu_tuning* out) {
+  KI_REQUIRE(ctx && out, "null argument");
+  to_abi(ctx->c.tune, out);
+  return KIGPU_OK;
+}
+
+kigpu_status kigpu_set_tuning(kigpu_ctx* ctx, const kigpu_tuning* in) {
+  KI_REQUIRE(ctx && in, "null argument");
+  KI_REQUIRE(in->version == KIGPU_TUNING_VERSION, "tuning version mismatch");
+  KI_REQUIRE(in->apply_split_cap >= 1 && in->apply_split_cap <= 4096 && in->fwd_split_cap >= 1 &&
+                 in->fwd_split_cap <= 4096,
+             "split caps must be 1 .. 4096");
+  Tuning& t = ctx->c.tune;
+  for (int b = 0; b < 4; ++b) t.dec_wave[b] = in->dec_wave[b] != 0;
+  t.mid_on_side = in->mid_on_side != 0;
+  t.pair_sweeps = in->pair_sweeps != 0;
+  t.apply_split_cap = in->apply_split_cap;
+  t.fwd_split_cap = in->fwd_split_cap;
+  return KIGPU_OK;
+}
+
+kigpu_status kigpu_autotune(kigpu_ctx* ctx, kigpu_tree* tree, const kigpu_factor_opts* opts,
+                            int reps, kigpu_tuning* out) {
+  KI_REQUIRE(ctx && tree && opts, "null argument");
+  KI_REQUIRE(reps >= 1 && reps <= 100, "reps must be 1 .. 100");
+  return guard([&] {
+    Context& c = ctx->c;
+    TreeData& t = tree->t;
+    Tuning& tu = c.tune;
+    auto best_of = [&](auto&& run) {
+      double b = 1e30;
+      for (int r = 0; r < reps; ++r) b = std::min(b, run());
+      return b;
+    };
+    auto factor_time = [&] {
+      factor_tree(c, t, *opts);
+      return c.tim.t_decouple;
+    };
+    // elimination kernel per box-size class, then the side stream
+    for (int b = 0; b < 4; ++b) {
+      double tb[2];
+      for (int v = 0; v < 2; ++v) {
+        tu.dec_wave[b] = v;
+        tb[v] = best_of(factor_time);
+      }
+      tu.dec_wave[b] = tb[1] <= tb[0];
+    }
+    {
+      double ts[2];
+      for (int v = 0; v < 2; ++v) {
+        tu.mid_on_side = v;
+        ts[v] = best_of(factor_time);
+      }
+      tu.mid_on_side = ts[1] <= ts[0];
+    }
+    factor_tree(c, t, *opts);
+    // sweeps: one right-hand side of ones
+    const size_t nd = t.ndof;
+    std::vector<double> ones(nd, 1.0);
+    DBuf<double> db, dx, dy;
+    db.upload(ones, c.stream);
+    dx.alloc(nd);
+    dy.alloc(nd);
+    {
+      double tp[2];
+      for (int v = 0; v < 2; ++v) {
+        tu.pair_sweeps = v;
+        tp[v] = best_of([&] {
+          solve_tree(c, t, db.p, 1, dx.p, nullptr, nullptr, nullptr, false);
+          return c.tim.t_solve;
+        });
+      }
+      tu.pair_sweeps = tp[1] <= tp[0];
+    }
+    // direct K apply with one right-hand side: source splits (the Laplace
+    // double layer takes the symmetric apply, which has none)
+    if (t.opts.pde != KIGPU_LAPLACE) {
+      const int caps[] = {16, 32, 64, 128, 256};
+      double bt = 1e30;
+      int bc = tu.apply_split_cap;
+      for (int cap : caps) {
+        tu.apply_split_cap = cap;
+        const double tt = best_of([&] {
+          KI_HIP(hipEventRecord(c.ev[8], c.stream));
+          kernel_apply(c, t, t.opts.pde, db.p, 1, dy.p);
+          KI_HIP(hipEventRecord(c.ev[9], c.stream));
+          stream_wait(c.stream);
+          float ms = 0;
+          KI_HIP(hipEventElapsedTime(&ms, c.ev[8], c.ev[9]));
+          return (double)ms;
+        });
+        if (tt < bt) {
+          bt = tt;
+          bc = cap;
+        }
+      }
+      tu.apply_split_cap = bc;
+    }
+    if (out) to_abi(tu, out);
+  }, &ctx->c);
+}
+
 kigpu_status kigpu_set_graphs(kigpu_ctx* ctx, int on) {
   KI_REQUIRE(ctx, "null context");
   ctx->c.graphs = on != 0;

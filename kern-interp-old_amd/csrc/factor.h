@@ -40,6 +40,17 @@ struct LevelWork {
   DBuf<float> dws32;
 };
 
+// Launch choices the autotuner sets (kigpu_tuning in kigpu.h mirrors it).
+struct Tuning {
+  // elimination: per box-size class n in 1-16, 17-32, 33-48, 49-64 active
+  // DOFs, 1 = one wavefront per box, 0 = one workgroup per box
+  int dec_wave[4] = {1, 1, 1, 1};
+  int mid_on_side = 1;       // the 33-64 wavefront class on the side stream
+  int pair_sweeps = 1;       // two tree levels per sweep launch where LDS allows
+  int apply_split_cap = 64;  // source splits of the direct K apply (one RHS block)
+  int fwd_split_cap = 256;   // source splits of the forward / transposed applies
+};
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -50,6 +61,7 @@ struct Context {
   kigpu_timings tim{};
   int refine_apply = 0;     // refinement residual: 0 treecode where it applies, 1 direct sum
   bool graphs = false;      // replay the solve sweeps from captured HIP graphs
+  Tuning tune;
   DBuf<double> ring;        // proxy unit vectors cos | sin (host libm)
   size_t lds_max = 65536;
   PinnedHost pin_down;      // per-level downloads

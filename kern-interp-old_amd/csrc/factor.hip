@@ -1678,7 +1678,11 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
       std::stable_sort(ord.begin(), ord.end(), [&](int p, int q) { return h_n[p] > h_n[q]; });
       for (int i : ord) {
         const int nb = h_n[i];
-        if (nb <= kWdecMaxN / 2) {
+        const bool wave = nb <= kWdecMaxN && c.tune.dec_wave[std::max(0, (nb - 1) / 16)];
+        if (!wave) {
+          dec_large.push_back(a + i);
+          nmax_large = std::max(nmax_large, nb);
+        } else if (nb <= kWdecMaxN / 2) {
           dec_small.push_back(a + i);
           nmax_small = std::max(nmax_small, nb);
         } else if (nb <= kWdecMaxN) {
@@ -1796,7 +1800,8 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
       };
       // the middle size class on the side stream beside the small boxes and
       // then the large ones on the main stream
-      const bool side = !dec_mid.empty() && (!dec_small.empty() || !dec_large.empty());
+      const bool side = c.tune.mid_on_side && !dec_mid.empty() &&
+                        (!dec_small.empty() || !dec_large.empty());
       if (side) {
         KI_HIP(hipEventRecord(c.ev_fork, s));
         KI_HIP(hipStreamWaitEvent(c.side, c.ev_fork, 0));

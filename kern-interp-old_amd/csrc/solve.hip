@@ -955,6 +955,7 @@ __global__ void k_axpy_f32(double* x, const float* d, size_t n) {
 template <typename T>
 static bool pair_fits(const Context& c, const TreeData& t, int l, int rcol, int per_n,
                       size_t* lds, int* wstride) {
+  if (!c.tune.pair_sweeps) return false;
   if (l < 1 || l + 1 >= t.nlevels || !t.work[l] || !t.work[l + 1]) return false;
   const LevelWork& wp = *t.work[l];
   const LevelWork& wc = *t.work[l + 1];
@@ -1212,7 +1213,7 @@ void run_apply(Context& c, const TreeData& t, int pde, bool fwd, const Targets& 
     KI_HIP(hipGetLastError());
     return;
   }
-  const int split = tb.splits(TRANS ? 256 : (fwd ? 256 : 64));
+  const int split = tb.splits(TRANS || fwd ? c.tune.fwd_split_cap : c.tune.apply_split_cap);
   part.alloc(len * split);
   dim3 grid(tb.ntiles, split);
   const Tiles tl = tb.view();

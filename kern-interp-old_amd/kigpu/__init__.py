@@ -63,6 +63,7 @@ EXPORTS = [
     "kigpu_factor_allskel", "kigpu_solve", "kigpu_solve_device", "kigpu_solve_device_noalloc",
     "kigpu_proxy_block", "kigpu_interp_decomp", "kigpu_schur_eliminate",
     "kigpu_domain_mask", "kigpu_domain_solution", "kigpu_tree_import", "kigpu_set_graphs",
+    "kigpu_get_tuning", "kigpu_set_tuning", "kigpu_autotune",
     "kigpu_kernel_apply",
     "kigpu_kernel_apply_device", "kigpu_forward_apply", "kigpu_geometry_create",
     "kigpu_geometry_destroy", "kigpu_kernel_block", "kigpu_factor_export_active",
@@ -133,6 +134,9 @@ def load() -> C.CDLL:
         "kigpu_kernel_apply_treecode": (C.c_int, [P, P, _f64p, C.c_int, _f64p]),
         "kigpu_set_refine_apply": (C.c_int, [P, C.c_int]),
         "kigpu_set_graphs": (C.c_int, [P, C.c_int]),
+        "kigpu_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
+        "kigpu_set_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
+        "kigpu_autotune": (C.c_int, [P, P, C.POINTER(FactorOpts), C.c_int, C.POINTER(Tuning)]),
         "kigpu_kernel_apply_device": (C.c_int, [P, P, C.c_int, P, C.c_int, P]),
         "kigpu_forward_apply": (C.c_int, [P, P, C.c_int, _f64p, C.c_int, _f64p, C.c_int, _f64p]),
         "kigpu_solve_transpose": (C.c_int, [P, P, _f64p, C.c_int, C.c_int, _f64p, C.c_int,
@@ -152,6 +156,28 @@ def load() -> C.CDLL:
         f.argtypes = args
     _lib = lib
     return lib
+
+
+class Tuning(C.Structure):
+    """kigpu_tuning: launch choices (see include/kigpu.h)."""
+    _fields_ = [("version", C.c_int), ("dec_wave", C.c_int * 4), ("mid_on_side", C.c_int),
+                ("pair_sweeps", C.c_int), ("apply_split_cap", C.c_int),
+                ("fwd_split_cap", C.c_int)]
+
+    def as_dict(self):
+        return {"version": self.version, "dec_wave": list(self.dec_wave),
+                "mid_on_side": self.mid_on_side, "pair_sweeps": self.pair_sweeps,
+                "apply_split_cap": self.apply_split_cap, "fwd_split_cap": self.fwd_split_cap}
+
+    @classmethod
+    def from_dict(cls, d):
+        t = cls()
+        t.version = d["version"]
+        for i, v in enumerate(d["dec_wave"]):
+            t.dec_wave[i] = v
+        for k in ("mid_on_side", "pair_sweeps", "apply_split_cap", "fwd_split_cap"):
+            setattr(t, k, d[k])
+        return t
 
 
 class Region(C.Structure):
@@ -230,6 +256,25 @@ class Context:
     def set_refine_apply(self, mode: int):
         """Residual apply of the FP64 refinement: 0 treecode (default), 1 direct sum."""
         _check(self.lib.kigpu_set_refine_apply(self.h, mode), "kigpu_set_refine_apply")
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        _check(self.lib.kigpu_get_tuning(self.h, C.byref(t)), "kigpu_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, d: dict):
+        t = Tuning.from_dict(d)
+        _check(self.lib.kigpu_set_tuning(self.h, C.byref(t)), "kigpu_set_tuning")
+
+    def autotune(self, tree, pde=LAPLACE, id_tol=1e-6, precision=FP64, reps=3, min_dofs=16):
+        """Measure the launch choices on `tree` (left factored with the
+        winners) and keep them on this context; returns them."""
+        o = FactorOpts(pde, id_tol, precision, min_dofs)
+        t = Tuning()
+        _check(self.lib.kigpu_autotune(self.h, tree.h, C.byref(o), reps, C.byref(t)),
+               "kigpu_autotune")
+        tree.pde = pde
+        return t.as_dict()
 
     def set_graphs(self, on: bool):
         """Replay the solve sweeps from captured HIP graphs (repeated solves)."""
