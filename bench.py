@@ -388,12 +388,21 @@ def main():
     dec_s = tf["dec_kernel_s"] / max(tf["dec_launches"], 1)
     dec_f = tf["dec_flops"] / max(tf["dec_launches"], 1)
     dec_achieved = dec_f / dec_s / 1e12 if dec_s > 0 else 0.0
+    waves = tunings["fp32_refine"]["dec_wave"]
+    kernels = []
+    if not all(waves):
+        kernels.append("k_decouple<float> (one 1024-thread workgroup per box, products as dot "
+                       "products over LDS, block Gauss-Jordan)")
+    if any(waves):
+        kernels.append("k_decouple_wave<float> (one wavefront per box, products on "
+                       "v_mfma_f32_16x16x4f32, register Gauss-Jordan)")
     roofline = {"bound": "mfma", "achieved": dec_achieved, "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": dec_achieved / PEAK_FP32_TFLOPS,
                 "traffic": pmc_traffic("pmc_decouple_laplace_star_N65536.md"),
-                "kernel": "k_decouple_wave<float> (elimination and Schur update, one wavefront "
-                          "per box, products on v_mfma_f32_16x16x4f32; each box is a chain of "
-                          "dependent steps, so the launch is bound by latency)",
+                "kernel": "elimination and Schur update: " + "; ".join(kernels) +
+                          f" (per box-size class as tuned: dec_wave {waves})",
+                "limit": "latency of each box's dependent Gauss-Jordan steps; a level has few "
+                         "boxes per CU",
                 "avg_launch_us": dec_s * 1e6, "launches_per_step": tf["dec_launches"],
                 "flops_per_launch": dec_f}
     # the ID kernel beside it: FP64 vector arithmetic (no matrix cores),
