@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libkigpu.so")
 
-SOURCES = ["alloc.cpp", "scan.hip", "tree.hip", "tree_import.cpp", "factor.hip", "border.hip", "solve.hip", "treecode.hip", "blocks.hip", "adjoint.hip", "domain.hip", "capi.cpp"]
+SOURCES = ["alloc.cpp", "scan.hip", "tree.hip", "tree_import.cpp", "factor.hip", "border.hip", "dense_inv.hip", "solve.hip", "treecode.hip", "blocks.hip", "adjoint.hip", "domain.hip", "capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-std=c++17", "-O3", "-fPIC", "--offload-arch=gfx950",

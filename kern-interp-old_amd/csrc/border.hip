@@ -75,26 +75,6 @@ __global__ void k_assemble_s(int ns, int h, int ndof, const int* allskel, const 
   }
 }
 
-// dense FP64 inverse (Gauss-Jordan, dgetrf pivot order), LDS or in place
-template <bool LDS>
-__global__ __launch_bounds__(1024) void k_dense_inverse(const double* A, int n, double* Ainv,
-                                                        int* status) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ int piv[1024], sig[1024];
-  __shared__ double prow[1024], fcol[1024], orow[1024];
-  __shared__ int s_p;
-  double* W = LDS ? reinterpret_cast<double*>(smem) : Ainv;
-  for (int e = threadIdx.x; e < n * n; e += blockDim.x) W[e] = A[e];
-  __syncthreads();
-  if (LDS) {  // columns left permuted; applied on the way out
-    if (!block_gj_inverse_perm(W, n, piv, sig, prow, orow, fcol, &s_p) && threadIdx.x == 0)
-      atomicOr(status, 16);
-    for2d<16>(n, n, [&](int i, int j) { Ainv[i + (size_t)j * n] = W[i + (size_t)sig[j] * n]; });
-  } else if (!block_gj_inverse(W, n, piv, prow, fcol, &s_p) && threadIdx.x == 0) {
-    atomicOr(status, 16);
-  }
-}
-
 // M = [x(allskel, :); g - Psi'(:, red) x(red, :)] with gram = Psi'(:, red) x(red, :)
 template <typename T>
 __global__ void k_border_rhs(int ns, int h, const int* allskel, const T* x, int ldx,
@@ -176,18 +156,13 @@ void border_factor(Context& c, TreeData& t) {
   G.alloc((size_t)h * h);
   k_masked_gram<T, T><<<dim3(h, h), kTB, 0, s>>>(sP, nd, sD, nd, t.is_skel.p, nd, h, G.p);
   const int nS = ns + h;
-  if (nS > 1024) KI_FAIL(KIGPU_ERR_NOTSUPP, "bordered system larger than 1024");
+  if (nS > kDenseInvMax) KI_FAIL(KIGPU_ERR_NOTSUPP, "bordered system larger than 1024");
   S.alloc((size_t)nS * nS);
   k_assemble_s<T><<<nS, 256, 0, s>>>(ns, h, nd, t.root_dofs.p, t.root_mat.p, sU, sP, G.p, S.p);
-  t.Sinv.alloc((size_t)nS * nS);
-  const size_t lb = sizeof(double) * (size_t)nS * nS;
-  if (lb + 40960 <= c.lds_max) {
-    allow_lds(k_dense_inverse<true>, lb);
-    k_dense_inverse<true><<<1, 1024, lb, s>>>(S.p, nS, t.Sinv.p, t.status.p);
-  } else {
-    k_dense_inverse<false><<<1, 1024, 0, s>>>(S.p, nS, t.Sinv.p, t.status.p);
-  }
   KI_HIP(hipGetLastError());
+  t.Sinv.alloc((size_t)nS * nS);
+  dense_inverse(c, s, 1, nS, nullptr, nullptr, nullptr, nS, (size_t)nS * nS, (size_t)nS, S.p,
+                t.Sinv.p, nullptr, t.status.p, 16);
   stream_wait(s);
   t.nS = nS;
 }

@@ -237,126 +237,6 @@ __host__ __device__ __forceinline__ bool pivot_usable(T d) {
   return a > 0.0 && a <= 1.7976931348623157e308;
 }
 
-// In-place Gauss-Jordan inverse of the n x n matrix A (LDS, column-major,
-// leading dimension n) with partial row pivoting: the pivot of step c is the
-// first largest |A(i, c)|, i >= c, of the current matrix -- the same choice
-// as dgetrf's -- and piv[c] records it.  Executed by ONE wavefront; prow and
-// fcol are n-element LDS scratch vectors.  Returns false on a zero or non-finite pivot.
-template <typename T>
-__device__ bool wave_gj_inverse(T* A, int n, int* piv, T* prow, T* fcol) {
-  const int lane = threadIdx.x & 63;
-  bool ok = true;
-  for (int c = 0; c < n; ++c) {
-    double best = -1.0;
-    int bi = c;  // stays c when no candidate is finite (NaN compares false)
-    for (int i = c + lane; i < n; i += 64) {
-      double v = fabs((double)A[i + (size_t)c * n]);
-      if (v > best) {
-        best = v;
-        bi = i;
-      }
-    }
-    wave_argmax(best, bi);
-    const int p = bi;
-    if (lane == 0) piv[c] = p;
-    if (p != c)
-      for (int j = lane; j < n; j += 64) {
-        T t0 = A[c + (size_t)j * n];
-        A[c + (size_t)j * n] = A[p + (size_t)j * n];
-        A[p + (size_t)j * n] = t0;
-      }
-    wave_sync();
-    const T d = A[c + (size_t)c * n];
-    if (!pivot_usable(d)) ok = false;
-    const T inv = T(1) / d;
-    for (int j = lane; j < n; j += 64) prow[j] = j == c ? inv : A[c + (size_t)j * n] * inv;
-    for (int i = lane; i < n; i += 64) fcol[i] = A[i + (size_t)c * n];
-    wave_sync();
-    for (int j = 0; j < n; ++j) {
-      const T pj = prow[j];
-      for (int i = lane; i < n; i += 64) {
-        T* a = A + i + (size_t)j * n;
-        if (i == c) *a = pj;
-        else if (j == c) *a = -fcol[i] * inv;
-        else *a -= fcol[i] * pj;
-      }
-    }
-    wave_sync();
-  }
-  for (int c = n - 1; c >= 0; --c) {
-    const int p = piv[c];
-    if (p != c)
-      for (int i = lane; i < n; i += 64) {
-        T t0 = A[i + (size_t)c * n];
-        A[i + (size_t)c * n] = A[i + (size_t)p * n];
-        A[i + (size_t)p * n] = t0;
-      }
-    wave_sync();
-  }
-  return ok;
-}
-
-// The same executed by a whole workgroup (block barriers); for larger n.
-template <typename T>
-__device__ bool block_gj_inverse(T* A, int n, int* piv, T* prow, T* fcol, int* s_p) {
-  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6;
-  bool ok = true;
-  for (int c = 0; c < n; ++c) {
-    if (wv == 0) {
-      double best = -1.0;
-      int bi = c;  // stays c when no candidate is finite (NaN compares false)
-      for (int i = c + lane; i < n; i += 64) {
-        double v = fabs((double)A[i + (size_t)c * n]);
-        if (v > best) {
-          best = v;
-          bi = i;
-        }
-      }
-      wave_argmax(best, bi);
-      if (lane == 0) {
-        *s_p = bi;
-        piv[c] = bi;
-      }
-    }
-    __syncthreads();
-    const int p = *s_p;
-    if (p != c)
-      for (int j = tid; j < n; j += nt) {
-        T t0 = A[c + (size_t)j * n];
-        A[c + (size_t)j * n] = A[p + (size_t)j * n];
-        A[p + (size_t)j * n] = t0;
-      }
-    __syncthreads();
-    const T d = A[c + (size_t)c * n];
-    if (!pivot_usable(d)) ok = false;
-    const T inv = T(1) / d;
-    for (int j = tid; j < n; j += nt) prow[j] = j == c ? inv : A[c + (size_t)j * n] * inv;
-    for (int i = tid; i < n; i += nt) fcol[i] = A[i + (size_t)c * n];
-    __syncthreads();
-    const int nw = nt >> 6;
-    for (int j = wv; j < n; j += nw) {
-      const T pj = prow[j];
-      T* col = A + (size_t)j * n;
-      for (int i = lane; i < n; i += 64) {
-        if (i == c) col[i] = pj;
-        else if (j == c) col[i] = -fcol[i] * inv;
-        else col[i] -= fcol[i] * pj;
-      }
-    }
-    __syncthreads();
-  }
-  for (int c = n - 1; c >= 0; --c) {
-    const int p = piv[c];
-    if (p != c)
-      for (int i = tid; i < n; i += nt) {
-        T t0 = A[i + (size_t)c * n];
-        A[i + (size_t)c * n] = A[i + (size_t)p * n];
-        A[i + (size_t)p * n] = t0;
-      }
-    __syncthreads();
-  }
-  return ok;
-}
 
 // Gauss-Jordan inverse with two barriers per column.  The row interchange
 // of step c is folded into the elimination (the new pivot row is read from

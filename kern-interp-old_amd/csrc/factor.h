@@ -84,6 +84,18 @@ void proxy_block(Context& c, const GeomView& g, int pde, const double* h_pp, con
 void schur_eliminate(Context& c, const double* h_K, int n, int k, const int* h_perm,
                      const double* h_T, double* X_rr, double* X_lu, int* piv, double* L,
                      double* U, double* schur, int* singular);
+// Largest order of a root or bordered matrix that dense_inverse takes.
+constexpr int kDenseInvMax = 1024;
+// Inverses of ni dense FP64 matrices (dense_inv.hip).  Matrix x has order
+// d_n[x] and sits column-major at offset d_moff[x] of A; its inverse goes to
+// the same offset of Ainv and the pivot row of each elimination step to
+// offset d_doff[x] of piv (if not null).  d_n == nullptr: one matrix of
+// order n0 at offset 0.  mtot, dtot: the totals of the orders squared and of
+// the orders.  A zero or non-finite pivot sets `bit` in *status.  Queued on
+// stream s; no host wait.
+void dense_inverse(Context& c, hipStream_t s, int ni, int nmax, const int* d_n,
+                   const int64_t* d_doff, const int64_t* d_moff, int n0, size_t mtot, size_t dtot,
+                   const double* A, double* Ainv, int* piv, int* status, int bit);
 // bordered system (border.hip)
 template <typename T>
 void border_factor(Context& c, TreeData& t);

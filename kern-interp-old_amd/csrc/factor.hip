@@ -1444,54 +1444,57 @@ __global__ __launch_bounds__(kWave) void k_decouple_wave(
 }
 
 // ---------------------------------------------------------------------------
-// root: allskel_mat = K(allskel, allskel) - updates, FP64 LU (skel_factorization.cpp:116-128)
+// root: allskel_mat = K(allskel, allskel) - the children's updates
+// (skel_factorization.cpp:116-128); its inverse comes from dense_inverse.
 // ---------------------------------------------------------------------------
-// One block per instance; the root of instance b is node b.
-template <typename T, bool LDS>
-__global__ __launch_bounds__(1024) void k_root(GeomView g, int pde, const int* child,
-                                               const int* osize, const uint64_t* acc_ptr,
-                                               const int* acc_ld, const int* root_n,
-                                               const int64_t* root_doff, const int64_t* root_moff,
-                                               const int* dofs_all, double* Aws_all,
-                                               double* Ainv_all, int* piv_all, double* Amat_all,
-                                               int* status) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int x = blockIdx.x;
+// Grid (column groups, instances); the root of instance b is node b, and
+// child s's update covers the diagonal block at the offset of the children
+// before it (up to the first missing child).
+template <typename T>
+__global__ __launch_bounds__(kTB) void k_root_assemble(GeomView g, int pde, const int* child,
+                                                       const int* osize, const uint64_t* acc_ptr,
+                                                       const int* acc_ld, const int* root_n,
+                                                       const int64_t* root_doff,
+                                                       const int64_t* root_moff,
+                                                       const int* dofs_all, double* Amat_all) {
+  const int x = blockIdx.y;
   const int n = root_n[x];
-  if (n == 0) return;
   const int* dofs = dofs_all + root_doff[x];
-  double* Ainv = Ainv_all + root_moff[x];
   double* Amat = Amat_all + root_moff[x];
-  int* piv_out = piv_all + root_doff[x];
-  double* A = LDS ? reinterpret_cast<double*>(smem) : Aws_all + root_moff[x];
-  __shared__ int piv[1024];
-  const int tid = threadIdx.x, nt = blockDim.x;
-  for2d<16>(n, n, [&](int i, int j) { A[i + (size_t)j * n] = kernel_entry(g, pde, dofs[i], dofs[j]); });
-  __syncthreads();
-  int o = 0;
+  int o[5];
+  const T* acc[4];
+  int ld[4];
+  o[0] = 0;
+  bool open = true;
+#pragma unroll
   for (int s = 0; s < 4; ++s) {
-    int y = child[4 * x + s];
-    if (y < 0) break;
-    int sz = osize[y];
-    const T* acc = reinterpret_cast<const T*>(acc_ptr[y]);
-    if (acc) {
-      int ld = acc_ld[y];
-      for2d<16>(sz, sz, [&](int i, int j) {
-        A[(o + i) + (size_t)(o + j) * n] -= (double)acc[i + (size_t)j * ld];
-      });
-    }
-    o += sz;
+    const int y = child[4 * x + s];
+    open = open && y >= 0;
+    const int sz = open ? osize[y] : 0;
+    acc[s] = open ? reinterpret_cast<const T*>(acc_ptr[y]) : nullptr;
+    ld[s] = open ? acc_ld[y] : 0;
+    o[s + 1] = o[s] + sz;
   }
-  __syncthreads();
-  for (int e = tid; e < n * n; e += nt) Amat[e] = A[e];
-  // allskel_mat^{-1} in place (Gauss-Jordan, dgetrf's pivot order), then out
-  // with the column permutation applied
-  __shared__ int s_p;
-  __shared__ int sig[1024];
-  __shared__ double prow[1024], orow[1024], fcol[1024];
-  if (!block_gj_inverse_perm(A, n, piv, sig, prow, orow, fcol, &s_p) && tid == 0) flag(status, 8);
-  for2d<16>(n, n, [&](int i, int j) { Ainv[i + (size_t)j * n] = A[i + (size_t)sig[j] * n]; });
-  for (int i = tid; i < n; i += nt) piv_out[i] = piv[i];
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int j = blockIdx.x * kNW + (threadIdx.x >> 6); j < n; j += gridDim.x * kNW) {
+    const int sj = (j >= o[1]) + (j >= o[2]) + (j >= o[3]);
+    const bool inb = j < o[4];
+    const int b0 = o[sj];
+    const T* aj = nullptr;
+    int lj = 0;
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      if (s == sj) {
+        aj = acc[s];
+        lj = ld[s];
+      }
+    for (int i = lane; i < n; i += kWave) {
+      double v = kernel_entry(g, pde, dofs[i], dofs[j]);
+      if (inb && aj && i >= b0 && i < o[sj + 1])
+        v -= (double)aj[(i - b0) + (size_t)(j - b0) * lj];
+      Amat[i + (size_t)j * n] = v;
+    }
+  }
 }
 
 __global__ void k_set_ptr(uint64_t* dst, int a, int b, const int* nact, uint64_t base,
@@ -1846,7 +1849,7 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     t.nroot_max = std::max(t.nroot_max, (int)n0);
   }
   t.nroot = (int)t.h_root_doff[ni];
-  if (t.nroot_max > 1024) KI_FAIL(KIGPU_ERR_NOTSUPP, "root skeleton larger than 1024 DOFs");
+  if (t.nroot_max > kDenseInvMax) KI_FAIL(KIGPU_ERR_NOTSUPP, "root skeleton larger than 1024 DOFs");
   if (t.nroot > 0) {
     // the roots' active lists are contiguous in act_pool (level 0 is last)
     t.root_dofs.alloc(t.nroot);
@@ -1864,20 +1867,13 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     t.root_lu.alloc(msz);   // allskel_mat^{-1}
     t.root_mat.alloc(msz);  // allskel_mat
     t.root_piv.alloc(t.nroot);
-    const size_t lbytes = sizeof(double) * (size_t)t.nroot_max * t.nroot_max;
-    if (lbytes + 40960 <= lds_cap) {  // static LDS of k_root: 2 int + 3 scratch vectors
-      allow_lds(k_root<T, true>, lbytes);
-      k_root<T, true><<<ni, 1024, lbytes, s>>>(g, pde, t.child.p, t.osize.p, t.acc_ptr.p,
-                                               t.acc_ld.p, t.root_n_p, t.root_doff_p,
-                                               t.root_moff_p, t.root_dofs.p, nullptr, t.root_lu.p,
-                                               t.root_piv.p, t.root_mat.p, t.status.p);
-    } else {
-      t.ws64.alloc(msz);
-      k_root<T, false><<<ni, 1024, 0, s>>>(g, pde, t.child.p, t.osize.p, t.acc_ptr.p,
-                                           t.acc_ld.p, t.root_n_p, t.root_doff_p, t.root_moff_p,
-                                           t.root_dofs.p, t.ws64.p, t.root_lu.p, t.root_piv.p,
-                                           t.root_mat.p, t.status.p);
-    }
+    k_root_assemble<T><<<dim3(ceil_div(t.nroot_max, kNW), ni), kTB, 0, s>>>(
+        g, pde, t.child.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, t.root_n_p, t.root_doff_p,
+        t.root_moff_p, t.root_dofs.p, t.root_mat.p);
+    KI_HIP(hipGetLastError());
+    dense_inverse(c, s, ni, t.nroot_max, t.root_n_p, t.root_doff_p, t.root_moff_p, 0,
+                  (size_t)t.h_root_moff[ni], (size_t)t.nroot, t.root_mat.p, t.root_lu.p,
+                  t.root_piv.p, t.status.p, 8);
     KI_HIP(hipGetLastError());
   }
   const int e_root1 = mark();

@@ -1,17 +1,18 @@
 // CPU driver for the Gauss-Jordan inverses of
-// kern-interp-old_amd/csrc/device_util.h (wave_gj_inverse, block_gj_inverse,
-// block_gj_inverse_perm), run on emulated GPU threads (tests/cpp/simt) and
+// kern-interp-old_amd/csrc/device_util.h (block_gj_inverse_perm of the
+// workgroup elimination, wave_gj_reg of the one-wavefront elimination), run
+// on emulated GPU threads (tests/cpp/simt) and
 // built with AddressSanitizer by tests/test_gj_emulated.py.  Every array the
 // routines touch -- the matrix and the LDS scratch vectors -- is a separate
 // heap block of exactly the size the kernels give it (n elements, n * n for
 // the matrix), so an index past any of them is reported.
 //
-// stdin:  int32 variant (0 perm, 1 block, 2 wave, 3 register wave of
-//         wave_gj_reg, 4 the same in FP32), int32 n, int32 threads,
+// stdin:  int32 variant (0 perm, 3 register wave of wave_gj_reg, 4 the
+//         same in FP32), int32 n, int32 threads,
 //         then n * n doubles, column-major
 // stdout: int32 ok, n int32 pivots, n * n doubles of the inverse as the
 //         kernels write it out (for the perm variant with the column
-//         permutation applied, as k_root and k_dense_inverse do:
+//         permutation applied, as k_decouple does:
 //         inv(i, j) = A[i + sigma[j] * n]; the register variants write the
 //         inverse to a separate r x r array Z, as k_decouple_wave does)
 #include <cstdio>
@@ -33,7 +34,7 @@ int main() {
   int hdr[3];
   read_all(hdr, sizeof(hdr));
   const int variant = hdr[0], n = hdr[1], nthreads = hdr[2];
-  if (n <= 0 || nthreads <= 0 || nthreads % 64) return 2;
+  if (n <= 0 || nthreads <= 0 || nthreads % 64 || (variant != 0 && variant < 3)) return 2;
   double* A = new double[(size_t)n * n];
   read_all(A, sizeof(double) * (size_t)n * n);
   int* piv = new int[n];
@@ -87,15 +88,13 @@ int main() {
   }
   simt::launch(nthreads, [&] {
     bool ok;
-    if (variant == 0) ok = block_gj_inverse_perm(A, n, piv, sig, prow, orow, fcol, s_p);
-    else if (variant == 1) ok = block_gj_inverse(A, n, piv, prow, fcol, s_p);
-    else ok = wave_gj_inverse(A, n, piv, prow, fcol);
+    ok = block_gj_inverse_perm(A, n, piv, sig, prow, orow, fcol, s_p);
     if (threadIdx.x == 0) ok_all = ok;
   });
   double* out = new double[(size_t)n * n];
   for (int j = 0; j < n; ++j)
     for (int i = 0; i < n; ++i)
-      out[i + (size_t)j * n] = variant == 0 ? A[i + (size_t)sig[j] * n] : A[i + (size_t)j * n];
+      out[i + (size_t)j * n] = A[i + (size_t)sig[j] * n];
   std::fwrite(&ok_all, sizeof(int), 1, stdout);
   std::fwrite(piv, sizeof(int), n, stdout);
   std::fwrite(out, sizeof(double), (size_t)n * n, stdout);
