@@ -346,6 +346,16 @@ kigpu_status kigpu_kernel_apply_treecode(kigpu_ctx* ctx, const kigpu_tree* tree,
  * the adjoint), 1 always the direct sum of kigpu_kernel_apply.  No
  * counterpart in the reference. */
 kigpu_status kigpu_set_refine_apply(kigpu_ctx* ctx, int mode);
+/* How the factorizations on this context compute each box's ID (replaces
+ * the dgeqp3 + dtrtrs of ki_Mat::id, kern_interp/ki_mat.cpp:505-541):
+ * 0 (default) by precision -- FP32_REFINE with id_tol >= 1e-7 from the
+ * box's Gram matrix A^T A (FP64, matrix cores) and a pivoted Cholesky of it,
+ * otherwise by Householder QR with column pivoting; 1 always the pivoted QR;
+ * 2 always the Gram matrix (whose rank rule cannot resolve tolerances much
+ * below 1e-7: remaining norms squared below ~eps |A|^2 are noise).
+ * Both give the QR's pivots, rank and T in exact arithmetic; the Gram route
+ * can differ at pivots or ranks that FP64 cannot separate (near ties). */
+kigpu_status kigpu_set_id_method(kigpu_ctx* ctx, int method);
 /* u = Kernel::forward() mu (kernel.cpp:337-357): boundary -> domain targets,
  * targets: 2*ntgt interleaved.  u: (sd*ntgt) x nrhs. */
 kigpu_status kigpu_forward_apply(kigpu_ctx* ctx, const kigpu_tree* tree, int pde,

@@ -892,6 +892,12 @@ kigpu_status kigpu_set_refine_apply(kigpu_ctx* ctx, int mode) {
   return KIGPU_OK;
 }
 
+kigpu_status kigpu_set_id_method(kigpu_ctx* ctx, int method) {
+  KI_REQUIRE(ctx && method >= 0 && method <= 2, "invalid argument");
+  ctx->c.id_method = method;
+  return KIGPU_OK;
+}
+
 kigpu_status kigpu_kernel_apply(kigpu_ctx* ctx, const kigpu_tree* tree, int pde, const double* mu,
                                 int nrhs, double* y) {
   KI_REQUIRE(ctx && tree && mu && y && nrhs > 0, "invalid argument");

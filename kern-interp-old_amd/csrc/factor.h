@@ -60,6 +60,7 @@ struct Context {
   std::vector<hipEvent_t> evpool;  // per-level phase markers of a factorization
   kigpu_timings tim{};
   int refine_apply = 0;     // refinement residual: 0 treecode where it applies, 1 direct sum
+  int id_method = 0;        // ID: 0 by precision (FP32: Gram matrix, FP64: pivoted QR), 1 pivoted QR, 2 Gram matrix
   bool graphs = false;      // replay the solve sweeps from captured HIP graphs
   Tuning tune;
   DBuf<double> ring;        // proxy unit vectors cos | sin (host libm)

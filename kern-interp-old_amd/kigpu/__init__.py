@@ -70,7 +70,7 @@ EXPORTS = [
     "kigpu_solve_transpose", "kigpu_solve_transpose_device", "kigpu_kernel_apply_transpose",
     "kigpu_forward_apply_transpose", "kigpu_kernel_sensitivity", "kigpu_forward_sensitivity",
     "kigpu_tree_build_batch", "kigpu_tree_build_batch_device", "kigpu_tree_instances",
-    "kigpu_kernel_apply_treecode", "kigpu_set_refine_apply",
+    "kigpu_kernel_apply_treecode", "kigpu_set_refine_apply", "kigpu_set_id_method",
 ]
 
 _lib = None
@@ -133,6 +133,7 @@ def load() -> C.CDLL:
         "kigpu_kernel_apply": (C.c_int, [P, P, C.c_int, _f64p, C.c_int, _f64p]),
         "kigpu_kernel_apply_treecode": (C.c_int, [P, P, _f64p, C.c_int, _f64p]),
         "kigpu_set_refine_apply": (C.c_int, [P, C.c_int]),
+        "kigpu_set_id_method": (C.c_int, [P, C.c_int]),
         "kigpu_set_graphs": (C.c_int, [P, C.c_int]),
         "kigpu_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
         "kigpu_set_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
@@ -256,6 +257,10 @@ class Context:
     def set_refine_apply(self, mode: int):
         """Residual apply of the FP64 refinement: 0 treecode (default), 1 direct sum."""
         _check(self.lib.kigpu_set_refine_apply(self.h, mode), "kigpu_set_refine_apply")
+
+    def set_id_method(self, method: int):
+        """Box IDs: 0 by precision (FP32_REFINE Gram matrix, FP64 pivoted QR), 1 pivoted QR, 2 Gram."""
+        _check(self.lib.kigpu_set_id_method(self.h, method), "kigpu_set_id_method")
 
     def tuning(self) -> dict:
         t = Tuning()
