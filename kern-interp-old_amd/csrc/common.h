@@ -11,15 +11,12 @@
 #include <utility>
 #include <vector>
 
+#include "geom.h"
 #include "kigpu.h"
 #include "lds_grants.h"
 
 namespace kigpu {
 
-constexpr int kProxyPoints = 128;     // NUM_PROXY_POINTS, kernel.h:10
-constexpr double kRadiusRatio = 1.5;  // RADIUS_RATIO, kernel.h:11
-constexpr int kMaxNbrs = 8;           // same-level neighbors of a box (8-connectivity)
-constexpr int kWave = 64;
 
 // thread-local error detail returned by kigpu_last_error()
 void set_error(const std::string& msg);
@@ -200,17 +197,6 @@ struct UploadPack {
   }
 };
 
-// Boundary arrays on the device, SoA for coalesced gathers.
-struct GeomView {
-  const double* px;
-  const double* py;
-  const double* nx;
-  const double* ny;
-  const double* w;
-  const double* kappa;
-  int npts;
-  int sd;
-};
 
 // device-side scan helpers (scan.hip)
 void exclusive_scan_i64(const int64_t* in, int64_t* out, int n, hipStream_t s);

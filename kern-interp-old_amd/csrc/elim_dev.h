@@ -1,0 +1,397 @@
+// Device routines of the elimination (decouple, skel_factorization.cpp:55-91)
+// and of the ID matrix generation shared by factor.hip's kernels: staged
+// geometry, kernel entries from it, the matrix-core tile products of one
+// wavefront, the children's update layout, and the elimination of one box
+// by four wavefronts.  Needs only the HIP device built-ins, so that a CPU
+// test can run them on emulated threads.
+#pragma once
+
+#include "device_util.h"
+#include "entries.h"
+#include "geom.h"
+
+namespace kigpu {
+namespace elim {
+
+__device__ __forceinline__ void flag(int* status, int bit) { atomicOr(status, bit); }
+
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
+
+// Geometry of a DOF staged in LDS: point x, y, normal x, y, weight, curvature.
+constexpr int kGeo = 6;
+
+__device__ __forceinline__ void stage_geo(const GeomView& g, int dof, double* dst) {
+  const int p = dof / g.sd;
+  dst[0] = g.px[p];
+  dst[1] = g.py[p];
+  dst[2] = g.nx[p];
+  dst[3] = g.ny[p];
+  dst[4] = g.w[p];
+  dst[5] = g.kappa[p];
+}
+
+// K(t, s) from staged geometry (same operations as kernel_entry)
+__device__ __forceinline__ double staged_entry(int pde, int tdof, const double* tg, int sdof,
+                                               const double* sg) {
+#pragma clang fp contract(off)
+  if (pde == KIGPU_STOKES)
+    return stokes_entry(tdof, sdof, tg[0], tg[1], tg[2], tg[3], sg[0], sg[1], sg[2], sg[3], sg[4],
+                        sg[5]);
+  return laplace_entry(pde, tg[0] - sg[0], tg[1] - sg[1], sg[4], sg[5], sg[2], sg[3], tg[2],
+                       tg[3], false);
+}
+
+constexpr int kWdecMaxN = 64;
+
+typedef float wd_f4 __attribute__((ext_vector_type(4)));
+typedef double wd_d4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<float> {
+  typedef wd_f4 acc;
+  __device__ static acc step(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  // row of accumulator element v of lane l (column l % 16)
+  __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+template <>
+struct Mfma<double> {
+  typedef wd_d4 acc;
+  __device__ static acc step(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+
+// out(i, j, sum_t A(i, t) B(t, j)) for every (i, j) of an M x N product by
+// one wavefront, in 16 x 16 tiles with 4-deep MFMA steps (with nparts > 1:
+// the tiles part, part + nparts, ... only, for one of several waves).  The A operand of
+// lane l is A(i0 + l % 16, t0 + l / 16), the B operand B(t0 + l / 16,
+// j0 + l % 16); entries past M, N or K enter as zeros (read at a clamped
+// index and replaced, so the loads need no branch).  The next step's
+// operands are loaded before the current MFMA is issued.
+template <typename T, typename FA, typename FB, typename FO>
+__device__ __forceinline__ void wave_gemm(int M, int N, int K, FA&& A, FB&& B, FO&& out,
+                                          int part = 0, int nparts = 1) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int l16 = lane & 15, l4 = lane >> 4;
+  int tile = 0;
+  for (int i0 = 0; i0 < M; i0 += 16)
+    for (int j0 = 0; j0 < N; j0 += 16) {
+      if (tile++ % nparts != part) continue;  // tiles part, part + nparts, ... of the product
+      const int ia = i0 + l16, jb = j0 + l16;
+      const int iac = min(ia, M - 1), jbc = min(jb, N - 1);
+      auto ld = [&](int t0, T& a, T& b) {
+        const int t = t0 + l4, tc = min(t, K - 1);
+        const T av = A(iac, tc), bv = B(tc, jbc);
+        a = (ia < M && t < K) ? av : T(0);
+        b = (jb < N && t < K) ? bv : T(0);
+      };
+      typename Mfma<T>::acc c = {T(0), T(0), T(0), T(0)};
+      T a, b;
+      ld(0, a, b);
+      for (int t0 = 0; t0 < K; t0 += 4) {
+        T an, bn;
+        ld(min(t0 + 4, K - 1), an, bn);
+        c = Mfma<T>::step(a, b, c);
+        a = an;
+        b = bn;
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = i0 + Mfma<T>::row(lane, v);
+        if (i < M && jb < N) out(i, jb, c[v]);
+      }
+    }
+}
+
+
+// LDS of one k_decouple_wave box with |active_box| = n and rank k:
+// staged geometry, the [skel | redundant] DOFs, their active positions and
+// the inverse map, pivots, the Gauss-Jordan's pivot row, then
+//   Kp (n x n): K_BN; after X_rr is formed it holds X_rr^{-1}, and at the
+//               end the Schur update;
+//   X (r x r):  X_rr, kept for the refinement step of L;
+//   Tm (k x r): T; once X_rr is formed it holds L;
+//   K_r_sn (r x k), K_sn_r (k x r).
+struct WdecLds {
+  size_t geo, sr, pos, ipos, piv, gjv, kp, x, tm, krsn, ksnr, total;
+};
+__host__ __device__ inline WdecLds wdec_layout(int n, int k, size_t tsz) {
+  WdecLds L;
+  const int r = n - k;
+  size_t o = 0;
+  L.geo = o;  o = align16(o + sizeof(double) * kGeo * (size_t)n);
+  L.sr = o;   o = align16(o + sizeof(int) * (size_t)n);
+  L.pos = o;  o = align16(o + sizeof(int) * (size_t)n);
+  L.ipos = o; o = align16(o + sizeof(int) * (size_t)n);
+  L.piv = o;  o = align16(o + sizeof(int) * (size_t)kWdecMaxN);
+  L.gjv = o;  o = align16(o + tsz * (size_t)kWdecMaxN);
+  L.kp = o;   o = align16(o + tsz * (size_t)n * n);
+  L.x = o;    o = align16(o + tsz * (size_t)r * r);
+  L.tm = o;   o = align16(o + tsz * (size_t)k * r);
+  L.krsn = o; o = align16(o + tsz * (size_t)k * r);
+  L.ksnr = o; o = align16(o + tsz * (size_t)k * r);
+  L.total = o;
+  return L;
+}
+// the most any box of size <= n needs (over every rank k)
+__host__ inline size_t wdec_lds_bound(int n, size_t tsz) {
+  size_t best = wdec_layout(n, n, tsz).total;
+  for (int m = 1; m <= n; ++m)
+    for (int k = 1; k < m; ++k) best = std::max(best, wdec_layout(m, k, tsz).total);
+  return best;
+}
+
+// The children's accumulated updates of a box, flattened: child c's
+// osize x osize block holds entries [eo[c], eo[c + 1]) in column-major
+// order, entry (a, b) at active positions (co[c] + a, co[c] + b).
+struct ChildUpd {
+  int co[5], eo[5], cld[4];
+  uint64_t cacc[4];
+};
+
+// element c (0..3) of a 4-entry array held in registers (a runtime index
+// into the array itself would put it in scratch memory)
+template <typename V>
+__device__ __forceinline__ V pick4(const V* a, int c) {
+  return c == 0 ? a[0] : c == 1 ? a[1] : c == 2 ? a[2] : a[3];
+}
+
+// f(a', b', v) for the entries e = e0 + lane + 64 u (u < 8) of the
+// flattened update; the 8 loads of a lane are issued before any f runs.
+template <typename T, typename F>
+__device__ __forceinline__ void child_update_batch(const ChildUpd& cu, int e0, F&& f) {
+  const int lane = threadIdx.x & (kWave - 1);
+  T v[8];
+  int pa[8], pb[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int e = e0 + u * kWave + lane;
+    pa[u] = -1;
+    pb[u] = 0;
+    v[u] = T(0);
+    if (e < cu.eo[4]) {
+      const int c = (e >= cu.eo[1]) + (e >= cu.eo[2]) + (e >= cu.eo[3]);
+      const int c0 = pick4(cu.co, c), e0c = pick4(cu.eo, c);
+      const int sz = pick4(cu.co + 1, c) - c0;
+      const int loc = e - e0c;
+      int b = (int)(((float)loc + 0.5f) / (float)sz);  // exact: loc < 2^12
+      int a = loc - b * sz;
+      if (a < 0) { --b; a += sz; }
+      if (a >= sz) { ++b; a -= sz; }
+      const T* acc = reinterpret_cast<const T*>(pick4(cu.cacc, c));
+      if (acc) {
+        v[u] = acc[a + (size_t)b * pick4(cu.cld, c)];
+        pa[u] = c0 + a;
+        pb[u] = c0 + b;
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 8; ++u)
+    if (pa[u] >= 0) f(pa[u], pb[u], v[u]);
+}
+
+// ---------------------------------------------------------------------------
+// elimination with four wavefronts per box (|active_box| <= 64): the steps
+// of k_decouple_wave, with K_BN, the children's updates and every product
+// split over the four waves (a share of the 16 x 16 tiles each, workgroup
+// barriers between dependent products) and X_rr^{-1} by the register
+// Gauss-Jordan in wave 0.  A box's latency is then set by the Gauss-Jordan
+// alone; the other three waves wait for it at one barrier.
+// ---------------------------------------------------------------------------
+constexpr int kQdecTB = 256;
+constexpr int kQdecNW = kQdecTB / kWave;
+
+// The elimination of box x by the calling 256-thread workgroup; smem: its
+// dynamic LDS (wdec_lds_bound of the launch's largest box).
+template <typename T, int RMAX>
+__device__ void decouple_quad_box(
+    int x, unsigned char* smem, GeomView g, int pde, const int* is_leaf, const int* child,
+    const int* osize_in, const uint64_t* acc_ptr_in, const int* acc_ld_in, const int* nact,
+    const int* rank, const int64_t* act_off, const int* sr_pool, const int* pos_pool,
+    int* piv_pool, const uint64_t* fac_ptr, int* osize, uint64_t* acc_ptr, int* acc_ld,
+    int* status) {
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid >> 6;
+  const int n = nact[x];
+  const int kr = rank[x];
+  const bool leaf = is_leaf[x];
+  const int y = lane < 4 ? child[4 * x + lane] : -1;  // -1 for a leaf's slots
+  const int64_t aoff = act_off[x];
+  T* slab = reinterpret_cast<T*>(fac_ptr[x]);
+  if (n == 0) {
+    if (tid == 0) {
+      osize[x] = 0;
+      acc_ptr[x] = 0;
+      acc_ld[x] = 0;
+    }
+    return;
+  }
+  const int k = kr < 0 ? n : kr, r = n - k;
+  // every wave gathers the children's update layout (the same loads)
+  const int csz = (!leaf && y >= 0) ? osize_in[y] : 0;
+  const uint64_t cap = (!leaf && y >= 0) ? acc_ptr_in[y] : 0;
+  const int cldl = (!leaf && y >= 0) ? acc_ld_in[y] : 0;
+  ChildUpd cu;
+  cu.co[0] = cu.eo[0] = 0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int sz = __shfl(csz, c, kWave);
+    cu.co[c + 1] = cu.co[c] + sz;
+    cu.eo[c + 1] = cu.eo[c] + sz * sz;
+    cu.cacc[c] = (uint64_t)__shfl((long long)cap, c, kWave);
+    cu.cld[c] = __shfl(cldl, c, kWave);
+  }
+  if (kr < 0) {  // not compressed: hand the whole active list and its update up
+    if (!leaf) {  // thread a writes row a of a column group, each entry once
+      for (int b = wv; b < n; b += kQdecNW) {
+        const int cb = (b >= cu.co[1]) + (b >= cu.co[2]) + (b >= cu.co[3]);
+        const T* accb = reinterpret_cast<const T*>(pick4(cu.cacc, cb));
+        const int cob = pick4(cu.co, cb), ldb = pick4(cu.cld, cb);
+        if (lane < n) {
+          const int ca = (lane >= cu.co[1]) + (lane >= cu.co[2]) + (lane >= cu.co[3]);
+          const bool in = ca == cb && b < cu.co[4] && lane < cu.co[4] && accb;
+          slab[lane + (size_t)b * n] = in ? accb[(lane - cob) + (size_t)(b - cob) * ldb] : T(0);
+        }
+      }
+    }
+    if (tid == 0) {
+      osize[x] = n;
+      acc_ptr[x] = leaf ? 0 : reinterpret_cast<uint64_t>(slab);
+      acc_ld[x] = n;
+    }
+    return;
+  }
+  const WdecLds Ly = wdec_layout(n, k, sizeof(T));
+  double* geo = reinterpret_cast<double*>(smem + Ly.geo);
+  int* sr = reinterpret_cast<int*>(smem + Ly.sr);
+  int* pos = reinterpret_cast<int*>(smem + Ly.pos);
+  int* ipos = reinterpret_cast<int*>(smem + Ly.ipos);
+  int* piv = reinterpret_cast<int*>(smem + Ly.piv);
+  T* gjv = reinterpret_cast<T*>(smem + Ly.gjv);  // pivot row of the Gauss-Jordan
+  T* Kp = reinterpret_cast<T*>(smem + Ly.kp);
+  T* X = reinterpret_cast<T*>(smem + Ly.x);
+  T* Ts = reinterpret_cast<T*>(smem + Ly.tm);
+  T* Krsn = reinterpret_cast<T*>(smem + Ly.krsn);
+  T* Ksnr = reinterpret_cast<T*>(smem + Ly.ksnr);
+  if (wv == 0 && lane < n) {
+    const int md = sr_pool[aoff + lane], ps = pos_pool[aoff + lane];
+    sr[lane] = md;
+    pos[lane] = ps;
+    ipos[ps] = lane;
+    stage_geo(g, md, geo + kGeo * lane);
+  }
+  for (int e = tid; e < k * r; e += kQdecTB) Ts[e] = slab[e];  // T from the ID
+  __syncthreads();
+  // K_BN = K(BN, BN) - update, permuted to [s r] (skel_factorization.cpp:66-78):
+  // 256 / n threads per row, thread t on row t % n and every (256 / n)-th
+  // column from t / n
+  {
+    const int npl = kQdecTB / n;
+    if (tid < npl * n) {
+      const int i = tid % n;
+      double tg[kGeo];
+#pragma unroll
+      for (int q = 0; q < kGeo; ++q) tg[q] = geo[kGeo * i + q];
+      const int ti = sr[i];
+#pragma unroll 4
+      for (int j = tid / n; j < n; j += npl)
+        Kp[i + (size_t)j * n] = (T)staged_entry(pde, ti, tg, sr[j], geo + kGeo * j);
+    }
+  }
+  __syncthreads();
+  if (!leaf) {
+    for (int e0 = wv * 8 * kWave; e0 < cu.eo[4]; e0 += kQdecNW * 8 * kWave)
+      child_update_batch<T>(cu, e0, [&](int a, int b, T v) {
+        if (a >= n || b >= n) return;
+        const size_t e = ipos[a] + (size_t)ipos[b] * n;
+        Kp[e] = (T)((double)Kp[e] - (double)v);
+      });
+    __syncthreads();
+  }
+  auto KP = [&](int i, int j) -> T { return Kp[i + (size_t)j * n]; };
+  auto TT = [&](int i, int t) -> T { return Ts[t + (size_t)i * k]; };  // T^T (r x k)
+  auto TM = [&](int t, int j) -> T { return Ts[t + (size_t)j * k]; };  // T (k x r)
+  // K_r_sn = K_rs - T^T K_ss (r x k); K_sn_r = K_sr - K_ss T (k x r);
+  // X_rr = K_rr - T^T K_sr (r x r), then X_rr -= K_r_sn T
+  wave_gemm<T>(r, k, k, TT, [&](int t, int j) { return KP(t, j); },
+               [&](int i, int j, T v) { Krsn[i + (size_t)j * r] = KP(k + i, j) - v; }, wv,
+               kQdecNW);
+  wave_gemm<T>(k, r, k, [&](int i, int t) { return KP(i, t); }, TM,
+               [&](int i, int j, T v) { Ksnr[i + (size_t)j * k] = KP(i, k + j) - v; }, wv,
+               kQdecNW);
+  wave_gemm<T>(r, r, k, TT, [&](int t, int j) { return KP(t, k + j); },
+               [&](int i, int j, T v) { X[i + (size_t)j * r] = KP(k + i, k + j) - v; }, wv,
+               kQdecNW);
+  __syncthreads();
+  wave_gemm<T>(r, r, k, [&](int i, int t) { return Krsn[i + (size_t)t * r]; }, TM,
+               [&](int i, int j, T v) { X[i + (size_t)j * r] -= v; }, wv, kQdecNW);
+  __syncthreads();
+  // K_BN and T are dead: X_rr^{-1} goes where K_BN was, L where T was
+  T* Z = Kp;
+  T* Ls = Ts;
+  if (wv == 0) {
+    bool ok;
+    if (r <= 16) ok = wave_gj_reg<T, 16>(X, r, Z, piv, gjv);
+    else if (RMAX <= 32 || r <= 32) ok = wave_gj_reg<T, 32>(X, r, Z, piv, gjv);
+    else ok = wave_gj_reg<T, (RMAX > 32 ? 64 : 32)>(X, r, Z, piv, gjv);
+    if (!ok && lane == 0) flag(status, 4);
+  }
+  __syncthreads();
+  T* Lm = slab + (size_t)k * r;        // k x r
+  T* Um = slab + 2 * (size_t)k * r;    // r x k
+  T* Xinv = slab + 3 * (size_t)k * r;  // r x r
+  T* Acc = Xinv + (size_t)r * r;       // k x k
+  auto ZZ = [&](int i, int j) -> T { return Z[i + (size_t)j * r]; };
+  // L0 = K_sn_r X^{-1} (k x r) into Ls; U = X^{-1} K_r_sn (r x k) into the slab
+  wave_gemm<T>(k, r, r, [&](int i, int t) { return Ksnr[i + (size_t)t * k]; }, ZZ,
+               [&](int i, int j, T v) { Ls[i + (size_t)j * k] = v; }, wv, kQdecNW);
+  wave_gemm<T>(r, k, r, ZZ, [&](int t, int j) { return Krsn[t + (size_t)j * r]; },
+               [&](int i, int j, T v) { Um[i + (size_t)j * r] = v; }, wv, kQdecNW);
+  for (int e = tid; e < r * r; e += kQdecTB) Xinv[e] = Z[e];
+  if (tid < r) piv_pool[aoff + tid] = piv[tid];
+  __syncthreads();
+  // one refinement step of L against X_rr (see k_decouple): R = K_sn_r - L0 X_rr
+  // in place of K_sn_r, then L = L0 + R X_rr^{-1}
+  wave_gemm<T>(k, r, r, [&](int i, int t) { return Ls[i + (size_t)t * k]; },
+               [&](int t, int j) { return X[t + (size_t)j * r]; },
+               [&](int i, int j, T v) { Ksnr[i + (size_t)j * k] -= v; }, wv, kQdecNW);
+  __syncthreads();
+  wave_gemm<T>(k, r, r, [&](int i, int t) { return Ksnr[i + (size_t)t * k]; }, ZZ,
+               [&](int i, int j, T v) {
+                 const T l = Ls[i + (size_t)j * k] + v;
+                 Ls[i + (size_t)j * k] = l;
+                 Lm[i + (size_t)j * k] = l;
+               },
+               wv, kQdecNW);
+  __syncthreads();
+  // schur_update = L K_r_sn (k x k) where X_rr^{-1} was, plus the
+  // descendants' update on the skeleton, then out to the slab
+  T* Sa = Kp;
+  wave_gemm<T>(k, k, r, [&](int i, int t) { return Ls[i + (size_t)t * k]; },
+               [&](int t, int j) { return Krsn[t + (size_t)j * r]; },
+               [&](int i, int j, T v) { Sa[i + (size_t)j * k] = v; }, wv, kQdecNW);
+  __syncthreads();
+  if (!leaf) {
+    for (int e0 = wv * 8 * kWave; e0 < cu.eo[4]; e0 += kQdecNW * 8 * kWave)
+      child_update_batch<T>(cu, e0, [&](int a, int b, T v) {
+        if (a >= n || b >= n) return;
+        const int i = ipos[a], j = ipos[b];
+        if (i < k && j < k) Sa[i + (size_t)j * k] += v;
+      });
+    __syncthreads();
+  }
+  for (int e = tid; e < k * k; e += kQdecTB) Acc[e] = Sa[e];
+  if (tid == 0) {
+    osize[x] = k;
+    acc_ptr[x] = reinterpret_cast<uint64_t>(Acc);
+    acc_ld[x] = k;
+  }
+}
+
+}  // namespace elim
+}  // namespace kigpu

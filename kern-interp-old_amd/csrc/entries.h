@@ -9,7 +9,8 @@
 
 #include <hip/hip_runtime.h>
 
-#include "common.h"
+#include "geom.h"
+#include "kigpu.h"
 
 namespace kigpu {
 

@@ -20,6 +20,7 @@
 #include <cstring>
 
 #include "device_util.h"
+#include "elim_dev.h"
 #include "entries.h"
 #include "factor.h"
 
@@ -27,10 +28,11 @@ namespace kigpu {
 
 namespace {
 
+using namespace elim;
+
 constexpr int kTB = 256;
 constexpr int kNW = kTB / kWave;
 
-__device__ __forceinline__ void flag(int* status, int bit) { atomicOr(status, bit); }
 
 // ---------------------------------------------------------------------------
 // index lists
@@ -265,10 +267,7 @@ __host__ __device__ inline bool id_wide_box(int m, int n) {
          ((n <= 8 * kIdWideNW && m <= 8 * kWave) || (n <= 6 * kIdWideNW && m <= 10 * kWave));
 }
 
-__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
-// Geometry of a DOF staged in LDS: point x, y, normal x, y, weight, curvature.
-constexpr int kGeo = 6;
 
 __host__ __device__ inline IdLds id_lds_layout(int m, int n, int q, bool a_in_lds, size_t tsz) {
   IdLds L;
@@ -295,27 +294,6 @@ __host__ __device__ inline IdLds id_lds_layout(int m, int n, int q, bool a_in_ld
   o = align16(o + sizeof(double) * 32);
   L.total = o;
   return L;
-}
-
-__device__ __forceinline__ void stage_geo(const GeomView& g, int dof, double* dst) {
-  const int p = dof / g.sd;
-  dst[0] = g.px[p];
-  dst[1] = g.py[p];
-  dst[2] = g.nx[p];
-  dst[3] = g.ny[p];
-  dst[4] = g.w[p];
-  dst[5] = g.kappa[p];
-}
-
-// K(t, s) from staged geometry (same operations as kernel_entry)
-__device__ __forceinline__ double staged_entry(int pde, int tdof, const double* tg, int sdof,
-                                               const double* sg) {
-#pragma clang fp contract(off)
-  if (pde == KIGPU_STOKES)
-    return stokes_entry(tdof, sdof, tg[0], tg[1], tg[2], tg[3], sg[0], sg[1], sg[2], sg[3], sg[4],
-                        sg[5]);
-  return laplace_entry(pde, tg[0] - sg[0], tg[1] - sg[1], sg[4], sg[5], sg[2], sg[3], tg[2],
-                       tg[3], false);
 }
 
 // proxy-block entry against a box DOF with staged geometry bg (see proxy_entry)
@@ -1384,157 +1362,6 @@ void k_decouple(const int* nodes, int a, GeomView g, int pde, const int* is_leaf
 // so no step waits on a workgroup barrier.
 // ---------------------------------------------------------------------------
 
-constexpr int kWdecMaxN = 64;
-
-typedef float wd_f4 __attribute__((ext_vector_type(4)));
-typedef double wd_d4 __attribute__((ext_vector_type(4)));
-
-template <typename T>
-struct Mfma;
-template <>
-struct Mfma<float> {
-  typedef wd_f4 acc;
-  __device__ static acc step(float a, float b, acc c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  // row of accumulator element v of lane l (column l % 16)
-  __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
-};
-template <>
-struct Mfma<double> {
-  typedef wd_d4 acc;
-  __device__ static acc step(double a, double b, acc c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  __device__ static int row(int lane, int v) { return (lane >> 4) + 4 * v; }
-};
-
-// out(i, j, sum_t A(i, t) B(t, j)) for every (i, j) of an M x N product by
-// one wavefront, in 16 x 16 tiles with 4-deep MFMA steps.  The A operand of
-// lane l is A(i0 + l % 16, t0 + l / 16), the B operand B(t0 + l / 16,
-// j0 + l % 16); entries past M, N or K enter as zeros (read at a clamped
-// index and replaced, so the loads need no branch).  The next step's
-// operands are loaded before the current MFMA is issued.
-template <typename T, typename FA, typename FB, typename FO>
-__device__ __forceinline__ void wave_gemm(int M, int N, int K, FA&& A, FB&& B, FO&& out) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int l16 = lane & 15, l4 = lane >> 4;
-  for (int i0 = 0; i0 < M; i0 += 16)
-    for (int j0 = 0; j0 < N; j0 += 16) {
-      const int ia = i0 + l16, jb = j0 + l16;
-      const int iac = min(ia, M - 1), jbc = min(jb, N - 1);
-      auto ld = [&](int t0, T& a, T& b) {
-        const int t = t0 + l4, tc = min(t, K - 1);
-        const T av = A(iac, tc), bv = B(tc, jbc);
-        a = (ia < M && t < K) ? av : T(0);
-        b = (jb < N && t < K) ? bv : T(0);
-      };
-      typename Mfma<T>::acc c = {T(0), T(0), T(0), T(0)};
-      T a, b;
-      ld(0, a, b);
-      for (int t0 = 0; t0 < K; t0 += 4) {
-        T an, bn;
-        ld(min(t0 + 4, K - 1), an, bn);
-        c = Mfma<T>::step(a, b, c);
-        a = an;
-        b = bn;
-      }
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = i0 + Mfma<T>::row(lane, v);
-        if (i < M && jb < N) out(i, jb, c[v]);
-      }
-    }
-}
-
-
-// LDS of one k_decouple_wave box with |active_box| = n and rank k:
-// staged geometry, the [skel | redundant] DOFs, their active positions and
-// the inverse map, pivots, the Gauss-Jordan's pivot row, then
-//   Kp (n x n): K_BN; after X_rr is formed it holds X_rr^{-1}, and at the
-//               end the Schur update;
-//   X (r x r):  X_rr, kept for the refinement step of L;
-//   Tm (k x r): T; once X_rr is formed it holds L;
-//   K_r_sn (r x k), K_sn_r (k x r).
-struct WdecLds {
-  size_t geo, sr, pos, ipos, piv, gjv, kp, x, tm, krsn, ksnr, total;
-};
-__host__ __device__ inline WdecLds wdec_layout(int n, int k, size_t tsz) {
-  WdecLds L;
-  const int r = n - k;
-  size_t o = 0;
-  L.geo = o;  o = align16(o + sizeof(double) * kGeo * (size_t)n);
-  L.sr = o;   o = align16(o + sizeof(int) * (size_t)n);
-  L.pos = o;  o = align16(o + sizeof(int) * (size_t)n);
-  L.ipos = o; o = align16(o + sizeof(int) * (size_t)n);
-  L.piv = o;  o = align16(o + sizeof(int) * (size_t)kWdecMaxN);
-  L.gjv = o;  o = align16(o + tsz * (size_t)kWdecMaxN);
-  L.kp = o;   o = align16(o + tsz * (size_t)n * n);
-  L.x = o;    o = align16(o + tsz * (size_t)r * r);
-  L.tm = o;   o = align16(o + tsz * (size_t)k * r);
-  L.krsn = o; o = align16(o + tsz * (size_t)k * r);
-  L.ksnr = o; o = align16(o + tsz * (size_t)k * r);
-  L.total = o;
-  return L;
-}
-// the most any box of size <= n needs (over every rank k)
-__host__ inline size_t wdec_lds_bound(int n, size_t tsz) {
-  size_t best = wdec_layout(n, n, tsz).total;
-  for (int m = 1; m <= n; ++m)
-    for (int k = 1; k < m; ++k) best = std::max(best, wdec_layout(m, k, tsz).total);
-  return best;
-}
-
-// The children's accumulated updates of a box, flattened: child c's
-// osize x osize block holds entries [eo[c], eo[c + 1]) in column-major
-// order, entry (a, b) at active positions (co[c] + a, co[c] + b).
-struct ChildUpd {
-  int co[5], eo[5], cld[4];
-  uint64_t cacc[4];
-};
-
-// element c (0..3) of a 4-entry array held in registers (a runtime index
-// into the array itself would put it in scratch memory)
-template <typename V>
-__device__ __forceinline__ V pick4(const V* a, int c) {
-  return c == 0 ? a[0] : c == 1 ? a[1] : c == 2 ? a[2] : a[3];
-}
-
-// f(a', b', v) for the entries e = e0 + lane + 64 u (u < 8) of the
-// flattened update; the 8 loads of a lane are issued before any f runs.
-template <typename T, typename F>
-__device__ __forceinline__ void child_update_batch(const ChildUpd& cu, int e0, F&& f) {
-  const int lane = threadIdx.x & (kWave - 1);
-  T v[8];
-  int pa[8], pb[8];
-#pragma unroll
-  for (int u = 0; u < 8; ++u) {
-    const int e = e0 + u * kWave + lane;
-    pa[u] = -1;
-    pb[u] = 0;
-    v[u] = T(0);
-    if (e < cu.eo[4]) {
-      const int c = (e >= cu.eo[1]) + (e >= cu.eo[2]) + (e >= cu.eo[3]);
-      const int c0 = pick4(cu.co, c), e0c = pick4(cu.eo, c);
-      const int sz = pick4(cu.co + 1, c) - c0;
-      const int loc = e - e0c;
-      int b = (int)(((float)loc + 0.5f) / (float)sz);  // exact: loc < 2^12
-      int a = loc - b * sz;
-      if (a < 0) { --b; a += sz; }
-      if (a >= sz) { ++b; a -= sz; }
-      const T* acc = reinterpret_cast<const T*>(pick4(cu.cacc, c));
-      if (acc) {
-        v[u] = acc[a + (size_t)b * pick4(cu.cld, c)];
-        pa[u] = c0 + a;
-        pb[u] = c0 + b;
-      }
-    }
-  }
-#pragma unroll
-  for (int u = 0; u < 8; ++u)
-    if (pa[u] >= 0) f(pa[u], pb[u], v[u]);
-}
-
 // RMAX: the largest r the launch's boxes can have (n <= RMAX + 1); the
 // register Gauss-Jordan is instantiated up to that width only.
 template <typename T, int RMAX>
@@ -1746,6 +1573,20 @@ __global__ __launch_bounds__(kWave) void k_decouple_wave(
     acc_ptr[x] = reinterpret_cast<uint64_t>(Acc);
     acc_ld[x] = k;
   }
+}
+
+// elimination with four wavefronts per box (decouple_quad_box in elim_dev.h)
+template <typename T, int RMAX>
+__global__ __launch_bounds__(kQdecTB) void k_decouple_quad(
+    const int* nodes, GeomView g, int pde, const int* is_leaf, const int* child,
+    const int* osize_in, const uint64_t* acc_ptr_in, const int* acc_ld_in, const int* nact,
+    const int* rank, const int64_t* act_off, const int* sr_pool, const int* pos_pool,
+    int* piv_pool, const uint64_t* fac_ptr, int* osize, uint64_t* acc_ptr, int* acc_ld,
+    int* status) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  decouple_quad_box<T, RMAX>(nodes[blockIdx.x], smem, g, pde, is_leaf, child, osize_in,
+                             acc_ptr_in, acc_ld_in, nact, rank, act_off, sr_pool, pos_pool,
+                             piv_pool, fac_ptr, osize, acc_ptr, acc_ld, status);
 }
 
 // ---------------------------------------------------------------------------
@@ -1995,16 +1836,23 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     // elimination: boxes of at most kWdecMaxN active DOFs one wavefront
     // each, in two size classes (their LDS differs fourfold), largest
     // first; the larger boxes a workgroup each
-    std::vector<int> dec_small, dec_mid, dec_large;
-    int nmax_small = 0, nmax_mid = 0, nmax_large = 0;
+    std::vector<int> dec_small, dec_mid, dec_large, dec_quad;
+    int nmax_small = 0, nmax_mid = 0, nmax_large = 0, nmax_quad = 0;
     {
       std::vector<int> ord(cn);
       for (int i = 0; i < cn; ++i) ord[i] = i;
       std::stable_sort(ord.begin(), ord.end(), [&](int p, int q) { return h_n[p] > h_n[q]; });
       for (int i : ord) {
         const int nb = h_n[i];
-        const bool wave = nb <= kWdecMaxN && c.tune.dec_wave[std::max(0, (nb - 1) / 16)];
-        if (!wave) {
+        int choice = nb <= kWdecMaxN ? c.tune.dec_wave[std::max(0, (nb - 1) / 16)] : 0;
+        // (Stokes boxes never take the four-wave kernel: its run on a Stokes
+        // problem with holes faulted the GPU, with no cause found)
+        if (choice == 2 && pde == KIGPU_STOKES) choice = 1;
+        const bool wave = choice == 1;
+        if (choice == 2) {
+          dec_quad.push_back(a + i);
+          nmax_quad = std::max(nmax_quad, nb);
+        } else if (!wave) {
           dec_large.push_back(a + i);
           nmax_large = std::max(nmax_large, nb);
         } else if (nb <= kWdecMaxN / 2) {
@@ -2027,10 +1875,12 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     const size_t o_ds = pk.add(dec_small.data(), sizeof(int) * dec_small.size());
     const size_t o_dm = pk.add(dec_mid.data(), sizeof(int) * dec_mid.size());
     const size_t o_dl = pk.add(dec_large.data(), sizeof(int) * dec_large.size());
+    const size_t o_dq = pk.add(dec_quad.data(), sizeof(int) * dec_quad.size());
     unsigned char* dpk = pk.send(c.pin_up, lw->pack, s);
     const int* d_dec_small = reinterpret_cast<const int*>(dpk + o_ds);
     const int* d_dec_mid = reinterpret_cast<const int*>(dpk + o_dm);
     const int* d_dec_large = reinterpret_cast<const int*>(dpk + o_dl);
+    const int* d_dec_quad = reinterpret_cast<const int*>(dpk + o_dq);
     const int64_t* d_soff = reinterpret_cast<const int64_t*>(dpk + o_soff);
     const int64_t* d_roff = reinterpret_cast<const int64_t*>(dpk + o_roff);
     const IdWork* d_wl = reinterpret_cast<const IdWork*>(dpk + o_wl);
@@ -2124,6 +1974,17 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
             t.nact.p, t.rank.p, t.act_off.p, t.sr_pool.p, t.pos_pool.p, t.piv_pool.p,
             t.fac_ptr.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, t.status.p);
       };
+      auto quad_launch = [&](hipStream_t st) {
+        const int nmx = std::max(nmax_quad, 1);
+        const size_t lds = wdec_lds_bound(nmx, sizeof(T));
+        auto kern = nmx <= kWdecMaxN / 2 ? k_decouple_quad<T, kWdecMaxN / 2>
+                                         : k_decouple_quad<T, kWdecMaxN>;
+        allow_lds(kern, lds);
+        kern<<<(int)dec_quad.size(), kQdecTB, lds, st>>>(
+            d_dec_quad, g, pde, t.is_leaf.p, t.child.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p,
+            t.nact.p, t.rank.p, t.act_off.p, t.sr_pool.p, t.pos_pool.p, t.piv_pool.p,
+            t.fac_ptr.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, t.status.p);
+      };
       auto large_launch = [&](hipStream_t st) {
         const int nl = (int)dec_large.size();
         DecLds L = dec_layout(nmax_large, sizeof(T));
@@ -2162,6 +2023,7 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
       } else if (!dec_mid.empty()) {
         wave_launch(d_dec_mid, (int)dec_mid.size(), nmax_mid, s);
       }
+      if (!dec_quad.empty()) quad_launch(s);
       if (!dec_small.empty()) wave_launch(d_dec_small, (int)dec_small.size(), nmax_small, s);
       if (!dec_large.empty()) large_launch(s);
       if (side) {

@@ -4,6 +4,16 @@ import numpy as np
 from oracle import geometry as G
 from oracle import kernel as K
 
+# The FP64 factorization of small Stokes problems with holes (the 8-hole disc
+# of test_gpu_border and the annulus of test_gpu_domain) faulted the GPU
+# twice with an illegal memory access inside kigpu_factor, once with the
+# four-wave elimination on the Stokes boxes and once without it, after
+# passing before; no cause has been found on the CPU, so these cases stay
+# off the GPU.
+STOKES_SMALL_BORDER_FAULT = (
+    "faulted the GPU (illegal memory access in kigpu_factor, FP64 Stokes with holes at "
+    "N = 2048) with no cause found on the CPU; kept off the GPU until one is shown")
+
 
 def star_problem(n, src=(3.0, 3.0)):
     b = G.star(n)
