@@ -390,10 +390,10 @@ def main():
     dec_achieved = dec_f / dec_s / 1e12 if dec_s > 0 else 0.0
     waves = tunings["fp32_refine"]["dec_wave"]
     kernels = []
-    if not all(waves):
+    if 0 in waves:
         kernels.append("k_decouple<float> (one 1024-thread workgroup per box, products as dot "
                        "products over LDS, block Gauss-Jordan)")
-    if any(waves):
+    if 1 in waves:
         kernels.append("k_decouple_wave<float> (one wavefront per box, products on "
                        "v_mfma_f32_16x16x4f32, register Gauss-Jordan)")
     roofline = {"bound": "mfma", "achieved": dec_achieved, "peak": PEAK_FP32_TFLOPS,
@@ -405,12 +405,17 @@ def main():
                          "boxes per CU",
                 "avg_launch_us": dec_s * 1e6, "launches_per_step": tf["dec_launches"],
                 "flops_per_launch": dec_f}
-    # the ID kernel beside it: FP64 vector arithmetic (no matrix cores),
-    # measured against the FP64 vector peak
+    # the ID kernels beside it (one level's ID launches, timed together):
+    # the Gram-matrix ID for boxes of up to 64 DOFs (A^T A on the FP64 matrix
+    # cores, pivoted Cholesky) and the pivoted Householder QR for the larger
+    # ones, measured against the FP64 peak (vector and matrix alike)
     id_s = tf["id_kernel_s"] / max(tf["id_launches"], 1)
     id_f = tf["id_flops"] / max(tf["id_launches"], 1)
-    id_kernel = {"kernel": "k_id<double,float> (ID matrix + pivoted QR + T in FP64)",
-                 "limit": "latency of dependent Householder steps on the FP64 vector units",
+    id_kernel = {"kernel": "k_id_gram<float> (ID matrix generated into A^T A on "
+                           "v_mfma_f64_16x16x4f64, pivoted Cholesky, T) and k_id<double,float> "
+                           "(ID matrix + pivoted Householder QR + T in FP64, boxes above 64 DOFs)",
+                 "limit": "latency of the dependent pivot steps; few boxes per launch at the "
+                          "coarse levels",
                  "achieved": id_f / id_s / 1e12 if id_s > 0 else 0.0,
                  "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                  "avg_launch_us": id_s * 1e6, "launches_per_step": tf["id_launches"],

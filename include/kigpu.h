@@ -118,11 +118,8 @@ const char* kigpu_last_error(void);
  * not change what is computed, only how it is cut into launches:
  *   dec_wave[b]      elimination of boxes with 16 b + 1 .. 16 b + 16 active
  *                    DOFs: 1 one wavefront per box (matrix-core products,
- *                    register Gauss-Jordan; the default), 2 four wavefronts
- *                    per box (the products split over them, the
- *                    Gauss-Jordan in one; Laplace and Laplace-Neumann only,
- *                    Stokes boxes take 1 instead), 0 one 1024-thread
- *                    workgroup per box;
+ *                    register Gauss-Jordan; the default), 0 one 1024-thread
+ *                    workgroup per box (other values are rejected);
  *   mid_on_side      the 33-64 wavefront class on the context's side stream,
  *                    beside the other classes;
  *   pair_sweeps      two tree levels per solve-sweep launch where LDS allows;

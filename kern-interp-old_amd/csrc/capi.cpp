@@ -204,7 +204,7 @@ kigpu_status kigpu_set_tuning(kigpu_ctx* ctx, const kigpu_tuning* in) {
                  in->fwd_split_cap <= 4096,
              "split caps must be 1 .. 4096");
   for (int b = 0; b < 4; ++b)
-    KI_REQUIRE(in->dec_wave[b] >= 0 && in->dec_wave[b] <= 2, "dec_wave entries must be 0, 1 or 2");
+    KI_REQUIRE(in->dec_wave[b] == 0 || in->dec_wave[b] == 1, "dec_wave entries must be 0 or 1");
   Tuning& t = ctx->c.tune;
   for (int b = 0; b < 4; ++b) t.dec_wave[b] = in->dec_wave[b];
   t.mid_on_side = in->mid_on_side != 0;
@@ -233,9 +233,9 @@ kigpu_status kigpu_autotune(kigpu_ctx* ctx, kigpu_tree* tree, const kigpu_factor
     };
     // elimination kernel per box-size class, then the side stream
     for (int b = 0; b < 4; ++b) {
-      double tb[3];
+      double tb[2];
       int bv = 0;
-      for (int v = 0; v < 3; ++v) {
+      for (int v = 0; v < 2; ++v) {
         tu.dec_wave[b] = v;
         tb[v] = best_of(factor_time);
         if (tb[v] < tb[bv]) bv = v;

@@ -1,9 +1,10 @@
 // Device routines of the elimination (decouple, skel_factorization.cpp:55-91)
 // and of the ID matrix generation shared by factor.hip's kernels: staged
 // geometry, kernel entries from it, the matrix-core tile products of one
-// wavefront, the children's update layout, and the elimination of one box
-// by four wavefronts.  Needs only the HIP device built-ins, so that a CPU
-// test can run them on emulated threads.
+// wavefront and the children's update layout; and an elimination of one box
+// by four wavefronts that the library does not launch (see below).  Needs
+// only the HIP device built-ins, so that a CPU test can run them on
+// emulated threads.
 #pragma once
 
 #include "device_util.h"
@@ -203,6 +204,11 @@ __device__ __forceinline__ void child_update_batch(const ChildUpd& cu, int e0, F
 // barriers between dependent products) and X_rr^{-1} by the register
 // Gauss-Jordan in wave 0.  A box's latency is then set by the Gauss-Jordan
 // alone; the other three waves wait for it at one barrier.
+// Not launched by the library: it cut the FP32 elimination from 1.10 to
+// 0.72 ms on the N = 65536 star, but every run in which it had run was
+// followed by a GPU memory fault in a later factorization, with no cause
+// found (its emulation under ASan, tests/test_elim_emulated.py, stays in
+// bounds on every box shape tried).  The routine stays for that test.
 // ---------------------------------------------------------------------------
 constexpr int kQdecTB = 256;
 constexpr int kQdecNW = kQdecTB / kWave;

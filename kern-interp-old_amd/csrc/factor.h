@@ -30,6 +30,7 @@ struct LevelWork {
   int a = 0, b = 0;         // node id range
   int nmax = 0;             // largest |active_box| on the level
   std::vector<IdWork> id_boxes;  // the level's ID work list (flop accounting)
+  int ngram = 0;                 // its last ngram boxes took the Gram-matrix ID
   DBuf<double> slab64;      // factor slabs of this level's nodes
   DBuf<float> slab32;
   // per-level device temporaries, kept until the tree is freed so that no
@@ -43,9 +44,7 @@ struct LevelWork {
 // Launch choices the autotuner sets (kigpu_tuning in kigpu.h mirrors it).
 struct Tuning {
   // elimination: per box-size class n in 1-16, 17-32, 33-48, 49-64 active
-  // DOFs, 1 = one wavefront per box, 2 = four wavefronts per box (opt-in,
-  // Laplace and Laplace-Neumann only: Stokes boxes take 1), 0 = one
-  // 1024-thread workgroup per box
+  // DOFs, 1 = one wavefront per box, 0 = one 1024-thread workgroup per box
   int dec_wave[4] = {1, 1, 1, 1};
   int mid_on_side = 1;       // the 33-64 wavefront class on the side stream
   int pair_sweeps = 1;       // two tree levels per sweep launch where LDS allows

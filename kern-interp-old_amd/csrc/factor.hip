@@ -1575,20 +1575,6 @@ __global__ __launch_bounds__(kWave) void k_decouple_wave(
   }
 }
 
-// elimination with four wavefronts per box (decouple_quad_box in elim_dev.h)
-template <typename T, int RMAX>
-__global__ __launch_bounds__(kQdecTB) void k_decouple_quad(
-    const int* nodes, GeomView g, int pde, const int* is_leaf, const int* child,
-    const int* osize_in, const uint64_t* acc_ptr_in, const int* acc_ld_in, const int* nact,
-    const int* rank, const int64_t* act_off, const int* sr_pool, const int* pos_pool,
-    int* piv_pool, const uint64_t* fac_ptr, int* osize, uint64_t* acc_ptr, int* acc_ld,
-    int* status) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  decouple_quad_box<T, RMAX>(nodes[blockIdx.x], smem, g, pde, is_leaf, child, osize_in,
-                             acc_ptr_in, acc_ld_in, nact, rank, act_off, sr_pool, pos_pool,
-                             piv_pool, fac_ptr, osize, acc_ptr, acc_ld, status);
-}
-
 // ---------------------------------------------------------------------------
 // root: allskel_mat = K(allskel, allskel) - the children's updates
 // (skel_factorization.cpp:116-128); its inverse comes from dense_inverse.
@@ -1836,23 +1822,16 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     // elimination: boxes of at most kWdecMaxN active DOFs one wavefront
     // each, in two size classes (their LDS differs fourfold), largest
     // first; the larger boxes a workgroup each
-    std::vector<int> dec_small, dec_mid, dec_large, dec_quad;
-    int nmax_small = 0, nmax_mid = 0, nmax_large = 0, nmax_quad = 0;
+    std::vector<int> dec_small, dec_mid, dec_large;
+    int nmax_small = 0, nmax_mid = 0, nmax_large = 0;
     {
       std::vector<int> ord(cn);
       for (int i = 0; i < cn; ++i) ord[i] = i;
       std::stable_sort(ord.begin(), ord.end(), [&](int p, int q) { return h_n[p] > h_n[q]; });
       for (int i : ord) {
         const int nb = h_n[i];
-        int choice = nb <= kWdecMaxN ? c.tune.dec_wave[std::max(0, (nb - 1) / 16)] : 0;
-        // (Stokes boxes never take the four-wave kernel: its run on a Stokes
-        // problem with holes faulted the GPU, with no cause found)
-        if (choice == 2 && pde == KIGPU_STOKES) choice = 1;
-        const bool wave = choice == 1;
-        if (choice == 2) {
-          dec_quad.push_back(a + i);
-          nmax_quad = std::max(nmax_quad, nb);
-        } else if (!wave) {
+        const bool wave = nb <= kWdecMaxN && c.tune.dec_wave[std::max(0, (nb - 1) / 16)];
+        if (!wave) {
           dec_large.push_back(a + i);
           nmax_large = std::max(nmax_large, nb);
         } else if (nb <= kWdecMaxN / 2) {
@@ -1875,12 +1854,10 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
     const size_t o_ds = pk.add(dec_small.data(), sizeof(int) * dec_small.size());
     const size_t o_dm = pk.add(dec_mid.data(), sizeof(int) * dec_mid.size());
     const size_t o_dl = pk.add(dec_large.data(), sizeof(int) * dec_large.size());
-    const size_t o_dq = pk.add(dec_quad.data(), sizeof(int) * dec_quad.size());
     unsigned char* dpk = pk.send(c.pin_up, lw->pack, s);
     const int* d_dec_small = reinterpret_cast<const int*>(dpk + o_ds);
     const int* d_dec_mid = reinterpret_cast<const int*>(dpk + o_dm);
     const int* d_dec_large = reinterpret_cast<const int*>(dpk + o_dl);
-    const int* d_dec_quad = reinterpret_cast<const int*>(dpk + o_dq);
     const int64_t* d_soff = reinterpret_cast<const int64_t*>(dpk + o_soff);
     const int64_t* d_roff = reinterpret_cast<const int64_t*>(dpk + o_roff);
     const IdWork* d_wl = reinterpret_cast<const IdWork*>(dpk + o_wl);
@@ -1955,6 +1932,7 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
       }
       E.idl1 = mark();
       lw->id_boxes = wl;
+      lw->ngram = ngram;
     }
     E.id1 = mark();
     // the next level's lists, then this level's elimination
@@ -1971,17 +1949,6 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
         allow_lds(kern, lds);
         kern<<<count, kWave, lds, st>>>(
             nodes, g, pde, t.is_leaf.p, t.child.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p,
-            t.nact.p, t.rank.p, t.act_off.p, t.sr_pool.p, t.pos_pool.p, t.piv_pool.p,
-            t.fac_ptr.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, t.status.p);
-      };
-      auto quad_launch = [&](hipStream_t st) {
-        const int nmx = std::max(nmax_quad, 1);
-        const size_t lds = wdec_lds_bound(nmx, sizeof(T));
-        auto kern = nmx <= kWdecMaxN / 2 ? k_decouple_quad<T, kWdecMaxN / 2>
-                                         : k_decouple_quad<T, kWdecMaxN>;
-        allow_lds(kern, lds);
-        kern<<<(int)dec_quad.size(), kQdecTB, lds, st>>>(
-            d_dec_quad, g, pde, t.is_leaf.p, t.child.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p,
             t.nact.p, t.rank.p, t.act_off.p, t.sr_pool.p, t.pos_pool.p, t.piv_pool.p,
             t.fac_ptr.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, t.status.p);
       };
@@ -2023,7 +1990,6 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
       } else if (!dec_mid.empty()) {
         wave_launch(d_dec_mid, (int)dec_mid.size(), nmax_mid, s);
       }
-      if (!dec_quad.empty()) quad_launch(s);
       if (!dec_small.empty()) wave_launch(d_dec_small, (int)dec_small.size(), nmax_small, s);
       if (!dec_large.empty()) large_launch(s);
       if (side) {
@@ -2180,6 +2146,17 @@ static double id_flops(int m, int n, int k, double c_gen) {
   return f;
 }
 
+// The same for the Gram-matrix ID: generation, one triangle of A^T A
+// (m n (n + 1)), the pivoted Cholesky's rows of R up to the rank-revealing
+// step (2 i (n - i) at step i) and the T solve.
+static double gram_id_flops(int m, int n, int k, double c_gen) {
+  double f = c_gen * m * n + (double)m * n * (n + 1);
+  const double st = k > 0 ? k + 1 : n;
+  f += 2.0 * n * (st * (st - 1) / 2) - 2.0 * ((st - 1) * st * (2 * st - 1) / 6);
+  if (k > 0) f += (double)k * k * (n - k);
+  return f;
+}
+
 // elimination: K_BN generation plus 6k^2r + 8kr^2 + 2/3 r^3
 static double dec_flops(int n, int k, double c_gen) {
   if (k <= 0) return 0;
@@ -2225,7 +2202,11 @@ void factor_tree(Context& c, TreeData& t, const kigpu_factor_opts& o) {
   const double c_gen = o.pde == KIGPU_STOKES ? 20.0 : 11.0;
   for (auto& lw : t.work) {
     if (!lw) continue;
-    for (const IdWork& w : lw->id_boxes) c.tim.id_flops += id_flops(w.m, w.n, t.h_rank[w.node], c_gen);
+    const size_t g0 = lw->id_boxes.size() - lw->ngram;
+    for (size_t i = 0; i < lw->id_boxes.size(); ++i) {
+      const IdWork& w = lw->id_boxes[i];
+      c.tim.id_flops += (i >= g0 ? gram_id_flops : id_flops)(w.m, w.n, t.h_rank[w.node], c_gen);
+    }
     for (int x = lw->a; x < lw->b; ++x) c.tim.dec_flops += dec_flops(t.h_nact[x], t.h_rank[x], c_gen);
   }
   t.opts = o;

@@ -5,11 +5,11 @@ from oracle import geometry as G
 from oracle import kernel as K
 
 # The FP64 factorization of small Stokes problems with holes (the 8-hole disc
-# of test_gpu_border and the annulus of test_gpu_domain) faulted the GPU
-# twice with an illegal memory access inside kigpu_factor, once with the
-# four-wave elimination on the Stokes boxes and once without it, after
-# passing before; no cause has been found on the CPU, so these cases stay
-# off the GPU.
+# of test_gpu_border and the annulus of test_gpu_domain) faulted the GPU with
+# an illegal memory access inside kigpu_factor, in runs where a four-wave
+# elimination kernel (since taken off the GPU) had run earlier in the same
+# process, after passing before; no cause has been found on the CPU, so these
+# cases stay off the GPU.
 STOKES_SMALL_BORDER_FAULT = (
     "faulted the GPU (illegal memory access in kigpu_factor, FP64 Stokes with holes at "
     "N = 2048) with no cause found on the CPU; kept off the GPU until one is shown")
