@@ -1057,7 +1057,8 @@ void sweeps_launch(Context& c, TreeData& t, T* x, int ldx, int nrhs, const doubl
 
 // x <- F^{-1} x, or F^{-T} x with trans (no border then).  With graphs on,
 // the launches of a sweep are captured on its second call with the same
-// vector, shape and border vector, and replayed from then on: the first call
+// vector, shape and border vector, and replayed from then on while the
+// bordered step's scratch stays where the capture found it: the first call
 // runs them directly (it sets the LDS limits and sizes the border scratch, so
 // the captured launches allocate nothing), and the sweeps hold no memset,
 // host wait or allocation.
@@ -1079,7 +1080,16 @@ void sweeps(Context& c, TreeData& t, T* x, int ldx, int nrhs, const double* g = 
       if (t.sweep_graphs.front().exec) (void)hipGraphExecDestroy(t.sweep_graphs.front().exec);
       t.sweep_graphs.erase(t.sweep_graphs.begin());
     }
-    t.sweep_graphs.push_back({(int)sizeof(T), x, ldx, nrhs, g, trans, nullptr});
+    t.sweep_graphs.push_back({(int)sizeof(T), x, ldx, nrhs, g, trans, nullptr, {}});
+    sweeps_launch<T>(c, t, x, ldx, nrhs, g, trans);
+    return;
+  }
+  const void* scr[3] = {t.bgram.p, t.bM.p, t.alpha_dev.p};
+  if (e->exec && (e->scr[0] != scr[0] || e->scr[1] != scr[1] || e->scr[2] != scr[2])) {
+    // the border scratch moved since the capture: the graph would write the
+    // freed blocks.  Run directly and capture again on the next call.
+    (void)hipGraphExecDestroy(e->exec);
+    e->exec = nullptr;
     sweeps_launch<T>(c, t, x, ldx, nrhs, g, trans);
     return;
   }
@@ -1099,6 +1109,9 @@ void sweeps(Context& c, TreeData& t, T* x, int ldx, int nrhs, const double* g = 
     (void)hipGraphDestroy(graph);
     KI_HIP(ie);
     e->exec = exec;
+    e->scr[0] = t.bgram.p;
+    e->scr[1] = t.bM.p;
+    e->scr[2] = t.alpha_dev.p;
     ++c.tim.graph_captures;
   }
   KI_HIP(hipGraphLaunch(e->exec, s));

@@ -79,6 +79,9 @@ struct TreeData {
     const void* g;
     bool trans;
     hipGraphExec_t exec;
+    // the bordered step's scratch (bgram, bM, alpha_dev) the capture used: a
+    // later solve with more right-hand sides can move it
+    const void* scr[3];
   };
   std::vector<SweepGraph> sweep_graphs;
   DBuf<double> gx;  // FP64 solves with graphs: the sweeps' vector, kept at one address

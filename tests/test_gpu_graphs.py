@@ -98,6 +98,31 @@ def test_bordered_replays(ctx, gctx):
 
 
 @pytest.mark.gpu
+def test_bordered_replays_after_scratch_moves(ctx, gctx):
+    # solves of 3, 3, 4, 3, 3 right-hand sides: the 4-column solve grows the
+    # bordered step's scratch, so the 3-column graph captured before it is
+    # not replayed (it would write the freed blocks): the next 3-column solve
+    # runs directly and the one after captures again
+    print("[graphs] annulus 2048 with border, 3/3/4/3/3 right-hand sides", flush=True)
+    b = G.annulus(2048)
+    b.boundary_values = G.bc_single_electron(b)
+    border = (LS.initialize_U(K.LAPLACE, b.holes, b.points), LS.initialize_Psi(K.LAPLACE, b.holes, b))
+    rng = np.random.default_rng(5)
+    F = [rng.standard_normal((b.num_points, k)) for k in (3, 3, 4, 3, 3)]
+    t0, want = solves(ctx, b, K.LAPLACE, 1, 32, 1e-10, kigpu.FP64, F, border=border)
+    tg = gpu_tree(gctx, b, 1, 32)
+    tg.set_border(*border)
+    tg.factor(K.LAPLACE, 1e-10, kigpu.FP64)
+    counts = []
+    for i, f in enumerate(F):
+        assert np.array_equal(tg.solve(f), want[i]), f"solve {i}"
+        tm = gctx.timings()
+        counts.append((tm["graph_captures"], tm["graph_replays"]))
+    assert counts[1] == (1, 1) and counts[2] == (0, 0)
+    assert counts[3] == (0, 0) and counts[4] == (1, 1)
+
+
+@pytest.mark.gpu
 def test_repeated_solve_time(ctx, gctx):
     # many solves with one factorization: direct launches against replays
     print("[graphs] star 65536: 50 solves, direct vs replayed", flush=True)
