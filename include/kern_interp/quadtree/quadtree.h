@@ -28,7 +28,7 @@ struct InteractionLists {
 };
 
 struct QuadTreeNode {
-  int level, dofs_below;
+  int level = 0, dofs_below = 0;
   bool is_leaf, X_rr_is_LU_factored = false, compressed = false;
   double side_length;
   QuadTreeNode* parent;

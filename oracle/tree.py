@@ -4,8 +4,10 @@ TEST INFRASTRUCTURE ONLY (see the package docstring).
 
 Restates ``kern_interp/quadtree/quadtree.cpp`` (construction ``:72-114``,
 point insertion ``:164-197``, subdivision ``:203-285``, neighbor lists
-``:27-69`` and ``:119-161``, active/near DOF lists ``:656-720``) and the
-per-node DOF bookkeeping of ``quadtree.h:14-61``.  The leaf capacity, fixed
+``:27-69`` and ``:119-161``, active/near DOF lists ``:656-720``, the
+incremental edits ``perturb``, ``mark_neighbors_and_parents`` and
+``consolidate_node`` ``:288-560``) and the per-node DOF bookkeeping of
+``quadtree.h:14-61``.  The leaf capacity, fixed
 at 32 DOFs by ``quadtree.h:10``, is a parameter here.
 
 The insertion is simulated point by point, exactly as the reference does, so
@@ -14,7 +16,9 @@ tree uses a different, level-synchronous algorithm; agreement of the two is
 one of the pins.  Other pins (``tests/test_oracle_tree.py``): a brute-force
 recursive partition gives the same node set and DOF lists, every leaf holds
 at most ``leaf_dofs`` DOFs in ascending order, and neighbor sets match a
-geometric adjacency test.
+geometric adjacency test.  The edits are pinned by two perturbations of the
+four-point tree traced by hand (``tests/golden/perturb_trace_4pts.md``) and
+compared with the drop-in's C++ host code (``tests/test_perturb_host.py``).
 """
 from __future__ import annotations
 
@@ -29,9 +33,9 @@ SQRT2 = math.sqrt(2)
 class Node:
     __slots__ = ("level", "index", "is_leaf", "parent", "children", "corners",
                  "side", "neighbors", "original_box", "active_box", "skel",
-                 "redundant", "near", "permutation", "compressed", "T", "L",
-                 "U", "X_lu", "X_piv", "schur", "path", "split_time", "id",
-                 "m_rows")
+                 "redundant", "near", "skelnear", "permutation", "compressed",
+                 "T", "L", "U", "X_lu", "X_piv", "schur", "path", "split_time",
+                 "id", "m_rows", "dofs_below")
 
     def __init__(self, level, corners, side, parent):
         self.level = level
@@ -47,6 +51,7 @@ class Node:
         self.skel = []
         self.redundant = []
         self.near = []
+        self.skelnear = []
         self.permutation = []
         self.compressed = False
         self.T = self.L = self.U = self.X_lu = self.X_piv = self.schur = None
@@ -54,6 +59,7 @@ class Node:
         self.split_time = -1
         self.id = -1
         self.m_rows = 0
+        self.dofs_below = 0
 
     def mid(self):
         c = self.corners
@@ -242,6 +248,129 @@ class QuadTree:
         """``remove_inactive_dofs_at_all_boxes`` (quadtree.cpp:656-661)."""
         for level in range(len(self.levels) - 1, -1, -1):
             self.refresh_level(level)
+
+    # -- incremental edits (quadtree.cpp:288-560) -----------------------------
+    @staticmethod
+    def mark_neighbors_and_parents(node):
+        """``mark_neighbors_and_parents`` (quadtree.cpp:288-313): the box, its
+        neighbors, and every ancestor with its neighbors lose their
+        compression and factor blocks."""
+        while node is not None:
+            for n in [node] + node.neighbors:
+                n.compressed = False
+                n.active_box = []
+                n.skel = []
+                n.skelnear = []
+                n.redundant = []
+                n.permutation = []
+                n.T = n.U = n.L = None
+            node = node.parent
+
+    def consolidate_node(self, node):
+        """``consolidate_node`` (quadtree.cpp:316-356): the box becomes a leaf
+        holding its leaves' DOFs in breadth-first order; its descendants leave
+        their levels."""
+        node.original_box = []
+        queue, gone = [node], []
+        i = 0
+        while i < len(queue):
+            cur = queue[i]
+            i += 1
+            if cur.is_leaf:
+                node.original_box.extend(cur.original_box)
+            else:
+                queue.extend(cur.children)
+            if cur is not node:
+                gone.append(cur)
+        for g in gone:
+            lvl = self.levels[g.level]
+            for k, m in enumerate(lvl):
+                if m is g:
+                    del lvl[k]
+                    break
+        node.children = [None, None, None, None]
+        node.is_leaf = True
+
+    def _leaf_of(self, x, y):
+        cur = self.root
+        while not cur.is_leaf:
+            midx, midy = cur.mid()
+            cur = cur.children[_quadrant(x, y, midx, midy)]
+        return cur
+
+    def perturb(self, new_points):
+        """``QuadTree::perturb`` (quadtree.cpp:359-560).  Points are matched by
+        exact coordinates (a repeated point keeps its last new index, as the
+        reference's hash map does); the DOF lists are renumbered (a leaf's
+        original box, every box's active, skeleton, redundant and skelnear
+        lists; a non-leaf's original box is left as it was); added points go
+        to the leaves holding them, deleted points mark the first box on
+        their path holding fewer than ``leaf_dofs`` DOFs, which is then
+        merged into one leaf; overfull leaves subdivide; the neighbor lists
+        are computed afresh.  Empty trailing levels are kept, as there."""
+        sd = self.sd
+        old = self.points.reshape(-1, 2)
+        new = np.asarray(new_points, dtype=np.float64).reshape(-1, 2)
+        new_index = {}
+        for i in range(new.shape[0]):
+            new_index[(float(new[i, 0]), float(new[i, 1]))] = i
+        found = [False] * new.shape[0]
+        old_to_new, deletions = {}, []
+        for i in range(old.shape[0]):
+            j = new_index.get((float(old[i, 0]), float(old[i, 1])))
+            if j is None:
+                deletions.append(i)
+            else:
+                old_to_new[i] = j
+                found[j] = True
+        additions = [i for i in range(new.shape[0]) if not found[i]]
+
+        def remap(v):
+            return [sd * old_to_new[d // sd] + d % sd for d in v if d // sd in old_to_new]
+        for lvl in self.levels:
+            for nd in lvl:
+                if nd.is_leaf:
+                    nd.original_box = remap(nd.original_box)
+                nd.active_box = remap(nd.active_box)
+                nd.skel = remap(nd.skel)
+                nd.redundant = remap(nd.redundant)
+                nd.skelnear = remap(nd.skelnear)
+        bursting = []
+        for a in additions:
+            leaf = self._leaf_of(float(new[a, 0]), float(new[a, 1]))
+            leaf.original_box.extend(sd * a + j for j in range(sd))
+            bursting.append(leaf)
+            self.mark_neighbors_and_parents(leaf)
+        for lvl in self.levels:
+            for nd in lvl:
+                nd.dofs_below = len(nd.original_box) if nd.is_leaf else 0
+        for lv in range(len(self.levels) - 1, 0, -1):
+            for nd in self.levels[lv]:
+                nd.parent.dofs_below += nd.dofs_below
+        sparse = []
+        for d in deletions:
+            x, y = float(old[d, 0]), float(old[d, 1])
+            cur, marked = self.root, False
+            while not cur.is_leaf:
+                if cur.dofs_below < self.leaf_dofs and not marked:
+                    marked = True
+                    if not any(s is cur for s in sparse):
+                        sparse.append(cur)
+                midx, midy = cur.mid()
+                cur = cur.children[_quadrant(x, y, midx, midy)]
+            self.mark_neighbors_and_parents(cur)
+        self.points = new.reshape(-1).copy()
+        self._x = new[:, 0].tolist()
+        self._y = new[:, 1].tolist()
+        for nd in bursting:
+            if nd.is_leaf and len(nd.original_box) > self.leaf_dofs:
+                self._subdivide(nd)
+        for nd in sparse:
+            self.consolidate_node(nd)
+        for nd in self.all_nodes():
+            nd.neighbors = []
+        self._neighbor_lists()
+        self._number()
 
     # -- views used by tests ------------------------------------------------
     def all_nodes(self):

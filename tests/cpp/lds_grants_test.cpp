@@ -5,7 +5,9 @@
 // kern-interp-old_amd/csrc/lds_grants.h, and a record kept per thread.
 // Prints the number of launches that found the limit below their size.
 #include <atomic>
+#include <condition_variable>
 #include <cstdio>
+#include <mutex>
 #include <random>
 #include <thread>
 #include <vector>
@@ -39,21 +41,47 @@ void allow_per_thread(const void* f, size_t bytes) {
   set_limit(f, bytes);
   granted.push_back({f, bytes});
 }
+
+// all threads meet here: every thread makes its request of a round before
+// any thread launches, whatever the host's scheduling, so the requests of
+// different threads interleave in every round
+struct Barrier {
+  std::mutex mu;
+  std::condition_variable cv;
+  int count = 0, gen = 0;
+  const int n;
+  explicit Barrier(int n_) : n(n_) {}
+  void wait() {
+    std::unique_lock<std::mutex> lk(mu);
+    const int g = gen;
+    if (++count == n) {
+      count = 0;
+      ++gen;
+      cv.notify_all();
+    } else {
+      cv.wait(lk, [&] { return gen != g; });
+    }
+  }
+};
 }  // namespace
 
 int main(int argc, char** argv) {
   const bool per_thread = argc > 1 && argv[1][0] == 't';
   static int kernel;  // stands for the kernel's address
   kigpu::LdsGrants grants;
+  constexpr int kThreads = 8;
+  Barrier bar(kThreads);
   std::vector<std::thread> th;
-  for (int t = 0; t < 8; ++t)
+  for (int t = 0; t < kThreads; ++t)
     th.emplace_back([&, t] {
       std::mt19937 rng(t);
-      for (int i = 0; i < 20000; ++i) {
+      for (int i = 0; i < 2000; ++i) {
         const size_t bytes = 1024 * (1 + rng() % 150);
         if (per_thread) allow_per_thread(&kernel, bytes);
         else grants.allow(&kernel, 0, bytes, set_limit);
+        bar.wait();
         launch(bytes);
+        bar.wait();
       }
     });
   for (auto& x : th) x.join();
