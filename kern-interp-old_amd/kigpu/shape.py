@@ -132,3 +132,32 @@ def loss_and_gradient(ctx, star: FourierStar, c, targets, bc=quadratic_bc, id_to
     grad += np.einsum("n,nc->c", lam * fx, J[:, 0, :])
     grad += np.einsum("n,nc->c", lam * fy, J[:, 1, :])
     return float(np.sum(u * u)), grad, info
+
+
+def shape_loss(ctx, star: FourierStar, c, targets, **kw):
+    """L(c) as a torch scalar whose backward pass is the adjoint gradient.
+
+    ``c`` is a 1-D float64 tensor of the star's coefficients (any device; the
+    computation itself runs through the library on its own context and
+    stream).  The forward pass calls :func:`loss_and_gradient` once and keeps
+    dL/dc; ``L.backward()`` then scales it by the incoming gradient, so no
+    second factorization runs.  Keyword arguments go to
+    :func:`loss_and_gradient`.  torch is imported here, not by the package.
+    """
+    import torch
+
+    class _ShapeLoss(torch.autograd.Function):
+        @staticmethod
+        def forward(fctx, coeffs):
+            L, g, info = loss_and_gradient(ctx, star, coeffs.detach().cpu().double().numpy(),
+                                           targets, **kw)
+            fctx.save_for_backward(torch.as_tensor(g, dtype=coeffs.dtype, device=coeffs.device))
+            fctx.info = info
+            return torch.tensor(L, dtype=coeffs.dtype, device=coeffs.device)
+
+        @staticmethod
+        def backward(fctx, gout):
+            (g,) = fctx.saved_tensors
+            return gout * g
+
+    return _ShapeLoss.apply(c)

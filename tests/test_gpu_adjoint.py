@@ -144,3 +144,22 @@ def test_shape_gradient_n32768_directional(ctx):
     fd = (Lp - Lm) / (2 * h)
     print(f"  grad.d {grad @ d:.10e} fd {fd:.10e}", flush=True)
     assert abs(grad @ d - fd) <= 1e-5 * abs(fd)
+
+
+@pytest.mark.gpu
+def test_shape_loss_autograd(ctx):
+    # the torch.autograd wrapper: backward leaves the adjoint gradient of the
+    # same call in c.grad (scaled by the incoming gradient), against the
+    # dense complex-step gradient
+    import torch
+    print("[adjoint] torch.autograd wrapper, shape gradient N=2048", flush=True)
+    n, c = 2048, coeffs()
+    tgt = G.interior_disc_targets(128, 21, radius=0.3)
+    star = shape.FourierStar(n, c.size)
+    ct = torch.tensor(c, dtype=torch.float64, requires_grad=True)
+    L = shape.shape_loss(ctx, star, ct, tgt, id_tol=1e-10, precision=kigpu.FP32_REFINE,
+                         refine_iters=10, rel_residual=1e-13)
+    (2.0 * L).backward()
+    ref = A.complex_step_grad(n, c, tgt)
+    assert abs(L.item() - A.dense_loss(n, c, tgt)) <= 1e-11 * L.item()
+    assert np.linalg.norm(ct.grad.numpy() / 2.0 - ref) <= 1e-7 * np.linalg.norm(ref)
