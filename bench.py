@@ -221,15 +221,15 @@ def bench_adjoint(kigpu, ctx, evals):
             "note": "host arrays in and out (includes PCIe copies)"}
 
 
-def pmc_traffic(name):
-    """HBM bytes per launch of a kernel from its committed rocprofv3 --pmc
-    summary under profiles/ (FETCH_SIZE + WRITE_SIZE, scaled by the
-    copy-kernel calibration), or None when the summary is absent."""
+def pmc_traffic(name, key="traffic_bytes_per_launch"):
+    """A figure from a committed rocprofv3 --pmc summary under profiles/:
+    by default the HBM bytes per launch of a kernel (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration); None when the summary is absent."""
     path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as fh:
             for line in fh:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key + ":"):
                     return float(line.split(":", 1)[1])
     except OSError:
         pass
@@ -419,7 +419,9 @@ def main():
                  "achieved": id_f / id_s / 1e12 if id_s > 0 else 0.0,
                  "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                  "avg_launch_us": id_s * 1e6, "launches_per_step": tf["id_launches"],
-                 "traffic": pmc_traffic("pmc_id_laplace_star_N65536.md")}
+                 "traffic": pmc_traffic("pmc_id_laplace_star_N65536.md"),
+                 "mfma_util_percent": pmc_traffic("pmc_mfma_laplace_star_N65536.md",
+                                                  "mfma_util_id_percent")}
     # the refinement's residual apply: the treecode (box series + direct near
     # field).  Algorithmic flops: 8 (P - 1) + 15 per (target, series box) for
     # the Horner evaluation of the P <= 48 terms the box needs for the
