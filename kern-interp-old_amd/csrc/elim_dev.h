@@ -1,10 +1,9 @@
 // Device routines of the elimination (decouple, skel_factorization.cpp:55-91)
 // and of the ID matrix generation shared by factor.hip's kernels: staged
 // geometry, kernel entries from it, the matrix-core tile products of one
-// wavefront and the children's update layout; and an elimination of one box
-// by four wavefronts that the library does not launch (see below).  Needs
-// only the HIP device built-ins, so that a CPU test can run them on
-// emulated threads.
+// wavefront, the children's update layout, and the elimination of one box
+// by one wavefront (the body of k_decouple_wave).  Needs only the HIP device
+// built-ins, so that a CPU test can run them on emulated threads.
 #pragma once
 
 #include "device_util.h"
@@ -68,21 +67,17 @@ struct Mfma<double> {
 };
 
 // out(i, j, sum_t A(i, t) B(t, j)) for every (i, j) of an M x N product by
-// one wavefront, in 16 x 16 tiles with 4-deep MFMA steps (with nparts > 1:
-// the tiles part, part + nparts, ... only, for one of several waves).  The A operand of
+// one wavefront, in 16 x 16 tiles with 4-deep MFMA steps.  The A operand of
 // lane l is A(i0 + l % 16, t0 + l / 16), the B operand B(t0 + l / 16,
 // j0 + l % 16); entries past M, N or K enter as zeros (read at a clamped
 // index and replaced, so the loads need no branch).  The next step's
 // operands are loaded before the current MFMA is issued.
 template <typename T, typename FA, typename FB, typename FO>
-__device__ __forceinline__ void wave_gemm(int M, int N, int K, FA&& A, FB&& B, FO&& out,
-                                          int part = 0, int nparts = 1) {
+__device__ __forceinline__ void wave_gemm(int M, int N, int K, FA&& A, FB&& B, FO&& out) {
   const int lane = threadIdx.x & (kWave - 1);
   const int l16 = lane & 15, l4 = lane >> 4;
-  int tile = 0;
   for (int i0 = 0; i0 < M; i0 += 16)
     for (int j0 = 0; j0 < N; j0 += 16) {
-      if (tile++ % nparts != part) continue;  // tiles part, part + nparts, ... of the product
       const int ia = i0 + l16, jb = j0 + l16;
       const int iac = min(ia, M - 1), jbc = min(jb, N - 1);
       auto ld = [&](int t0, T& a, T& b) {
@@ -198,31 +193,26 @@ __device__ __forceinline__ void child_update_batch(const ChildUpd& cu, int e0, F
 }
 
 // ---------------------------------------------------------------------------
-// elimination with four wavefronts per box (|active_box| <= 64): the steps
-// of k_decouple_wave, with K_BN, the children's updates and every product
-// split over the four waves (a share of the 16 x 16 tiles each, workgroup
-// barriers between dependent products) and X_rr^{-1} by the register
-// Gauss-Jordan in wave 0.  A box's latency is then set by the Gauss-Jordan
-// alone; the other three waves wait for it at one barrier.
-// Not launched by the library: it cut the FP32 elimination from 1.10 to
-// 0.72 ms on the N = 65536 star, but every run in which it had run was
-// followed by a GPU memory fault in a later factorization, with no cause
-// found (its emulation under ASan, tests/test_elim_emulated.py, stays in
-// bounds on every box shape tried).  The routine stays for that test.
+// elimination with one wavefront per box (|active_box| <= 64): the same
+// steps as k_decouple, with the products on the matrix cores
+// (v_mfma_f32_16x16x4f32 for the FP32 factor, v_mfma_f64_16x16x4f64 for
+// FP64) and X_rr^{-1} by a Gauss-Jordan held in registers, row i in lane i,
+// so no step waits on a workgroup barrier.
 // ---------------------------------------------------------------------------
-constexpr int kQdecTB = 256;
-constexpr int kQdecNW = kQdecTB / kWave;
 
-// The elimination of box x by the calling 256-thread workgroup; smem: its
-// dynamic LDS (wdec_lds_bound of the launch's largest box).
+// The elimination of box x by the calling wavefront; smem: its dynamic LDS
+// (wdec_lds_bound of the launch's largest box).  RMAX: the largest r the
+// launch's boxes can have (n <= RMAX + 1); the register Gauss-Jordan is
+// instantiated up to that width only.
 template <typename T, int RMAX>
-__device__ void decouple_quad_box(
+__device__ __forceinline__ void decouple_wave_box(
     int x, unsigned char* smem, GeomView g, int pde, const int* is_leaf, const int* child,
     const int* osize_in, const uint64_t* acc_ptr_in, const int* acc_ld_in, const int* nact,
     const int* rank, const int64_t* act_off, const int* sr_pool, const int* pos_pool,
     int* piv_pool, const uint64_t* fac_ptr, int* osize, uint64_t* acc_ptr, int* acc_ld,
     int* status) {
-  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid >> 6;
+  const int lane = threadIdx.x;  // a 64-thread block: the lane
+  // loads that need only x, issued together
   const int n = nact[x];
   const int kr = rank[x];
   const bool leaf = is_leaf[x];
@@ -230,7 +220,7 @@ __device__ void decouple_quad_box(
   const int64_t aoff = act_off[x];
   T* slab = reinterpret_cast<T*>(fac_ptr[x]);
   if (n == 0) {
-    if (tid == 0) {
+    if (lane == 0) {
       osize[x] = 0;
       acc_ptr[x] = 0;
       acc_ld[x] = 0;
@@ -238,10 +228,23 @@ __device__ void decouple_quad_box(
     return;
   }
   const int k = kr < 0 ? n : kr, r = n - k;
-  // every wave gathers the children's update layout (the same loads)
+  // the second round: children (lanes 0..3), the [skel | redundant] lists
+  // (one DOF per lane, n <= 64) and T (k x r, 8 values per lane per batch)
   const int csz = (!leaf && y >= 0) ? osize_in[y] : 0;
   const uint64_t cap = (!leaf && y >= 0) ? acc_ptr_in[y] : 0;
   const int cldl = (!leaf && y >= 0) ? acc_ld_in[y] : 0;
+  int md = 0, ps = 0;
+  if (kr >= 0 && lane < n) {
+    md = sr_pool[aoff + lane];
+    ps = pos_pool[aoff + lane];
+  }
+  const int kt = kr >= 0 ? k * r : 0;
+  T tv[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int e = u * kWave + lane;
+    tv[u] = e < kt ? slab[e] : T(0);
+  }
   ChildUpd cu;
   cu.co[0] = cu.eo[0] = 0;
 #pragma unroll
@@ -253,19 +256,26 @@ __device__ void decouple_quad_box(
     cu.cld[c] = __shfl(cldl, c, kWave);
   }
   if (kr < 0) {  // not compressed: hand the whole active list and its update up
-    if (!leaf) {  // thread a writes row a of a column group, each entry once
-      for (int b = wv; b < n; b += kQdecNW) {
-        const int cb = (b >= cu.co[1]) + (b >= cu.co[2]) + (b >= cu.co[3]);
-        const T* accb = reinterpret_cast<const T*>(pick4(cu.cacc, cb));
-        const int cob = pick4(cu.co, cb), ldb = pick4(cu.cld, cb);
-        if (lane < n) {
-          const int ca = (lane >= cu.co[1]) + (lane >= cu.co[2]) + (lane >= cu.co[3]);
-          const bool in = ca == cb && b < cu.co[4] && lane < cu.co[4] && accb;
-          slab[lane + (size_t)b * n] = in ? accb[(lane - cob) + (size_t)(b - cob) * ldb] : T(0);
+    if (!leaf) {  // lane a writes row a, each entry once
+      const int ca = (lane >= cu.co[1]) + (lane >= cu.co[2]) + (lane >= cu.co[3]);
+      const T* acca = reinterpret_cast<const T*>(pick4(cu.cacc, ca));
+      const int coa = pick4(cu.co, ca), lda = pick4(cu.cld, ca);
+      for (int b0 = 0; b0 < n; b0 += 8) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int b = b0 + u;
+          const int cb = (b >= cu.co[1]) + (b >= cu.co[2]) + (b >= cu.co[3]);
+          const bool in = lane < n && b < n && cb == ca && b < cu.co[4] && lane < cu.co[4] &&
+                          acca;
+          v[u] = in ? acca[(lane - coa) + (size_t)(b - coa) * lda] : T(0);
         }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (lane < n && b0 + u < n) slab[lane + (size_t)(b0 + u) * n] = v[u];
       }
     }
-    if (tid == 0) {
+    if (lane == 0) {
       osize[x] = n;
       acc_ptr[x] = leaf ? 0 : reinterpret_cast<uint64_t>(slab);
       acc_ld[x] = n;
@@ -284,70 +294,75 @@ __device__ void decouple_quad_box(
   T* Ts = reinterpret_cast<T*>(smem + Ly.tm);
   T* Krsn = reinterpret_cast<T*>(smem + Ly.krsn);
   T* Ksnr = reinterpret_cast<T*>(smem + Ly.ksnr);
-  if (wv == 0 && lane < n) {
-    const int md = sr_pool[aoff + lane], ps = pos_pool[aoff + lane];
+  // the third round: this lane's DOF geometry (kept in registers for its row)
+  double mg[kGeo] = {0, 0, 0, 0, 0, 0};
+  if (lane < n) stage_geo(g, md, mg);
+  if (lane < n) {
     sr[lane] = md;
     pos[lane] = ps;
     ipos[ps] = lane;
-    stage_geo(g, md, geo + kGeo * lane);
+#pragma unroll
+    for (int q = 0; q < kGeo; ++q) geo[kGeo * lane + q] = mg[q];
   }
-  for (int e = tid; e < k * r; e += kQdecTB) Ts[e] = slab[e];  // T from the ID
-  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int e = u * kWave + lane;
+    if (e < kt) Ts[e] = tv[u];
+  }
+  for (int e0 = 8 * kWave; e0 < kt; e0 += kWave) {  // T past the first batch
+    const int e = e0 + lane;
+    if (e < kt) Ts[e] = slab[e];
+  }
+  wave_sync();
   // K_BN = K(BN, BN) - update, permuted to [s r] (skel_factorization.cpp:66-78):
-  // 256 / n threads per row, thread t on row t % n and every (256 / n)-th
-  // column from t / n
+  // 64 / n lanes per row, lane l on row l % n and every (64 / n)-th column
+  // from l / n, four columns in flight
   {
-    const int npl = kQdecTB / n;
-    if (tid < npl * n) {
-      const int i = tid % n;
+    const int npl = kWave / n;
+    if (lane < npl * n) {
+      const int i = lane % n;
       double tg[kGeo];
 #pragma unroll
       for (int q = 0; q < kGeo; ++q) tg[q] = geo[kGeo * i + q];
       const int ti = sr[i];
 #pragma unroll 4
-      for (int j = tid / n; j < n; j += npl)
+      for (int j = lane / n; j < n; j += npl)
         Kp[i + (size_t)j * n] = (T)staged_entry(pde, ti, tg, sr[j], geo + kGeo * j);
     }
   }
-  __syncthreads();
+  wave_sync();
   if (!leaf) {
-    for (int e0 = wv * 8 * kWave; e0 < cu.eo[4]; e0 += kQdecNW * 8 * kWave)
+    for (int e0 = 0; e0 < cu.eo[4]; e0 += 8 * kWave)
       child_update_batch<T>(cu, e0, [&](int a, int b, T v) {
         if (a >= n || b >= n) return;
         const size_t e = ipos[a] + (size_t)ipos[b] * n;
         Kp[e] = (T)((double)Kp[e] - (double)v);
       });
-    __syncthreads();
+    wave_sync();
   }
   auto KP = [&](int i, int j) -> T { return Kp[i + (size_t)j * n]; };
   auto TT = [&](int i, int t) -> T { return Ts[t + (size_t)i * k]; };  // T^T (r x k)
   auto TM = [&](int t, int j) -> T { return Ts[t + (size_t)j * k]; };  // T (k x r)
-  // K_r_sn = K_rs - T^T K_ss (r x k); K_sn_r = K_sr - K_ss T (k x r);
-  // X_rr = K_rr - T^T K_sr (r x r), then X_rr -= K_r_sn T
+  // K_r_sn = K_rs - T^T K_ss (r x k); K_sn_r = K_sr - K_ss T (k x r)
   wave_gemm<T>(r, k, k, TT, [&](int t, int j) { return KP(t, j); },
-               [&](int i, int j, T v) { Krsn[i + (size_t)j * r] = KP(k + i, j) - v; }, wv,
-               kQdecNW);
+               [&](int i, int j, T v) { Krsn[i + (size_t)j * r] = KP(k + i, j) - v; });
   wave_gemm<T>(k, r, k, [&](int i, int t) { return KP(i, t); }, TM,
-               [&](int i, int j, T v) { Ksnr[i + (size_t)j * k] = KP(i, k + j) - v; }, wv,
-               kQdecNW);
+               [&](int i, int j, T v) { Ksnr[i + (size_t)j * k] = KP(i, k + j) - v; });
+  // X_rr = K_rr - T^T K_sr - K_r_sn T (r x r), in two products
   wave_gemm<T>(r, r, k, TT, [&](int t, int j) { return KP(t, k + j); },
-               [&](int i, int j, T v) { X[i + (size_t)j * r] = KP(k + i, k + j) - v; }, wv,
-               kQdecNW);
-  __syncthreads();
+               [&](int i, int j, T v) { X[i + (size_t)j * r] = KP(k + i, k + j) - v; });
+  wave_sync();
   wave_gemm<T>(r, r, k, [&](int i, int t) { return Krsn[i + (size_t)t * r]; }, TM,
-               [&](int i, int j, T v) { X[i + (size_t)j * r] -= v; }, wv, kQdecNW);
-  __syncthreads();
+               [&](int i, int j, T v) { X[i + (size_t)j * r] -= v; });
+  wave_sync();
   // K_BN and T are dead: X_rr^{-1} goes where K_BN was, L where T was
   T* Z = Kp;
   T* Ls = Ts;
-  if (wv == 0) {
-    bool ok;
-    if (r <= 16) ok = wave_gj_reg<T, 16>(X, r, Z, piv, gjv);
-    else if (RMAX <= 32 || r <= 32) ok = wave_gj_reg<T, 32>(X, r, Z, piv, gjv);
-    else ok = wave_gj_reg<T, (RMAX > 32 ? 64 : 32)>(X, r, Z, piv, gjv);
-    if (!ok && lane == 0) flag(status, 4);
-  }
-  __syncthreads();
+  bool ok;
+  if (r <= 16) ok = wave_gj_reg<T, 16>(X, r, Z, piv, gjv);
+  else if (RMAX <= 32 || r <= 32) ok = wave_gj_reg<T, 32>(X, r, Z, piv, gjv);
+  else ok = wave_gj_reg<T, (RMAX > 32 ? 64 : 32)>(X, r, Z, piv, gjv);
+  if (!ok && lane == 0) flag(status, 4);
   T* Lm = slab + (size_t)k * r;        // k x r
   T* Um = slab + 2 * (size_t)k * r;    // r x k
   T* Xinv = slab + 3 * (size_t)k * r;  // r x r
@@ -355,44 +370,43 @@ __device__ void decouple_quad_box(
   auto ZZ = [&](int i, int j) -> T { return Z[i + (size_t)j * r]; };
   // L0 = K_sn_r X^{-1} (k x r) into Ls; U = X^{-1} K_r_sn (r x k) into the slab
   wave_gemm<T>(k, r, r, [&](int i, int t) { return Ksnr[i + (size_t)t * k]; }, ZZ,
-               [&](int i, int j, T v) { Ls[i + (size_t)j * k] = v; }, wv, kQdecNW);
+               [&](int i, int j, T v) { Ls[i + (size_t)j * k] = v; });
   wave_gemm<T>(r, k, r, ZZ, [&](int t, int j) { return Krsn[t + (size_t)j * r]; },
-               [&](int i, int j, T v) { Um[i + (size_t)j * r] = v; }, wv, kQdecNW);
-  for (int e = tid; e < r * r; e += kQdecTB) Xinv[e] = Z[e];
-  if (tid < r) piv_pool[aoff + tid] = piv[tid];
-  __syncthreads();
+               [&](int i, int j, T v) { Um[i + (size_t)j * r] = v; });
+  for (int e = lane; e < r * r; e += kWave) Xinv[e] = Z[e];
+  if (lane < r) piv_pool[aoff + lane] = piv[lane];
+  wave_sync();
   // one refinement step of L against X_rr (see k_decouple): R = K_sn_r - L0 X_rr
   // in place of K_sn_r, then L = L0 + R X_rr^{-1}
   wave_gemm<T>(k, r, r, [&](int i, int t) { return Ls[i + (size_t)t * k]; },
                [&](int t, int j) { return X[t + (size_t)j * r]; },
-               [&](int i, int j, T v) { Ksnr[i + (size_t)j * k] -= v; }, wv, kQdecNW);
-  __syncthreads();
+               [&](int i, int j, T v) { Ksnr[i + (size_t)j * k] -= v; });
+  wave_sync();
   wave_gemm<T>(k, r, r, [&](int i, int t) { return Ksnr[i + (size_t)t * k]; }, ZZ,
                [&](int i, int j, T v) {
                  const T l = Ls[i + (size_t)j * k] + v;
                  Ls[i + (size_t)j * k] = l;
                  Lm[i + (size_t)j * k] = l;
-               },
-               wv, kQdecNW);
-  __syncthreads();
+               });
+  wave_sync();
   // schur_update = L K_r_sn (k x k) where X_rr^{-1} was, plus the
   // descendants' update on the skeleton, then out to the slab
   T* Sa = Kp;
   wave_gemm<T>(k, k, r, [&](int i, int t) { return Ls[i + (size_t)t * k]; },
                [&](int t, int j) { return Krsn[t + (size_t)j * r]; },
-               [&](int i, int j, T v) { Sa[i + (size_t)j * k] = v; }, wv, kQdecNW);
-  __syncthreads();
+               [&](int i, int j, T v) { Sa[i + (size_t)j * k] = v; });
+  wave_sync();
   if (!leaf) {
-    for (int e0 = wv * 8 * kWave; e0 < cu.eo[4]; e0 += kQdecNW * 8 * kWave)
+    for (int e0 = 0; e0 < cu.eo[4]; e0 += 8 * kWave)
       child_update_batch<T>(cu, e0, [&](int a, int b, T v) {
         if (a >= n || b >= n) return;
         const int i = ipos[a], j = ipos[b];
         if (i < k && j < k) Sa[i + (size_t)j * k] += v;
       });
-    __syncthreads();
+    wave_sync();
   }
-  for (int e = tid; e < k * k; e += kQdecTB) Acc[e] = Sa[e];
-  if (tid == 0) {
+  for (int e = lane; e < k * k; e += kWave) Acc[e] = Sa[e];
+  if (lane == 0) {
     osize[x] = k;
     acc_ptr[x] = reinterpret_cast<uint64_t>(Acc);
     acc_ld[x] = k;

@@ -1354,16 +1354,7 @@ void k_decouple(const int* nodes, int a, GeomView g, int pde, const int* is_leaf
   }
 }
 
-// ---------------------------------------------------------------------------
-// elimination with one wavefront per box (|active_box| <= 64): the same
-// steps as k_decouple, with the products on the matrix cores
-// (v_mfma_f32_16x16x4f32 for the FP32 factor, v_mfma_f64_16x16x4f64 for
-// FP64) and X_rr^{-1} by a Gauss-Jordan held in registers, row i in lane i,
-// so no step waits on a workgroup barrier.
-// ---------------------------------------------------------------------------
-
-// RMAX: the largest r the launch's boxes can have (n <= RMAX + 1); the
-// register Gauss-Jordan is instantiated up to that width only.
+// elimination with one wavefront per box (decouple_wave_box in elim_dev.h)
 template <typename T, int RMAX>
 __global__ __launch_bounds__(kWave) void k_decouple_wave(
     const int* nodes, GeomView g, int pde, const int* is_leaf, const int* child,
@@ -1372,207 +1363,9 @@ __global__ __launch_bounds__(kWave) void k_decouple_wave(
     int* piv_pool, const uint64_t* fac_ptr, int* osize, uint64_t* acc_ptr, int* acc_ld,
     int* status) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int lane = threadIdx.x;
-  const int x = nodes[blockIdx.x];
-  // loads that need only x, issued together
-  const int n = nact[x];
-  const int kr = rank[x];
-  const bool leaf = is_leaf[x];
-  const int y = lane < 4 ? child[4 * x + lane] : -1;  // -1 for a leaf's slots
-  const int64_t aoff = act_off[x];
-  T* slab = reinterpret_cast<T*>(fac_ptr[x]);
-  if (n == 0) {
-    if (lane == 0) {
-      osize[x] = 0;
-      acc_ptr[x] = 0;
-      acc_ld[x] = 0;
-    }
-    return;
-  }
-  const int k = kr < 0 ? n : kr, r = n - k;
-  // the second round: children (lanes 0..3), the [skel | redundant] lists
-  // (one DOF per lane, n <= 64) and T (k x r, 8 values per lane per batch)
-  const int csz = (!leaf && y >= 0) ? osize_in[y] : 0;
-  const uint64_t cap = (!leaf && y >= 0) ? acc_ptr_in[y] : 0;
-  const int cldl = (!leaf && y >= 0) ? acc_ld_in[y] : 0;
-  int md = 0, ps = 0;
-  if (kr >= 0 && lane < n) {
-    md = sr_pool[aoff + lane];
-    ps = pos_pool[aoff + lane];
-  }
-  const int kt = kr >= 0 ? k * r : 0;
-  T tv[8];
-#pragma unroll
-  for (int u = 0; u < 8; ++u) {
-    const int e = u * kWave + lane;
-    tv[u] = e < kt ? slab[e] : T(0);
-  }
-  ChildUpd cu;
-  cu.co[0] = cu.eo[0] = 0;
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int sz = __shfl(csz, c, kWave);
-    cu.co[c + 1] = cu.co[c] + sz;
-    cu.eo[c + 1] = cu.eo[c] + sz * sz;
-    cu.cacc[c] = (uint64_t)__shfl((long long)cap, c, kWave);
-    cu.cld[c] = __shfl(cldl, c, kWave);
-  }
-  if (kr < 0) {  // not compressed: hand the whole active list and its update up
-    if (!leaf) {  // lane a writes row a, each entry once
-      const int ca = (lane >= cu.co[1]) + (lane >= cu.co[2]) + (lane >= cu.co[3]);
-      const T* acca = reinterpret_cast<const T*>(pick4(cu.cacc, ca));
-      const int coa = pick4(cu.co, ca), lda = pick4(cu.cld, ca);
-      for (int b0 = 0; b0 < n; b0 += 8) {
-        T v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int b = b0 + u;
-          const int cb = (b >= cu.co[1]) + (b >= cu.co[2]) + (b >= cu.co[3]);
-          const bool in = lane < n && b < n && cb == ca && b < cu.co[4] && lane < cu.co[4] &&
-                          acca;
-          v[u] = in ? acca[(lane - coa) + (size_t)(b - coa) * lda] : T(0);
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-          if (lane < n && b0 + u < n) slab[lane + (size_t)(b0 + u) * n] = v[u];
-      }
-    }
-    if (lane == 0) {
-      osize[x] = n;
-      acc_ptr[x] = leaf ? 0 : reinterpret_cast<uint64_t>(slab);
-      acc_ld[x] = n;
-    }
-    return;
-  }
-  const WdecLds Ly = wdec_layout(n, k, sizeof(T));
-  double* geo = reinterpret_cast<double*>(smem + Ly.geo);
-  int* sr = reinterpret_cast<int*>(smem + Ly.sr);
-  int* pos = reinterpret_cast<int*>(smem + Ly.pos);
-  int* ipos = reinterpret_cast<int*>(smem + Ly.ipos);
-  int* piv = reinterpret_cast<int*>(smem + Ly.piv);
-  T* gjv = reinterpret_cast<T*>(smem + Ly.gjv);  // pivot row of the Gauss-Jordan
-  T* Kp = reinterpret_cast<T*>(smem + Ly.kp);
-  T* X = reinterpret_cast<T*>(smem + Ly.x);
-  T* Ts = reinterpret_cast<T*>(smem + Ly.tm);
-  T* Krsn = reinterpret_cast<T*>(smem + Ly.krsn);
-  T* Ksnr = reinterpret_cast<T*>(smem + Ly.ksnr);
-  // the third round: this lane's DOF geometry (kept in registers for its row)
-  double mg[kGeo] = {0, 0, 0, 0, 0, 0};
-  if (lane < n) stage_geo(g, md, mg);
-  if (lane < n) {
-    sr[lane] = md;
-    pos[lane] = ps;
-    ipos[ps] = lane;
-#pragma unroll
-    for (int q = 0; q < kGeo; ++q) geo[kGeo * lane + q] = mg[q];
-  }
-#pragma unroll
-  for (int u = 0; u < 8; ++u) {
-    const int e = u * kWave + lane;
-    if (e < kt) Ts[e] = tv[u];
-  }
-  for (int e0 = 8 * kWave; e0 < kt; e0 += kWave) {  // T past the first batch
-    const int e = e0 + lane;
-    if (e < kt) Ts[e] = slab[e];
-  }
-  wave_sync();
-  // K_BN = K(BN, BN) - update, permuted to [s r] (skel_factorization.cpp:66-78):
-  // 64 / n lanes per row, lane l on row l % n and every (64 / n)-th column
-  // from l / n, four columns in flight
-  {
-    const int npl = kWave / n;
-    if (lane < npl * n) {
-      const int i = lane % n;
-      double tg[kGeo];
-#pragma unroll
-      for (int q = 0; q < kGeo; ++q) tg[q] = geo[kGeo * i + q];
-      const int ti = sr[i];
-#pragma unroll 4
-      for (int j = lane / n; j < n; j += npl)
-        Kp[i + (size_t)j * n] = (T)staged_entry(pde, ti, tg, sr[j], geo + kGeo * j);
-    }
-  }
-  wave_sync();
-  if (!leaf) {
-    for (int e0 = 0; e0 < cu.eo[4]; e0 += 8 * kWave)
-      child_update_batch<T>(cu, e0, [&](int a, int b, T v) {
-        if (a >= n || b >= n) return;
-        const size_t e = ipos[a] + (size_t)ipos[b] * n;
-        Kp[e] = (T)((double)Kp[e] - (double)v);
-      });
-    wave_sync();
-  }
-  auto KP = [&](int i, int j) -> T { return Kp[i + (size_t)j * n]; };
-  auto TT = [&](int i, int t) -> T { return Ts[t + (size_t)i * k]; };  // T^T (r x k)
-  auto TM = [&](int t, int j) -> T { return Ts[t + (size_t)j * k]; };  // T (k x r)
-  // K_r_sn = K_rs - T^T K_ss (r x k); K_sn_r = K_sr - K_ss T (k x r)
-  wave_gemm<T>(r, k, k, TT, [&](int t, int j) { return KP(t, j); },
-               [&](int i, int j, T v) { Krsn[i + (size_t)j * r] = KP(k + i, j) - v; });
-  wave_gemm<T>(k, r, k, [&](int i, int t) { return KP(i, t); }, TM,
-               [&](int i, int j, T v) { Ksnr[i + (size_t)j * k] = KP(i, k + j) - v; });
-  // X_rr = K_rr - T^T K_sr - K_r_sn T (r x r), in two products
-  wave_gemm<T>(r, r, k, TT, [&](int t, int j) { return KP(t, k + j); },
-               [&](int i, int j, T v) { X[i + (size_t)j * r] = KP(k + i, k + j) - v; });
-  wave_sync();
-  wave_gemm<T>(r, r, k, [&](int i, int t) { return Krsn[i + (size_t)t * r]; }, TM,
-               [&](int i, int j, T v) { X[i + (size_t)j * r] -= v; });
-  wave_sync();
-  // K_BN and T are dead: X_rr^{-1} goes where K_BN was, L where T was
-  T* Z = Kp;
-  T* Ls = Ts;
-  bool ok;
-  if (r <= 16) ok = wave_gj_reg<T, 16>(X, r, Z, piv, gjv);
-  else if (RMAX <= 32 || r <= 32) ok = wave_gj_reg<T, 32>(X, r, Z, piv, gjv);
-  else ok = wave_gj_reg<T, (RMAX > 32 ? 64 : 32)>(X, r, Z, piv, gjv);
-  if (!ok && lane == 0) flag(status, 4);
-  T* Lm = slab + (size_t)k * r;        // k x r
-  T* Um = slab + 2 * (size_t)k * r;    // r x k
-  T* Xinv = slab + 3 * (size_t)k * r;  // r x r
-  T* Acc = Xinv + (size_t)r * r;       // k x k
-  auto ZZ = [&](int i, int j) -> T { return Z[i + (size_t)j * r]; };
-  // L0 = K_sn_r X^{-1} (k x r) into Ls; U = X^{-1} K_r_sn (r x k) into the slab
-  wave_gemm<T>(k, r, r, [&](int i, int t) { return Ksnr[i + (size_t)t * k]; }, ZZ,
-               [&](int i, int j, T v) { Ls[i + (size_t)j * k] = v; });
-  wave_gemm<T>(r, k, r, ZZ, [&](int t, int j) { return Krsn[t + (size_t)j * r]; },
-               [&](int i, int j, T v) { Um[i + (size_t)j * r] = v; });
-  for (int e = lane; e < r * r; e += kWave) Xinv[e] = Z[e];
-  if (lane < r) piv_pool[aoff + lane] = piv[lane];
-  wave_sync();
-  // one refinement step of L against X_rr (see k_decouple): R = K_sn_r - L0 X_rr
-  // in place of K_sn_r, then L = L0 + R X_rr^{-1}
-  wave_gemm<T>(k, r, r, [&](int i, int t) { return Ls[i + (size_t)t * k]; },
-               [&](int t, int j) { return X[t + (size_t)j * r]; },
-               [&](int i, int j, T v) { Ksnr[i + (size_t)j * k] -= v; });
-  wave_sync();
-  wave_gemm<T>(k, r, r, [&](int i, int t) { return Ksnr[i + (size_t)t * k]; }, ZZ,
-               [&](int i, int j, T v) {
-                 const T l = Ls[i + (size_t)j * k] + v;
-                 Ls[i + (size_t)j * k] = l;
-                 Lm[i + (size_t)j * k] = l;
-               });
-  wave_sync();
-  // schur_update = L K_r_sn (k x k) where X_rr^{-1} was, plus the
-  // descendants' update on the skeleton, then out to the slab
-  T* Sa = Kp;
-  wave_gemm<T>(k, k, r, [&](int i, int t) { return Ls[i + (size_t)t * k]; },
-               [&](int t, int j) { return Krsn[t + (size_t)j * r]; },
-               [&](int i, int j, T v) { Sa[i + (size_t)j * k] = v; });
-  wave_sync();
-  if (!leaf) {
-    for (int e0 = 0; e0 < cu.eo[4]; e0 += 8 * kWave)
-      child_update_batch<T>(cu, e0, [&](int a, int b, T v) {
-        if (a >= n || b >= n) return;
-        const int i = ipos[a], j = ipos[b];
-        if (i < k && j < k) Sa[i + (size_t)j * k] += v;
-      });
-    wave_sync();
-  }
-  for (int e = lane; e < k * k; e += kWave) Acc[e] = Sa[e];
-  if (lane == 0) {
-    osize[x] = k;
-    acc_ptr[x] = reinterpret_cast<uint64_t>(Acc);
-    acc_ld[x] = k;
-  }
+  decouple_wave_box<T, RMAX>(nodes[blockIdx.x], smem, g, pde, is_leaf, child, osize_in,
+                             acc_ptr_in, acc_ld_in, nact, rank, act_off, sr_pool, pos_pool,
+                             piv_pool, fac_ptr, osize, acc_ptr, acc_ld, status);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,6 +1,6 @@
-// CPU driver for the elimination of one box by four wavefronts
-// (decouple_quad_box, kern-interp-old_amd/csrc/elim_dev.h, the body of
-// k_decouple_quad), run on emulated GPU threads (tests/cpp/simt) and built
+// CPU driver for the elimination of one box by one wavefront
+// (decouple_wave_box, kern-interp-old_amd/csrc/elim_dev.h, the body of
+// k_decouple_wave), run on emulated GPU threads (tests/cpp/simt) and built
 // with AddressSanitizer by tests/test_elim_emulated.py.  The box is node 0 of
 // a small made-up tree whose children are nodes 1..4; every array the
 // routine touches -- geometry, pools, the factor slab, the children's
@@ -82,8 +82,8 @@ static int run(int pde, int sd, int npts, int n, int k, int leaf, const int* csz
   int status = 0;
   const size_t lds = k < n ? wdec_layout(n, k, sizeof(T)).total : 16;
   unsigned char* smem = new unsigned char[lds];
-  simt::launch(kQdecTB, [&] {
-    decouple_quad_box<T, RMAX>(0, smem, g, pde, is_leaf, child, osize_in, acc_in, acc_ld_in,
+  simt::launch(kWave, [&] {
+    decouple_wave_box<T, RMAX>(0, smem, g, pde, is_leaf, child, osize_in, acc_in, acc_ld_in,
                                nact, rank, act_off, sr_pool, pos_pool, piv_pool, fac_ptr,
                                osize, acc_ptr, acc_ld, &status);
   });

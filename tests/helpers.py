@@ -7,7 +7,7 @@ from oracle import kernel as K
 # The FP64 factorization of small Stokes problems with holes (the 8-hole disc
 # of test_gpu_border and the annulus of test_gpu_domain) faulted the GPU with
 # an illegal memory access inside kigpu_factor, in runs where a four-wave
-# elimination kernel (since taken off the GPU) had run earlier in the same
+# elimination kernel (since removed) had run earlier in the same
 # process, after passing before; no cause has been found on the CPU, so these
 # cases stay off the GPU.
 STOKES_SMALL_BORDER_FAULT = (

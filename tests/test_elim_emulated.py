@@ -1,5 +1,5 @@
-"""The elimination of one box by four wavefronts (decouple_quad_box in
-kern-interp-old_amd/csrc/elim_dev.h, the body of k_decouple_quad) on the CPU,
+"""The elimination of one box by one wavefront (decouple_wave_box in
+kern-interp-old_amd/csrc/elim_dev.h, the body of k_decouple_wave) on the CPU,
 on emulated GPU threads (tests/cpp/simt) under AddressSanitizer and UBSan:
 every array it indexes -- geometry, pools, the factor slab, the children's
 updates, the dynamic LDS -- is a heap block of exactly the box's size.
