@@ -1141,217 +1141,25 @@ __global__ __launch_bounds__(kGramTB) void k_id_gram(const IdWork* work, GeomVie
   if (tid == 0) rank[w.node] = k;
 }
 
-// ---------------------------------------------------------------------------
-// elimination of the redundant DOFs (decouple, skel_factorization.cpp:55-91)
-// ---------------------------------------------------------------------------
-
-struct DecLds {
-  size_t uact, kp, x, krsn, ksnr, tm, lm, sr, pos, piv, sig, total;
-};
-
-__host__ __device__ inline DecLds dec_layout(int n, size_t tsz) {
-  DecLds L;
-  size_t o = 0;
-  const size_t nn = (size_t)n * n, q4 = nn / 4 + 1;
-  L.uact = o; o = align16(o + nn * tsz);
-  L.kp = o;   o = align16(o + nn * tsz);
-  L.x = o;    o = align16(o + nn * tsz);
-  L.krsn = o; o = align16(o + q4 * tsz);
-  L.ksnr = o; o = align16(o + q4 * tsz);
-  L.tm = o;   o = align16(o + q4 * tsz);
-  L.lm = o;   o = align16(o + q4 * tsz);
-  L.sr = o;   o = align16(o + sizeof(int) * (size_t)n);
-  L.pos = o;  o = align16(o + sizeof(int) * (size_t)n);
-  L.piv = o;  o = align16(o + sizeof(int) * (size_t)n);
-  L.sig = o;  o = align16(o + sizeof(int) * (size_t)n);
-  L.total = o;
-  return L;
-}
-
-// Factor slab of a compressed node (k skeleton, r redundant DOFs):
-//   T (k x r) | L (k x r) | U (r x k) | X_rr^{-1} (r x r) | Acc (k x k)
-// all column-major with leading dimension = height.
-__host__ __device__ inline size_t slab_elems(int n) { return ((size_t)n * n * 5 + 3) / 4 + 4; }
-
-constexpr int kDecTB = 1024;
-constexpr int kDecNW = kDecTB / kWave;
-
-// At most 64 VGPRs (8 waves per SIMD) so that two boxes share a CU when
-// their LDS fits: the elimination of one box is a chain of latency-bound
-// steps, and a second box fills the gaps.
+// elimination with one 1024-thread workgroup per box (decouple_block_box in
+// elim_dev.h).  At most 64 VGPRs (8 waves per SIMD) so that two boxes share
+// a CU when their LDS fits: the elimination of one box is a chain of
+// latency-bound steps, and a second box fills the gaps.
 template <typename T, bool LDS>
 __global__ __launch_bounds__(kDecTB) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void k_decouple(const int* nodes, int a, GeomView g, int pde, const int* is_leaf,
-                                                     const int* child, const int* osize_in,
-                                                     const uint64_t* acc_ptr_in,
-                                                     const int* acc_ld_in, const int* nact,
-                                                     const int* rank, const int64_t* act_off,
-                                                     const int* act_pool, const int* sr_pool,
-                                                     const int* pos_pool, int* piv_pool,
-                                                     const uint64_t* fac_ptr, int* osize,
-                                                     uint64_t* acc_ptr, int* acc_ld, T* ws,
-                                                     size_t ws_stride, int* status) {
+                const int* child, const int* osize_in, const uint64_t* acc_ptr_in,
+                const int* acc_ld_in, const int* nact, const int* rank, const int64_t* act_off,
+                const int* sr_pool, const int* pos_pool, int* piv_pool, const uint64_t* fac_ptr,
+                int* osize, uint64_t* acc_ptr, int* acc_ld, T* ws, size_t ws_stride,
+                int* status) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int x = nodes ? nodes[blockIdx.x] : a + blockIdx.x;
-  const int n = nact[x];
-  const int k = rank[x];
-  const int tid = threadIdx.x;
-  if (n == 0) {
-    if (tid == 0) {
-      osize[x] = 0;
-      acc_ptr[x] = 0;
-      acc_ld[x] = 0;
-    }
-    return;
-  }
-  const DecLds Ly = dec_layout(n, sizeof(T));
-  unsigned char* base = LDS ? smem : reinterpret_cast<unsigned char*>(ws + ws_stride * blockIdx.x);
-  T* Uact = reinterpret_cast<T*>(base + Ly.uact);
-  T* slab = reinterpret_cast<T*>(fac_ptr[x]);
-  const bool leaf = is_leaf[x];
-
-  // accumulated updates of the children, block diagonal in active order
-  if (!leaf) {
-    for (int i = tid; i < n * n; i += kDecTB) Uact[i] = T(0);
-    __syncthreads();
-    int o = 0;
-    for (int s = 0; s < 4; ++s) {
-      int y = child[4 * x + s];
-      int sz = osize_in[y];
-      const T* acc = reinterpret_cast<const T*>(acc_ptr_in[y]);
-      if (acc) {
-        int ld = acc_ld_in[y];
-        for2d<kDecNW>(sz, sz, [&](int i, int j) {
-          Uact[(o + i) + (size_t)(o + j) * n] = acc[i + (size_t)j * ld];
-        });
-      }
-      o += sz;
-    }
-    __syncthreads();
-  }
-  if (k < 0) {  // not compressed: hand the whole active list and its update up
-    if (!leaf) {
-      for (int i = tid; i < n * n; i += kDecTB) slab[i] = Uact[i];
-    }
-    if (tid == 0) {
-      osize[x] = n;
-      acc_ptr[x] = leaf ? 0 : reinterpret_cast<uint64_t>(slab);
-      acc_ld[x] = n;
-    }
-    return;
-  }
-  const int r = n - k;
-  T* Kp = reinterpret_cast<T*>(base + Ly.kp);
-  T* X = reinterpret_cast<T*>(base + Ly.x);
-  T* Krsn = reinterpret_cast<T*>(base + Ly.krsn);
-  T* Ksnr = reinterpret_cast<T*>(base + Ly.ksnr);
-  T* Ts = reinterpret_cast<T*>(base + Ly.tm);
-  T* Ls = reinterpret_cast<T*>(base + Ly.lm);
-  int* sr = reinterpret_cast<int*>(base + Ly.sr);
-  int* pos = reinterpret_cast<int*>(base + Ly.pos);
-  int* piv = reinterpret_cast<int*>(base + Ly.piv);
-  int* sig = reinterpret_cast<int*>(base + Ly.sig);
-  const int64_t aoff = act_off[x];
-  for (int j = tid; j < n; j += kDecTB) {
-    sr[j] = sr_pool[aoff + j];
-    pos[j] = pos_pool[aoff + j];
-  }
-  for (int e = tid; e < k * r; e += kDecTB) Ts[e] = slab[e];  // T from the ID
-  __syncthreads();
-  // K_BN = K(BN, BN) - update, permuted to [s r] (skel_factorization.cpp:66-78)
-  for2d<kDecNW>(n, n, [&](int i, int j) {
-    double v = kernel_entry(g, pde, sr[i], sr[j]);
-    if (!leaf) v -= (double)Uact[pos[i] + (size_t)pos[j] * n];
-    Kp[i + (size_t)j * n] = (T)v;
-  });
-  __syncthreads();
-  T* Lm = slab + (size_t)k * r;             // k x r
-  T* Um = slab + 2 * (size_t)k * r;         // r x k
-  T* Xinv = slab + 3 * (size_t)k * r;       // r x r
-  T* Acc = Xinv + (size_t)r * r;            // k x k
-  auto KP = [&](int i, int j) -> T { return Kp[i + (size_t)j * n]; };
-  // K_r_sn = K_rs - T^T K_ss (r x k); K_sn_r = K_sr - K_ss T (k x r)
-  for2d<kDecNW>(r, k, [&](int i, int j) {
-    T s = KP(k + i, j);
-    for (int t = 0; t < k; ++t) s -= Ts[t + (size_t)i * k] * KP(t, j);
-    Krsn[i + (size_t)j * r] = s;
-  });
-  for2d<kDecNW>(k, r, [&](int i, int j) {
-    T s = KP(i, k + j);
-    for (int t = 0; t < k; ++t) s -= KP(i, t) * Ts[t + (size_t)j * k];
-    Ksnr[i + (size_t)j * k] = s;
-  });
-  __syncthreads();
-  // X_rr = K_rr - T^T K_sr - K_r_sn T (r x r)
-  for2d<kDecNW>(r, r, [&](int i, int j) {
-    T s = KP(k + i, k + j);
-    for (int t = 0; t < k; ++t) s -= Ts[t + (size_t)i * k] * KP(t, k + j);
-    for (int t = 0; t < k; ++t) s -= Krsn[i + (size_t)t * r] * Ts[t + (size_t)j * k];
-    X[i + (size_t)j * r] = s;
-  });
-  __syncthreads();
-  // K_BN and T are dead from here on: the K_BN area holds a copy of X_rr and
-  // the two pivot scratch vectors (r^2 + 2r <= n^2 since k >= 1)
-  T* Xc = Kp;
-  for (int e = tid; e < r * r; e += kDecTB) Xc[e] = X[e];
-  __syncthreads();
-  // X_rr^{-1} in place (Gauss-Jordan with dgetrf's pivot order), columns
-  // left permuted: X_rr^{-1}(i, j) = Z[i + sig[j] r].  Scratch: two vectors
-  // after the copy of X_rr, one after X_rr (r^2 + r <= n^2).
-  T* Z = X;
   __shared__ int s_p;
-  if (!block_gj_inverse_perm(X, r, piv, sig, Xc + (size_t)r * r, X + (size_t)r * r,
-                             Xc + (size_t)r * r + r, &s_p) &&
-      tid == 0)
-    flag(status, 4);
-  // L0 = K_sn_r X_rr^{-1} (k x r), U = X_rr^{-1} K_r_sn (r x k).  The right
-  // residual of an explicit inverse is small but its left residual is not:
-  // multiplied from the right, X_rr^{-1} loses digits when X_rr is
-  // ill-conditioned (tight id_tol), so L takes one refinement step against
-  // X_rr below, which brings it to the accuracy of the dgetrs solve of
-  // skel_factorization.cpp:98-103.
-  for2d<kDecNW>(k, r, [&](int i, int j) {
-    const T* zj = Z + (size_t)sig[j] * r;
-    T s = T(0);
-    for (int t = 0; t < r; ++t) s += Ksnr[i + (size_t)t * k] * zj[t];
-    Ls[i + (size_t)j * k] = s;
-  });
-  for2d<kDecNW>(r, k, [&](int i, int j) {
-    T s = T(0);
-    for (int t = 0; t < r; ++t) s += Z[i + (size_t)sig[t] * r] * Krsn[t + (size_t)j * r];
-    Um[i + (size_t)j * r] = s;
-  });
-  for2d<kDecNW>(r, r, [&](int i, int j) { Xinv[i + (size_t)j * r] = Z[i + (size_t)sig[j] * r]; });
-  for (int c = tid; c < r; c += kDecTB) piv_pool[aoff + c] = piv[c];
-  __syncthreads();
-  // R = K_sn_r - L0 X_rr in place of K_sn_r (each entry reads only itself)
-  for2d<kDecNW>(k, r, [&](int i, int j) {
-    T s = Ksnr[i + (size_t)j * k];
-    for (int t = 0; t < r; ++t) s -= Ls[i + (size_t)t * k] * Xc[t + (size_t)j * r];
-    Ksnr[i + (size_t)j * k] = s;
-  });
-  __syncthreads();
-  // L = L0 + R X_rr^{-1}
-  for2d<kDecNW>(k, r, [&](int i, int j) {
-    const T* zj = Z + (size_t)sig[j] * r;
-    T s = Ls[i + (size_t)j * k];
-    for (int t = 0; t < r; ++t) s += Ksnr[i + (size_t)t * k] * zj[t];
-    Ls[i + (size_t)j * k] = s;
-    Lm[i + (size_t)j * k] = s;
-  });
-  __syncthreads();
-  // schur_update = L K_r_sn (k x k), plus the descendants' update on the skeleton
-  for2d<kDecNW>(k, k, [&](int i, int j) {
-    T s = T(0);
-    for (int t = 0; t < r; ++t) s += Ls[i + (size_t)t * k] * Krsn[t + (size_t)j * r];
-    if (!leaf) s += Uact[pos[i] + (size_t)pos[j] * n];
-    Acc[i + (size_t)j * k] = s;
-  });
-  if (tid == 0) {
-    osize[x] = k;
-    acc_ptr[x] = reinterpret_cast<uint64_t>(Acc);
-    acc_ld[x] = k;
-  }
+  const int x = nodes ? nodes[blockIdx.x] : a + blockIdx.x;
+  unsigned char* base = LDS ? smem : reinterpret_cast<unsigned char*>(ws + ws_stride * blockIdx.x);
+  decouple_block_box<T>(x, base, &s_p, g, pde, is_leaf, child, osize_in, acc_ptr_in, acc_ld_in,
+                        nact, rank, act_off, sr_pool, pos_pool, piv_pool, fac_ptr, osize,
+                        acc_ptr, acc_ld, status);
 }
 
 // elimination with one wavefront per box (decouple_wave_box in elim_dev.h)
@@ -1752,8 +1560,8 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
           allow_lds(k_decouple<T, true>, L.total);
           k_decouple<T, true><<<nl, kDecTB, L.total, st>>>(
               d_dec_large, a, g, pde, t.is_leaf.p, t.child.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p,
-              t.nact.p, t.rank.p, t.act_off.p, t.act_pool.p, t.sr_pool.p, t.pos_pool.p,
-              t.piv_pool.p, t.fac_ptr.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, (T*)nullptr, 0,
+              t.nact.p, t.rank.p, t.act_off.p, t.sr_pool.p, t.pos_pool.p, t.piv_pool.p,
+              t.fac_ptr.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, (T*)nullptr, 0,
               t.status.p);
           return;
         }
@@ -1768,8 +1576,8 @@ static void factor_levels(Context& c, TreeData& t, const kigpu_factor_opts& o) {
         }
         k_decouple<T, false><<<nl, kDecTB, 0, st>>>(
             d_dec_large, a, g, pde, t.is_leaf.p, t.child.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p,
-            t.nact.p, t.rank.p, t.act_off.p, t.act_pool.p, t.sr_pool.p, t.pos_pool.p,
-            t.piv_pool.p, t.fac_ptr.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, ws, stride,
+            t.nact.p, t.rank.p, t.act_off.p, t.sr_pool.p, t.pos_pool.p, t.piv_pool.p,
+            t.fac_ptr.p, t.osize.p, t.acc_ptr.p, t.acc_ld.p, ws, stride,
             t.status.p);
       };
       // the middle size class on the side stream beside the small boxes and

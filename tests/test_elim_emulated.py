@@ -1,5 +1,6 @@
-"""The elimination of one box by one wavefront (decouple_wave_box in
-kern-interp-old_amd/csrc/elim_dev.h, the body of k_decouple_wave) on the CPU,
+"""The elimination of one box by one wavefront (decouple_wave_box, the body of
+k_decouple_wave) and by one 1024-thread workgroup (decouple_block_box, the
+body of k_decouple), both in kern-interp-old_amd/csrc/elim_dev.h, on the CPU,
 on emulated GPU threads (tests/cpp/simt) under AddressSanitizer and UBSan:
 every array it indexes -- geometry, pools, the factor slab, the children's
 updates, the dynamic LDS -- is a heap block of exactly the box's size.
@@ -51,8 +52,10 @@ def make_box(pde, n, k, children, seed):
     b = G.star(512)
     npts = 512
     kern = K.Kernel(sd, pde, b)
-    start = int(rng.integers(0, npts - 64))
-    pts = start + np.sort(rng.choice(64, size=(n + sd - 1) // sd, replace=False))
+    npb = (n + sd - 1) // sd
+    span = max(64, npb + 16)
+    start = int(rng.integers(0, npts - span))
+    pts = start + np.sort(rng.choice(span, size=npb, replace=False))
     dofs = np.array([p * sd + c for p in pts for c in range(sd)])[:n]
     perm = rng.permutation(n)           # [skel | redundant] -> active position
     T = rng.standard_normal((k, n - k)) * 0.5 if k < n else np.zeros((0, 0))
@@ -123,8 +126,19 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("pde,n,k,children,tsize,rmax", CASES)
-def test_quad_elimination(emu, pde, n, k, children, tsize, rmax):
+BLOCK_CASES = [
+    # the workgroup kernel (rmax 0): boxes beyond the wavefront kernel's 64 DOFs too
+    (K.LAPLACE, 20, 6, (), 8, 0),
+    (K.LAPLACE, 64, 9, (16, 16, 16, 16), 4, 0),
+    (K.LAPLACE, 130, 41, (30, 40, 0, 60), 8, 0),
+    (K.STOKES, 96, 40, (), 8, 0),
+    (K.STOKES, 150, 76, (40, 36, 38, 36), 4, 0),
+    (K.STOKES, 62, 61, (30, 32), 8, 0),
+]
+
+
+@pytest.mark.parametrize("pde,n,k,children,tsize,rmax", CASES + BLOCK_CASES)
+def test_elimination(emu, pde, n, k, children, tsize, rmax):
     box = make_box(pde, n, k, children, seed=n * 100 + k)
     status, osize, slab = run_box(emu, box, pde, n, k, children, tsize, rmax)
     assert status == 0 and osize == k
@@ -143,11 +157,12 @@ def test_quad_elimination(emu, pde, n, k, children, tsize, rmax):
         assert np.abs(got[name] - ref).max() <= tol * scale * max(1.0, np.linalg.cond(Xi)), name
 
 
+@pytest.mark.parametrize("rmax", [64, 0])
 @pytest.mark.parametrize("pde,children", [(K.LAPLACE, (10, 14)), (K.STOKES, (20, 24))])
-def test_uncompressed_box_hands_update_up(emu, pde, children):
+def test_uncompressed_box_hands_update_up(emu, pde, children, rmax):
     n = sum(children)
     box = make_box(pde, n, n, children, seed=7)
-    status, osize, slab = run_box(emu, box, pde, n, n, children, 8, 64)
+    status, osize, slab = run_box(emu, box, pde, n, n, children, 8, rmax)
     assert status == 0 and osize == n
     U = np.zeros((n, n))
     o = 0
