@@ -13,18 +13,6 @@ namespace kigpu {
 constexpr int kStatusSingular = 4 | 8 | 16;
 
 // one box to compress at a level
-struct IdWork {
-  int node;
-  int n;          // |active_box|
-  int q;          // #near (inner-circle) or outside rows
-  int m;          // ID matrix rows: 2q (+ proxy rows)
-  int proxy;      // proxy block appended
-  int in_lds;     // ID matrix kept in LDS
-  int64_t rows_off;
-  int64_t a_off;  // global workspace offset when !in_lds
-  double cx, cy, rad, pw;
-};
-
 struct LevelWork {
   int level = 0;
   int a = 0, b = 0;         // node id range

@@ -26,4 +26,17 @@ struct GeomView {
   int sd;
 };
 
+// One box's ID work item (host list, read by the ID kernels).
+struct IdWork {
+  int node;
+  int n;          // |active_box|
+  int q;          // #near (inner-circle) or outside rows
+  int m;          // ID matrix rows: 2q (+ proxy rows)
+  int proxy;      // proxy block appended
+  int in_lds;     // ID matrix kept in LDS
+  int64_t rows_off;
+  int64_t a_off;  // global workspace offset when !in_lds
+  double cx, cy, rad, pw;
+};
+
 }  // namespace kigpu

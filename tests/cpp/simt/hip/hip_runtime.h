@@ -22,6 +22,8 @@
 #define __host__
 #define __global__
 #define __forceinline__ inline
+// a variable in LDS: one copy for the process, which runs one workgroup
+#define __shared__ static
 
 namespace simt {
 
