@@ -813,13 +813,16 @@ __device__ __forceinline__ void id_gram_box(const IdWork& w, unsigned char* smem
       // row i of R: R(i, c) = (G(p, c) - sum_{l < i} R(l, p) R(l, c)) / R_ii
       // for the columns c at the logical positions after i; R(l, :) is row
       // s_pv[l] of G
-      double r = 0;
+      // Every lane runs the loop (the whole wavefront takes part in the
+      // readlane); lanes at other positions work on column p and drop the
+      // result, so the loop has no branch and every index is in G.
       const bool after = lane > i && lane < n;
-      if (after) r = G[p + (size_t)lp * n];
+      const int lc = after ? lp : p;
+      double r = G[p + (size_t)lc * n];
 #pragma unroll 4
       for (int l = 0; l < i; ++l) {
-        const int pl = __builtin_amdgcn_readlane(mypv, l);  // read by the whole wavefront
-        if (after) r -= G[pl + (size_t)p * n] * G[pl + (size_t)lp * n];
+        const int pl = __builtin_amdgcn_readlane(mypv, l);
+        r -= G[pl + (size_t)p * n] * G[pl + (size_t)lc * n];
       }
       if (after) {
         r /= rii;
